@@ -1,0 +1,63 @@
+"""ctypes binding of libafr.so (the C ABI declared in include/afr.h)."""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafr.so")
+ABI_VERSION = 1
+
+c_vp = ctypes.c_void_p
+
+
+class AfrMeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("order", ctypes.c_int32), ("dtype", ctypes.c_int32),
+        ("n_rows", ctypes.c_int32), ("n_cells", ctypes.c_int64), ("n_prism", ctypes.c_int64),
+        ("det", c_vp), ("adj", c_vp), ("adj_fine", c_vp),
+        ("slot_nb", c_vp), ("slot_blk", c_vp), ("slot_mask", c_vp), ("slot_bc", c_vp), ("slot_fallback", c_vp),
+        ("slot_tn", c_vp), ("slot_adjrow", c_vp), ("rows", c_vp),
+        ("c2f", c_vp * 2), ("op", c_vp * 2), ("E", c_vp * 2), ("D", c_vp * 2), ("filt", c_vp * 2),
+        ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp),
+        ("bc_params", c_vp), ("work_q", c_vp), ("work_grad", c_vp),
+    ]
+
+
+# every function declared in include/afr.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "afr_version": (ctypes.c_int, []),
+    "afr_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "afr_mesh_create": (ctypes.c_int, [ctypes.POINTER(AfrMeshDesc), ctypes.POINTER(c_vp)]),
+    "afr_mesh_destroy": (ctypes.c_int, [c_vp]),
+    "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, c_vp]),
+    "afr_inviscid_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
+    "afr_viscous_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int32, c_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libafr.so; raises if it has not been built (there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m autoflowcfd_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.afr_version() != ABI_VERSION:
+        raise RuntimeError("libafr.so ABI version mismatch; rebuild it")
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        buf = ctypes.create_string_buffer(512)
+        load().afr_last_error(buf, 512)
+        raise RuntimeError(f"{what} failed: {buf.value.decode(errors='replace')}")

@@ -1,0 +1,182 @@
+// Point physics of the FR residual path (device side, templated on precision).
+//
+// Each function restates one reference routine; the branch structure and the
+// floors are the reference's own:
+//   euler_flux      src/autoflowcfd/core/fr_operators/flux_kernels.py:29-68
+//   ausm_up         src/autoflowcfd/core/fr_operators/kernels.py:20-155
+//   ghost_state     src/autoflowcfd/boundary/fr_ghost_state.py:26-189
+//   viscous_flux    src/autoflowcfd/core/fr_operators/flux_kernels.py:72-137
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace afr {
+
+template <typename real> struct Tiny;
+template <> struct Tiny<float> { static constexpr float v = 1.17549435e-38f; };
+template <> struct Tiny<double> { static constexpr double v = 1e-300; };
+
+constexpr double kGamma = 1.4;
+constexpr double kRair = 287.0;
+
+enum BcCode { BC_INTERIOR = -1, BC_WALL = 0, BC_FARFIELD = 1, BC_INLET = 2, BC_OUTLET = 3,
+              BC_SYMMETRY = 4, BC_SLIP_WALL = 5, BC_EXTRAP = 6 };
+
+template <typename real>
+__device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
+template <typename real>
+__device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
+
+// conserved (rho, rho u, rho v, rho w, rho E) -> primitive (rho, u, v, w, p); rho floor 1e-10, no p floor
+template <typename real>
+__device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {
+    const real g1 = real(kGamma - 1.0);
+    real rho = rmax(U[0], real(1e-10));
+    real u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, E = U[4] / rho;
+    Q[0] = rho; Q[1] = u; Q[2] = v; Q[3] = w;
+    Q[4] = g1 * rho * (E - real(0.5) * (u * u + v * v + w * w));
+}
+
+// physical flux F[i][v] of primitive state q
+template <typename real>
+__device__ __forceinline__ void euler_flux(const real q[5], real F[3][5]) {
+    const real g1 = real(kGamma - 1.0);
+    real rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    real rs = rmax(rho, real(1e-10));
+    real ke = real(0.5) * (u * u + v * v + w * w);
+    real rhoE = rho * (p / (g1 * rs) + ke);
+    real H = (rhoE + p) / rs;
+    real rH = rho * H;
+    real vel[3] = {u, v, w};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        real mf = rho * vel[i];
+        F[i][0] = mf;
+        F[i][1] = mf * u + (i == 0 ? p : real(0));
+        F[i][2] = mf * v + (i == 1 ? p : real(0));
+        F[i][3] = mf * w + (i == 2 ? p : real(0));
+        F[i][4] = rH * vel[i];
+    }
+}
+
+// AUSM+up normal flux density for unit direction n
+template <typename real>
+__device__ __forceinline__ void ausm_up(const real qL[5], const real qR[5], const real n[3], real out[5]) {
+    const real g = real(1.4), alpha = real(0.1875), beta = real(0.5);
+    real rhoL = rmax(qL[0], real(1e-6)), rhoR = rmax(qR[0], real(1e-6));
+    real pL = rmax(qL[4], real(10.0)), pR = rmax(qR[4], real(10.0));
+    real uL = qL[1], vL = qL[2], wL = qL[3];
+    real uR = qR[1], vR = qR[2], wR = qR[3];
+    real unL = uL * n[0] + vL * n[1] + wL * n[2];
+    real unR = uR * n[0] + vR * n[1] + wR * n[2];
+    real aL = sqrt(rmax(g * pL / rhoL, real(1e-10)));
+    real aR = sqrt(rmax(g * pR / rhoR, real(1e-10)));
+    real ML = unL / rmax(aL, real(1e-10));
+    real MR = unR / rmax(aR, real(1e-10));
+    real ah = real(0.5) * (aL + aR);
+    real rh = real(0.5) * (rhoL + rhoR);
+    real Mbar2 = (unL * unL + unR * unR) / (real(2.0) * ah * ah);
+    real M0sq = rmin(real(1.0), rmax(Mbar2, real(0.01)));
+    real M0 = sqrt(M0sq);
+    real fa = rmax(M0 * (real(2.0) - M0), real(1e-6));
+    real Mp, Mm, Pp, Pm;
+    {
+        real M = ML;
+        real aM = M < 0 ? -M : M;
+        if (aM >= real(1)) {
+            Mp = real(0.5) * (M + aM);
+            Pp = real(0.5) * (real(1) + (M > 0 ? real(1) : (M < 0 ? real(-1) : real(0))));
+        } else {
+            real t = M * M - real(1);
+            Mp = real(0.25) * (M + real(1)) * (M + real(1)) + alpha * t * t;
+            Pp = real(0.25) * ((M + real(1)) * (M + real(1)) * (real(2) - M) + beta * M * t * t);
+        }
+    }
+    {
+        real M = MR;
+        real aM = M < 0 ? -M : M;
+        if (aM >= real(1)) {
+            Mm = real(0.5) * (M - aM);
+            Pm = real(0.5) * (real(1) - (M > 0 ? real(1) : (M < 0 ? real(-1) : real(0))));
+        } else {
+            real t = M * M - real(1);
+            Mm = real(-0.25) * (M - real(1)) * (M - real(1)) - alpha * t * t;
+            Pm = real(0.25) * ((M - real(1)) * (M - real(1)) * (real(2) + M) - beta * M * t * t);
+        }
+    }
+    const real Kp = real(0.25), Ku = real(0.75);
+    real Mh = Mp + Mm;
+    real Mpress = -(Kp / fa) * rmax(real(1.0) - Mbar2, real(0.0)) * (pR - pL) / (rh * ah * ah);
+    real mass = real(0.5) * (rhoL * aL + rhoR * aR) * (Mh + Mpress);
+    real ph = Pp * pL + Pm * pR - Ku * Pp * Pm * (rhoL + rhoR) * fa * ah * (unR - unL);
+    bool up = mass >= real(0);
+    out[0] = mass;
+    out[1] = mass * (up ? uL : uR) + ph * n[0];
+    out[2] = mass * (up ? vL : vR) + ph * n[1];
+    out[3] = mass * (up ? wL : wR) + ph * n[2];
+    const real gg = g / (g - real(1));
+    real hL = gg * pL / rhoL + real(0.5) * (uL * uL + vL * vL + wL * wL);
+    real hR = gg * pR / rhoR + real(0.5) * (uR * uR + vR * vR + wR * wR);
+    out[4] = mass * (up ? hL : hR);
+}
+
+// primitive ghost state of boundary code bc; bcp = (8,5) per-code reference state
+template <typename real>
+__device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real n[3], const real* bcp, real qg[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qg[v] = qi[v];
+    real vn = qi[1] * n[0] + qi[2] * n[1] + qi[3] * n[2];
+    switch (bc) {
+        case BC_WALL:
+            qg[1] = -qi[1]; qg[2] = -qi[2]; qg[3] = -qi[3];
+            break;
+        case BC_SLIP_WALL:
+        case BC_SYMMETRY:
+            qg[1] = qi[1] - real(2) * vn * n[0];
+            qg[2] = qi[2] - real(2) * vn * n[1];
+            qg[3] = qi[3] - real(2) * vn * n[2];
+            break;
+        case BC_FARFIELD:
+#pragma unroll
+            for (int v = 0; v < 5; ++v) qg[v] = bcp[BC_FARFIELD * 5 + v];
+            break;
+        case BC_INLET:
+            if (vn < real(0)) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qg[v] = bcp[BC_INLET * 5 + v];
+            }
+            break;
+        case BC_OUTLET:
+            if (vn > real(0)) qg[4] = bcp[BC_OUTLET * 5 + 4];
+            break;
+        default:  // BC_EXTRAP and anything else: zero-gradient copy
+            break;
+    }
+}
+
+// viscous flux G[i][v]; gv[i][j] = du_i/dx_j
+template <typename real>
+__device__ __forceinline__ void viscous_flux(const real q[5], const real gv[3][3], const real gT[3], real mu, real Pr,
+                                             real mut, real Prt, real G[3][5]) {
+    real mt = mu + mut;
+    real div = gv[0][0] + gv[1][1] + gv[2][2];
+    real lam = real(-2.0 / 3.0) * mt;
+    real tau[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tau[i][j] = real(2) * mt * (real(0.5) * (gv[i][j] + gv[j][i])) + (i == j ? lam * div : real(0));
+    const real cp = real(kGamma * kRair / (kGamma - 1.0));
+    real k = mu * cp / Pr + mut * cp / Prt;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        G[i][0] = real(0);
+        G[i][1] = tau[i][0];
+        G[i][2] = tau[i][1];
+        G[i][3] = tau[i][2];
+        real work = q[1] * tau[0][i] + q[2] * tau[1][i] + q[3] * tau[2][i];
+        G[i][4] = work - k * gT[i];
+    }
+}
+
+}  // namespace afr

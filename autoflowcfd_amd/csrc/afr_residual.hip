@@ -1,0 +1,425 @@
+// Cell-centric FR residual kernels for gfx950 and the afr C ABI.
+//
+// One workgroup of 256 threads processes tiles of CPB cells of one type
+// (prism or tet) at one order.  Per tile, everything stays in LDS:
+//   1. primitive states of the tile's cells,
+//   2. contravariant fluxes at the over-integration points
+//      (Q_fine = c2f Q, F~ = adj_fine F(Q_fine)),
+//   3. the FR interface jumps of every face slot (own trace E Q, other side
+//      by cross-interpolation rows or a ghost state, AUSM+up),
+//   4. residual = -(Op F~ + sum_slots g' jump) / det  with Op = f2c D_fine,
+//   5. outlier suppression (median over solution points),
+// and the tile is written once.  No scatter, coloring or atomics.
+//
+// Reference semantics: src/autoflowcfd/core/fr_residual/inviscid.py:139-345,
+// src/autoflowcfd/core/fr_residual/inviscid_kernel.py:87-284,
+// src/autoflowcfd/core/fr_operators/troubled_cell.py:339-408.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/afr.h"
+#include "afr_physics.hpp"
+
+using namespace afr;
+
+static thread_local std::string g_err;
+
+static int fail(const char* msg) {
+    g_err = msg;
+    return -1;
+}
+
+#define HIP_TRY(x)                                                              \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) {                                                 \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);             \
+            return -1;                                                          \
+        }                                                                       \
+    } while (0)
+
+struct AfrMesh {
+    AfrMeshDesc d;
+};
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSlots = 5;
+
+template <int P> struct Ord {
+    static constexpr int N1 = P + 1;
+    static constexpr int NS = N1 * N1 * N1;
+    static constexpr int NFP = N1 * N1;
+    static constexpr int OV = (2 * P < 3 ? 2 * P : 3);
+    static constexpr int NF = (OV + 1) * (OV + 1) * (OV + 1);
+    static constexpr int CPB = kThreads / NF;  // cells per tile
+};
+
+template <int CT> struct Cell {
+    static constexpr int NSLOT = CT == 0 ? 5 : 4;
+    __device__ static int code(int slot) {
+        // prism slots: a=-1, a=+1, b=-1, c=-1, c=+1 ; tet slots: a=-1, a=+1, b=-1, c=-1
+        return slot < 3 ? slot : (slot == 3 ? 4 : 5);
+    }
+};
+
+template <typename real>
+struct Args {
+    int64_t cell_begin, n_cells_type;
+    const real* Q;        // (C, NS, 5) primitive
+    const real* U;        // (C, NS, 5) conserved (outlier reference)
+    real* R;
+    const real* det;
+    const real* adjf;
+    const int* slot_nb;
+    const int* slot_blk;
+    const int* slot_mask;
+    const int* slot_bc;
+    const int* slot_fb;
+    const real* slot_tn;
+    const real* slot_adjrow;
+    const real* rows;
+    const real* c2f;
+    const real* op;
+    const real* E;
+    const real* gL;
+    const real* gR;
+    const int* fp_of_sp;
+    const int* coord_of_sp;
+    const real* bcp;
+    int suppress;
+    int n_rows;
+};
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_primitives(const real* __restrict__ U, real* __restrict__ Q, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        real u[5], q[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) u[v] = U[i * 5 + v];
+        cons2prim(u, q);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Q[i * 5 + v] = q[v];
+    }
+}
+
+// median of |x[0..n)| as in numpy.median (mean of the two middle values for even n)
+template <typename real, int N>
+__device__ real median_abs(const real* x, int stride) {
+    real a[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        real t = x[i * stride];
+        a[i] = t < real(0) ? -t : t;
+    }
+    const int k1 = (N - 1) / 2, k2 = N / 2;
+    real m1 = real(0), m2 = real(0);
+    for (int i = 0; i < N; ++i) {
+        int lt = 0, le = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            lt += a[j] < a[i];
+            le += a[j] <= a[i];
+        }
+        if (lt <= k1 && k1 < le) m1 = a[i];
+        if (lt <= k2 && k2 < le) m2 = a[i];
+    }
+    return (N % 2) ? m1 : real(0.5) * (m1 + m2);
+}
+
+template <int P, int CT, typename real>
+__global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
+    using O = Ord<P>;
+    constexpr int NS = O::NS, NF = O::NF, NFP = O::NFP, CPB = O::CPB, N1 = O::N1;
+    constexpr int NSLOT = Cell<CT>::NSLOT;
+    __shared__ real s_c2f[NF * NS];
+    __shared__ real s_op[NS * 3 * NF];
+    __shared__ real s_E[NSLOT * NFP * NS];
+    __shared__ real s_Q[CPB * NS * 5];
+    __shared__ real s_Ft[CPB * 3 * NF * 5];
+    __shared__ real s_J[CPB * NSLOT * NFP * 5];
+    __shared__ real s_R[CPB * NS * 5];
+    __shared__ real s_g[2 * N1];
+    __shared__ int s_fp[3 * NS];
+    __shared__ int s_co[3 * NS];
+
+    const int t = threadIdx.x;
+    for (int i = t; i < NF * NS; i += kThreads) s_c2f[i] = a.c2f[i];
+    for (int i = t; i < NS * 3 * NF; i += kThreads) s_op[i] = a.op[i];
+    for (int i = t; i < NSLOT * NFP * NS; i += kThreads) s_E[i] = a.E[i];
+    for (int i = t; i < 3 * NS; i += kThreads) {
+        s_fp[i] = a.fp_of_sp[i];
+        s_co[i] = a.coord_of_sp[i];
+    }
+    if (t < N1) {
+        s_g[t] = a.gL[t];
+        s_g[N1 + t] = a.gR[t];
+    }
+    __syncthreads();
+
+    const int64_t n_tiles = (a.n_cells_type + CPB - 1) / CPB;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t first = a.cell_begin + tile * CPB;
+        const int ncell = (int)((a.n_cells_type - tile * CPB) < CPB ? (a.n_cells_type - tile * CPB) : CPB);
+
+        // 1. primitive states of the tile (contiguous)
+        for (int i = t; i < CPB * NS * 5; i += kThreads)
+            s_Q[i] = (i < ncell * NS * 5) ? a.Q[first * NS * 5 + i] : real(0);
+        __syncthreads();
+
+        // 2. contravariant fluxes at the fine points
+        if (t < CPB * NF) {
+            const int c = t / NF, j = t % NF;
+            if (c < ncell) {
+                real q[5] = {0, 0, 0, 0, 0};
+                for (int s = 0; s < NS; ++s) {
+                    const real w = s_c2f[j * NS + s];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) q[v] += w * s_Q[(c * NS + s) * 5 + v];
+                }
+                real F[3][5];
+                euler_flux(q, F);
+                const real* ad = a.adjf + ((first + c) * NF + j) * 9;
+                real A[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) A[k] = ad[k];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        s_Ft[((c * 3 + m) * NF + j) * 5 + v] = A[m * 3 + 0] * F[0][v] + A[m * 3 + 1] * F[1][v] +
+                                                               A[m * 3 + 2] * F[2][v];
+            }
+        }
+
+        // 3. interface jumps, one thread per (cell, slot, flux point)
+        if (t < CPB * NSLOT * NFP) {
+            const int c = t / (NSLOT * NFP), r = t % (NSLOT * NFP);
+            const int slot = r / NFP, fp = r % NFP;
+            if (c < ncell) {
+                const int64_t gc = first + c;
+                const int64_t sidx = gc * kSlots + slot;
+                const int code = Cell<CT>::code(slot);
+                const real side = (code & 1) ? real(1) : real(-1);
+                real qo[5] = {0, 0, 0, 0, 0};
+                for (int s = 0; s < NS; ++s) {
+                    const real w = s_E[(slot * NFP + fp) * NS + s];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) qo[v] += w * s_Q[(c * NS + s) * 5 + v];
+                }
+                real tn[3] = {a.slot_tn[sidx * 3 + 0], a.slot_tn[sidx * 3 + 1], a.slot_tn[sidx * 3 + 2]};
+                real qn[5] = {0, 0, 0, 0, 0};
+                const int bc = a.slot_bc[sidx];
+                if (bc == BC_INTERIOR) {
+                    const int src = (a.slot_mask[sidx] >> fp) & 1;
+                    const int nbc = a.slot_nb[sidx * 2 + src];
+                    const int blk = a.slot_blk[sidx * 2 + src];
+                    const int row = blk * NFP + fp;
+                    if (nbc >= 0 && row < a.n_rows) {
+                        const real* rw = a.rows + (int64_t)row * NS;
+                        const real* qn_src = a.Q + (int64_t)nbc * NS * 5;
+                        for (int s = 0; s < NS; ++s) {
+                            const real w = rw[s];
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) qn[v] += w * qn_src[s * 5 + v];
+                        }
+                    }
+                } else {
+                    ghost_state(bc, qo, tn, a.bcp, qn);
+                }
+                const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
+                real A0 = ar[0], A1 = ar[1], A2 = ar[2];
+                real mag = rmax(sqrt(A0 * A0 + A1 * A1 + A2 * A2), Tiny<real>::v);
+                real d[3];
+                if ((a.slot_fb[sidx] >> fp) & 1) {
+                    d[0] = tn[0]; d[1] = tn[1]; d[2] = tn[2];
+                } else {
+                    d[0] = A0 / mag * side; d[1] = A1 / mag * side; d[2] = A2 / mag * side;
+                }
+                real fc[5];
+                ausm_up(qo, qn, d, fc);
+                real F[3][5];
+                euler_flux(qo, F);
+                const real sc = mag * side;
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    s_J[((c * NSLOT + slot) * NFP + fp) * 5 + v] = fc[v] * sc - (A0 * F[0][v] + A1 * F[1][v] + A2 * F[2][v]);
+            }
+        }
+        __syncthreads();
+
+        // 4. volume divergence (folded restriction) + interface correction
+        for (int o = t; o < CPB * NS * 5; o += kThreads) {
+            const int c = o / (NS * 5), rem = o % (NS * 5), s = rem / 5, v = rem % 5;
+            real val = real(0);
+            if (c < ncell) {
+                real acc = real(0);
+                const real* opr = s_op + s * 3 * NF;
+                const real* ft = s_Ft + (c * 3 * NF) * 5 + v;
+                for (int k = 0; k < 3 * NF; ++k) acc += opr[k] * ft[k * 5];
+                real corr = real(0);
+#pragma unroll
+                for (int slot = 0; slot < NSLOT; ++slot) {
+                    const int code = Cell<CT>::code(slot);
+                    const int axis = code >> 1;
+                    const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
+                    corr += g * s_J[((c * NSLOT + slot) * NFP + s_fp[axis * NS + s]) * 5 + v];
+                }
+                const real dj = a.det[(first + c) * NS + s];
+                val = -acc / dj - corr / dj;
+            }
+            s_R[o] = val;
+        }
+        __syncthreads();
+
+        // 5. outlier suppression per (cell, variable)
+        if (a.suppress && t < CPB * 5) {
+            const int c = t / 5, v = t % 5;
+            if (c < ncell) {
+                real med = median_abs<real, NS>(s_R + c * NS * 5 + v, 5);
+                real mu = real(0);
+                const real* u = a.U + (first + c) * NS * 5 + v;
+                for (int s = 0; s < NS; ++s) mu += (u[s * 5] < real(0) ? -u[s * 5] : u[s * 5]);
+                mu /= real(NS);
+                real ref = rmax(rmax(med, real(1e-9) * mu), Tiny<real>::v);
+                real thr = real(1e4) * ref;
+                for (int s = 0; s < NS; ++s) {
+                    real x = s_R[(c * NS + s) * 5 + v];
+                    if ((x < real(0) ? -x : x) > thr) s_R[(c * NS + s) * 5 + v] = real(0);
+                }
+            }
+        }
+        __syncthreads();
+
+        for (int i = t; i < ncell * NS * 5; i += kThreads) a.R[first * NS * 5 + i] = s_R[i];
+        __syncthreads();
+    }
+}
+
+int grid_for(int64_t tiles) {
+    int64_t g = tiles < 4096 ? tiles : 4096;
+    return (int)(g > 0 ? g : 1);
+}
+
+template <int P, typename real>
+int launch_inviscid(const AfrMeshDesc& d, const real* U, real* R, int suppress, hipStream_t st) {
+    using O = Ord<P>;
+    const real* Q = (const real*)d.work_q;
+    for (int ct = 0; ct < 2; ++ct) {
+        const int64_t begin = ct == 0 ? 0 : d.n_prism;
+        const int64_t n = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
+        if (n <= 0) continue;
+        Args<real> a;
+        a.cell_begin = begin;
+        a.n_cells_type = n;
+        a.Q = Q;
+        a.U = U;
+        a.R = R;
+        a.det = (const real*)d.det;
+        a.adjf = (const real*)d.adj_fine;
+        a.slot_nb = d.slot_nb;
+        a.slot_blk = d.slot_blk;
+        a.slot_mask = d.slot_mask;
+        a.slot_bc = d.slot_bc;
+        a.slot_fb = d.slot_fallback;
+        a.slot_tn = (const real*)d.slot_tn;
+        a.slot_adjrow = (const real*)d.slot_adjrow;
+        a.rows = (const real*)d.rows;
+        a.c2f = (const real*)d.c2f[ct];
+        a.op = (const real*)d.op[ct];
+        a.E = (const real*)d.E[ct];
+        a.gL = (const real*)d.gL;
+        a.gR = (const real*)d.gR;
+        a.fp_of_sp = d.fp_of_sp;
+        a.coord_of_sp = d.coord_of_sp;
+        a.bcp = (const real*)d.bc_params;
+        a.suppress = suppress;
+        a.n_rows = d.n_rows;
+        const int grid = grid_for((n + O::CPB - 1) / O::CPB);
+        if (ct == 0)
+            hipLaunchKernelGGL((k_inviscid<P, 0, real>), dim3(grid), dim3(kThreads), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_inviscid<P, 1, real>), dim3(grid), dim3(kThreads), 0, st, a);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+template <typename real>
+int launch_primitives(const void* U, void* Q, int64_t n, hipStream_t st) {
+    int64_t blocks = (n + kThreads - 1) / kThreads;
+    int grid = (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
+    hipLaunchKernelGGL((k_primitives<real>), dim3(grid), dim3(kThreads), 0, st, (const real*)U, (real*)Q, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int afr_version(void) { return AFR_ABI_VERSION; }
+
+int afr_last_error(char* buf, size_t n) {
+    if (!buf || n == 0) return -1;
+    std::strncpy(buf, g_err.c_str(), n - 1);
+    buf[n - 1] = 0;
+    return 0;
+}
+
+int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
+    if (!desc || !out) return fail("afr_mesh_create: null argument");
+    if (desc->abi_version != AFR_ABI_VERSION) return fail("afr_mesh_create: ABI version mismatch");
+    if (desc->order < 1 || desc->order > 2) return fail("afr_mesh_create: order must be 1 or 2");
+    if (desc->dtype != AFR_F32 && desc->dtype != AFR_F64) return fail("afr_mesh_create: bad dtype");
+    if (desc->n_cells <= 0 || desc->n_prism < 0 || desc->n_prism > desc->n_cells)
+        return fail("afr_mesh_create: bad cell counts");
+    if (!desc->det || !desc->adj_fine || !desc->slot_nb || !desc->slot_blk || !desc->slot_mask || !desc->slot_bc ||
+        !desc->slot_fallback || !desc->slot_tn || !desc->slot_adjrow || !desc->rows || !desc->gL || !desc->gR ||
+        !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q)
+        return fail("afr_mesh_create: missing device array");
+    AfrMesh* m = new AfrMesh;
+    m->d = *desc;
+    *out = m;
+    return 0;
+}
+
+int afr_mesh_destroy(AfrMesh* mesh) {
+    delete mesh;
+    return 0;
+}
+
+int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, void* stream) {
+    if (!U || !Q || n_points < 0) return fail("afr_primitives: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == AFR_F64 ? launch_primitives<double>(U, Q, n_points, st)
+                            : launch_primitives<float>(U, Q, n_points, st);
+}
+
+int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t suppress, void* stream) {
+    if (!mesh || !U || !R) return fail("afr_inviscid_residual: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n_pts = d.n_cells * (int64_t)((d.order + 1) * (d.order + 1) * (d.order + 1));
+    if (afr_primitives(U, d.work_q, n_pts, d.dtype, stream)) return -1;
+    if (d.dtype == AFR_F32) {
+        return d.order == 1 ? launch_inviscid<1, float>(d, (const float*)U, (float*)R, suppress, st)
+                            : launch_inviscid<2, float>(d, (const float*)U, (float*)R, suppress, st);
+    }
+    return d.order == 1 ? launch_inviscid<1, double>(d, (const double*)U, (double*)R, suppress, st)
+                        : launch_inviscid<2, double>(d, (const double*)U, (double*)R, suppress, st);
+}
+
+int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,
+                         double Pr_t, int32_t suppress, void* stream) {
+    (void)mesh; (void)U; (void)mu_t; (void)R; (void)mu; (void)Pr; (void)Pr_t; (void)suppress; (void)stream;
+    return fail("afr_viscous_residual: not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+"""Device-resident mesh/operator data and the residual entry points.
+
+PyTorch-ROCm tensors own every device buffer; the C ABI (libafr.so) receives
+raw device pointers.  Host-side set-up (operator tables, metrics, face slots)
+is FP64 and rounded once to the device precision.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .face_slots import NSLOT, SLOT_CODES, build_slots
+from .fr_tables import tables
+from .geometry import cell_block_metrics
+from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
+
+GAMMA = 1.4
+
+
+def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
+    """(8,5) per-BC-code reference states (src/autoflowcfd/core/fr_solver/boundary.py:52-145)."""
+    P = np.zeros((8, 5))
+    Q_free = np.asarray(Q_free, dtype=np.float64)
+    P[BC_FARFIELD] = Q_free
+    P[BC_INLET] = Q_free if Q_inlet is None else np.asarray(Q_inlet, dtype=np.float64)
+    P[BC_OUTLET, 4] = Q_free[4] if p_outlet is None else p_outlet
+    return P
+
+
+class DeviceMesh:
+    """One mesh at one order on one GPU."""
+
+    def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, slots=None, metrics=None):
+        self.lib = _lib.load()
+        self.mesh, self.order, self.dtype, self.device = mesh, order, dtype, torch.device(device)
+        T = tables(order)
+        self.T = T
+        C = mesh.n_cells
+        self.n_cells, self.ns, self.nfp, self.nf = C, T.ns, T.nfp, T.nf
+        geo = metrics if metrics is not None else cell_block_metrics(mesh, T, dtype=np.float64)
+        sl = slots if slots is not None else build_slots(mesh, T)
+        self.geo, self.slots = geo, sl
+        self._validate(sl, C, T)
+
+        def dev(x, dt=None):
+            return torch.as_tensor(np.ascontiguousarray(x), dtype=dt or dtype).to(self.device)
+
+        def devi(x):
+            return torch.as_tensor(np.ascontiguousarray(x, dtype=np.int32)).to(self.device)
+
+        self.det = dev(geo["det"])
+        self.adj = dev(geo["adj"])
+        self.adj_fine = dev(geo["adj_fine"])
+        self.volume = geo["volume"]
+        self.slot_nb, self.slot_blk = devi(sl.nb), devi(sl.blk)
+        self.slot_mask, self.slot_bc, self.slot_fb = devi(sl.mask), devi(sl.bc), devi(sl.fallback)
+        self.slot_tn, self.slot_adjrow = dev(sl.tn), dev(sl.adjrow)
+        self.rows = dev(sl.rows)
+        self.c2f = [dev(T.c2f[ct]) for ct in (0, 1)]
+        self.op = [dev(T.Op[ct]) for ct in (0, 1)]
+        self.E = []
+        for ct in (0, 1):
+            E = np.zeros((NSLOT, T.nfp, T.ns))
+            for si, code in enumerate(SLOT_CODES[ct]):
+                E[si] = T.E[ct, code // 2, code % 2]
+            self.E.append(dev(E))
+        self.D = [dev(np.transpose(T.D[ct], (2, 0, 1))) for ct in (0, 1)]
+        self.filt = [dev(T.filt[ct]) for ct in (0, 1)]
+        self.gL, self.gR = dev(T.gL), dev(T.gR)
+        self.fp_of_sp, self.coord_of_sp = devi(T.fp_of_sp), devi(T.coord_of_sp)
+        self.bc_params = dev(bc_params if bc_params is not None else np.zeros((8, 5)))
+        self.work_q = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_grad = torch.empty((C, T.ns, 13), dtype=dtype, device=self.device)
+
+        d = _lib.AfrMeshDesc()
+        d.abi_version = _lib.ABI_VERSION
+        d.order = order
+        d.dtype = 0 if dtype == torch.float32 else 1
+        d.n_rows = int(sl.rows.shape[0])
+        d.n_cells = C
+        d.n_prism = mesh.n_prism
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        d.det, d.adj, d.adj_fine = p(self.det), p(self.adj), p(self.adj_fine)
+        d.slot_nb, d.slot_blk, d.slot_mask = p(self.slot_nb), p(self.slot_blk), p(self.slot_mask)
+        d.slot_bc, d.slot_fallback = p(self.slot_bc), p(self.slot_fb)
+        d.slot_tn, d.slot_adjrow, d.rows = p(self.slot_tn), p(self.slot_adjrow), p(self.rows)
+        for ct in (0, 1):
+            d.c2f[ct], d.op[ct], d.E[ct] = p(self.c2f[ct]), p(self.op[ct]), p(self.E[ct])
+            d.D[ct], d.filt[ct] = p(self.D[ct]), p(self.filt[ct])
+        d.gL, d.gR = p(self.gL), p(self.gR)
+        d.fp_of_sp, d.coord_of_sp = p(self.fp_of_sp), p(self.coord_of_sp)
+        d.bc_params, d.work_q, d.work_grad = p(self.bc_params), p(self.work_q), p(self.work_grad)
+        self._desc = d
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
+        self.handle = handle
+
+    @staticmethod
+    def _validate(sl, C, T):
+        """Host-side bounds checks of every index the kernels dereference."""
+        nb = sl.nb
+        if nb.min() < -1 or nb.max() >= C:
+            raise ValueError("slot source cell out of range")
+        n_rows = sl.rows.shape[0]
+        used = (sl.bc == -1)
+        if np.any(used & (nb[..., 0] < 0)):
+            raise ValueError("interior slot without a source cell")
+        if np.any((sl.blk + 1) * T.nfp > n_rows):
+            raise ValueError("slot row block out of range")
+        if np.any((sl.bc < -2) | (sl.bc > 6)):
+            raise ValueError("unknown boundary code")
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.afr_mesh_destroy(h)
+            except Exception:
+                pass
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _check_state(self, U):
+        if U.dtype != self.dtype or U.device.type != "cuda" or not U.is_contiguous():
+            raise ValueError("state must be a contiguous device tensor of the mesh precision")
+        if tuple(U.shape[:2]) != (self.n_cells, self.ns) or U.shape[2] < 5:
+            raise ValueError(f"state shape {tuple(U.shape)} != ({self.n_cells}, {self.ns}, 5)")
+
+    def inviscid_residual(self, U, R=None, suppress=True):
+        """dU/dt inviscid (C, ns, 5) of the conserved state U (C, ns, 5)."""
+        self._check_state(U)
+        if U.shape[2] != 5:
+            raise ValueError("inviscid_residual takes the 5 Euler variables")
+        R = torch.empty_like(U) if R is None else R
+        _lib.check(self.lib.afr_inviscid_residual(self.handle, ctypes.c_void_p(U.data_ptr()),
+                                                  ctypes.c_void_p(R.data_ptr()), int(bool(suppress)),
+                                                  self._stream()), "afr_inviscid_residual")
+        return R

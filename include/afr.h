@@ -1,0 +1,106 @@
+/*
+ * afr -- MI355X (gfx950) flux-reconstruction residual-and-update C ABI.
+ *
+ * Drop-in device backend for AutoFlowCFD's `--backend gpu` path.  All entry
+ * points are plain C: device pointers, explicit sizes, an explicit
+ * hipStream_t (passed as void*), int status (0 = ok) and afr_last_error().
+ * Device buffers are owned by the caller (PyTorch-ROCm tensors on the Python
+ * side); nothing here allocates device memory inside a hot call.
+ *
+ * Reference interfaces replaced (paths under the reference repository):
+ *   afr_inviscid_residual  <- compute_inviscid_residual_fr_gpu
+ *                             src/autoflowcfd/core/gpu/gpu_inviscid.py:31
+ *                             (semantics of the CPU path
+ *                             src/autoflowcfd/core/fr_residual/inviscid.py:139-345)
+ *   afr_viscous_residual   <- compute_viscous_residual_fr_gpu
+ *                             src/autoflowcfd/core/gpu/gpu_viscous.py:36
+ *                             (semantics: src/autoflowcfd/core/fr_residual/viscous_flux.py:113-302)
+ *   afr_primitives         <- conserved_to_primitive
+ *                             src/autoflowcfd/core/fr_residual/inviscid.py:38-47
+ *   afr_mesh_create        <- GPUArrayManager.upload_mesh_data + flat face arrays
+ *                             src/autoflowcfd/core/gpu/array_manager.py:27-364,
+ *                             src/autoflowcfd/core/gpu/gpu_solver_init.py:19-28
+ */
+#ifndef AFR_H
+#define AFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFR_ABI_VERSION 1
+
+enum { AFR_F32 = 0, AFR_F64 = 1 };
+
+/* Device-resident description of one mesh at one order.  Every pointer is a
+ * device pointer.  Cells are ordered prisms first ([0, n_prism)) then tets.
+ * Real-valued arrays use the precision `dtype`.  Layouts (C = n_cells,
+ * ns = (order+1)^3, nfp = (order+1)^2, nf = fine points, S = 5 slots):
+ *   det        (C, ns)                 det(J) at solution points
+ *   adj        (C, ns, 3, 3)           adj(J)[m][i] = det * J^{-1}[m][i]
+ *   adj_fine   (C, nf, 3, 3)           adj(J) at over-integration points
+ *   slot_nb    (C, S, 2) int32         source cells of each face slot (-1: none)
+ *   slot_blk   (C, S, 2) int32         row-block ids into `rows`
+ *   slot_mask  (C, S)    int32         bit fp set: flux point fp uses source 1
+ *   slot_bc    (C, S)    int32         BC code, -1 interior, -2 unused slot
+ *   slot_fallback (C, S) int32         bit fp set: direction falls back to tn
+ *   slot_tn    (C, S, 3)               role-signed true normal
+ *   slot_adjrow (C, S, nfp, 3)         E @ adj[:, axis, :]
+ *   rows       (n_rows, ns)            cross-cell interpolation rows
+ *   c2f[t] (nf, ns), op[t] (ns, 3*nf), E[t] (S, nfp, ns), D[t] (3, ns, ns),
+ *   filt[t] (ns, ns) for t = 0 prism, 1 tet
+ *   gL, gR (order+1), fp_of_sp / coord_of_sp (3, ns) int32
+ *   bc_params  (8, 5)                  per BC code reference state
+ *   work_q     (C, ns, 5)              scratch for primitive variables
+ *   work_grad  (C, ns, 13)             scratch for (grad u (9), grad T (3), mu_t)
+ */
+typedef struct AfrMeshDesc {
+    int32_t abi_version;
+    int32_t order;
+    int32_t dtype;
+    int32_t n_rows;
+    int64_t n_cells;
+    int64_t n_prism;
+    const void *det, *adj, *adj_fine;
+    const int32_t *slot_nb, *slot_blk, *slot_mask, *slot_bc, *slot_fallback;
+    const void *slot_tn, *slot_adjrow;
+    const void *rows;
+    const void *c2f[2], *op[2], *E[2], *D[2], *filt[2];
+    const void *gL, *gR;
+    const int32_t *fp_of_sp, *coord_of_sp;
+    const void *bc_params;
+    void *work_q;
+    void *work_grad;
+} AfrMeshDesc;
+
+typedef struct AfrMesh AfrMesh;
+
+/* Library ABI version (AFR_ABI_VERSION). */
+int afr_version(void);
+
+/* Copies the message of the last failed call on this thread into buf. */
+int afr_last_error(char *buf, size_t n);
+
+/* Validates and records a mesh description (no device allocation). */
+int afr_mesh_create(const AfrMeshDesc *desc, AfrMesh **out);
+int afr_mesh_destroy(AfrMesh *mesh);
+
+/* Q = conserved_to_primitive(U) for n_points points of 5 variables. */
+int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, void *stream);
+
+/* Inviscid FR residual dU/dt (C, ns, 5): over-integrated volume term, AUSM+up
+ * interface correction, optional outlier suppression.  U is (C, ns, 5). */
+int afr_inviscid_residual(const AfrMesh *mesh, const void *U, void *R, int32_t suppress, void *stream);
+
+/* Viscous BR1 FR residual dU/dt (C, ns, 5); mu_t (C, ns) or NULL.
+ * mu, Pr, Pr_t as in the reference (1.8e-5, 0.72, 0.9 by default). */
+int afr_viscous_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu,
+                         double Pr, double Pr_t, int32_t suppress, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AFR_H */

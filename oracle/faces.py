@@ -1,0 +1,376 @@
+"""Flat face flux-point geometry with dense cross-cell interpolation (oracle).
+
+Restates src/autoflowcfd/fr/face_flux_points_merge.py:51-413 together with
+face_flux_points_numba.py:30-251, face_flux_points_ms_numba.py:30-317 and
+face_flux_points_data.py:145-276, and the flattened form of
+src/autoflowcfd/core/fr_operators/face_kernels.py:125-351:
+
+  * primary flags: per (cell, prism-quad cube face) group of records, only the
+    lowest record index is primary (face_flux_points_merge.py:83-124);
+  * for a primary side, the physical positions of that side's native face
+    flux points are located on the other cell's face (tet: closed-form
+    barycentric, face_flux_points_locate.py:53-111; prism: Newton
+    least squares, :114-246) and the other cell's collapsed modal basis is
+    evaluated there: row = V_other(abc) @ V_sps^{-1}
+    (face_flux_points.py:164-236);
+  * a prism quad face whose two triangle records lead to (possibly)
+    different cells splits its flux points along the quad diagonal
+    (face_flux_points_data.py:145-217) into source 0 (primary record's
+    cell) and source 1 (the other record's cell).
+
+The oracle solves the point location with an analytic Jacobian and runs
+Newton to machine precision; the reference's finite-difference Newton stops
+at 1e-10 of the face scale, so the two agree to ~1e-10 on interpolation
+weights, far below every tolerance used against the GPU path.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from autoflowcfd_amd.mesh import (CUBE_FACE_AXIS, CUBE_FACE_SIDE, PRISM_CUBE_FACES,
+                                  PRISM_QUAD_CODES)
+
+from . import fr_ops
+from .geometry import map_points
+
+_TET_FACE_VERTS = {(0, -1.0): (0, 2, 3), (0, 1.0): (1, 2, 3), (1, -1.0): (0, 1, 3), (2, -1.0): (0, 1, 2)}
+
+
+def _cell_nodes(mesh, cells):
+    """(K, 6|4, 3) nodes for a homogeneous batch of cells."""
+    if len(cells) == 0:
+        return np.zeros((0, 6, 3))
+    if cells[0] < mesh.n_prism:
+        return mesh.nodes[mesh.prism_conn[cells]]
+    return mesh.nodes[mesh.tet_conn[cells - mesh.n_prism]]
+
+
+def _full_ref(axis, side, free, n1d):
+    """Insert the fixed coordinate: free (K,n,2) -> (K,n,3)."""
+    K, n, _ = free.shape
+    out = np.zeros((K, n, 3))
+    for k in range(K):
+        ax = axis[k]
+        other = [a for a in range(3) if a != ax]
+        out[k, :, ax] = side[k]
+        out[k, :, other[0]] = free[k, :, 0]
+        out[k, :, other[1]] = free[k, :, 1]
+    return out
+
+
+def _prism_jac_free(ref, nodes):
+    """d x / d xi for per-cell reference points: ref (K,n,3), nodes (K,6,3) -> (K,n,3,3).
+
+    Same closed form as geometry.exact_jacobian (curved_mapping_exact_jacobian.py:74-119).
+    """
+    a, b, c = ref[..., 0:1], ref[..., 1:2], ref[..., 2:3]
+    P = nodes[:, None]
+    p0, p1, p2, p3, p4, p5 = (P[:, :, i] for i in range(6))
+    dbda = (1.0 - b) / 4.0 * (p1 - p0)
+    dtda = (1.0 - b) / 4.0 * (p4 - p3)
+    dbdb = -(1.0 - a) / 4.0 * p0 - (1.0 + a) / 4.0 * p1 + 0.5 * p2
+    dtdb = -(1.0 - a) / 4.0 * p3 - (1.0 + a) / 4.0 * p4 + 0.5 * p5
+    r = (1.0 + a) * (1.0 - b) / 2.0 - 1.0
+    s = b
+    l1, l2, l3 = -(r + s) / 2.0, (1.0 + r) / 2.0, (1.0 + s) / 2.0
+    bot = l1 * p0 + l2 * p1 + l3 * p2
+    top = l1 * p3 + l2 * p4 + l3 * p5
+    J = np.zeros(ref.shape[:2] + (3, 3))
+    J[..., 0] = 0.5 * (1.0 - c) * dbda + 0.5 * (1.0 + c) * dtda
+    J[..., 1] = 0.5 * (1.0 - c) * dbdb + 0.5 * (1.0 + c) * dtdb
+    J[..., 2] = 0.5 * (top - bot)
+    return J
+
+
+def locate(mesh, cells, axis, side, targets):
+    """Free coordinates (K,n,2) on face axis=side of each cell hitting ``targets`` (K,n,3)."""
+    K, n, _ = targets.shape
+    free = np.zeros((K, n, 2))
+    if K == 0:
+        return free
+    nodes = _cell_nodes(mesh, cells)
+    is_tet = cells[0] >= mesh.n_prism
+    for ax in range(3):
+        for sd in (-1.0, 1.0):
+            sel = np.flatnonzero((axis == ax) & (side == sd))
+            if len(sel) == 0:
+                continue
+            other = [a for a in range(3) if a != ax]
+            nd = nodes[sel]
+            tg = targets[sel]
+            if is_tet:
+                vi, vj, vk = _TET_FACE_VERTS[(ax, sd)]
+                Pi, Pj, Pk = nd[:, vi], nd[:, vj], nd[:, vk]
+                e1, e2 = Pj - Pi, Pk - Pi
+                n1 = np.linalg.norm(e1, axis=1)[:, None]
+                n2 = np.linalg.norm(e2, axis=1)[:, None]
+                h1, h2 = e1 / n1, e2 / n2
+                a11 = np.sum(h1 * h1, 1)[:, None]
+                a12 = np.sum(h1 * h2, 1)[:, None]
+                a22 = np.sum(h2 * h2, 1)[:, None]
+                rr = tg - Pi[:, None, :]
+                b1 = np.einsum("kni,ki->kn", rr, h1)
+                b2 = np.einsum("kni,ki->kn", rr, h2)
+                det = a11 * a22 - a12 * a12
+                alpha = (b1 * a22 - b2 * a12) / det / n1
+                beta = (a11 * b2 - a12 * b1) / det / n2
+                L = np.zeros(alpha.shape + (4,))
+                L[..., vi] = 1.0 - alpha - beta
+                L[..., vj] = alpha
+                L[..., vk] = beta
+                r = 2.0 * L[..., 1] - 1.0
+                s = 2.0 * L[..., 2] - 1.0
+                t = 2.0 * L[..., 3] - 1.0
+                c = t
+                db = np.where(np.abs(1.0 - c) > 1e-300, 1.0 - c, 1e-300)
+                b = 2.0 * (s + 1.0) / db - 1.0
+                da = np.where(np.abs(s + t) > 1e-300, s + t, 1e-300)
+                a = -2.0 * (r + 1.0) / da - 1.0
+                full = np.stack([a, b, c], axis=-1)
+                free[sel] = full[..., other]
+                continue
+            x = np.zeros((len(sel), n, 2))
+            for _ in range(60):
+                ref = np.zeros((len(sel), n, 3))
+                ref[..., ax] = sd
+                ref[..., other[0]] = x[..., 0]
+                ref[..., other[1]] = x[..., 1]
+                res = map_points(True, ref, nd) - tg
+                Jf = _prism_jac_free(ref, nd)[..., other]
+                JtJ = np.einsum("knia,knib->knab", Jf, Jf)
+                Jtr = np.einsum("knia,kni->kna", Jf, res)
+                dx = -np.linalg.solve(JtJ, Jtr[..., None])[..., 0]
+                x = x + dx
+                if np.max(np.abs(dx)) < 1e-15:
+                    break
+            free[sel] = x
+    return free
+
+
+def interp_rows(mesh, cells, axis, side, free, ops):
+    """Rows V_cell(abc) V_sps^{-1}: (K, n, n_sps)."""
+    K, n, _ = free.shape
+    if K == 0:
+        return np.zeros((0, n, ops.n_sps))
+    ctype = "prism" if cells[0] < mesh.n_prism else "tet"
+    abc = _full_ref(axis, side, free, ops.n1d).reshape(-1, 3)
+    Vt = fr_ops.modal_basis(ctype, abc[:, 0], abc[:, 1], abc[:, 2], ops.order)[0]
+    Vs = fr_ops.modal_basis(ctype, ops.ref_sps[:, 0], ops.ref_sps[:, 1], ops.ref_sps[:, 2], ops.order)[0]
+    return fr_ops._right_solve(Vt, Vs).reshape(K, n, ops.n_sps)
+
+
+def _quad_half(mesh, cell, code, tri_nodes):
+    """('lower'|'upper', standard_diagonal) of a quad-face record (face_flux_points_data.py:145-217)."""
+    cn = mesh.prism_conn[cell]
+    quad = PRISM_CUBE_FACES[code]
+    bottom = [int(cn[i]) for i in (0, 1, 2)]
+    order = sorted(range(3), key=lambda k: bottom[k])
+    vs = [(0, 1, 2)[order[k]] for k in range(3)]
+    ws = [(3, 4, 5)[order[k]] for k in range(3)]
+    edge = {quad[0], quad[1]}
+    if edge == {vs[0], vs[1]}:
+        diag = {vs[0], ws[1]}
+    elif edge == {vs[1], vs[2]}:
+        diag = {vs[1], ws[2]}
+    elif edge == {vs[0], vs[2]}:
+        diag = {vs[0], ws[2]}
+    else:
+        raise RuntimeError("unexpected prism quad connectivity")
+    i0, i1, i2, i3 = quad
+    if diag == {i0, i2}:
+        std, lower, upper = True, (i0, i1, i2), (i0, i2, i3)
+    elif diag == {i1, i3}:
+        std, lower, upper = False, (i0, i1, i3), (i1, i2, i3)
+    else:
+        raise RuntimeError("invalid quad diagonal")
+    fs = set(int(x) for x in tri_nodes)
+    if fs == set(int(cn[i]) for i in lower):
+        return "lower", std
+    if fs == set(int(cn[i]) for i in upper):
+        return "upper", std
+    raise RuntimeError("record does not match a quad half")
+
+
+@dataclass
+class FlatFaces:
+    n_faces: int
+    n_fp: int
+    n_sps: int
+    n_prism: int
+    owner: np.ndarray
+    neighbor: np.ndarray
+    is_boundary: np.ndarray
+    owner_axis: np.ndarray
+    owner_side: np.ndarray
+    neighbor_axis: np.ndarray
+    neighbor_side: np.ndarray
+    owner_primary: np.ndarray
+    neighbor_primary: np.ndarray
+    true_normal: np.ndarray     # (F, n_fp, 3)
+    nb_src0_cell: np.ndarray
+    nb_src0_mat: np.ndarray     # (F, n_fp, n_sps)
+    nb_src1_idx: np.ndarray
+    nb_src1_cell: np.ndarray
+    nb_src1_mat: np.ndarray
+    ow_src0_cell: np.ndarray
+    ow_src0_mat: np.ndarray
+    ow_src1_idx: np.ndarray
+    ow_src1_cell: np.ndarray
+    ow_src1_mat: np.ndarray
+    bc: np.ndarray
+    area: np.ndarray
+
+
+def _side_sources(mesh, ops, fr, self_cells, self_codes, faces_idx, role, groups):
+    """Sources for the native FP grid of ``self`` sides of records ``faces_idx``.
+
+    role 'owner': self = owner, other = neighbour (nb_src); 'neighbor' the reverse.
+    Returns src0_cell, src0_mat, list of (face, cell, mat) for source 1.
+    """
+    n1d, nfp, nsp = ops.n1d, ops.n_fp, ops.n_sps
+    F = fr.n_faces
+    src0_cell = np.full(F, -1, dtype=np.int64)
+    src0_mat = np.zeros((F, nfp, nsp))
+    extra = []
+    x1d, _ = fr_ops.gauss_legendre(n1d)
+    g1, g2 = np.meshgrid(x1d, x1d, indexing="ij")
+    lower_std = (g1.ravel() >= g2.ravel())
+    lower_flip = (g1.ravel() + g2.ravel()) <= 0.0
+
+    other_cell = fr.neighbor if role == "owner" else fr.owner
+    other_code = fr.neighbor_code if role == "owner" else fr.owner_code
+    sign = 1.0 if role == "owner" else -1.0  # search = phys + sign*translation
+
+    # per-FP (target cell, target code) assignment
+    tasks = []  # (face, fp_mask, target_cell, target_code)
+    for f in faces_idx:
+        sc, code = int(self_cells[f]), int(self_codes[f])
+        if sc < mesh.n_prism and code in PRISM_QUAD_CODES:
+            grp = sorted(groups[(sc, code)])
+            half, std = _quad_half(mesh, sc, code, fr.node_ids[grp[0]])
+            low = lower_std if std else lower_flip
+            m0 = low if half == "lower" else ~low
+            tasks.append((f, m0, int(other_cell[grp[0]]), int(other_code[grp[0]]), 0))
+            if len(grp) > 1:
+                tasks.append((f, ~m0, int(other_cell[grp[1]]), int(other_code[grp[1]]), 1))
+        else:
+            tasks.append((f, np.ones(nfp, dtype=bool), int(other_cell[f]), int(other_code[f]), 0))
+
+    # physical FP positions of the self side
+    phys = {}
+    for is_prism in (True, False):
+        sel = [f for f in faces_idx if (self_cells[f] < mesh.n_prism) == is_prism]
+        if not sel:
+            continue
+        sel = np.array(sel)
+        cells = self_cells[sel]
+        nodes = _cell_nodes(mesh, cells)
+        for k, f in enumerate(sel):
+            code = int(self_codes[f])
+            ref = fr_ops.face_ref_grid(n1d, CUBE_FACE_AXIS[code], CUBE_FACE_SIDE[code])
+            phys[f] = map_points(is_prism, ref, nodes[k]) + sign * fr.translation[f]
+
+    for is_prism in (True, False):
+        tk = [t for t in tasks if (t[2] < mesh.n_prism) == is_prism]
+        if not tk:
+            continue
+        cells = np.array([t[2] for t in tk])
+        axis = CUBE_FACE_AXIS[np.array([t[3] for t in tk])]
+        side = CUBE_FACE_SIDE[np.array([t[3] for t in tk])]
+        targets = np.stack([phys[t[0]] for t in tk])
+        free = locate(mesh, cells, axis, side, targets)
+        rows = interp_rows(mesh, cells, axis, side, free, ops)
+        for k, (f, mask, tc, _, slot) in enumerate(tk):
+            mat = np.zeros((nfp, nsp))
+            mat[mask] = rows[k][mask]
+            if slot == 0:
+                src0_cell[f] = tc
+                src0_mat[f] = mat
+            else:
+                extra.append((f, tc, mat))
+    return src0_cell, src0_mat, extra
+
+
+def build_flat_faces(mesh, ops) -> FlatFaces:
+    fr = mesh.faces
+    F = fr.n_faces
+    nfp, nsp = ops.n_fp, ops.n_sps
+    bnd = fr.is_boundary
+    oax = CUBE_FACE_AXIS[fr.owner_code]
+    osd = CUBE_FACE_SIDE[fr.owner_code]
+    nax = np.where(bnd, -1, CUBE_FACE_AXIS[np.maximum(fr.neighbor_code, 0)])
+    nsd = np.where(bnd, 0.0, CUBE_FACE_SIDE[np.maximum(fr.neighbor_code, 0)])
+
+    owner_groups, neighbor_groups, bnd_groups = {}, {}, {}
+    for f in range(F):
+        oc, ocode = int(fr.owner[f]), int(fr.owner_code[f])
+        quad_o = oc < mesh.n_prism and ocode in PRISM_QUAD_CODES
+        if bnd[f]:
+            if quad_o:
+                bnd_groups.setdefault((oc, ocode), []).append(f)
+            continue
+        if quad_o:
+            owner_groups.setdefault((oc, ocode), []).append(f)
+        nc, ncode = int(fr.neighbor[f]), int(fr.neighbor_code[f])
+        if nc < mesh.n_prism and ncode in PRISM_QUAD_CODES:
+            neighbor_groups.setdefault((nc, ncode), []).append(f)
+    op = np.ones(F, dtype=bool)
+    npri = np.ones(F, dtype=bool)
+    for g in owner_groups.values():
+        op[sorted(g)[1:]] = False
+    for g in bnd_groups.values():
+        op[sorted(g)[1:]] = False
+    for g in neighbor_groups.values():
+        npri[sorted(g)[1:]] = False
+
+    nb_faces = np.flatnonzero(~bnd & op)
+    ow_faces = np.flatnonzero(~bnd & npri)
+    nb0c, nb0m, nbx = _side_sources(mesh, ops, fr, fr.owner, fr.owner_code, nb_faces, "owner", owner_groups)
+    ow0c, ow0m, owx = _side_sources(mesh, ops, fr, fr.neighbor, fr.neighbor_code, ow_faces, "neighbor",
+                                    neighbor_groups)
+
+    def pack_extra(extra):
+        idx = np.full(F, -1, dtype=np.int64)
+        cells = np.zeros(len(extra), dtype=np.int64)
+        mats = np.zeros((len(extra), nfp, nsp))
+        for i, (f, c, m) in enumerate(extra):
+            idx[f] = i
+            cells[i] = c
+            mats[i] = m
+        return idx, cells, mats
+
+    nb1i, nb1c, nb1m = pack_extra(nbx)
+    ow1i, ow1c, ow1m = pack_extra(owx)
+    return FlatFaces(
+        n_faces=F, n_fp=nfp, n_sps=nsp, n_prism=mesh.n_prism,
+        owner=fr.owner.copy(), neighbor=fr.neighbor.copy(), is_boundary=bnd.copy(),
+        owner_axis=oax, owner_side=osd, neighbor_axis=nax, neighbor_side=nsd,
+        owner_primary=op, neighbor_primary=npri,
+        true_normal=np.repeat(fr.normal[:, None, :], nfp, axis=1),
+        nb_src0_cell=nb0c, nb_src0_mat=nb0m, nb_src1_idx=nb1i, nb_src1_cell=nb1c, nb_src1_mat=nb1m,
+        ow_src0_cell=ow0c, ow_src0_mat=ow0m, ow_src1_idx=ow1i, ow_src1_cell=ow1c, ow_src1_mat=ow1m,
+        bc=fr.bc.copy(), area=fr.area.copy(),
+    )
+
+
+def gather_sources(fl: FlatFaces, role: str, field: np.ndarray) -> np.ndarray:
+    """Other-side field values at the self side's native FPs: (F, n_fp, ...).
+
+    role 'owner' uses nb_src (values of the neighbour at owner FPs);
+    role 'neighbor' uses ow_src.  Rows of faces without a source are zero.
+    """
+    if role == "owner":
+        c0, m0, i1, c1, m1 = fl.nb_src0_cell, fl.nb_src0_mat, fl.nb_src1_idx, fl.nb_src1_cell, fl.nb_src1_mat
+    else:
+        c0, m0, i1, c1, m1 = fl.ow_src0_cell, fl.ow_src0_mat, fl.ow_src1_idx, fl.ow_src1_cell, fl.ow_src1_mat
+    trailing = field.shape[2:]
+    flat = field.reshape(field.shape[0], field.shape[1], -1)
+    out = np.zeros((fl.n_faces, fl.n_fp, flat.shape[2]))
+    has0 = c0 >= 0
+    out[has0] = np.einsum("fps,fsv->fpv", m0[has0], flat[c0[has0]])
+    has1 = i1 >= 0
+    if np.any(has1):
+        e = i1[has1]
+        out[has1] += np.einsum("fps,fsv->fpv", m1[e], flat[c1[e]])
+    return out.reshape((fl.n_faces, fl.n_fp) + trailing)

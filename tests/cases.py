@@ -1,0 +1,58 @@
+"""Seeded meshes and states shared by the CPU and GPU tests."""
+
+import numpy as np
+
+from autoflowcfd_amd.mesh import (BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, hex_box,
+                                  mesh_from_cells)
+
+GAMMA = 1.4
+RHO, P_INF, A_INF = 1.225, 101325.0, np.sqrt(1.4 * 101325.0 / 1.225)
+
+
+def mixed_mesh():
+    """2 tets sharing a face + 2 prisms sharing a quad (tests/unit/test_fr_residual_inviscid.py:42-70)."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1],
+                      [10, 0, 0], [11, 0, 0], [10, 1, 0], [10, 0, 1], [11, 0, 1], [10, 1, 1],
+                      [9, -1, 0], [9, -1, 1]], dtype=float)
+    return mesh_from_cells(nodes, np.array([[5, 6, 7, 8, 9, 10], [5, 7, 11, 8, 10, 12]]),
+                           np.array([[0, 1, 2, 3], [1, 2, 3, 4]]))
+
+
+WALL_BOX_BC = {"y0": BC_WALL, "x0": BC_INLET, "x1": BC_OUTLET, "y1": BC_SYMMETRY,
+               "z0": BC_SYMMETRY, "z1": BC_SYMMETRY}
+
+
+def wall_box(n=(4, 4, 3), L=(4.0, 1.0, 1.0), grading=1.1, seed=0):
+    """Small version of the graded wall-bounded prism box (y-extruded prisms, no-slip y=0)."""
+    return hex_box(*n, *L, split="prism", grading_y=grading, seed=seed, bc_by_plane=WALL_BOX_BC)
+
+
+def tet_box(n=3, seed=0, displacement=0.1, periodic=False):
+    return hex_box(n, n, n, split="tet", displacement=displacement, seed=seed, periodic=periodic)
+
+
+def freestream(mach=0.15, yaw_deg=0.0):
+    u = mach * A_INF
+    y = np.deg2rad(yaw_deg)
+    return np.array([RHO, u * np.cos(y), 0.0, u * np.sin(y), P_INF])
+
+
+def prim_to_cons(Q):
+    rho, u, v, w, p = (Q[..., i] for i in range(5))
+    E = p / ((GAMMA - 1.0) * rho) + 0.5 * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, rho * E], axis=-1)
+
+
+def smooth_state(xyz, Q0, amp=1e-2, seed=0):
+    """Freestream plus a seeded smooth perturbation of relative amplitude ``amp`` (conserved)."""
+    rng = np.random.default_rng(seed)
+    L = np.ptp(xyz.reshape(-1, 3), axis=0) + 1e-30
+    x = (xyz - xyz.reshape(-1, 3).min(axis=0)) / L
+    Q = np.broadcast_to(Q0, xyz.shape[:-1] + (5,)).copy()
+    scale = np.array([Q0[0], np.abs(Q0[1:4]).max() + 1.0, np.abs(Q0[1:4]).max() + 1.0,
+                      np.abs(Q0[1:4]).max() + 1.0, Q0[4]])
+    for v in range(5):
+        k = rng.uniform(0.5, 2.0, size=3)
+        ph = rng.uniform(0, 2 * np.pi)
+        Q[..., v] += amp * scale[v] * np.sin(2 * np.pi * (x @ k) + ph)
+    return prim_to_cons(Q)
