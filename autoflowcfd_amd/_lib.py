@@ -19,7 +19,8 @@ class AfrMeshDesc(ctypes.Structure):
         ("slot_tn", c_vp), ("slot_adjrow", c_vp), ("rows", c_vp),
         ("c2f", c_vp * 2), ("op", c_vp * 2), ("E", c_vp * 2), ("D", c_vp * 2), ("filt", c_vp * 2),
         ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp),
-        ("bc_params", c_vp), ("work_q", c_vp), ("work_grad", c_vp),
+        ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
+        ("work_q", c_vp), ("work_grad", c_vp),
     ]
 
 
@@ -29,7 +30,8 @@ SIGNATURES = {
     "afr_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "afr_mesh_create": (ctypes.c_int, [ctypes.POINTER(AfrMeshDesc), ctypes.POINTER(c_vp)]),
     "afr_mesh_destroy": (ctypes.c_int, [c_vp]),
-    "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, c_vp]),
+    "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_double), c_vp]),
     "afr_inviscid_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "afr_viscous_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int32, c_vp]),

@@ -22,13 +22,13 @@ enum BcCode { BC_INTERIOR = -1, BC_WALL = 0, BC_FARFIELD = 1, BC_INLET = 2, BC_O
               BC_SYMMETRY = 4, BC_SLIP_WALL = 5, BC_EXTRAP = 6 };
 
 template <typename real>
-__device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
 template <typename real>
-__device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
 
 // conserved (rho, rho u, rho v, rho w, rho E) -> primitive (rho, u, v, w, p); rho floor 1e-10, no p floor
 template <typename real>
-__device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {
+__host__ __device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {
     const real g1 = real(kGamma - 1.0);
     real rho = rmax(U[0], real(1e-10));
     real u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, E = U[4] / rho;
@@ -38,7 +38,7 @@ __device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {
 
 // physical flux F[i][v] of primitive state q
 template <typename real>
-__device__ __forceinline__ void euler_flux(const real q[5], real F[3][5]) {
+__host__ __device__ __forceinline__ void euler_flux(const real q[5], real F[3][5]) {
     const real g1 = real(kGamma - 1.0);
     real rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
     real rs = rmax(rho, real(1e-10));
@@ -60,7 +60,7 @@ __device__ __forceinline__ void euler_flux(const real q[5], real F[3][5]) {
 
 // AUSM+up normal flux density for unit direction n
 template <typename real>
-__device__ __forceinline__ void ausm_up(const real qL[5], const real qR[5], const real n[3], real out[5]) {
+__host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR[5], const real n[3], real out[5]) {
     const real g = real(1.4), alpha = real(0.1875), beta = real(0.5);
     real rhoL = rmax(qL[0], real(1e-6)), rhoR = rmax(qR[0], real(1e-6));
     real pL = rmax(qL[4], real(10.0)), pR = rmax(qR[4], real(10.0));
@@ -121,7 +121,7 @@ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR[5], cons
 
 // primitive ghost state of boundary code bc; bcp = (8,5) per-code reference state
 template <typename real>
-__device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real n[3], const real* bcp, real qg[5]) {
+__host__ __device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real n[3], const real* bcp, real qg[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) qg[v] = qi[v];
     real vn = qi[1] * n[0] + qi[2] * n[1] + qi[3] * n[2];
@@ -155,7 +155,7 @@ __device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real
 
 // viscous flux G[i][v]; gv[i][j] = du_i/dx_j
 template <typename real>
-__device__ __forceinline__ void viscous_flux(const real q[5], const real gv[3][3], const real gT[3], real mu, real Pr,
+__host__ __device__ __forceinline__ void viscous_flux(const real q[5], const real gv[3][3], const real gT[3], real mu, real Pr,
                                              real mut, real Prt, real G[3][5]) {
     real mt = mu + mut;
     real div = gv[0][0] + gv[1][1] + gv[2][2];

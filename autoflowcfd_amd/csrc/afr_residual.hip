@@ -90,22 +90,30 @@ struct Args {
     const real* gR;
     const int* fp_of_sp;
     const int* coord_of_sp;
-    const real* bcp;
+    const real* gcl;      // (C, NS, 3) constant-state divergence
+    const double* bcp;    // (8, 5) per-BC reference states
+    double qref[5];       // primitive reference state
+    double fref[15];      // F(qref)[i][v]
     int suppress;
     int n_rows;
 };
 
+struct Ref5 {
+    double q[5];
+};
+
 template <typename real>
-__global__ void __launch_bounds__(kThreads) k_primitives(const real* __restrict__ U, real* __restrict__ Q, int64_t n) {
+__global__ void __launch_bounds__(kThreads) k_primitives(const real* __restrict__ U, real* __restrict__ Q, int64_t n,
+                                                         Ref5 ref) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
-        real u[5], q[5];
+        double u[5], q[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] = U[i * 5 + v];
+        for (int v = 0; v < 5; ++v) u[v] = (double)U[i * 5 + v];
         cons2prim(u, q);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Q[i * 5 + v] = q[v];
+        for (int v = 0; v < 5; ++v) Q[i * 5 + v] = (real)(q[v] - ref.q[v]);
     }
 }
 
@@ -177,14 +185,22 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
         if (t < CPB * NF) {
             const int c = t / NF, j = t % NF;
             if (c < ncell) {
-                real q[5] = {0, 0, 0, 0, 0};
+                real dq[5] = {0, 0, 0, 0, 0};
                 for (int s = 0; s < NS; ++s) {
                     const real w = s_c2f[j * NS + s];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) q[v] += w * s_Q[(c * NS + s) * 5 + v];
+                    for (int v = 0; v < 5; ++v) dq[v] += w * s_Q[(c * NS + s) * 5 + v];
                 }
+                // point flux in FP64 about the reference state, then back to storage precision
+                double q[5], Fd[3][5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) q[v] = a.qref[v] + (double)dq[v];
+                euler_flux(q, Fd);
                 real F[3][5];
-                euler_flux(q, F);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) F[i][v] = (real)(Fd[i][v] - a.fref[i * 5 + v]);
                 const real* ad = a.adjf + ((first + c) * NF + j) * 9;
                 real A[9];
 #pragma unroll
@@ -206,17 +222,21 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
                 const int64_t gc = first + c;
                 const int64_t sidx = gc * kSlots + slot;
                 const int code = Cell<CT>::code(slot);
-                const real side = (code & 1) ? real(1) : real(-1);
-                real qo[5] = {0, 0, 0, 0, 0};
+                const double side = (code & 1) ? 1.0 : -1.0;
+                real dqo[5] = {0, 0, 0, 0, 0};
                 for (int s = 0; s < NS; ++s) {
                     const real w = s_E[(slot * NFP + fp) * NS + s];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) qo[v] += w * s_Q[(c * NS + s) * 5 + v];
+                    for (int v = 0; v < 5; ++v) dqo[v] += w * s_Q[(c * NS + s) * 5 + v];
                 }
-                real tn[3] = {a.slot_tn[sidx * 3 + 0], a.slot_tn[sidx * 3 + 1], a.slot_tn[sidx * 3 + 2]};
-                real qn[5] = {0, 0, 0, 0, 0};
+                double qo[5], qn[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qo[v] = a.qref[v] + (double)dqo[v];
+                double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
+                                (double)a.slot_tn[sidx * 3 + 2]};
                 const int bc = a.slot_bc[sidx];
                 if (bc == BC_INTERIOR) {
+                    real dqn[5] = {0, 0, 0, 0, 0};
                     const int src = (a.slot_mask[sidx] >> fp) & 1;
                     const int nbc = a.slot_nb[sidx * 2 + src];
                     const int blk = a.slot_blk[sidx * 2 + src];
@@ -227,29 +247,32 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
                         for (int s = 0; s < NS; ++s) {
                             const real w = rw[s];
 #pragma unroll
-                            for (int v = 0; v < 5; ++v) qn[v] += w * qn_src[s * 5 + v];
+                            for (int v = 0; v < 5; ++v) dqn[v] += w * qn_src[s * 5 + v];
                         }
                     }
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) qn[v] = a.qref[v] + (double)dqn[v];
                 } else {
                     ghost_state(bc, qo, tn, a.bcp, qn);
                 }
                 const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
-                real A0 = ar[0], A1 = ar[1], A2 = ar[2];
-                real mag = rmax(sqrt(A0 * A0 + A1 * A1 + A2 * A2), Tiny<real>::v);
-                real d[3];
+                double A0 = ar[0], A1 = ar[1], A2 = ar[2];
+                double mag = rmax(sqrt(A0 * A0 + A1 * A1 + A2 * A2), 1e-300);
+                double d[3];
                 if ((a.slot_fb[sidx] >> fp) & 1) {
                     d[0] = tn[0]; d[1] = tn[1]; d[2] = tn[2];
                 } else {
                     d[0] = A0 / mag * side; d[1] = A1 / mag * side; d[2] = A2 / mag * side;
                 }
-                real fc[5];
+                double fc[5];
                 ausm_up(qo, qn, d, fc);
-                real F[3][5];
+                double F[3][5];
                 euler_flux(qo, F);
-                const real sc = mag * side;
+                const double sc = mag * side;
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
-                    s_J[((c * NSLOT + slot) * NFP + fp) * 5 + v] = fc[v] * sc - (A0 * F[0][v] + A1 * F[1][v] + A2 * F[2][v]);
+                    s_J[((c * NSLOT + slot) * NFP + fp) * 5 + v] =
+                        (real)(fc[v] * sc - (A0 * F[0][v] + A1 * F[1][v] + A2 * F[2][v]));
             }
         }
         __syncthreads();
@@ -263,6 +286,8 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
                 const real* opr = s_op + s * 3 * NF;
                 const real* ft = s_Ft + (c * 3 * NF) * 5 + v;
                 for (int k = 0; k < 3 * NF; ++k) acc += opr[k] * ft[k * 5];
+                const real* gc = a.gcl + ((first + c) * NS + s) * 3;
+                acc += gc[0] * (real)a.fref[v] + gc[1] * (real)a.fref[5 + v] + gc[2] * (real)a.fref[10 + v];
                 real corr = real(0);
 #pragma unroll
                 for (int slot = 0; slot < NSLOT; ++slot) {
@@ -338,7 +363,15 @@ int launch_inviscid(const AfrMeshDesc& d, const real* U, real* R, int suppress, 
         a.gR = (const real*)d.gR;
         a.fp_of_sp = d.fp_of_sp;
         a.coord_of_sp = d.coord_of_sp;
-        a.bcp = (const real*)d.bc_params;
+        a.gcl = (const real*)d.gcl;
+        a.bcp = d.bc_params;
+        for (int v = 0; v < 5; ++v) a.qref[v] = d.ref_state[v];
+        {
+            double F[3][5];
+            euler_flux(d.ref_state, F);
+            for (int i = 0; i < 3; ++i)
+                for (int v = 0; v < 5; ++v) a.fref[i * 5 + v] = F[i][v];
+        }
         a.suppress = suppress;
         a.n_rows = d.n_rows;
         const int grid = grid_for((n + O::CPB - 1) / O::CPB);
@@ -352,10 +385,12 @@ int launch_inviscid(const AfrMeshDesc& d, const real* U, real* R, int suppress, 
 }
 
 template <typename real>
-int launch_primitives(const void* U, void* Q, int64_t n, hipStream_t st) {
+int launch_primitives(const void* U, void* Q, int64_t n, const double* ref, hipStream_t st) {
     int64_t blocks = (n + kThreads - 1) / kThreads;
     int grid = (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
-    hipLaunchKernelGGL((k_primitives<real>), dim3(grid), dim3(kThreads), 0, st, (const real*)U, (real*)Q, n);
+    Ref5 r;
+    for (int v = 0; v < 5; ++v) r.q[v] = ref ? ref[v] : 0.0;
+    hipLaunchKernelGGL((k_primitives<real>), dim3(grid), dim3(kThreads), 0, st, (const real*)U, (real*)Q, n, r);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -382,7 +417,7 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
         return fail("afr_mesh_create: bad cell counts");
     if (!desc->det || !desc->adj_fine || !desc->slot_nb || !desc->slot_blk || !desc->slot_mask || !desc->slot_bc ||
         !desc->slot_fallback || !desc->slot_tn || !desc->slot_adjrow || !desc->rows || !desc->gL || !desc->gR ||
-        !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q)
+        !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q || !desc->gcl)
         return fail("afr_mesh_create: missing device array");
     AfrMesh* m = new AfrMesh;
     m->d = *desc;
@@ -395,11 +430,11 @@ int afr_mesh_destroy(AfrMesh* mesh) {
     return 0;
 }
 
-int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, void* stream) {
+int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, const double* ref, void* stream) {
     if (!U || !Q || n_points < 0) return fail("afr_primitives: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    return dtype == AFR_F64 ? launch_primitives<double>(U, Q, n_points, st)
-                            : launch_primitives<float>(U, Q, n_points, st);
+    return dtype == AFR_F64 ? launch_primitives<double>(U, Q, n_points, ref, st)
+                            : launch_primitives<float>(U, Q, n_points, ref, st);
 }
 
 int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t suppress, void* stream) {
@@ -407,7 +442,7 @@ int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t s
     const AfrMeshDesc& d = mesh->d;
     hipStream_t st = (hipStream_t)stream;
     const int64_t n_pts = d.n_cells * (int64_t)((d.order + 1) * (d.order + 1) * (d.order + 1));
-    if (afr_primitives(U, d.work_q, n_pts, d.dtype, stream)) return -1;
+    if (afr_primitives(U, d.work_q, n_pts, d.dtype, d.ref_state, stream)) return -1;
     if (d.dtype == AFR_F32) {
         return d.order == 1 ? launch_inviscid<1, float>(d, (const float*)U, (float*)R, suppress, st)
                             : launch_inviscid<2, float>(d, (const float*)U, (float*)R, suppress, st);

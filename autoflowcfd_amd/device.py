@@ -32,7 +32,8 @@ def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
 class DeviceMesh:
     """One mesh at one order on one GPU."""
 
-    def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, slots=None, metrics=None):
+    def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, ref_state=None,
+                 slots=None, metrics=None):
         self.lib = _lib.load()
         self.mesh, self.order, self.dtype, self.device = mesh, order, dtype, torch.device(device)
         T = tables(order)
@@ -70,7 +71,11 @@ class DeviceMesh:
         self.filt = [dev(T.filt[ct]) for ct in (0, 1)]
         self.gL, self.gR = dev(T.gL), dev(T.gR)
         self.fp_of_sp, self.coord_of_sp = devi(T.fp_of_sp), devi(T.coord_of_sp)
-        self.bc_params = dev(bc_params if bc_params is not None else np.zeros((8, 5)))
+        self.gcl = dev(geo["gcl"])
+        bcp = np.zeros((8, 5)) if bc_params is None else np.asarray(bc_params, dtype=np.float64)
+        self.bc_params = dev(bcp, torch.float64)
+        # reference state of the perturbation form: the freestream by default
+        self.ref_state = np.asarray(ref_state if ref_state is not None else bcp[BC_FARFIELD], dtype=np.float64)
         self.work_q = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
         self.work_grad = torch.empty((C, T.ns, 13), dtype=dtype, device=self.device)
 
@@ -91,7 +96,10 @@ class DeviceMesh:
             d.D[ct], d.filt[ct] = p(self.D[ct]), p(self.filt[ct])
         d.gL, d.gR = p(self.gL), p(self.gR)
         d.fp_of_sp, d.coord_of_sp = p(self.fp_of_sp), p(self.coord_of_sp)
-        d.bc_params, d.work_q, d.work_grad = p(self.bc_params), p(self.work_q), p(self.work_grad)
+        d.gcl, d.bc_params = p(self.gcl), p(self.bc_params)
+        for v in range(5):
+            d.ref_state[v] = float(self.ref_state[v])
+        d.work_q, d.work_grad = p(self.work_q), p(self.work_grad)
         self._desc = d
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")

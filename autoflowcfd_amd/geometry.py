@@ -82,6 +82,7 @@ def cell_block_metrics(mesh, T, dtype=np.float32, chunk=65536):
     vol = np.empty(C, np.float64)
     det_mean = np.empty(C, np.float64)
     mfs = np.empty((C, T.ns), np.float64)
+    gcl = np.empty((C, T.ns, 3), dtype)
     for is_prism, conn, off in ((True, mesh.prism_conn, 0), (False, mesh.tet_conn, mesh.n_prism)):
         for s0 in range(0, len(conn), chunk):
             sl = slice(s0, min(s0 + chunk, len(conn)))
@@ -97,4 +98,8 @@ def cell_block_metrics(mesh, T, dtype=np.float32, chunk=65536):
             vol[g] = d @ T.w_sps
             det_mean[g] = d.mean(axis=1)
             mfs[g] = np.linalg.norm(a, axis=-1).sum(axis=-1)
-    return dict(det=det, adj=adj, adj_fine=adjf, volume=vol, det_mean=det_mean, metric_flux_scale=mfs)
+            # constant-state divergence sum_{m,j} Op[s, m*nf+j] adj_fine[j, m, i] (FP64)
+            ct = 0 if is_prism else 1
+            Ak = np.transpose(af, (0, 2, 1, 3)).reshape(af.shape[0], 3 * T.nf, 3)
+            gcl[g] = np.einsum("sk,cki->csi", T.Op[ct], Ak)
+    return dict(det=det, adj=adj, adj_fine=adjf, volume=vol, det_mean=det_mean, metric_flux_scale=mfs, gcl=gcl)

@@ -53,8 +53,12 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   c2f[t] (nf, ns), op[t] (ns, 3*nf), E[t] (S, nfp, ns), D[t] (3, ns, ns),
  *   filt[t] (ns, ns) for t = 0 prism, 1 tet
  *   gL, gR (order+1), fp_of_sp / coord_of_sp (3, ns) int32
- *   bc_params  (8, 5)                  per BC code reference state
- *   work_q     (C, ns, 5)              scratch for primitive variables
+ *   gcl        (C, ns, 3)              constant-state divergence sum_{m,j} Op[s][m,j] adj_fine[j][m][i]
+ *   bc_params  (8, 5) FP64             per BC code reference state (always double)
+ *   ref_state  (5) FP64, host values   primitive reference state Q_ref; the kernels carry
+ *                                      dQ = Q - Q_ref in the storage precision and evaluate
+ *                                      point fluxes / Riemann fluxes in FP64
+ *   work_q     (C, ns, 5)              scratch for dQ = primitive - Q_ref
  *   work_grad  (C, ns, 13)             scratch for (grad u (9), grad T (3), mu_t)
  */
 typedef struct AfrMeshDesc {
@@ -71,7 +75,9 @@ typedef struct AfrMeshDesc {
     const void *c2f[2], *op[2], *E[2], *D[2], *filt[2];
     const void *gL, *gR;
     const int32_t *fp_of_sp, *coord_of_sp;
-    const void *bc_params;
+    const void *gcl;
+    const double *bc_params;
+    double ref_state[5];
     void *work_q;
     void *work_grad;
 } AfrMeshDesc;
@@ -88,8 +94,8 @@ int afr_last_error(char *buf, size_t n);
 int afr_mesh_create(const AfrMeshDesc *desc, AfrMesh **out);
 int afr_mesh_destroy(AfrMesh *mesh);
 
-/* Q = conserved_to_primitive(U) for n_points points of 5 variables. */
-int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, void *stream);
+/* dQ = conserved_to_primitive(U) - ref (ref: 5 host doubles, may be NULL) for n_points points. */
+int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, const double *ref, void *stream);
 
 /* Inviscid FR residual dU/dt (C, ns, 5): over-integrated volume term, AUSM+up
  * interface correction, optional outlier suppression.  U is (C, ns, 5). */
