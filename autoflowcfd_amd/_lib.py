@@ -24,12 +24,25 @@ class AfrMeshDesc(ctypes.Structure):
     ]
 
 
+class AfrGeomDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("order", ctypes.c_int32), ("dtype", ctypes.c_int32),
+        ("pad_", ctypes.c_int32), ("n_cells", ctypes.c_int64), ("n_prism", ctypes.c_int64),
+        ("nodes", c_vp), ("prism_conn", c_vp), ("tet_conn", c_vp),
+        ("sps", c_vp), ("fine", c_vp), ("w_sps", c_vp), ("op64", c_vp * 2), ("E64", c_vp * 2),
+        ("slot_tn64", c_vp),
+        ("det", c_vp), ("adj", c_vp), ("adj_fine", c_vp), ("gcl", c_vp), ("metric_flux_scale", c_vp),
+        ("slot_adjrow", c_vp), ("volume", c_vp), ("det_mean", c_vp), ("slot_fallback", c_vp), ("bad_det", c_vp),
+    ]
+
+
 # every function declared in include/afr.h: name -> (restype, argtypes)
 SIGNATURES = {
     "afr_version": (ctypes.c_int, []),
     "afr_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "afr_mesh_create": (ctypes.c_int, [ctypes.POINTER(AfrMeshDesc), ctypes.POINTER(c_vp)]),
     "afr_mesh_destroy": (ctypes.c_int, [c_vp]),
+    "afr_metrics": (ctypes.c_int, [ctypes.POINTER(AfrGeomDesc), c_vp]),
     "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_double), c_vp]),
     "afr_inviscid_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),

@@ -32,6 +32,8 @@ static int fail(const char* msg) {
     return -1;
 }
 
+void afr_set_error(const std::string& msg) { g_err = msg; }
+
 #define HIP_TRY(x)                                                              \
     do {                                                                        \
         hipError_t e_ = (x);                                                    \

@@ -33,16 +33,17 @@ class DeviceMesh:
     """One mesh at one order on one GPU."""
 
     def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, ref_state=None,
-                 slots=None, metrics=None):
+                 slots=None, metrics="gpu"):
+        """metrics: "gpu" (FP64 precompute on the device) or "host" (NumPy FP64, small meshes)."""
         self.lib = _lib.load()
         self.mesh, self.order, self.dtype, self.device = mesh, order, dtype, torch.device(device)
         T = tables(order)
         self.T = T
         C = mesh.n_cells
         self.n_cells, self.ns, self.nfp, self.nf = C, T.ns, T.nfp, T.nf
-        geo = metrics if metrics is not None else cell_block_metrics(mesh, T, dtype=np.float64)
-        sl = slots if slots is not None else build_slots(mesh, T)
-        self.geo, self.slots = geo, sl
+        on_gpu = metrics == "gpu"
+        sl = slots if slots is not None else build_slots(mesh, T, with_geometry=not on_gpu)
+        self.slots = sl
         self._validate(sl, C, T)
 
         def dev(x, dt=None):
@@ -51,27 +52,37 @@ class DeviceMesh:
         def devi(x):
             return torch.as_tensor(np.ascontiguousarray(x, dtype=np.int32)).to(self.device)
 
-        self.det = dev(geo["det"])
-        self.adj = dev(geo["adj"])
-        self.adj_fine = dev(geo["adj_fine"])
-        self.volume = geo["volume"]
         self.slot_nb, self.slot_blk = devi(sl.nb), devi(sl.blk)
-        self.slot_mask, self.slot_bc, self.slot_fb = devi(sl.mask), devi(sl.bc), devi(sl.fallback)
-        self.slot_tn, self.slot_adjrow = dev(sl.tn), dev(sl.adjrow)
+        self.slot_mask, self.slot_bc = devi(sl.mask), devi(sl.bc)
+        self.slot_tn = dev(sl.tn)
         self.rows = dev(sl.rows)
-        self.c2f = [dev(T.c2f[ct]) for ct in (0, 1)]
-        self.op = [dev(T.Op[ct]) for ct in (0, 1)]
         self.E = []
+        self.E64 = []
         for ct in (0, 1):
             E = np.zeros((NSLOT, T.nfp, T.ns))
             for si, code in enumerate(SLOT_CODES[ct]):
                 E[si] = T.E[ct, code // 2, code % 2]
             self.E.append(dev(E))
+            self.E64.append(E)
+        if on_gpu:
+            self._gpu_metrics(sl)
+        else:
+            geo = metrics if isinstance(metrics, dict) else cell_block_metrics(mesh, T, dtype=np.float64)
+            self.det = dev(geo["det"])
+            self.adj = dev(geo["adj"])
+            self.adj_fine = dev(geo["adj_fine"])
+            self.gcl = dev(geo["gcl"])
+            self.metric_flux_scale = dev(geo["metric_flux_scale"])
+            self.volume = np.asarray(geo["volume"], dtype=np.float64)
+            self.det_mean = np.asarray(geo["det_mean"], dtype=np.float64)
+            self.slot_fb = devi(sl.fallback)
+            self.slot_adjrow = dev(sl.adjrow)
+        self.c2f = [dev(T.c2f[ct]) for ct in (0, 1)]
+        self.op = [dev(T.Op[ct]) for ct in (0, 1)]
         self.D = [dev(np.transpose(T.D[ct], (2, 0, 1))) for ct in (0, 1)]
         self.filt = [dev(T.filt[ct]) for ct in (0, 1)]
         self.gL, self.gR = dev(T.gL), dev(T.gR)
         self.fp_of_sp, self.coord_of_sp = devi(T.fp_of_sp), devi(T.coord_of_sp)
-        self.gcl = dev(geo["gcl"])
         bcp = np.zeros((8, 5)) if bc_params is None else np.asarray(bc_params, dtype=np.float64)
         self.bc_params = dev(bcp, torch.float64)
         # reference state of the perturbation form: the freestream by default
@@ -104,6 +115,55 @@ class DeviceMesh:
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
         self.handle = handle
+
+    def _gpu_metrics(self, sl):
+        """FP64 metric precompute on the device (afr_metrics)."""
+        mesh, T, dt, C = self.mesh, self.T, self.dtype, self.n_cells
+        f64 = torch.float64
+        dv = self.device
+
+        def t64(x):
+            return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64)).to(dv)
+
+        nodes = t64(mesh.nodes)
+        pconn = torch.as_tensor(np.ascontiguousarray(mesh.prism_conn.reshape(-1, 6), dtype=np.int32)).to(dv)
+        tconn = torch.as_tensor(np.ascontiguousarray(mesh.tet_conn.reshape(-1, 4), dtype=np.int32)).to(dv)
+        if mesh.n_prism and int(mesh.prism_conn.max()) >= len(mesh.nodes):
+            raise ValueError("prism connectivity out of range")
+        if mesh.n_tet and int(mesh.tet_conn.max()) >= len(mesh.nodes):
+            raise ValueError("tet connectivity out of range")
+        sps, fine, w = t64(T.sps), t64(T.fine), t64(T.w_sps)
+        op64 = [t64(T.Op[ct]) for ct in (0, 1)]
+        E64 = [t64(e) for e in self.E64]
+        tn64 = t64(sl.tn)
+        self.det = torch.empty((C, T.ns), dtype=dt, device=dv)
+        self.adj = torch.empty((C, T.ns, 3, 3), dtype=dt, device=dv)
+        self.adj_fine = torch.empty((C, T.nf, 3, 3), dtype=dt, device=dv)
+        self.gcl = torch.empty((C, T.ns, 3), dtype=dt, device=dv)
+        self.metric_flux_scale = torch.empty((C, T.ns), dtype=dt, device=dv)
+        self.slot_adjrow = torch.empty((C, NSLOT, T.nfp, 3), dtype=dt, device=dv)
+        vol = torch.empty(C, dtype=f64, device=dv)
+        dmean = torch.empty(C, dtype=f64, device=dv)
+        self.slot_fb = torch.empty((C, NSLOT), dtype=torch.int32, device=dv)
+        bad = torch.zeros(1, dtype=torch.int32, device=dv)
+        g = _lib.AfrGeomDesc()
+        g.abi_version, g.order, g.dtype = _lib.ABI_VERSION, self.order, 0 if dt == torch.float32 else 1
+        g.n_cells, g.n_prism = C, mesh.n_prism
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        g.nodes, g.prism_conn, g.tet_conn = p(nodes), p(pconn), p(tconn)
+        g.sps, g.fine, g.w_sps = p(sps), p(fine), p(w)
+        for ct in (0, 1):
+            g.op64[ct], g.E64[ct] = p(op64[ct]), p(E64[ct])
+        g.slot_tn64 = p(tn64)
+        g.det, g.adj, g.adj_fine, g.gcl = p(self.det), p(self.adj), p(self.adj_fine), p(self.gcl)
+        g.metric_flux_scale, g.slot_adjrow = p(self.metric_flux_scale), p(self.slot_adjrow)
+        g.volume, g.det_mean, g.slot_fallback, g.bad_det = p(vol), p(dmean), p(self.slot_fb), p(bad)
+        _lib.check(self.lib.afr_metrics(ctypes.byref(g), self._stream()), "afr_metrics")
+        torch.cuda.synchronize(dv)
+        if int(bad.item()) != 0:
+            raise ValueError("non-positive Jacobian determinant (mesh distortion)")
+        self.volume = vol.cpu().numpy()
+        self.det_mean = dmean.cpu().numpy()
 
     @staticmethod
     def _validate(sl, C, T):

@@ -156,7 +156,8 @@ def _quad_lower_mask(mesh, cell, code, tri_nodes, n1):
     return low if is_lower else ~low
 
 
-def build_slots(mesh, T, check_samples=4, rng_seed=0) -> Slots:
+def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slots:
+    """with_geometry=False leaves adjrow/fallback to the device metric precompute."""
     fr = mesh.faces
     F, C, n1, nfp = fr.n_faces, mesh.n_cells, T.n1, T.nfp
     bnd = fr.is_boundary
@@ -336,12 +337,12 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0) -> Slots:
     rows = np.concatenate(rows_list, axis=0) if rows_list else np.zeros((nfp, T.ns))
 
     # ---- extrapolated adjugate rows and direction fallback
-    adjrow = np.zeros((C, NSLOT, nfp, 3))
-    fallback = np.zeros((C, NSLOT), dtype=np.int32)
+    adjrow = np.zeros((C, NSLOT, nfp, 3)) if with_geometry else None
+    fallback = np.zeros((C, NSLOT), dtype=np.int32) if with_geometry else None
     chunk = 65536
     for is_prism, conn, off in ((True, mesh.prism_conn, 0), (False, mesh.tet_conn, mesh.n_prism)):
         ct = 0 if is_prism else 1
-        for s0 in range(0, len(conn), chunk):
+        for s0 in range(0, len(conn) if with_geometry else 0, chunk):
             sl = slice(s0, min(s0 + chunk, len(conn)))
             _, adj = det_adj(jacobian(is_prism, T.sps, mesh.nodes[conn[sl]]))
             g = np.arange(off + sl.start, off + sl.stop)

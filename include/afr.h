@@ -84,6 +84,32 @@ typedef struct AfrMeshDesc {
 
 typedef struct AfrMesh AfrMesh;
 
+/* Inputs/outputs of the FP64 metric precompute (afr_metrics).  nodes (N,3)
+ * FP64, prism_conn (n_prism,6) / tet_conn (n_tet,4) int32 (orientation-fixed),
+ * sps (ns,3) / fine (nf,3) / w_sps (ns) FP64 reference points and weights,
+ * op64[t] (ns, 3*nf) and E64[t] (5, nfp, ns) FP64 tables in slot order,
+ * slot_tn64 (C,5,3) FP64.  Outputs in the storage precision `dtype`: det,
+ * adj, adj_fine, gcl, metric_flux_scale, slot_adjrow (layouts as in
+ * AfrMeshDesc); FP64 volume (C) and det_mean (C); int32 slot_fallback (C,5);
+ * bad_det (1 int32, set non-zero if any det(J) <= 0; caller zeroes it). */
+typedef struct AfrGeomDesc {
+    int32_t abi_version;
+    int32_t order;
+    int32_t dtype;
+    int32_t pad_;
+    int64_t n_cells;
+    int64_t n_prism;
+    const double *nodes;
+    const int32_t *prism_conn, *tet_conn;
+    const double *sps, *fine, *w_sps;
+    const double *op64[2], *E64[2];
+    const double *slot_tn64;
+    void *det, *adj, *adj_fine, *gcl, *metric_flux_scale, *slot_adjrow;
+    double *volume, *det_mean;
+    int32_t *slot_fallback;
+    int32_t *bad_det;
+} AfrGeomDesc;
+
 /* Library ABI version (AFR_ABI_VERSION). */
 int afr_version(void);
 
@@ -93,6 +119,11 @@ int afr_last_error(char *buf, size_t n);
 /* Validates and records a mesh description (no device allocation). */
 int afr_mesh_create(const AfrMeshDesc *desc, AfrMesh **out);
 int afr_mesh_destroy(AfrMesh *mesh);
+
+/* FP64 metric precompute on the device (replaces the per-cell Python loops of
+ * src/autoflowcfd/grid/high_order/high_order_mesh_order.py:47-99,152-160 and
+ * the flat-face adjugate extrapolation of inviscid_kernel.py:131-133). */
+int afr_metrics(const AfrGeomDesc *geom, void *stream);
 
 /* dQ = conserved_to_primitive(U) - ref (ref: 5 host doubles, may be NULL) for n_points points. */
 int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, const double *ref, void *stream);
