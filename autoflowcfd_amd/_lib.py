@@ -46,6 +46,7 @@ SIGNATURES = {
     "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_double), c_vp]),
     "afr_inviscid_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
+    "afr_inviscid_from_primitives": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "afr_viscous_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int32, c_vp]),
 }

@@ -335,9 +335,8 @@ int grid_for(int64_t tiles) {
 }
 
 template <int P, typename real>
-int launch_inviscid(const AfrMeshDesc& d, const real* U, real* R, int suppress, hipStream_t st) {
+int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* U, real* R, int suppress, hipStream_t st) {
     using O = Ord<P>;
-    const real* Q = (const real*)d.work_q;
     for (int ct = 0; ct < 2; ++ct) {
         const int64_t begin = ct == 0 ? 0 : d.n_prism;
         const int64_t n = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
@@ -439,18 +438,25 @@ int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, cons
                             : launch_primitives<float>(U, Q, n_points, ref, st);
 }
 
+int afr_inviscid_from_primitives(const AfrMesh* mesh, const void* dQ, const void* U, void* R, int32_t suppress,
+                                 void* stream) {
+    if (!mesh || !dQ || !U || !R) return fail("afr_inviscid_from_primitives: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    hipStream_t st = (hipStream_t)stream;
+    if (d.dtype == AFR_F32) {
+        return d.order == 1 ? launch_inviscid<1, float>(d, (const float*)dQ, (const float*)U, (float*)R, suppress, st)
+                            : launch_inviscid<2, float>(d, (const float*)dQ, (const float*)U, (float*)R, suppress, st);
+    }
+    return d.order == 1 ? launch_inviscid<1, double>(d, (const double*)dQ, (const double*)U, (double*)R, suppress, st)
+                        : launch_inviscid<2, double>(d, (const double*)dQ, (const double*)U, (double*)R, suppress, st);
+}
+
 int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t suppress, void* stream) {
     if (!mesh || !U || !R) return fail("afr_inviscid_residual: null argument");
     const AfrMeshDesc& d = mesh->d;
-    hipStream_t st = (hipStream_t)stream;
     const int64_t n_pts = d.n_cells * (int64_t)((d.order + 1) * (d.order + 1) * (d.order + 1));
     if (afr_primitives(U, d.work_q, n_pts, d.dtype, d.ref_state, stream)) return -1;
-    if (d.dtype == AFR_F32) {
-        return d.order == 1 ? launch_inviscid<1, float>(d, (const float*)U, (float*)R, suppress, st)
-                            : launch_inviscid<2, float>(d, (const float*)U, (float*)R, suppress, st);
-    }
-    return d.order == 1 ? launch_inviscid<1, double>(d, (const double*)U, (double*)R, suppress, st)
-                        : launch_inviscid<2, double>(d, (const double*)U, (double*)R, suppress, st);
+    return afr_inviscid_from_primitives(mesh, d.work_q, U, R, suppress, stream);
 }
 
 int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,

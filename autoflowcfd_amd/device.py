@@ -197,6 +197,23 @@ class DeviceMesh:
         if tuple(U.shape[:2]) != (self.n_cells, self.ns) or U.shape[2] < 5:
             raise ValueError(f"state shape {tuple(U.shape)} != ({self.n_cells}, {self.ns}, 5)")
 
+    def primitives(self, U, dQ=None):
+        """dQ = primitive(U) - ref_state into dQ (default: the mesh work buffer)."""
+        self._check_state(U)
+        dQ = self.work_q if dQ is None else dQ
+        ref = (ctypes.c_double * 5)(*[float(x) for x in self.ref_state])
+        _lib.check(self.lib.afr_primitives(ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(dQ.data_ptr()),
+                                           self.n_cells * self.ns, 0 if self.dtype == torch.float32 else 1,
+                                           ref, self._stream()), "afr_primitives")
+        return dQ
+
+    def inviscid_from_primitives(self, dQ, U, R, suppress=True):
+        _lib.check(self.lib.afr_inviscid_from_primitives(self.handle, ctypes.c_void_p(dQ.data_ptr()),
+                                                         ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(R.data_ptr()),
+                                                         int(bool(suppress)), self._stream()),
+                   "afr_inviscid_from_primitives")
+        return R
+
     def inviscid_residual(self, U, R=None, suppress=True):
         """dU/dt inviscid (C, ns, 5) of the conserved state U (C, ns, 5)."""
         self._check_state(U)

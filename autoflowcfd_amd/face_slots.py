@@ -288,15 +288,26 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slo
         own_corners = _face_corners(mesh, kc, own_code[kc, ks])
         oth_corners = _face_corners(mesh, other0[kc, ks], ocode0[kc, ks])
         ncorner = np.where(own_corners[:, 3] >= 0, 4, 3)
-        Xo = mesh.nodes[np.maximum(own_corners, 0)] + trans[kc, ks][:, None, :]
-        Xt = mesh.nodes[np.maximum(oth_corners, 0)]
-        d2 = np.sum((Xo[:, :, None, :] - Xt[:, None, :, :]) ** 2, axis=-1)
-        d2[:, :, 3][oth_corners[:, 3] < 0] = np.inf
-        pi = np.argmin(d2, axis=2)
+        # corner correspondence: shared node ids, or positions for periodic images
+        hit = own_corners[:, :, None] == oth_corners[:, None, :]
+        hit &= (own_corners[:, :, None] >= 0)
+        pi = np.argmax(hit, axis=2)
+        ok = np.all(hit.any(axis=2) | (own_corners < 0), axis=1)
+        per = np.flatnonzero(~ok)
+        if len(per):
+            Xo = mesh.nodes[np.maximum(own_corners[per], 0)] + trans[kc[per], ks[per]][:, None, :]
+            Xt = mesh.nodes[np.maximum(oth_corners[per], 0)]
+            d2 = np.zeros((len(per), 4, 4))
+            for i in range(4):
+                for j in range(4):
+                    dd = Xo[:, i] - Xt[:, j]
+                    d2[:, i, j] = np.einsum("ki,ki->k", dd, dd)
+            d2[:, :, 3][oth_corners[per, 3] < 0] = np.inf
+            pi[per] = np.argmin(d2, axis=2)
+            scale = np.einsum("ki,ki->k", Xo[:, 1] - Xo[:, 0], Xo[:, 1] - Xo[:, 0])
+            if np.any(np.min(d2, axis=2)[:, :3].max(axis=1) > 1e-12 * scale + 1e-300):
+                raise RuntimeError("face corners of the two sides do not coincide")
         pi[ncorner == 3, 3] = 0
-        scale = np.sum((Xo[:, 1] - Xo[:, 0]) ** 2, axis=-1)
-        if np.any(np.min(d2, axis=2)[:, :3].max(axis=1) > 1e-12 * scale + 1e-300):
-            raise RuntimeError("face corners of the two sides do not coincide")
         key = (((ctype[kc, ks] * 6 + own_code[kc, ks]) * 2 + (other0[kc, ks] >= mesh.n_prism)) * 6
                + ocode0[kc, ks]) * 256 + pi[:, 0] + 4 * pi[:, 1] + 16 * pi[:, 2] + 64 * pi[:, 3]
         ukeys, first, inv = np.unique(key, return_index=True, return_inverse=True)

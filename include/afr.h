@@ -132,6 +132,11 @@ int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, cons
  * interface correction, optional outlier suppression.  U is (C, ns, 5). */
 int afr_inviscid_residual(const AfrMesh *mesh, const void *U, void *R, int32_t suppress, void *stream);
 
+/* Same as afr_inviscid_residual with dQ = primitive(U) - ref_state already in
+ * dQ (as written by afr_primitives); U is still read for the outlier reference. */
+int afr_inviscid_from_primitives(const AfrMesh *mesh, const void *dQ, const void *U, void *R, int32_t suppress,
+                                 void *stream);
+
 /* Viscous BR1 FR residual dU/dt (C, ns, 5); mu_t (C, ns) or NULL.
  * mu, Pr, Pr_t as in the reference (1.8e-5, 0.72, 0.9 by default). */
 int afr_viscous_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu,
