@@ -5,7 +5,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafr.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_vp = ctypes.c_void_p
 
@@ -17,10 +17,12 @@ class AfrMeshDesc(ctypes.Structure):
         ("det", c_vp), ("adj", c_vp), ("adj_fine", c_vp),
         ("slot_nb", c_vp), ("slot_blk", c_vp), ("slot_mask", c_vp), ("slot_bc", c_vp), ("slot_fallback", c_vp),
         ("slot_tn", c_vp), ("slot_adjrow", c_vp), ("rows", c_vp),
+        ("rows_nnz", c_vp), ("rows_idx", c_vp), ("rows_w", c_vp), ("rows_knz", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
         ("c2f", c_vp * 2), ("op", c_vp * 2), ("E", c_vp * 2), ("D", c_vp * 2), ("filt", c_vp * 2),
-        ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp),
+        ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp), ("prism_tab", c_vp),
         ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
-        ("work_q", c_vp), ("work_grad", c_vp),
+        ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
     ]
 
 
@@ -45,6 +47,7 @@ SIGNATURES = {
     "afr_metrics": (ctypes.c_int, [ctypes.POINTER(AfrGeomDesc), c_vp]),
     "afr_primitives": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_double), c_vp]),
+    "afr_prepare_state": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "afr_inviscid_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "afr_inviscid_from_primitives": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "afr_viscous_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,

@@ -1,7 +1,8 @@
 // Cell-centric FR residual kernels for gfx950 and the afr C ABI.
 //
-// One workgroup of 256 threads processes tiles of CPB cells of one type
-// (prism or tet) at one order.  Per tile, everything stays in LDS:
+// Tets: one workgroup of 256 threads processes tiles of CPB cells at one
+// order with the dense operators (prisms use the Kronecker-factored kernel of
+// afr_prism.hip).  Per tile, everything stays in LDS:
 //   1. primitive states of the tile's cells,
 //   2. contravariant fluxes at the over-integration points
 //      (Q_fine = c2f Q, F~ = adj_fine F(Q_fine)),
@@ -22,6 +23,7 @@
 
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
+#include "afr_prism.hpp"
 
 using namespace afr;
 
@@ -73,7 +75,7 @@ template <typename real>
 struct Args {
     int64_t cell_begin, n_cells_type;
     const real* Q;        // (C, NS, 5) primitive
-    const real* U;        // (C, NS, 5) conserved (outlier reference)
+    const real* umean;    // (C, 5) mean |U| per cell and variable (outlier reference)
     real* R;
     const real* det;
     const real* adjf;
@@ -98,10 +100,6 @@ struct Args {
     double fref[15];      // F(qref)[i][v]
     int suppress;
     int n_rows;
-};
-
-struct Ref5 {
-    double q[5];
 };
 
 template <typename real>
@@ -310,10 +308,7 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
             const int c = t / 5, v = t % 5;
             if (c < ncell) {
                 real med = median_abs<real, NS>(s_R + c * NS * 5 + v, 5);
-                real mu = real(0);
-                const real* u = a.U + (first + c) * NS * 5 + v;
-                for (int s = 0; s < NS; ++s) mu += (u[s * 5] < real(0) ? -u[s * 5] : u[s * 5]);
-                mu /= real(NS);
+                const real mu = a.umean[(first + c) * 5 + v];
                 real ref = rmax(rmax(med, real(1e-9) * mu), Tiny<real>::v);
                 real thr = real(1e4) * ref;
                 for (int s = 0; s < NS; ++s) {
@@ -335,9 +330,50 @@ int grid_for(int64_t tiles) {
 }
 
 template <int P, typename real>
-int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* U, real* R, int suppress, hipStream_t st) {
+int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real* R, int suppress, hipStream_t st) {
     using O = Ord<P>;
-    for (int ct = 0; ct < 2; ++ct) {
+    double fref[15];
+    {
+        double F[3][5];
+        euler_flux(d.ref_state, F);
+        for (int i = 0; i < 3; ++i)
+            for (int v = 0; v < 5; ++v) fref[i * 5 + v] = F[i][v];
+    }
+    if (d.n_prism > 0) {
+        // prisms: Kronecker-factored wave-per-cell kernel
+        afr_prism::PrismArgs<real> p;
+        p.n_cells = d.n_prism;
+        p.dQ = Q;
+        p.umean = umean;
+        p.R = R;
+        p.det = (const real*)d.det;
+        p.adjf = (const real*)d.adj_fine;
+        p.gcl = (const real*)d.gcl;
+        p.slot_nb = d.slot_nb;
+        p.slot_blk = d.slot_blk;
+        p.slot_mask = d.slot_mask;
+        p.slot_bc = d.slot_bc;
+        p.slot_fb = d.slot_fallback;
+        p.slot_tn = (const real*)d.slot_tn;
+        p.slot_adjrow = (const real*)d.slot_adjrow;
+        p.rows_nnz = d.rows_nnz;
+        p.rows_idx = d.rows_idx;
+        p.rows_w = (const real*)d.rows_w;
+        p.knz = d.rows_knz;
+        p.n_rows = d.n_rows;
+        p.tab = (const real*)d.prism_tab;
+        p.gL = (const real*)d.gL;
+        p.gR = (const real*)d.gR;
+        p.fp_of_sp = d.fp_of_sp;
+        p.coord_of_sp = d.coord_of_sp;
+        p.bcp = d.bc_params;
+        for (int v = 0; v < 5; ++v) p.qref[v] = d.ref_state[v];
+        for (int k = 0; k < 15; ++k) p.fref[k] = fref[k];
+        p.suppress = suppress;
+        if (afr_prism::launch_prism_inviscid<P, real>(p, st)) return fail("k_prism_inviscid launch failed");
+    }
+    // tets: tile-per-workgroup kernel with dense operators
+    for (int ct = 1; ct < 2; ++ct) {
         const int64_t begin = ct == 0 ? 0 : d.n_prism;
         const int64_t n = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
         if (n <= 0) continue;
@@ -345,7 +381,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* U, real* R,
         a.cell_begin = begin;
         a.n_cells_type = n;
         a.Q = Q;
-        a.U = U;
+        a.umean = umean;
         a.R = R;
         a.det = (const real*)d.det;
         a.adjf = (const real*)d.adj_fine;
@@ -367,19 +403,11 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* U, real* R,
         a.gcl = (const real*)d.gcl;
         a.bcp = d.bc_params;
         for (int v = 0; v < 5; ++v) a.qref[v] = d.ref_state[v];
-        {
-            double F[3][5];
-            euler_flux(d.ref_state, F);
-            for (int i = 0; i < 3; ++i)
-                for (int v = 0; v < 5; ++v) a.fref[i * 5 + v] = F[i][v];
-        }
+        for (int k = 0; k < 15; ++k) a.fref[k] = fref[k];
         a.suppress = suppress;
         a.n_rows = d.n_rows;
         const int grid = grid_for((n + O::CPB - 1) / O::CPB);
-        if (ct == 0)
-            hipLaunchKernelGGL((k_inviscid<P, 0, real>), dim3(grid), dim3(kThreads), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_inviscid<P, 1, real>), dim3(grid), dim3(kThreads), 0, st, a);
+        hipLaunchKernelGGL((k_inviscid<P, 1, real>), dim3(grid), dim3(kThreads), 0, st, a);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -418,7 +446,8 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
         return fail("afr_mesh_create: bad cell counts");
     if (!desc->det || !desc->adj_fine || !desc->slot_nb || !desc->slot_blk || !desc->slot_mask || !desc->slot_bc ||
         !desc->slot_fallback || !desc->slot_tn || !desc->slot_adjrow || !desc->rows || !desc->gL || !desc->gR ||
-        !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q || !desc->gcl)
+        !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q || !desc->gcl ||
+        !desc->work_umean || !desc->prism_tab || !desc->rows_nnz || !desc->rows_idx || !desc->rows_w)
         return fail("afr_mesh_create: missing device array");
     AfrMesh* m = new AfrMesh;
     m->d = *desc;
@@ -438,25 +467,50 @@ int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, cons
                             : launch_primitives<float>(U, Q, n_points, ref, st);
 }
 
-int afr_inviscid_from_primitives(const AfrMesh* mesh, const void* dQ, const void* U, void* R, int32_t suppress,
-                                 void* stream) {
-    if (!mesh || !dQ || !U || !R) return fail("afr_inviscid_from_primitives: null argument");
+int afr_prepare_state(const AfrMesh* mesh, const void* U, void* dQ, void* umean, void* stream) {
+    if (!mesh || !U || !dQ || !umean) return fail("afr_prepare_state: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    hipStream_t st = (hipStream_t)stream;
+    const int ns = (d.order + 1) * (d.order + 1) * (d.order + 1);
+    int rc;
+    if (d.dtype == AFR_F32) {
+        auto* u = (const float*)U;
+        auto* q = (float*)dQ;
+        auto* m = (float*)umean;
+        rc = ns == 8 ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, st)
+                     : afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, st);
+    } else {
+        auto* u = (const double*)U;
+        auto* q = (double*)dQ;
+        auto* m = (double*)umean;
+        rc = ns == 8 ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, st)
+                     : afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, st);
+    }
+    return rc ? fail("k_prim_umean launch failed") : 0;
+}
+
+int afr_inviscid_from_primitives(const AfrMesh* mesh, const void* dQ, const void* umean, void* R,
+                                 int32_t suppress, void* stream) {
+    if (!mesh || !dQ || !umean || !R) return fail("afr_inviscid_from_primitives: null argument");
     const AfrMeshDesc& d = mesh->d;
     hipStream_t st = (hipStream_t)stream;
     if (d.dtype == AFR_F32) {
-        return d.order == 1 ? launch_inviscid<1, float>(d, (const float*)dQ, (const float*)U, (float*)R, suppress, st)
-                            : launch_inviscid<2, float>(d, (const float*)dQ, (const float*)U, (float*)R, suppress, st);
+        auto* q = (const float*)dQ;
+        auto* m = (const float*)umean;
+        return d.order == 1 ? launch_inviscid<1, float>(d, q, m, (float*)R, suppress, st)
+                            : launch_inviscid<2, float>(d, q, m, (float*)R, suppress, st);
     }
-    return d.order == 1 ? launch_inviscid<1, double>(d, (const double*)dQ, (const double*)U, (double*)R, suppress, st)
-                        : launch_inviscid<2, double>(d, (const double*)dQ, (const double*)U, (double*)R, suppress, st);
+    auto* q = (const double*)dQ;
+    auto* m = (const double*)umean;
+    return d.order == 1 ? launch_inviscid<1, double>(d, q, m, (double*)R, suppress, st)
+                        : launch_inviscid<2, double>(d, q, m, (double*)R, suppress, st);
 }
 
 int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t suppress, void* stream) {
     if (!mesh || !U || !R) return fail("afr_inviscid_residual: null argument");
     const AfrMeshDesc& d = mesh->d;
-    const int64_t n_pts = d.n_cells * (int64_t)((d.order + 1) * (d.order + 1) * (d.order + 1));
-    if (afr_primitives(U, d.work_q, n_pts, d.dtype, d.ref_state, stream)) return -1;
-    return afr_inviscid_from_primitives(mesh, d.work_q, U, R, suppress, stream);
+    if (afr_prepare_state(mesh, U, d.work_q, d.work_umean, stream)) return -1;
+    return afr_inviscid_from_primitives(mesh, d.work_q, d.work_umean, R, suppress, stream);
 }
 
 int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,

@@ -12,11 +12,26 @@ import torch
 
 from . import _lib
 from .face_slots import NSLOT, SLOT_CODES, build_slots
-from .fr_tables import tables
+from .fr_tables import prism_factors, tables
 from .geometry import cell_block_metrics
 from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
 
 GAMMA = 1.4
+ROW_DROP = 1e-11   # cross-interpolation weights below this are round-off of structural zeros
+
+
+def sparse_rows(rows, drop=ROW_DROP):
+    """Dense (n_rows, ns) interpolation rows -> (nnz (n_rows,), idx (n_rows, k), w (n_rows, k))."""
+    keep = np.abs(rows) > drop
+    lost = np.abs(np.where(keep, 0.0, rows)).sum(axis=1)
+    if lost.size and lost.max() > 1e-9:
+        raise ValueError(f"sparse rows drop weight {lost.max():.2e}")
+    nnz = keep.sum(axis=1).astype(np.int32)
+    k = max(int(nnz.max()) if nnz.size else 1, 1)
+    order = np.argsort(~keep, axis=1, kind="stable")[:, :k]
+    idx = np.where(np.arange(k)[None, :] < nnz[:, None], order, 0).astype(np.int32)
+    w = np.where(np.arange(k)[None, :] < nnz[:, None], np.take_along_axis(rows, order, axis=1), 0.0)
+    return nnz, idx, w
 
 
 def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
@@ -44,7 +59,7 @@ class DeviceMesh:
         on_gpu = metrics == "gpu"
         sl = slots if slots is not None else build_slots(mesh, T, with_geometry=not on_gpu)
         self.slots = sl
-        self._validate(sl, C, T)
+        self._validate(sl, C, T, mesh.n_prism)
 
         def dev(x, dt=None):
             return torch.as_tensor(np.ascontiguousarray(x), dtype=dt or dtype).to(self.device)
@@ -56,6 +71,10 @@ class DeviceMesh:
         self.slot_mask, self.slot_bc = devi(sl.mask), devi(sl.bc)
         self.slot_tn = dev(sl.tn)
         self.rows = dev(sl.rows)
+        nnz, idx, w = sparse_rows(sl.rows)
+        self.rows_nnz, self.rows_idx, self.rows_w = devi(nnz), devi(idx), dev(w)
+        self.rows_knz = int(idx.shape[1])
+        self.prism_tab = dev(prism_factors(order).packed())
         self.E = []
         self.E64 = []
         for ct in (0, 1):
@@ -89,6 +108,7 @@ class DeviceMesh:
         self.ref_state = np.asarray(ref_state if ref_state is not None else bcp[BC_FARFIELD], dtype=np.float64)
         self.work_q = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
         self.work_grad = torch.empty((C, T.ns, 13), dtype=dtype, device=self.device)
+        self.work_umean = torch.empty((C, 5), dtype=dtype, device=self.device)
 
         d = _lib.AfrMeshDesc()
         d.abi_version = _lib.ABI_VERSION
@@ -102,6 +122,9 @@ class DeviceMesh:
         d.slot_nb, d.slot_blk, d.slot_mask = p(self.slot_nb), p(self.slot_blk), p(self.slot_mask)
         d.slot_bc, d.slot_fallback = p(self.slot_bc), p(self.slot_fb)
         d.slot_tn, d.slot_adjrow, d.rows = p(self.slot_tn), p(self.slot_adjrow), p(self.rows)
+        d.rows_nnz, d.rows_idx, d.rows_w = p(self.rows_nnz), p(self.rows_idx), p(self.rows_w)
+        d.rows_knz = self.rows_knz
+        d.prism_tab = p(self.prism_tab)
         for ct in (0, 1):
             d.c2f[ct], d.op[ct], d.E[ct] = p(self.c2f[ct]), p(self.op[ct]), p(self.E[ct])
             d.D[ct], d.filt[ct] = p(self.D[ct]), p(self.filt[ct])
@@ -110,7 +133,7 @@ class DeviceMesh:
         d.gcl, d.bc_params = p(self.gcl), p(self.bc_params)
         for v in range(5):
             d.ref_state[v] = float(self.ref_state[v])
-        d.work_q, d.work_grad = p(self.work_q), p(self.work_grad)
+        d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
         self._desc = d
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
@@ -166,7 +189,7 @@ class DeviceMesh:
         self.det_mean = dmean.cpu().numpy()
 
     @staticmethod
-    def _validate(sl, C, T):
+    def _validate(sl, C, T, n_prism):
         """Host-side bounds checks of every index the kernels dereference."""
         nb = sl.nb
         if nb.min() < -1 or nb.max() >= C:
@@ -179,6 +202,8 @@ class DeviceMesh:
             raise ValueError("slot row block out of range")
         if np.any((sl.bc < -2) | (sl.bc > 6)):
             raise ValueError("unknown boundary code")
+        if np.any(sl.bc[:n_prism] == -2):
+            raise ValueError("prism with an unused face slot")
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -207,9 +232,20 @@ class DeviceMesh:
                                            ref, self._stream()), "afr_primitives")
         return dQ
 
-    def inviscid_from_primitives(self, dQ, U, R, suppress=True):
+    def prepare_state(self, U, dQ=None, umean=None):
+        """dQ = primitive(U) - ref_state and the per-cell mean |U| (outlier reference)."""
+        self._check_state(U)
+        dQ = self.work_q if dQ is None else dQ
+        umean = self.work_umean if umean is None else umean
+        _lib.check(self.lib.afr_prepare_state(self.handle, ctypes.c_void_p(U.data_ptr()),
+                                              ctypes.c_void_p(dQ.data_ptr()), ctypes.c_void_p(umean.data_ptr()),
+                                              self._stream()), "afr_prepare_state")
+        return dQ, umean
+
+    def inviscid_from_primitives(self, dQ, umean, R, suppress=True):
         _lib.check(self.lib.afr_inviscid_from_primitives(self.handle, ctypes.c_void_p(dQ.data_ptr()),
-                                                         ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(R.data_ptr()),
+                                                         ctypes.c_void_p(umean.data_ptr()),
+                                                         ctypes.c_void_p(R.data_ptr()),
                                                          int(bool(suppress)), self._stream()),
                    "afr_inviscid_from_primitives")
         return R

@@ -199,3 +199,90 @@ def tables(order: int) -> Tables:
                sps=sps, fine=fine, interp=tuple(interps))
     _CACHE[order] = t
     return t
+
+
+def _tri_basis(pts_ab, order, deriv=False):
+    """Collapsed triangle modes g_ij(a,b) (i,j <= order) at (a,b) points: (V, Va, Vb) (n, n1^2)."""
+    a, b = pts_ab[:, 0], pts_ab[:, 1]
+    n1 = order + 1
+    V, Va, Vb = (np.zeros((len(a), n1 * n1)) for _ in range(3))
+    hb = 0.5 * (1.0 - b)
+    for i in range(n1):
+        fa, dfa = _jac(a, 0.0, 0.0, i), _djac(a, 0.0, 0.0, i)
+        wb = hb ** i if i else np.ones_like(b)
+        dwb = -0.5 * i * hb ** (i - 1) if i else np.zeros_like(b)
+        for j in range(n1):
+            pj, dpj = _jac(b, 2.0 * i + 1, 0.0, j), _djac(b, 2.0 * i + 1, 0.0, j)
+            V[:, i * n1 + j] = fa * wb * pj
+            Va[:, i * n1 + j] = dfa * wb * pj
+            Vb[:, i * n1 + j] = fa * (dwb * pj + wb * dpj)
+    return V, Va, Vb
+
+
+def _line_basis(x, order):
+    n1 = order + 1
+    return (np.stack([_jac(x, 0.0, 0.0, k) for k in range(n1)], 1),
+            np.stack([_djac(x, 0.0, 0.0, k) for k in range(n1)], 1))
+
+
+def _rs(V_at, V_nodes):
+    return np.linalg.solve(V_nodes.T, V_at.T).T
+
+
+@dataclass
+class PrismFactors:
+    """Prism operators in (a,b)-triangle x c-line factored form (FP64)."""
+    c2f_ab: np.ndarray   # (nabf, nab)
+    c2f_c: np.ndarray    # (nf1, n1)
+    A_a: np.ndarray      # (nab, nabf) = f2c_ab @ D_ab,a (fine)
+    A_b: np.ndarray      # (nab, nabf)
+    B_c: np.ndarray      # (n1, nf1)   = f2c_c @ D_c (fine)
+    f2c_c: np.ndarray    # (n1, nf1)
+    f2c_ab: np.ndarray   # (nab, nabf)
+    E_ab: np.ndarray     # (3, n1, nab): quad slots a=-1, a=+1, b=-1; row = free (a|b) index
+    e_c: np.ndarray      # (2, n1): c=-1, c=+1
+    D_ab: np.ndarray     # (2, nab, nab) coarse d/da, d/db on the triangle
+    D_c: np.ndarray      # (n1, n1) coarse d/dc
+
+    def packed(self):
+        return np.concatenate([x.ravel() for x in (self.c2f_ab, self.c2f_c, self.A_a, self.A_b, self.B_c,
+                                                   self.f2c_c, self.f2c_ab, self.E_ab, self.e_c, self.D_ab,
+                                                   self.D_c)])
+
+
+def prism_factors(order: int) -> PrismFactors:
+    n1 = order + 1
+    over = min(2 * order, OVER_MAX)
+    nf1 = over + 1
+    x, _ = gl(n1)
+    xf, _ = gl(nf1)
+    A, B = np.meshgrid(x, x, indexing="ij")
+    ab = np.stack([A.ravel(), B.ravel()], 1)
+    Af, Bf = np.meshgrid(xf, xf, indexing="ij")
+    abf = np.stack([Af.ravel(), Bf.ravel()], 1)
+    Vab = _tri_basis(ab, order)[0]
+    Vab_f_P = _tri_basis(abf, order)[0]
+    Vo_f, Vo_fa, Vo_fb = _tri_basis(abf, over)
+    Vo_c = _tri_basis(ab, over)[0]
+    Vc = _line_basis(x, order)[0]
+    Vc_fP = _line_basis(xf, order)[0]
+    Vco_f, dVco_f = _line_basis(xf, over)
+    Vco_c = _line_basis(x, over)[0]
+    c2f_ab = _rs(Vab_f_P, Vab)
+    c2f_c = _rs(Vc_fP, Vc)
+    f2c_ab = _rs(Vo_c, Vo_f)
+    f2c_c = _rs(Vco_c, Vco_f)
+    Dfa, Dfb = _rs(Vo_fa, Vo_f), _rs(Vo_fb, Vo_f)
+    Dfc = _rs(dVco_f, Vco_f)
+    E_ab = np.zeros((3, n1, n1 * n1))
+    for si, (fix, val) in enumerate(((0, -1.0), (0, 1.0), (1, -1.0))):
+        p = np.zeros((n1, 2))
+        p[:, fix] = val
+        p[:, 1 - fix] = x
+        E_ab[si] = _rs(_tri_basis(p, order)[0], Vab)
+    e_c = np.stack([_rs(_line_basis(np.array([s]), order)[0], Vc)[0] for s in (-1.0, 1.0)])
+    Vab_c, Vab_a, Vab_b = _tri_basis(ab, order)
+    Vcc, dVcc = _line_basis(x, order)
+    return PrismFactors(c2f_ab=c2f_ab, c2f_c=c2f_c, A_a=f2c_ab @ Dfa, A_b=f2c_ab @ Dfb, B_c=f2c_c @ Dfc,
+                        f2c_c=f2c_c, f2c_ab=f2c_ab, E_ab=E_ab, e_c=e_c,
+                        D_ab=np.stack([_rs(Vab_a, Vab_c), _rs(Vab_b, Vab_c)]), D_c=_rs(dVcc, Vcc))

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define AFR_ABI_VERSION 1
+#define AFR_ABI_VERSION 2
 
 enum { AFR_F32 = 0, AFR_F64 = 1 };
 
@@ -50,9 +50,15 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   slot_tn    (C, S, 3)               role-signed true normal
  *   slot_adjrow (C, S, nfp, 3)         E @ adj[:, axis, :]
  *   rows       (n_rows, ns)            cross-cell interpolation rows
+ *   rows_nnz   (n_rows) int32          non-zeros of each row (<= rows_knz)
+ *   rows_idx   (n_rows, rows_knz) int32  source solution point of each non-zero
+ *   rows_w     (n_rows, rows_knz)      weight of each non-zero
  *   c2f[t] (nf, ns), op[t] (ns, 3*nf), E[t] (S, nfp, ns), D[t] (3, ns, ns),
  *   filt[t] (ns, ns) for t = 0 prism, 1 tet
  *   gL, gR (order+1), fp_of_sp / coord_of_sp (3, ns) int32
+ *   prism_tab  packed prism factors (a,b)-triangle x c-line: c2f_ab, c2f_c,
+ *              A_a = f2c_ab D_a, A_b = f2c_ab D_b, B_c = f2c_c D_c, f2c_c,
+ *              f2c_ab, E_ab (3 quad slots), e_c (2 c slots), D_ab, D_c
  *   gcl        (C, ns, 3)              constant-state divergence sum_{m,j} Op[s][m,j] adj_fine[j][m][i]
  *   bc_params  (8, 5) FP64             per BC code reference state (always double)
  *   ref_state  (5) FP64, host values   primitive reference state Q_ref; the kernels carry
@@ -60,6 +66,7 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *                                      point fluxes / Riemann fluxes in FP64
  *   work_q     (C, ns, 5)              scratch for dQ = primitive - Q_ref
  *   work_grad  (C, ns, 13)             scratch for (grad u (9), grad T (3), mu_t)
+ *   work_umean (C, 5)                  scratch for the per-cell mean |U| (outlier reference)
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
@@ -72,14 +79,20 @@ typedef struct AfrMeshDesc {
     const int32_t *slot_nb, *slot_blk, *slot_mask, *slot_bc, *slot_fallback;
     const void *slot_tn, *slot_adjrow;
     const void *rows;
+    const int32_t *rows_nnz, *rows_idx;
+    const void *rows_w;
+    int32_t rows_knz;
+    int32_t pad_;
     const void *c2f[2], *op[2], *E[2], *D[2], *filt[2];
     const void *gL, *gR;
     const int32_t *fp_of_sp, *coord_of_sp;
+    const void *prism_tab;
     const void *gcl;
     const double *bc_params;
     double ref_state[5];
     void *work_q;
     void *work_grad;
+    void *work_umean;
 } AfrMeshDesc;
 
 typedef struct AfrMesh AfrMesh;
@@ -128,14 +141,20 @@ int afr_metrics(const AfrGeomDesc *geom, void *stream);
 /* dQ = conserved_to_primitive(U) - ref (ref: 5 host doubles, may be NULL) for n_points points. */
 int afr_primitives(const void *U, void *Q, int64_t n_points, int32_t dtype, const double *ref, void *stream);
 
+/* Per-step state preparation of one mesh: dQ = primitive(U) - ref_state
+ * (C, ns, 5) and umean[c][v] = mean over solution points of |U[c][s][v]| (C, 5),
+ * the reference magnitude of the outlier test (troubled_cell.py:371-380). */
+int afr_prepare_state(const AfrMesh *mesh, const void *U, void *dQ, void *umean, void *stream);
+
 /* Inviscid FR residual dU/dt (C, ns, 5): over-integrated volume term, AUSM+up
- * interface correction, optional outlier suppression.  U is (C, ns, 5). */
+ * interface correction, optional outlier suppression.  U is (C, ns, 5).
+ * Uses the mesh work buffers work_q and work_umean. */
 int afr_inviscid_residual(const AfrMesh *mesh, const void *U, void *R, int32_t suppress, void *stream);
 
-/* Same as afr_inviscid_residual with dQ = primitive(U) - ref_state already in
- * dQ (as written by afr_primitives); U is still read for the outlier reference. */
-int afr_inviscid_from_primitives(const AfrMesh *mesh, const void *dQ, const void *U, void *R, int32_t suppress,
-                                 void *stream);
+/* Same as afr_inviscid_residual from the prepared state (dQ, umean) of
+ * afr_prepare_state. */
+int afr_inviscid_from_primitives(const AfrMesh *mesh, const void *dQ, const void *umean, void *R,
+                                 int32_t suppress, void *stream);
 
 /* Viscous BR1 FR residual dU/dt (C, ns, 5); mu_t (C, ns) or NULL.
  * mu, Pr, Pr_t as in the reference (1.8e-5, 0.72, 0.9 by default). */
