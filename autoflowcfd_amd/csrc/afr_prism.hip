@@ -23,20 +23,116 @@ using namespace afr;
 
 namespace afr_prism {
 
-template <int P, typename real>
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void gbl_void_t;
+
+// Asynchronous global -> LDS copy of one dword per lane (gfx950 global_load_lds_dword):
+// lane i writes dst[i]; completion is awaited with vm_wait_all().
+__device__ __forceinline__ void lds_dma_dword(const void* src_lane, void* dst_wave) {
+    __builtin_amdgcn_global_load_lds((gbl_void_t*)src_lane, (lds_void_t*)dst_wave, 4, 0, 0);
+}
+
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
+
+// Exact median-based outlier zeroing of the (NS x 5) residual block of one
+// cell held in s_R ([s][v]) and in rv (lane-strided copy), as in
+// troubled_cell.py:339-408: R[s][v] = 0 where |R| > 1e4 max(median_s |R[:,v]|,
+// 1e-9 mean_s |U[:,v]|, tiny).  A cheap certificate runs first: with
+// M = max_s |R[:,v]| and t = 1.0001e-4 M, if at most k1 = (NS-1)/2 values lie
+// below t then median >= t, so no value can exceed the threshold and the
+// variable is left as is.  Only when the certificate fails for some variable
+// is the median selected exactly (rank counting).
+template <int NS, typename real>
+__device__ __forceinline__ void outlier_zeroing(const real* s_R, real* s_med, real* rv, const real* umean, int t) {
+    constexpr int NR = (NS * 5 + 63) / 64;
+    constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
+    constexpr int G = 8, J = (NS + G - 1) / G;  // 8 lanes per variable, J values each
+    const int v = t >> 3, g = t & 7;
+    real mx = 0;
+    if (t < 5 * G) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int s = g + G * j;
+            if (s < NS) {
+                const real x = fabs(s_R[s * 5 + v]);
+                mx = x > mx ? x : mx;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        const real o = __shfl_xor(mx, m);
+        mx = o > mx ? o : mx;
+    }
+    const real tt = mx * real(1.0001e-4);
+    int cnt = 0;
+    if (t < 5 * G) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int s = g + G * j;
+            if (s < NS) cnt += fabs(s_R[s * 5 + v]) < tt;
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) cnt += __shfl_xor(cnt, m);
+    const bool need = (t < 5 * G) && cnt > k1;
+    if (!__any(need)) return;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int o = t + 64 * k;
+        if (o < NS * 5) {
+            const int vv = o % 5;
+            const real x = fabs(rv[k]);
+            int lt = 0, le = 0;
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const real y = fabs(s_R[s2 * 5 + vv]);
+                lt += y < x;
+                le += y <= x;
+            }
+            if (lt <= k1 && k1 < le) s_med[vv * 2 + 0] = x;
+            if (lt <= k2 && k2 < le) s_med[vv * 2 + 1] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int o = t + 64 * k;
+        if (o < NS * 5) {
+            const int vv = o % 5;
+            const real med = (NS % 2) ? s_med[vv * 2] : real(0.5) * (s_med[vv * 2] + s_med[vv * 2 + 1]);
+            const real um = real(1e-9) * umean[vv];
+            real ref = med > um ? med : um;
+            ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
+            if (fabs(rv[k]) > real(1e4) * ref) rv[k] = real(0);
+        }
+    }
+}
+
+template <int P, int KMAX, typename real>
 __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
     using K = PK<P>;
     constexpr int N1 = K::N1, NAB = K::NAB, NF1 = K::NF1, NABF = K::NABF, NF = K::NF, NS = K::NS, NFP = K::NFP;
+    constexpr int NSV = NS * 5;
+    constexpr int NR = (NSV + 63) / 64;
+    constexpr int NNB = 10;                             // source cells: 5 slots x 2 sources
+    constexpr int WPR = sizeof(real) / 4;               // dwords per real
+    constexpr int NDW = NSV * WPR;                      // dwords of one cell's state
+    constexpr int NCH = (NDW + 63) / 64;                // 64-dword chunks per cell
+    __shared__ double s_ref[20];                        // qref (5), F(qref) (15)
     __shared__ real s_tab[K::TABN];
     __shared__ real s_g[2 * N1];
     __shared__ int s_fp[3 * NS];
     __shared__ int s_co[3 * NS];
-    __shared__ real s_Q[NS * 5];
+    __shared__ int s_nb[NNB];
+    __shared__ real s_Q[NSV];
+    __shared__ __attribute__((aligned(16))) uint32_t s_N[NNB * NCH * 64];  // source-cell states (raw dwords)
+    __shared__ int s_ridx[KMAX * 64];                  // interpolation row of each interface lane
+    __shared__ uint32_t s_rw[KMAX * 64 * WPR];
     __shared__ real s_X[NAB * NF1 * 5];
     __shared__ real s_Ft[3 * NF * 5];
     __shared__ real s_YW[(NAB * NF1 + NABF * N1) * 5];
     __shared__ real s_J[5 * NFP * 5];
-    __shared__ real s_R[NS * 5];
+    __shared__ real s_R[NSV];
     __shared__ real s_med[10];
 
     const int t = threadIdx.x;
@@ -49,6 +145,8 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
         s_g[t] = a.gL[t];
         s_g[N1 + t] = a.gR[t];
     }
+    if (t < 5) s_ref[t] = a.qref[t];
+    if (t < 15) s_ref[5 + t] = a.fref[t];
     __syncthreads();
     const real* c2f_ab = s_tab + K::O_C2FAB;
     const real* c2f_c = s_tab + K::O_C2FC;
@@ -62,11 +160,47 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
     real* s_Y = s_YW;
     real* s_W = s_YW + NAB * NF1 * 5;
 
+    // interface lane (slot, flux point) bookkeeping
+    const bool iface = t < 5 * NFP;
+    const int slot = iface ? t / NFP : 0, fp = iface ? t % NFP : 0;
+    const uint32_t* dQw = reinterpret_cast<const uint32_t*>(a.dQ);
+    const uint32_t* rww = reinterpret_cast<const uint32_t*>(a.rows_w);
+
     for (int64_t cell = blockIdx.x; cell < a.n_cells; cell += gridDim.x) {
-        // 1. delta primitive state of the cell
-        for (int i = t; i < NS * 5; i += 64) s_Q[i] = a.dQ[cell * NS * 5 + i];
+        const int64_t sidx = cell * 5 + slot;
+        // 1. delta primitive state of the cell, its source cells
+        for (int i = t; i < NSV; i += 64) s_Q[i] = a.dQ[cell * NSV + i];
+        if (t < NNB) s_nb[t] = a.slot_nb[cell * NNB + t];
         if (t < 10) s_med[t] = real(0);
         __syncthreads();
+        // source-cell states and interpolation rows go to LDS asynchronously (global_load_lds),
+        // awaited before the interface stage
+        {
+#pragma unroll
+            for (int j = 0; j < NNB; ++j) {
+                const int nbc = s_nb[j];
+                const int64_t base = nbc >= 0 ? (int64_t)nbc * NDW : 0;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) {
+                    const int w = ch * 64 + t;
+                    lds_dma_dword(dQw + base + (w < NDW ? w : NDW - 1), s_N + (j * NCH + ch) * 64);
+                }
+            }
+            const int bc0 = iface ? a.slot_bc[sidx] : BC_INTERIOR;
+            int64_t row = a.n_rows;  // the all-zero row
+            if (iface && bc0 == BC_INTERIOR) {
+                const int src = (a.slot_mask[sidx] >> fp) & 1;
+                const int r = a.slot_blk[sidx * 2 + src] * NFP + fp;
+                if (r >= 0 && r < a.n_rows && s_nb[slot * 2 + src] >= 0) row = r;
+            }
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                lds_dma_dword(a.rows_idx + row * a.knz + k, s_ridx + k * 64);
+#pragma unroll
+                for (int h = 0; h < WPR; ++h)
+                    lds_dma_dword(rww + (row * a.knz + k) * WPR + h, s_rw + (k * WPR + h) * 64);
+            }
+        }
         // 2. coarse -> fine along c
         for (int o = t; o < NAB * NF1 * 5; o += 64) {
             const int ab = o / (NF1 * 5), r = o % (NF1 * 5), cf = r / 5, v = r % 5;
@@ -88,13 +222,13 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
             }
             double q[5], Fd[3][5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) q[v] = a.qref[v] + (double)dq[v];
+            for (int v = 0; v < 5; ++v) q[v] = s_ref[v] + (double)dq[v];
             euler_flux(q, Fd);
             real F[3][5];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) F[i][v] = (real)(Fd[i][v] - a.fref[i * 5 + v]);
+                for (int v = 0; v < 5; ++v) F[i][v] = (real)(Fd[i][v] - s_ref[5 + i * 5 + v]);
             const real* ad = a.adjf + (cell * NF + t) * 9;
             real A[9];
 #pragma unroll
@@ -125,11 +259,11 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
             }
             s_YW[o] = acc;
         }
+        vm_wait_all();
         __syncthreads();
         // 5a. interface jumps: lane = (slot, flux point)
-        if (t < 5 * NFP) {
-            const int slot = t / NFP, fp = t % NFP;
-            const int64_t sidx = cell * 5 + slot;
+        if (iface) {
+            const int bc = a.slot_bc[sidx];
             const int code = slot < 3 ? slot : (slot == 3 ? 4 : 5);
             const double side = (code & 1) ? 1.0 : -1.0;
             real dqo[5] = {0, 0, 0, 0, 0};
@@ -151,27 +285,28 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
             }
             double qo[5], qn[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) qo[v] = a.qref[v] + (double)dqo[v];
+            for (int v = 0; v < 5; ++v) qo[v] = s_ref[v] + (double)dqo[v];
             double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
                             (double)a.slot_tn[sidx * 3 + 2]};
-            const int bc = a.slot_bc[sidx];
             if (bc == BC_INTERIOR) {
                 const int src = (a.slot_mask[sidx] >> fp) & 1;
-                const int nbc = a.slot_nb[sidx * 2 + src];
-                const int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
+                const real* qs = reinterpret_cast<const real*>(s_N + (slot * 2 + src) * NCH * 64);
+                const real* rwv = reinterpret_cast<const real*>(s_rw);
                 real dqn[5] = {0, 0, 0, 0, 0};
-                if (nbc >= 0 && row >= 0 && row < a.n_rows) {
-                    const int nnz = a.rows_nnz[row];
-                    const real* qsrc = a.dQ + (int64_t)nbc * NS * 5;
-                    for (int k = 0; k < nnz; ++k) {
-                        const int s = a.rows_idx[row * a.knz + k];
-                        const real w = a.rows_w[row * a.knz + k];
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) dqn[v] += w * qsrc[s * 5 + v];
+                for (int k = 0; k < KMAX; ++k) {
+                    const int sp = s_ridx[k * 64 + t];
+                    real w;
+                    if constexpr (WPR == 1) {
+                        w = rwv[k * 64 + t];
+                    } else {
+                        w = __hiloint2double((int)s_rw[(k * WPR + 1) * 64 + t], (int)s_rw[(k * WPR) * 64 + t]);
                     }
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) dqn[v] += w * qs[sp * 5 + v];
                 }
 #pragma unroll
-                for (int v = 0; v < 5; ++v) qn[v] = a.qref[v] + (double)dqn[v];
+                for (int v = 0; v < 5; ++v) qn[v] = s_ref[v] + (double)dqn[v];
             } else {
                 ghost_state(bc, qo, tn, a.bcp, qn);
             }
@@ -195,12 +330,12 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
                 s_J[(slot * NFP + fp) * 5 + v] = (real)(fc[v] * sc - (A0 * F[0][v] + A1 * F[1][v] + A2 * F[2][v]));
         }
         // 5b. volume divergence per (solution point, variable)
-        real rv[(NS * 5 + 63) / 64];
+        real rv[NR];
 #pragma unroll
-        for (int k = 0; k < (NS * 5 + 63) / 64; ++k) {
+        for (int k = 0; k < NR; ++k) {
             const int o = t + 64 * k;
             rv[k] = 0;
-            if (o < NS * 5) {
+            if (o < NSV) {
                 const int s = o / 5, v = o % 5, ab = s / N1, c = s % N1;
                 real acc = 0;
 #pragma unroll
@@ -208,24 +343,24 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
 #pragma unroll 4
                 for (int abf = 0; abf < NABF; ++abf) acc += f2c_ab[ab * NABF + abf] * s_W[(abf * N1 + c) * 5 + v];
                 const real* gc = a.gcl + (cell * NS + s) * 3;
-                acc += gc[0] * (real)a.fref[v] + gc[1] * (real)a.fref[5 + v] + gc[2] * (real)a.fref[10 + v];
+                acc += gc[0] * (real)s_ref[5 + v] + gc[1] * (real)s_ref[10 + v] + gc[2] * (real)s_ref[15 + v];
                 rv[k] = acc;
             }
         }
         __syncthreads();
         // 6. interface correction, residual
 #pragma unroll
-        for (int k = 0; k < (NS * 5 + 63) / 64; ++k) {
+        for (int k = 0; k < NR; ++k) {
             const int o = t + 64 * k;
-            if (o < NS * 5) {
+            if (o < NSV) {
                 const int s = o / 5, v = o % 5;
                 real corr = 0;
 #pragma unroll
-                for (int slot = 0; slot < 5; ++slot) {
-                    const int code = slot < 3 ? slot : (slot == 3 ? 4 : 5);
+                for (int sl = 0; sl < 5; ++sl) {
+                    const int code = sl < 3 ? sl : (sl == 3 ? 4 : 5);
                     const int axis = code >> 1;
                     const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
-                    corr += g * s_J[(slot * NFP + s_fp[axis * NS + s]) * 5 + v];
+                    corr += g * s_J[(sl * NFP + s_fp[axis * NS + s]) * 5 + v];
                 }
                 const real dj = a.det[cell * NS + s];
                 rv[k] = -rv[k] / dj - corr / dj;
@@ -233,44 +368,12 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
             }
         }
         __syncthreads();
-        if (a.suppress) {
-            // 7. median of |R| over solution points per variable (parallel rank selection)
-            constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
+        // 7. outlier suppression
+        if (a.suppress) outlier_zeroing<NS, real>(s_R, s_med, rv, a.umean + cell * 5, t);
 #pragma unroll
-            for (int k = 0; k < (NS * 5 + 63) / 64; ++k) {
-                const int o = t + 64 * k;
-                if (o < NS * 5) {
-                    const int v = o % 5;
-                    const real x = rv[k] < real(0) ? -rv[k] : rv[k];
-                    int lt = 0, le = 0;
-                    for (int s2 = 0; s2 < NS; ++s2) {
-                        real y = s_R[s2 * 5 + v];
-                        y = y < real(0) ? -y : y;
-                        lt += y < x;
-                        le += y <= x;
-                    }
-                    if (lt <= k1 && k1 < le) s_med[v * 2 + 0] = x;
-                    if (lt <= k2 && k2 < le) s_med[v * 2 + 1] = x;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < (NS * 5 + 63) / 64; ++k) {
-                const int o = t + 64 * k;
-                if (o < NS * 5) {
-                    const int v = o % 5;
-                    const real med = (NS % 2) ? s_med[v * 2] : real(0.5) * (s_med[v * 2] + s_med[v * 2 + 1]);
-                    real ref = med > real(1e-9) * a.umean[cell * 5 + v] ? med : real(1e-9) * a.umean[cell * 5 + v];
-                    ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
-                    const real x = rv[k] < real(0) ? -rv[k] : rv[k];
-                    if (x > real(1e4) * ref) rv[k] = real(0);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < (NS * 5 + 63) / 64; ++k) {
+        for (int k = 0; k < NR; ++k) {
             const int o = t + 64 * k;
-            if (o < NS * 5) a.R[cell * NS * 5 + o] = rv[k];
+            if (o < NSV) a.R[cell * NSV + o] = rv[k];
         }
         __syncthreads();
     }
@@ -315,7 +418,15 @@ __global__ void __launch_bounds__(256) k_prim_umean(const real* __restrict__ U, 
 template <int P, typename real>
 int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
     int64_t g = a.n_cells < 65536 ? a.n_cells : 65536;
-    hipLaunchKernelGGL((k_prism_inviscid<P, real>), dim3((unsigned)(g > 0 ? g : 1)), dim3(64), 0, st, a);
+    const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64);
+    // interpolation rows: Kronecker-sparse (prism sources: P2 <= 9, P1 <= 4 non-zeros) or dense (tet sources)
+    constexpr int KS = (P + 1) * (P + 1), NS = PK<P>::NS;
+    if (a.knz <= KS)
+        hipLaunchKernelGGL((k_prism_inviscid<P, KS, real>), grid, block, 0, st, a);
+    else if (a.knz <= NS)
+        hipLaunchKernelGGL((k_prism_inviscid<P, NS, real>), grid, block, 0, st, a);
+    else
+        return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -71,7 +71,8 @@ class DeviceMesh:
         self.slot_mask, self.slot_bc = devi(sl.mask), devi(sl.bc)
         self.slot_tn = dev(sl.tn)
         self.rows = dev(sl.rows)
-        nnz, idx, w = sparse_rows(sl.rows)
+        # sparse rows plus one trailing all-zero row (index n_rows) for lanes without a source
+        nnz, idx, w = sparse_rows(np.concatenate([sl.rows, np.zeros((1, sl.rows.shape[1]))]))
         self.rows_nnz, self.rows_idx, self.rows_w = devi(nnz), devi(idx), dev(w)
         self.rows_knz = int(idx.shape[1])
         self.prism_tab = dev(prism_factors(order).packed())
