@@ -21,6 +21,7 @@ class AfrMeshDesc(ctypes.Structure):
         ("pad_", ctypes.c_int32),
         ("c2f", c_vp * 2), ("op", c_vp * 2), ("E", c_vp * 2), ("D", c_vp * 2), ("filt", c_vp * 2),
         ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp), ("prism_tab", c_vp),
+        ("prism_schedule", c_vp),
         ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
         ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
     ]

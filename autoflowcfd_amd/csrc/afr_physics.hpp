@@ -23,6 +23,28 @@ enum BcCode { BC_INTERIOR = -1, BC_WALL = 0, BC_FARFIELD = 1, BC_INLET = 2, BC_O
 
 template <typename real>
 __host__ __device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
+
+// 1/x for positive normal x.  Device: hardware reciprocal estimate refined by
+// Newton steps (two for FP64, one for FP32), within a few ulp of the correctly
+// rounded quotient; host: IEEE division.
+__host__ __device__ __forceinline__ double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+#else
+    return 1.0 / x;
+#endif
+}
+__host__ __device__ __forceinline__ float recip(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+#else
+    return 1.0f / x;
+#endif
+}
 template <typename real>
 __host__ __device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
 
@@ -41,10 +63,10 @@ template <typename real>
 __host__ __device__ __forceinline__ void euler_flux(const real q[5], real F[3][5]) {
     const real g1 = real(kGamma - 1.0);
     real rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
-    real rs = rmax(rho, real(1e-10));
+    real irs = recip(rmax(rho, real(1e-10)));
     real ke = real(0.5) * (u * u + v * v + w * w);
-    real rhoE = rho * (p / (g1 * rs) + ke);
-    real H = (rhoE + p) / rs;
+    real rhoE = rho * (p * real(1.0 / (kGamma - 1.0)) * irs + ke);
+    real H = (rhoE + p) * irs;
     real rH = rho * H;
     real vel[3] = {u, v, w};
 #pragma unroll
@@ -58,6 +80,23 @@ __host__ __device__ __forceinline__ void euler_flux(const real q[5], real F[3][5
     }
 }
 
+// directional flux sum_i n_i F[i][v] of primitive state q (n need not be unit)
+template <typename real>
+__host__ __device__ __forceinline__ void euler_flux_dir(const real q[5], const real n[3], real F[5]) {
+    real rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    real irs = recip(rmax(rho, real(1e-10)));
+    real ke = real(0.5) * (u * u + v * v + w * w);
+    real rhoE = rho * (p * real(1.0 / (kGamma - 1.0)) * irs + ke);
+    real rH = rho * ((rhoE + p) * irs);
+    real un = u * n[0] + v * n[1] + w * n[2];
+    real mf = rho * un;
+    F[0] = mf;
+    F[1] = mf * u + p * n[0];
+    F[2] = mf * v + p * n[1];
+    F[3] = mf * w + p * n[2];
+    F[4] = rH * un;
+}
+
 // AUSM+up normal flux density for unit direction n
 template <typename real>
 __host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR[5], const real n[3], real out[5]) {
@@ -68,13 +107,15 @@ __host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR
     real uR = qR[1], vR = qR[2], wR = qR[3];
     real unL = uL * n[0] + vL * n[1] + wL * n[2];
     real unR = uR * n[0] + vR * n[1] + wR * n[2];
-    real aL = sqrt(rmax(g * pL / rhoL, real(1e-10)));
-    real aR = sqrt(rmax(g * pR / rhoR, real(1e-10)));
-    real ML = unL / rmax(aL, real(1e-10));
-    real MR = unR / rmax(aR, real(1e-10));
+    real irhoL = recip(rhoL), irhoR = recip(rhoR);
+    real aL = sqrt(rmax(g * pL * irhoL, real(1e-10)));
+    real aR = sqrt(rmax(g * pR * irhoR, real(1e-10)));
+    real ML = unL * recip(rmax(aL, real(1e-10)));
+    real MR = unR * recip(rmax(aR, real(1e-10)));
     real ah = real(0.5) * (aL + aR);
     real rh = real(0.5) * (rhoL + rhoR);
-    real Mbar2 = (unL * unL + unR * unR) / (real(2.0) * ah * ah);
+    real iah2 = recip(ah * ah);
+    real Mbar2 = (unL * unL + unR * unR) * real(0.5) * iah2;
     real M0sq = rmin(real(1.0), rmax(Mbar2, real(0.01)));
     real M0 = sqrt(M0sq);
     real fa = rmax(M0 * (real(2.0) - M0), real(1e-6));
@@ -105,7 +146,7 @@ __host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR
     }
     const real Kp = real(0.25), Ku = real(0.75);
     real Mh = Mp + Mm;
-    real Mpress = -(Kp / fa) * rmax(real(1.0) - Mbar2, real(0.0)) * (pR - pL) / (rh * ah * ah);
+    real Mpress = -Kp * rmax(real(1.0) - Mbar2, real(0.0)) * (pR - pL) * recip(fa * rh) * iah2;
     real mass = real(0.5) * (rhoL * aL + rhoR * aR) * (Mh + Mpress);
     real ph = Pp * pL + Pm * pR - Ku * Pp * Pm * (rhoL + rhoR) * fa * ah * (unR - unL);
     bool up = mass >= real(0);
@@ -114,8 +155,8 @@ __host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR
     out[2] = mass * (up ? vL : vR) + ph * n[1];
     out[3] = mass * (up ? wL : wR) + ph * n[2];
     const real gg = g / (g - real(1));
-    real hL = gg * pL / rhoL + real(0.5) * (uL * uL + vL * vL + wL * wL);
-    real hR = gg * pR / rhoR + real(0.5) * (uR * uR + vR * vR + wR * wR);
+    real hL = gg * pL * irhoL + real(0.5) * (uL * uL + vL * vL + wL * wL);
+    real hR = gg * pR * irhoR + real(0.5) * (uR * uR + vR * vR + wR * wR);
     out[4] = mass * (up ? hL : hR);
 }
 

@@ -34,17 +34,83 @@ __device__ __forceinline__ void lds_dma_dword(const void* src_lane, void* dst_wa
 
 __device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
 
-// Exact median-based outlier zeroing of the (NS x 5) residual block of one
-// cell held in s_R ([s][v]) and in rv (lane-strided copy), as in
-// troubled_cell.py:339-408: R[s][v] = 0 where |R| > 1e4 max(median_s |R[:,v]|,
-// 1e-9 mean_s |U[:,v]|, tiny).  A cheap certificate runs first: with
-// M = max_s |R[:,v]| and t = 1.0001e-4 M, if at most k1 = (NS-1)/2 values lie
-// below t then median >= t, so no value can exceed the threshold and the
-// variable is left as is.  Only when the certificate fails for some variable
-// is the median selected exactly (rank counting).
+// LDS ordering inside one wavefront: LDS operations of a wave execute in
+// program order, so a compiler-level barrier (no s_barrier) separates stages
+// whose lanes exchange data through LDS.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The lane index re-materialised per stage: an empty asm that the optimiser must treat
+// as redefining it, so per-stage index arithmetic is not hoisted out of the cell
+// loop (where it would pin dozens of VGPRs for the whole kernel).
+__device__ __forceinline__ int lane_fresh(int t) {
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+template <typename real> struct V4T;
+template <> struct V4T<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct V4T<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+template <typename real>
+__device__ __forceinline__ typename V4T<real>::type ld4(const real* p) {
+    return *reinterpret_cast<const typename V4T<real>::type*>(p);
+}
+template <typename real>
+__device__ __forceinline__ void st4(real* p, typename V4T<real>::type v) {
+    *reinterpret_cast<typename V4T<real>::type*>(p) = v;
+}
+
+constexpr int r4(int n) { return (n + 3) / 4 * 4; }
+constexpr int r64(int n) { return (n + 63) / 64 * 64; }
+
+// Padded LDS layout of the prism kernel (units: reals).
+template <int P, typename real> struct PL {
+    using K = PK<P>;
+    static constexpr int N1 = K::N1, NAB = K::NAB, NF1 = K::NF1, NABF = K::NABF, NF = K::NF, NS = K::NS;
+    static constexpr int NFP = K::NFP, NSV = NS * 5;
+    static constexpr int NABF4 = r4(NABF), NABP4 = r4(NAB), NG = (NAB + 2) / 3, NX = r4(NAB * 5);
+    static constexpr int WPR = sizeof(real) / 4;
+    // block-shared operator tables
+    static constexpr int T_C2FAB = 0;                          // [NABF][NABP4]   c2f_ab rows
+    static constexpr int T_A = T_C2FAB + NABF * NABP4;         // [2][NG][NABF4][4] A_m[3g+r][abf]
+    static constexpr int T_F2CAB = T_A + 2 * NG * NABF4 * 4;   // [NG][NABF4][4]  f2c_ab[3g+r][abf]
+    static constexpr int T_BC = T_F2CAB + NG * NABF4 * 4;      // [N1][4]         B_c[c][cf]
+    static constexpr int T_F2CC = T_BC + N1 * 4;               // [N1][4]         f2c_c[c][cf]
+    static constexpr int T_C2FC = T_F2CC + N1 * 4;             // [NF1][N1]
+    static constexpr int T_EAB = T_C2FC + NF1 * N1;            // [3][N1][NAB]
+    static constexpr int T_EC = T_EAB + 3 * N1 * NAB;          // [2][N1]
+    static constexpr int T_G = T_EC + 2 * N1;                  // [2][N1] gL, gR
+    static constexpr int T_N = r4(T_G + 2 * N1);
+    // per-wave buffers
+    static constexpr int QN_DW = r64(6 * NSV * WPR);           // own cell + 5 slot sources, raw dwords
+    static constexpr int W_QN = 0;
+    static constexpr int W_XY = W_QN + QN_DW / WPR;            // X [NF1][NX] | Y [NAB][5][4]
+    static constexpr int XY_N = r4(NF1 * NX > NAB * 20 ? NF1 * NX : NAB * 20);
+    static constexpr int W_FT = W_XY + XY_N;                   // Ft [3][NF1][5][NABF4] | J, R
+    static constexpr int FT_N = 3 * NF1 * 5 * NABF4;
+    static constexpr int W_J = W_FT;                           // J [5][NFP][5]
+    static constexpr int W_R = W_FT + r4(5 * NFP * 5);         // R [NS][5]
+    static constexpr int W_W = W_FT + FT_N;                    // W [N1][5][NABF4]
+    static constexpr int W_N = W_W + N1 * 5 * NABF4;
+    static_assert(r4(5 * NFP * 5) + NSV <= FT_N, "J/R alias does not fit");
+};
+
+constexpr int kWaves = 4;  // wavefronts per workgroup
+
+// Exact median-based outlier zeroing of the (NS x 5) residual block held in
+// sR ([s][v]), as in troubled_cell.py:339-408: R[s][v] = 0 where
+// |R| > 1e4 max(median_s |R[:,v]|, 1e-9 mean_s |U[:,v]|, tiny).  A cheap
+// certificate runs first: with M = max_s |R[:,v]| and t = 1.0001e-4 M, if at
+// most k1 = (NS-1)/2 values lie below t then median >= t, so no value can
+// exceed the threshold and the variable is left as is.  Only when the
+// certificate fails for some variable is the median selected exactly.
 template <int NS, typename real>
-__device__ __forceinline__ void outlier_zeroing(const real* s_R, real* s_med, real* rv, const real* umean, int t) {
-    constexpr int NR = (NS * 5 + 63) / 64;
+__device__ __forceinline__ void outlier_zeroing(real* sR, const real* umean, int t) {
+    constexpr int NSV = NS * 5, NR = (NSV + 63) / 64;
     constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
     constexpr int G = 8, J = (NS + G - 1) / G;  // 8 lanes per variable, J values each
     const int v = t >> 3, g = t & 7;
@@ -54,7 +120,7 @@ __device__ __forceinline__ void outlier_zeroing(const real* s_R, real* s_med, re
         for (int j = 0; j < J; ++j) {
             const int s = g + G * j;
             if (s < NS) {
-                const real x = fabs(s_R[s * 5 + v]);
+                const real x = fabs(sR[s * 5 + v]);
                 mx = x > mx ? x : mx;
             }
         }
@@ -70,199 +136,276 @@ __device__ __forceinline__ void outlier_zeroing(const real* s_R, real* s_med, re
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const int s = g + G * j;
-            if (s < NS) cnt += fabs(s_R[s * 5 + v]) < tt;
+            if (s < NS) cnt += fabs(sR[s * 5 + v]) < tt;
         }
     }
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) cnt += __shfl_xor(cnt, m);
     const bool need = (t < 5 * G) && cnt > k1;
     if (!__any(need)) return;
+    // exact selection: rank counting, then zeroing
+    real x[NR], m1[NR], m2[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int o = t + 64 * k;
-        if (o < NS * 5) {
+        x[k] = real(0);
+        m1[k] = m2[k] = real(0);
+        if (o < NSV) {
             const int vv = o % 5;
-            const real x = fabs(rv[k]);
+            x[k] = fabs(sR[o]);
             int lt = 0, le = 0;
             for (int s2 = 0; s2 < NS; ++s2) {
-                const real y = fabs(s_R[s2 * 5 + vv]);
-                lt += y < x;
-                le += y <= x;
+                const real y = fabs(sR[s2 * 5 + vv]);
+                lt += y < x[k];
+                le += y <= x[k];
             }
-            if (lt <= k1 && k1 < le) s_med[vv * 2 + 0] = x;
-            if (lt <= k2 && k2 < le) s_med[vv * 2 + 1] = x;
+            m1[k] = (lt <= k1 && k1 < le) ? real(1) : real(0);
+            m2[k] = (lt <= k2 && k2 < le) ? real(1) : real(0);
         }
     }
-    __syncthreads();
+    // the median value of variable vv is held by the lanes flagged above: gather it by
+    // a max over flagged values (all flagged values of one rank are equal)
+    real med1[5], med2[5];
+#pragma unroll
+    for (int vv = 0; vv < 5; ++vv) {
+        real a1 = real(-1), a2 = real(-1);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int o = t + 64 * k;
+            if (o < NSV && o % 5 == vv) {
+                if (m1[k] > real(0)) a1 = x[k] > a1 ? x[k] : a1;
+                if (m2[k] > real(0)) a2 = x[k] > a2 ? x[k] : a2;
+            }
+        }
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const real o1 = __shfl_xor(a1, m), o2 = __shfl_xor(a2, m);
+            a1 = o1 > a1 ? o1 : a1;
+            a2 = o2 > a2 ? o2 : a2;
+        }
+        med1[vv] = a1;
+        med2[vv] = a2;
+    }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int o = t + 64 * k;
-        if (o < NS * 5) {
+        if (o < NSV) {
             const int vv = o % 5;
-            const real med = (NS % 2) ? s_med[vv * 2] : real(0.5) * (s_med[vv * 2] + s_med[vv * 2 + 1]);
+            real m1v = med1[0], m2v = med2[0];
+#pragma unroll
+            for (int q = 1; q < 5; ++q)
+                if (vv == q) { m1v = med1[q]; m2v = med2[q]; }
+            const real med = (NS % 2) ? m1v : real(0.5) * (m1v + m2v);
             const real um = real(1e-9) * umean[vv];
             real ref = med > um ? med : um;
             ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
-            if (fabs(rv[k]) > real(1e4) * ref) rv[k] = real(0);
+            if (x[k] > real(1e4) * ref) sR[o] = real(0);
         }
     }
 }
 
 template <int P, int KMAX, typename real>
-__global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
-    using K = PK<P>;
-    constexpr int N1 = K::N1, NAB = K::NAB, NF1 = K::NF1, NABF = K::NABF, NF = K::NF, NS = K::NS, NFP = K::NFP;
-    constexpr int NSV = NS * 5;
-    constexpr int NR = (NSV + 63) / 64;
-    constexpr int NNB = 10;                             // source cells: 5 slots x 2 sources
-    constexpr int WPR = sizeof(real) / 4;               // dwords per real
-    constexpr int NDW = NSV * WPR;                      // dwords of one cell's state
-    constexpr int NCH = (NDW + 63) / 64;                // 64-dword chunks per cell
-    __shared__ double s_ref[20];                        // qref (5), F(qref) (15)
-    __shared__ real s_tab[K::TABN];
-    __shared__ real s_g[2 * N1];
-    __shared__ int s_fp[3 * NS];
-    __shared__ int s_co[3 * NS];
-    __shared__ int s_nb[NNB];
-    __shared__ real s_Q[NSV];
-    __shared__ __attribute__((aligned(16))) uint32_t s_N[NNB * NCH * 64];  // source-cell states (raw dwords)
-    __shared__ int s_ridx[KMAX * 64];                  // interpolation row of each interface lane
-    __shared__ uint32_t s_rw[KMAX * 64 * WPR];
-    __shared__ real s_X[NAB * NF1 * 5];
-    __shared__ real s_Ft[3 * NF * 5];
-    __shared__ real s_YW[(NAB * NF1 + NABF * N1) * 5];
-    __shared__ real s_J[5 * NFP * 5];
-    __shared__ real s_R[NSV];
-    __shared__ real s_med[10];
+__global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> a) {
+    using L = PL<P, real>;
+    using v4 = typename V4T<real>::type;
+    constexpr int N1 = L::N1, NAB = L::NAB, NF1 = L::NF1, NABF = L::NABF, NF = L::NF, NS = L::NS;
+    constexpr int NFP = L::NFP, NSV = L::NSV, NABF4 = L::NABF4, NABP4 = L::NABP4, NG = L::NG, NX = L::NX;
+    constexpr int WPR = L::WPR;
+    constexpr int CELL_DW = NSV * WPR;
+    __shared__ __attribute__((aligned(16))) real s_tab[L::T_N];
+    __shared__ __attribute__((aligned(16))) real s_wave[kWaves][L::W_N];
+    __shared__ double s_ref[20];  // qref (5), F(qref) (15)
 
-    const int t = threadIdx.x;
-    for (int i = t; i < K::TABN; i += 64) s_tab[i] = a.tab[i];
-    for (int i = t; i < 3 * NS; i += 64) {
-        s_fp[i] = a.fp_of_sp[i];
-        s_co[i] = a.coord_of_sp[i];
+    const int tid = threadIdx.x, wid = tid >> 6;
+    int t = tid & 63;
+    // operator tables in padded layouts (sources: packed factors, PK offsets)
+    {
+        using K = PK<P>;
+        const real* tb = a.tab;
+        for (int i = tid; i < L::T_N; i += 64 * kWaves) s_tab[i] = real(0);
+        __syncthreads();
+        for (int i = tid; i < NABF * NAB; i += 64 * kWaves) {
+            const int abf = i / NAB, ab = i % NAB;
+            s_tab[L::T_C2FAB + abf * NABP4 + ab] = tb[K::O_C2FAB + i];
+        }
+        for (int i = tid; i < NAB * NABF; i += 64 * kWaves) {
+            const int ab = i / NABF, abf = i % NABF, g = ab / 3, r = ab % 3;
+            s_tab[L::T_A + ((0 * NG + g) * NABF4 + abf) * 4 + r] = tb[K::O_AA + i];
+            s_tab[L::T_A + ((1 * NG + g) * NABF4 + abf) * 4 + r] = tb[K::O_AB + i];
+            s_tab[L::T_F2CAB + (g * NABF4 + abf) * 4 + r] = tb[K::O_F2CAB + i];
+        }
+        for (int i = tid; i < N1 * NF1; i += 64 * kWaves) {
+            const int c = i / NF1, cf = i % NF1;
+            s_tab[L::T_BC + c * 4 + cf] = tb[K::O_BC + i];
+            s_tab[L::T_F2CC + c * 4 + cf] = tb[K::O_F2CC + i];
+            s_tab[L::T_C2FC + i] = tb[K::O_C2FC + i];  // [NF1][N1] has N1*NF1 entries too
+        }
+        for (int i = tid; i < 3 * N1 * NAB; i += 64 * kWaves) s_tab[L::T_EAB + i] = tb[K::O_EAB + i];
+        for (int i = tid; i < 2 * N1; i += 64 * kWaves) s_tab[L::T_EC + i] = tb[K::O_EC + i];
+        if (tid < N1) {
+            s_tab[L::T_G + tid] = a.gL[tid];
+            s_tab[L::T_G + N1 + tid] = a.gR[tid];
+        }
+        if (tid < 5) s_ref[tid] = a.qref[tid];
+        if (tid < 15) s_ref[5 + tid] = a.fref[tid];
+        __syncthreads();
     }
-    if (t < N1) {
-        s_g[t] = a.gL[t];
-        s_g[N1 + t] = a.gR[t];
-    }
-    if (t < 5) s_ref[t] = a.qref[t];
-    if (t < 15) s_ref[5 + t] = a.fref[t];
-    __syncthreads();
-    const real* c2f_ab = s_tab + K::O_C2FAB;
-    const real* c2f_c = s_tab + K::O_C2FC;
-    const real* A_a = s_tab + K::O_AA;
-    const real* A_b = s_tab + K::O_AB;
-    const real* B_c = s_tab + K::O_BC;
-    const real* f2c_c = s_tab + K::O_F2CC;
-    const real* f2c_ab = s_tab + K::O_F2CAB;
-    const real* E_ab = s_tab + K::O_EAB;
-    const real* e_c = s_tab + K::O_EC;
-    real* s_Y = s_YW;
-    real* s_W = s_YW + NAB * NF1 * 5;
-
-    // interface lane (slot, flux point) bookkeeping
-    const bool iface = t < 5 * NFP;
-    const int slot = iface ? t / NFP : 0, fp = iface ? t % NFP : 0;
+    real* const W = s_wave[wid];
+    uint32_t* const sQNw = reinterpret_cast<uint32_t*>(W + L::W_QN);
+    const real* const sQ = W + L::W_QN;  // own cell [NS][5]; source j at sQ + (1 + j) * NSV
+    real* const sX = W + L::W_XY;
+    real* const sY = W + L::W_XY;
+    real* const sFt = W + L::W_FT;
+    real* const sJ = W + L::W_J;
+    real* const sR = W + L::W_R;
+    real* const sW = W + L::W_W;
     const uint32_t* dQw = reinterpret_cast<const uint32_t*>(a.dQ);
-    const uint32_t* rww = reinterpret_cast<const uint32_t*>(a.rows_w);
 
-    for (int64_t cell = blockIdx.x; cell < a.n_cells; cell += gridDim.x) {
-        const int64_t sidx = cell * 5 + slot;
-        // 1. delta primitive state of the cell, its source cells
-        for (int i = t; i < NSV; i += 64) s_Q[i] = a.dQ[cell * NSV + i];
-        if (t < NNB) s_nb[t] = a.slot_nb[cell * NNB + t];
-        if (t < 10) s_med[t] = real(0);
-        __syncthreads();
-        // source-cell states and interpolation rows go to LDS asynchronously (global_load_lds),
-        // awaited before the interface stage
-        {
+    // XCD-aware schedule: workgroups are dispatched round-robin over the 8 XCDs; each
+    // XCD walks one contiguous range of the locality-ordered cell list so that the
+    // source cells of neighbouring cells are shared in its L2.
+    const int64_t n = a.n_cells;
+    const int nxcd = gridDim.x >= 8 ? 8 : 1;
+    const int xcd = blockIdx.x % nxcd, lb = blockIdx.x / nxcd;
+    const int64_t waves_x = (int64_t)(gridDim.x / nxcd) * kWaves;
+    const int64_t lo = n * xcd / nxcd, hi = n * (xcd + 1) / nxcd;
+
+    for (int64_t it = lo + (int64_t)lb * kWaves + wid; it < hi; it += waves_x) {
+        const int64_t cell = a.schedule[it];
+        t = lane_fresh(t);
+        // ---- 1. own state and slot source states -> LDS (asynchronous copies)
+        int nbv = -1;
+        if (t < 10) nbv = a.slot_nb[cell * 10 + t];
+        int64_t base[6];
+        base[0] = cell * CELL_DW;
 #pragma unroll
-            for (int j = 0; j < NNB; ++j) {
-                const int nbc = s_nb[j];
-                const int64_t base = nbc >= 0 ? (int64_t)nbc * NDW : 0;
-#pragma unroll
-                for (int ch = 0; ch < NCH; ++ch) {
-                    const int w = ch * 64 + t;
-                    lds_dma_dword(dQw + base + (w < NDW ? w : NDW - 1), s_N + (j * NCH + ch) * 64);
-                }
-            }
-            const int bc0 = iface ? a.slot_bc[sidx] : BC_INTERIOR;
-            int64_t row = a.n_rows;  // the all-zero row
-            if (iface && bc0 == BC_INTERIOR) {
-                const int src = (a.slot_mask[sidx] >> fp) & 1;
-                const int r = a.slot_blk[sidx * 2 + src] * NFP + fp;
-                if (r >= 0 && r < a.n_rows && s_nb[slot * 2 + src] >= 0) row = r;
-            }
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                lds_dma_dword(a.rows_idx + row * a.knz + k, s_ridx + k * 64);
-#pragma unroll
-                for (int h = 0; h < WPR; ++h)
-                    lds_dma_dword(rww + (row * a.knz + k) * WPR + h, s_rw + (k * WPR + h) * 64);
-            }
+        for (int j = 0; j < 5; ++j) {
+            const int nb = __builtin_amdgcn_readlane(nbv, 2 * j);
+            base[1 + j] = nb >= 0 ? (int64_t)nb * CELL_DW : cell * CELL_DW;
         }
-        // 2. coarse -> fine along c
-        for (int o = t; o < NAB * NF1 * 5; o += 64) {
-            const int ab = o / (NF1 * 5), r = o % (NF1 * 5), cf = r / 5, v = r % 5;
-            real acc = 0;
 #pragma unroll
-            for (int c = 0; c < N1; ++c) acc += c2f_c[cf * N1 + c] * s_Q[(ab * N1 + c) * 5 + v];
-            s_X[o] = acc;
+        for (int ch = 0; ch < L::QN_DW / 64; ++ch) {
+            const int w = ch * 64 + t;
+            const int j = w / CELL_DW;
+            int off = w - j * CELL_DW;
+            int64_t b = base[0];
+#pragma unroll
+            for (int q = 1; q < 6; ++q)
+                if (j == q) b = base[q];
+            if (j >= 6) off = 0;
+            lds_dma_dword(dQw + b + off, sQNw + ch * 64);
         }
-        __syncthreads();
-        // 3. fine points: along (a,b), FP64 point flux about the reference, contravariant flux
+        vm_wait_all();
+        wave_sync();
+        // ---- 2. coarse -> fine along c: X[cf][ab*5+v]
+        t = lane_fresh(t);
+        if (t < NAB * 5) {
+            const int ab = t / 5, v = t % 5;
+            real q[N1];
+#pragma unroll
+            for (int c = 0; c < N1; ++c) q[c] = sQ[(ab * N1 + c) * 5 + v];
+#pragma unroll
+            for (int cf = 0; cf < NF1; ++cf) {
+                real acc = 0;
+#pragma unroll
+                for (int c = 0; c < N1; ++c) acc += s_tab[L::T_C2FC + cf * N1 + c] * q[c];
+                sX[cf * NX + t] = acc;
+            }
+        } else if (t < NX) {
+#pragma unroll
+            for (int cf = 0; cf < NF1; ++cf) sX[cf * NX + t] = real(0);
+        }
+        wave_sync();
+        // ---- 3. fine points (abf, cf): along (a,b), FP64 point flux, contravariant flux
+        t = lane_fresh(t);
         if (t < NF) {
             const int abf = t / NF1, cf = t % NF1;
             real dq[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-            for (int ab = 0; ab < NAB; ++ab) {
-                const real w = c2f_ab[abf * NAB + ab];
+            for (int qa = 0; qa < NABP4 / 4; ++qa) {
+                const v4 w = ld4(s_tab + L::T_C2FAB + abf * NABP4 + 4 * qa);
 #pragma unroll
-                for (int v = 0; v < 5; ++v) dq[v] += w * s_X[(ab * NF1 + cf) * 5 + v];
+                for (int e = 0; e < 4; ++e) {
+                    const int ab = 4 * qa + e;
+                    if (ab < NAB) {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) dq[v] += w[e] * sX[cf * NX + ab * 5 + v];
+                    }
+                }
             }
-            double q[5], Fd[3][5];
+            double q[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) q[v] = s_ref[v] + (double)dq[v];
-            euler_flux(q, Fd);
-            real F[3][5];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int v = 0; v < 5; ++v) F[i][v] = (real)(Fd[i][v] - s_ref[5 + i * 5 + v]);
             const real* ad = a.adjf + (cell * NF + t) * 9;
-            real A[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) A[k] = ad[k];
+            for (int m = 0; m < 3; ++m) {
+                // F~_m = adj_m . (F(q) - F(q_ref)), the difference taken in FP64
+                const double am[3] = {(double)ad[m * 3 + 0], (double)ad[m * 3 + 1], (double)ad[m * 3 + 2]};
+                double Fq[5];
+                euler_flux_dir(q, am, Fq);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    const double f0 = am[0] * s_ref[5 + v] + am[1] * s_ref[10 + v] + am[2] * s_ref[15 + v];
+                    sFt[((m * NF1 + cf) * 5 + v) * NABF4 + abf] = (real)(Fq[v] - f0);
+                }
+            }
+        } else if (t < NF + (NABF4 - NABF) * NF1) {
+            const int abf = NABF + (t - NF) / NF1, cf = (t - NF) % NF1;
 #pragma unroll
             for (int m = 0; m < 3; ++m)
 #pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    s_Ft[((m * NABF + abf) * NF1 + cf) * 5 + v] =
-                        A[m * 3 + 0] * F[0][v] + A[m * 3 + 1] * F[1][v] + A[m * 3 + 2] * F[2][v];
+                for (int v = 0; v < 5; ++v) sFt[((m * NF1 + cf) * 5 + v) * NABF4 + abf] = real(0);
         }
-        __syncthreads();
-        // 4. divergence partial products
-        for (int o = t; o < (NAB * NF1 + NABF * N1) * 5; o += 64) {
-            real acc = 0;
-            if (o < NAB * NF1 * 5) {
-                const int ab = o / (NF1 * 5), r = o % (NF1 * 5), cf = r / 5, v = r % 5;
-#pragma unroll 4
-                for (int abf = 0; abf < NABF; ++abf) {
-                    acc += A_a[ab * NABF + abf] * s_Ft[((0 * NABF + abf) * NF1 + cf) * 5 + v];
-                    acc += A_b[ab * NABF + abf] * s_Ft[((1 * NABF + abf) * NF1 + cf) * 5 + v];
-                }
-            } else {
-                const int o2 = o - NAB * NF1 * 5;
-                const int abf = o2 / (N1 * 5), r = o2 % (N1 * 5), c = r / 5, v = r % 5;
+        wave_sync();
+        // ---- 4a. Y[ab][v][cf] = sum_abf A_a[ab][abf] Ft_a + A_b[ab][abf] Ft_b  (lanes: row group, cf, v)
+        t = lane_fresh(t);
+        if (t < NG * NF1 * 5) {
+            const int g = t / (NF1 * 5), r = t % (NF1 * 5), cf = r / 5, v = r % 5;
+            real acc[3] = {0, 0, 0};
 #pragma unroll
-                for (int cf = 0; cf < NF1; ++cf) acc += B_c[c * NF1 + cf] * s_Ft[((2 * NABF + abf) * NF1 + cf) * 5 + v];
+            for (int m = 0; m < 2; ++m) {
+#pragma unroll
+                for (int qa = 0; qa < NABF4 / 4; ++qa) {
+                    const v4 f = ld4(sFt + ((m * NF1 + cf) * 5 + v) * NABF4 + 4 * qa);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const v4 c = ld4(s_tab + L::T_A + ((m * NG + g) * NABF4 + 4 * qa + e) * 4);
+                        acc[0] += c[0] * f[e];
+                        acc[1] += c[1] * f[e];
+                        acc[2] += c[2] * f[e];
+                    }
+                }
             }
-            s_YW[o] = acc;
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+                const int ab = 3 * g + rr;
+                if (ab < NAB) {
+                    sY[(ab * 5 + v) * 4 + cf] = acc[rr];
+                    if (cf == NF1 - 1)
+                        for (int cp = NF1; cp < 4; ++cp) sY[(ab * 5 + v) * 4 + cp] = real(0);
+                }
+            }
         }
-        vm_wait_all();
-        __syncthreads();
-        // 5a. interface jumps: lane = (slot, flux point)
-        if (iface) {
+        // ---- 4b. W[c][v][abf] = sum_cf B_c[c][cf] Ft_c[cf][v][abf]  (lanes: v, abf quad, c)
+        t = lane_fresh(t);
+        constexpr int QB = NABF4 / 4;
+        if (t < 5 * QB * N1) {
+            const int v = t / (QB * N1), r = t % (QB * N1), qa = r / N1, c = r % N1;
+            const v4 bc = ld4(s_tab + L::T_BC + c * 4);
+            v4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int cf = 0; cf < NF1; ++cf) {
+                const v4 f = ld4(sFt + ((2 * NF1 + cf) * 5 + v) * NABF4 + 4 * qa);
+                acc += bc[cf] * f;
+            }
+            st4(sW + (c * 5 + v) * NABF4 + 4 * qa, acc);
+        }
+        wave_sync();
+        // ---- 5a. interface jumps: lane = (slot, flux point)
+        t = lane_fresh(t);
+        if (t < 5 * NFP) {
+            const int slot = t / NFP, fp = t % NFP;
+            const int64_t sidx = cell * 5 + slot;
             const int bc = a.slot_bc[sidx];
             const int code = slot < 3 ? slot : (slot == 3 ? 4 : 5);
             const double side = (code & 1) ? 1.0 : -1.0;
@@ -271,16 +414,16 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
                 const int x = fp / N1, ic = fp % N1;
 #pragma unroll
                 for (int ab = 0; ab < NAB; ++ab) {
-                    const real w = E_ab[(slot * N1 + x) * NAB + ab];
+                    const real w = s_tab[L::T_EAB + (slot * N1 + x) * NAB + ab];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) dqo[v] += w * s_Q[(ab * N1 + ic) * 5 + v];
+                    for (int v = 0; v < 5; ++v) dqo[v] += w * sQ[(ab * N1 + ic) * 5 + v];
                 }
             } else {
 #pragma unroll
                 for (int c = 0; c < N1; ++c) {
-                    const real w = e_c[(slot - 3) * N1 + c];
+                    const real w = s_tab[L::T_EC + (slot - 3) * N1 + c];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) dqo[v] += w * s_Q[(fp * N1 + c) * 5 + v];
+                    for (int v = 0; v < 5; ++v) dqo[v] += w * sQ[(fp * N1 + c) * 5 + v];
                 }
             }
             double qo[5], qn[5];
@@ -290,20 +433,33 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
                             (double)a.slot_tn[sidx * 3 + 2]};
             if (bc == BC_INTERIOR) {
                 const int src = (a.slot_mask[sidx] >> fp) & 1;
-                const real* qs = reinterpret_cast<const real*>(s_N + (slot * 2 + src) * NCH * 64);
-                const real* rwv = reinterpret_cast<const real*>(s_rw);
-                real dqn[5] = {0, 0, 0, 0, 0};
+                int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
+                const int nbc = a.slot_nb[sidx * 2 + src];
+                if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
+                int sp[KMAX];
+                real rw[KMAX];
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k) {
-                    const int sp = s_ridx[k * 64 + t];
-                    real w;
-                    if constexpr (WPR == 1) {
-                        w = rwv[k * 64 + t];
-                    } else {
-                        w = __hiloint2double((int)s_rw[(k * WPR + 1) * 64 + t], (int)s_rw[(k * WPR) * 64 + t]);
+                    sp[k] = 0;
+                    rw[k] = real(0);
+                    if (k < a.knz) {
+                        sp[k] = a.rows_idx[(int64_t)row * a.knz + k];
+                        rw[k] = a.rows_w[(int64_t)row * a.knz + k];
                     }
+                }
+                real dqn[5] = {0, 0, 0, 0, 0};
+                if (src == 0) {  // primary source: staged in LDS
+                    const real* qs = sQ + (1 + slot) * NSV;
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) dqn[v] += w * qs[sp * 5 + v];
+                    for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) dqn[v] += rw[k] * qs[sp[k] * 5 + v];
+                } else {  // second source of a split quad: global memory
+                    const real* qs = a.dQ + (int64_t)(nbc >= 0 ? nbc : 0) * NSV;
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) dqn[v] += rw[k] * qs[sp[k] * 5 + v];
                 }
 #pragma unroll
                 for (int v = 0; v < 5; ++v) qn[v] = s_ref[v] + (double)dqn[v];
@@ -318,64 +474,71 @@ __global__ void __launch_bounds__(64) k_prism_inviscid(PrismArgs<real> a) {
             if ((a.slot_fb[sidx] >> fp) & 1) {
                 d[0] = tn[0]; d[1] = tn[1]; d[2] = tn[2];
             } else {
-                const double im = side / mag;
+                const double im = side * recip(mag);
                 d[0] = A0 * im; d[1] = A1 * im; d[2] = A2 * im;
             }
-            double fc[5], F[3][5];
+            double fc[5], FA[5];
             ausm_up(qo, qn, d, fc);
-            euler_flux(qo, F);
+            const double Av[3] = {A0, A1, A2};
+            euler_flux_dir(qo, Av, FA);
             const double sc = mag * side;
 #pragma unroll
-            for (int v = 0; v < 5; ++v)
-                s_J[(slot * NFP + fp) * 5 + v] = (real)(fc[v] * sc - (A0 * F[0][v] + A1 * F[1][v] + A2 * F[2][v]));
+            for (int v = 0; v < 5; ++v) sJ[(slot * NFP + fp) * 5 + v] = (real)(fc[v] * sc - FA[v]);
         }
-        // 5b. volume divergence per (solution point, variable)
-        real rv[NR];
+        wave_sync();
+        // ---- 5b/6. volume divergence + constant-state term + interface correction, / det
+        t = lane_fresh(t);
+        //      lanes: (c, v, row group g), rows ab = 3g..3g+2
+        if (t < N1 * 5 * NG) {
+            const int c = t / (5 * NG), r = t % (5 * NG), v = r / NG, g = r % NG;
+            real acc[3] = {0, 0, 0};
+            const v4 fc4 = ld4(s_tab + L::T_F2CC + c * 4);
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const int o = t + 64 * k;
-            rv[k] = 0;
-            if (o < NSV) {
-                const int s = o / 5, v = o % 5, ab = s / N1, c = s % N1;
-                real acc = 0;
-#pragma unroll
-                for (int cf = 0; cf < NF1; ++cf) acc += f2c_c[c * NF1 + cf] * s_Y[(ab * NF1 + cf) * 5 + v];
-#pragma unroll 4
-                for (int abf = 0; abf < NABF; ++abf) acc += f2c_ab[ab * NABF + abf] * s_W[(abf * N1 + c) * 5 + v];
-                const real* gc = a.gcl + (cell * NS + s) * 3;
-                acc += gc[0] * (real)s_ref[5 + v] + gc[1] * (real)s_ref[10 + v] + gc[2] * (real)s_ref[15 + v];
-                rv[k] = acc;
-            }
-        }
-        __syncthreads();
-        // 6. interface correction, residual
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const int o = t + 64 * k;
-            if (o < NSV) {
-                const int s = o / 5, v = o % 5;
-                real corr = 0;
-#pragma unroll
-                for (int sl = 0; sl < 5; ++sl) {
-                    const int code = sl < 3 ? sl : (sl == 3 ? 4 : 5);
-                    const int axis = code >> 1;
-                    const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
-                    corr += g * s_J[(sl * NFP + s_fp[axis * NS + s]) * 5 + v];
+            for (int rr = 0; rr < 3; ++rr) {
+                const int ab = 3 * g + rr;
+                if (ab < NAB) {
+                    const v4 y = ld4(sY + (ab * 5 + v) * 4);
+                    acc[rr] = fc4[0] * y[0] + fc4[1] * y[1] + fc4[2] * y[2] + fc4[3] * y[3];
                 }
-                const real dj = a.det[cell * NS + s];
-                rv[k] = -rv[k] / dj - corr / dj;
-                s_R[o] = rv[k];
+            }
+#pragma unroll
+            for (int qa = 0; qa < NABF4 / 4; ++qa) {
+                const v4 w = ld4(sW + (c * 5 + v) * NABF4 + 4 * qa);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const v4 cf = ld4(s_tab + L::T_F2CAB + (g * NABF4 + 4 * qa + e) * 4);
+                    acc[0] += cf[0] * w[e];
+                    acc[1] += cf[1] * w[e];
+                    acc[2] += cf[2] * w[e];
+                }
+            }
+            const real fr0 = (real)s_ref[5 + v], fr1 = (real)s_ref[10 + v], fr2 = (real)s_ref[15 + v];
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+                const int ab = 3 * g + rr;
+                if (ab < NAB) {
+                    const int i = ab / N1, j = ab % N1, s = ab * N1 + c;
+                    const real* gc = a.gcl + (cell * NS + s) * 3;
+                    const real vol = acc[rr] + gc[0] * fr0 + gc[1] * fr1 + gc[2] * fr2;
+                    const real* gL = s_tab + L::T_G;
+                    const real* gR = s_tab + L::T_G + N1;
+                    const real corr = gL[i] * sJ[(0 * NFP + j * N1 + c) * 5 + v] +
+                                      gR[i] * sJ[(1 * NFP + j * N1 + c) * 5 + v] +
+                                      gL[j] * sJ[(2 * NFP + i * N1 + c) * 5 + v] +
+                                      gL[c] * sJ[(3 * NFP + i * N1 + j) * 5 + v] +
+                                      gR[c] * sJ[(4 * NFP + i * N1 + j) * 5 + v];
+                    const real dj = a.det[cell * NS + s];
+                    sR[s * 5 + v] = -vol / dj - corr / dj;
+                }
             }
         }
-        __syncthreads();
-        // 7. outlier suppression
-        if (a.suppress) outlier_zeroing<NS, real>(s_R, s_med, rv, a.umean + cell * 5, t);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const int o = t + 64 * k;
-            if (o < NSV) a.R[cell * NSV + o] = rv[k];
-        }
-        __syncthreads();
+        wave_sync();
+        // ---- 7. outlier suppression, coalesced write
+        t = lane_fresh(t);
+        if (a.suppress) outlier_zeroing<NS, real>(sR, a.umean + cell * 5, t);
+        wave_sync();
+        for (int o = t; o < NSV; o += 64) a.R[cell * NSV + o] = sR[o];
+        wave_sync();
     }
 }
 
@@ -417,8 +580,11 @@ __global__ void __launch_bounds__(256) k_prim_umean(const real* __restrict__ U, 
 
 template <int P, typename real>
 int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
-    int64_t g = a.n_cells < 65536 ? a.n_cells : 65536;
-    const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64);
+    // persistent grid: a multiple of the 8 XCDs, up to 4 workgroups per CU of 256 CUs
+    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
+    int64_t g = need < 1024 ? need : 1024;
+    g = g >= 8 ? g / 8 * 8 : g;
+    const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64 * kWaves);
     // interpolation rows: Kronecker-sparse (prism sources: P2 <= 9, P1 <= 4 non-zeros) or dense (tet sources)
     constexpr int KS = (P + 1) * (P + 1), NS = PK<P>::NS;
     if (a.knz <= KS)

@@ -36,6 +36,7 @@ template <int P> struct PK {
 template <typename real>
 struct PrismArgs {
     int64_t n_cells;  // prisms occupy cells [0, n_cells)
+    const int* schedule;  // (n_cells) processing order of the prisms (locality order)
     const real* dQ;
     const real* umean;
     real* R;

@@ -343,6 +343,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         // prisms: Kronecker-factored wave-per-cell kernel
         afr_prism::PrismArgs<real> p;
         p.n_cells = d.n_prism;
+        p.schedule = d.prism_schedule;
         p.dQ = Q;
         p.umean = umean;
         p.R = R;
@@ -447,7 +448,8 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
     if (!desc->det || !desc->adj_fine || !desc->slot_nb || !desc->slot_blk || !desc->slot_mask || !desc->slot_bc ||
         !desc->slot_fallback || !desc->slot_tn || !desc->slot_adjrow || !desc->rows || !desc->gL || !desc->gR ||
         !desc->fp_of_sp || !desc->coord_of_sp || !desc->bc_params || !desc->work_q || !desc->gcl ||
-        !desc->work_umean || !desc->prism_tab || !desc->rows_nnz || !desc->rows_idx || !desc->rows_w)
+        !desc->work_umean || !desc->prism_tab || !desc->rows_nnz || !desc->rows_idx || !desc->rows_w ||
+        (desc->n_prism > 0 && !desc->prism_schedule))
         return fail("afr_mesh_create: missing device array");
     AfrMesh* m = new AfrMesh;
     m->d = *desc;

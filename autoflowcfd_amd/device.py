@@ -44,6 +44,27 @@ def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
     return P
 
 
+def _spread3(x):
+    """Insert two zero bits between the low 21 bits of x (uint64)."""
+    x = x & np.uint64(0x1FFFFF)
+    x = (x | (x << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    x = (x | (x << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    x = (x | (x << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    x = (x | (x << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    x = (x | (x << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return x
+
+
+def morton_order(points):
+    """Permutation of the points along a 3-D Morton (Z-order) curve of their bounding box."""
+    if len(points) == 0:
+        return np.zeros(0, dtype=np.int64)
+    lo, hi = points.min(axis=0), points.max(axis=0)
+    q = ((points - lo) / np.maximum(hi - lo, 1e-300) * (2 ** 21 - 1)).astype(np.uint64)
+    key = _spread3(q[:, 0]) | (_spread3(q[:, 1]) << np.uint64(1)) | (_spread3(q[:, 2]) << np.uint64(2))
+    return np.argsort(key, kind="stable")
+
+
 class DeviceMesh:
     """One mesh at one order on one GPU."""
 
@@ -76,6 +97,9 @@ class DeviceMesh:
         self.rows_nnz, self.rows_idx, self.rows_w = devi(nnz), devi(idx), dev(w)
         self.rows_knz = int(idx.shape[1])
         self.prism_tab = dev(prism_factors(order).packed())
+        # prisms are processed in Morton order of their centroids (L2 reuse of source cells)
+        cen = mesh.nodes[mesh.prism_conn].mean(axis=1) if mesh.n_prism else np.zeros((0, 3))
+        self.prism_schedule = devi(morton_order(cen))
         self.E = []
         self.E64 = []
         for ct in (0, 1):
@@ -126,6 +150,7 @@ class DeviceMesh:
         d.rows_nnz, d.rows_idx, d.rows_w = p(self.rows_nnz), p(self.rows_idx), p(self.rows_w)
         d.rows_knz = self.rows_knz
         d.prism_tab = p(self.prism_tab)
+        d.prism_schedule = p(self.prism_schedule) if mesh.n_prism else None
         for ct in (0, 1):
             d.c2f[ct], d.op[ct], d.E[ct] = p(self.c2f[ct]), p(self.op[ct]), p(self.E[ct])
             d.D[ct], d.filt[ct] = p(self.D[ct]), p(self.filt[ct])

@@ -56,6 +56,8 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   c2f[t] (nf, ns), op[t] (ns, 3*nf), E[t] (S, nfp, ns), D[t] (3, ns, ns),
  *   filt[t] (ns, ns) for t = 0 prism, 1 tet
  *   gL, gR (order+1), fp_of_sp / coord_of_sp (3, ns) int32
+ *   prism_schedule (n_prism) int32    processing order of the prisms (a permutation of
+ *                                      [0, n_prism) in spatial locality order)
  *   prism_tab  packed prism factors (a,b)-triangle x c-line: c2f_ab, c2f_c,
  *              A_a = f2c_ab D_a, A_b = f2c_ab D_b, B_c = f2c_c D_c, f2c_c,
  *              f2c_ab, E_ab (3 quad slots), e_c (2 c slots), D_ab, D_c
@@ -87,6 +89,7 @@ typedef struct AfrMeshDesc {
     const void *gL, *gR;
     const int32_t *fp_of_sp, *coord_of_sp;
     const void *prism_tab;
+    const int32_t *prism_schedule;
     const void *gcl;
     const double *bc_params;
     double ref_state[5];
