@@ -101,6 +101,49 @@ template <int P, typename real> struct PL {
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
 
+// Copies the states of up to 6 cells (dword offsets base[0..5] into src, each
+// CELL_DW dwords) back to back into dst[0 .. NCH*64) with one global_load_lds
+// per 64-dword chunk; the chunk -> cell split is resolved at compile time.
+template <int CH, int NCH, int CELL_DW>
+struct DmaCells {
+    __device__ __forceinline__ static void run(const uint32_t* src, const int64_t (&base)[6], uint32_t* dst, int t) {
+        constexpr int w0 = CH * 64;
+        int64_t b;
+        int off;
+        if constexpr (CELL_DW >= 64) {
+            constexpr int j0 = w0 / CELL_DW, bl = (j0 + 1) * CELL_DW - w0;
+            if constexpr (bl >= 64) {
+                b = base[j0];
+                off = w0 - j0 * CELL_DW + t;
+            } else {
+                const bool second = t >= bl;
+                if constexpr (j0 + 1 < 6) {
+                    b = second ? base[j0 + 1] : base[j0];
+                    off = second ? t - bl : w0 - j0 * CELL_DW + t;
+                } else {
+                    b = base[j0];
+                    off = second ? 0 : w0 - j0 * CELL_DW + t;
+                }
+            }
+        } else {
+            const int w = w0 + t, j = w / CELL_DW;
+            off = w - j * CELL_DW;
+            b = base[0];
+#pragma unroll
+            for (int q = 1; q < 6; ++q)
+                if (j == q) b = base[q];
+            if (j >= 6) off = 0;
+        }
+        lds_dma_dword(src + b + off, dst + w0);
+        if constexpr (CH + 1 < NCH) DmaCells<CH + 1, NCH, CELL_DW>::run(src, base, dst, t);
+    }
+};
+
+template <int NCH, int CELL_DW>
+__device__ __forceinline__ void dma_cells(const uint32_t* src, const int64_t (&base)[6], uint32_t* dst, int t) {
+    DmaCells<0, NCH, CELL_DW>::run(src, base, dst, t);
+}
+
 // Exact median-based outlier zeroing of the (NS x 5) residual block held in
 // sR ([s][v]), as in troubled_cell.py:339-408: R[s][v] = 0 where
 // |R| > 1e4 max(median_s |R[:,v]|, 1e-9 mean_s |U[:,v]|, tiny).  A cheap
@@ -215,8 +258,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
     __shared__ __attribute__((aligned(16))) real s_tab[L::T_N];
     __shared__ __attribute__((aligned(16))) real s_wave[kWaves][L::W_N];
     __shared__ double s_ref[20];  // qref (5), F(qref) (15)
+    __shared__ real s_c[6];       // qref (5) and its total enthalpy H0 in the storage precision
 
-    const int tid = threadIdx.x, wid = tid >> 6;
+    const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
     // operator tables in padded layouts (sources: packed factors, PK offsets)
     {
@@ -248,6 +292,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
         }
         if (tid < 5) s_ref[tid] = a.qref[tid];
         if (tid < 15) s_ref[5 + tid] = a.fref[tid];
+        if (tid < 5) s_c[tid] = (real)a.qref[tid];
+        if (tid == 0) {
+            const double* q = a.qref;
+            s_c[5] = (real)(kGamma / (kGamma - 1.0) * q[4] / q[0] + 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]));
+        }
         __syncthreads();
     }
     real* const W = s_wave[wid];
@@ -271,7 +320,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
     const int64_t lo = n * xcd / nxcd, hi = n * (xcd + 1) / nxcd;
 
     for (int64_t it = lo + (int64_t)lb * kWaves + wid; it < hi; it += waves_x) {
-        const int64_t cell = a.schedule[it];
+        const int64_t cell = __builtin_amdgcn_readfirstlane(a.schedule[it]);
         t = lane_fresh(t);
         // ---- 1. own state and slot source states -> LDS (asynchronous copies)
         int nbv = -1;
@@ -283,18 +332,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
             const int nb = __builtin_amdgcn_readlane(nbv, 2 * j);
             base[1 + j] = nb >= 0 ? (int64_t)nb * CELL_DW : cell * CELL_DW;
         }
-#pragma unroll
-        for (int ch = 0; ch < L::QN_DW / 64; ++ch) {
-            const int w = ch * 64 + t;
-            const int j = w / CELL_DW;
-            int off = w - j * CELL_DW;
-            int64_t b = base[0];
-#pragma unroll
-            for (int q = 1; q < 6; ++q)
-                if (j == q) b = base[q];
-            if (j >= 6) off = 0;
-            lds_dma_dword(dQw + b + off, sQNw + ch * 64);
-        }
+        dma_cells<L::QN_DW / 64, CELL_DW>(dQw, base, sQNw, t);
         vm_wait_all();
         wave_sync();
         // ---- 2. coarse -> fine along c: X[cf][ab*5+v]
@@ -333,21 +371,29 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
                     }
                 }
             }
-            double q[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) q[v] = s_ref[v] + (double)dq[v];
+            // F~_m = adj_m . (F(q_ref + dq) - F(q_ref)) by the exact algebraic expansion of the
+            // Euler flux about q_ref: every term carries a perturbation factor, so the
+            // difference is formed without cancellation in the storage precision
             const real* ad = a.adjf + (cell * NF + t) * 9;
+            const real r0 = s_c[0], u0 = s_c[1], v0 = s_c[2], w0 = s_c[3], p0 = s_c[4], H0 = s_c[5];
+            const real dr = dq[0], du = dq[1], dv = dq[2], dw = dq[3], dp = dq[4];
+            // dH = g/(g-1) d(p/rho) + d(|u|^2/2)
+            const real dpr = (dp * r0 - p0 * dr) * recip(r0 * (r0 + dr));
+            const real dH = real(kGamma / (kGamma - 1.0)) * dpr + (u0 * du + v0 * dv + w0 * dw) +
+                            real(0.5) * (du * du + dv * dv + dw * dw);
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                // F~_m = adj_m . (F(q) - F(q_ref)), the difference taken in FP64
-                const double am[3] = {(double)ad[m * 3 + 0], (double)ad[m * 3 + 1], (double)ad[m * 3 + 2]};
-                double Fq[5];
-                euler_flux_dir(q, am, Fq);
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    const double f0 = am[0] * s_ref[5 + v] + am[1] * s_ref[10 + v] + am[2] * s_ref[15 + v];
-                    sFt[((m * NF1 + cf) * 5 + v) * NABF4 + abf] = (real)(Fq[v] - f0);
-                }
+                const real a0 = ad[m * 3 + 0], a1 = ad[m * 3 + 1], a2 = ad[m * 3 + 2];
+                const real un0 = u0 * a0 + v0 * a1 + w0 * a2;
+                const real dun = du * a0 + dv * a1 + dw * a2;
+                const real m0 = r0 * un0;
+                const real dm = r0 * dun + dr * (un0 + dun);
+                real* o = sFt + (m * NF1 + cf) * 5 * NABF4 + abf;
+                o[0 * NABF4] = dm;
+                o[1 * NABF4] = m0 * du + dm * (u0 + du) + dp * a0;
+                o[2 * NABF4] = m0 * dv + dm * (v0 + dv) + dp * a1;
+                o[3 * NABF4] = m0 * dw + dm * (w0 + dw) + dp * a2;
+                o[4 * NABF4] = m0 * dH + dm * (H0 + dH);
             }
         } else if (t < NF + (NABF4 - NABF) * NF1) {
             const int abf = NABF + (t - NF) / NF1, cf = (t - NF) % NF1;
@@ -468,7 +514,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
             }
             const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
             const double A0 = ar[0], A1 = ar[1], A2 = ar[2];
-            double mag = sqrt(A0 * A0 + A1 * A1 + A2 * A2);
+            double mag = sqrt_r(A0 * A0 + A1 * A1 + A2 * A2);
             mag = mag > 1e-300 ? mag : 1e-300;
             double d[3];
             if ((a.slot_fb[sidx] >> fp) & 1) {
@@ -527,8 +573,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
                                       gL[j] * sJ[(2 * NFP + i * N1 + c) * 5 + v] +
                                       gL[c] * sJ[(3 * NFP + i * N1 + j) * 5 + v] +
                                       gR[c] * sJ[(4 * NFP + i * N1 + j) * 5 + v];
-                    const real dj = a.det[cell * NS + s];
-                    sR[s * 5 + v] = -vol / dj - corr / dj;
+                    sR[s * 5 + v] = -(vol + corr) * recip(a.det[cell * NS + s]);
                 }
             }
         }
