@@ -24,6 +24,7 @@ class AfrMeshDesc(ctypes.Structure):
         ("prism_schedule", c_vp),
         ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
         ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
+        ("work_rvol", c_vp), ("work_h", c_vp),
     ]
 
 
@@ -53,6 +54,8 @@ SIGNATURES = {
     "afr_inviscid_from_primitives": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "afr_viscous_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int32, c_vp]),
+    "afr_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int32, c_vp]),
 }
 
 _lib = None

@@ -24,6 +24,7 @@
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
 #include "afr_prism.hpp"
+#include "afr_viscous.hpp"
 
 using namespace afr;
 
@@ -515,10 +516,96 @@ int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t s
     return afr_inviscid_from_primitives(mesh, d.work_q, d.work_umean, R, suppress, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+template <typename real>
+int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, double mu, double Pr, double Prt,
+                int suppress, int accumulate, int phase, hipStream_t st) {
+    // phase 0: volume kernel only; 1: interface kernel only
+    for (int ct = 0; ct < 2; ++ct) {
+        afr_visc::ViscArgs<real> a;
+        a.cell_begin = ct == 0 ? 0 : d.n_prism;
+        a.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
+        if (a.n_cells <= 0) continue;
+        a.U = (const real*)U;
+        a.mu_t = (const real*)mu_t;
+        a.dQ = (real*)d.work_q;
+        a.umean = (real*)d.work_umean;
+        a.grad = (real*)d.work_grad;
+        a.rvol = (real*)d.work_rvol;
+        a.h_ip = (real*)d.work_h;
+        a.R = (real*)R;
+        a.det = (const real*)d.det;
+        a.adj = (const real*)d.adj;
+        a.D = (const real*)d.D[ct];
+        a.E = (const real*)d.E[ct];
+        a.slot_nb = d.slot_nb;
+        a.slot_blk = d.slot_blk;
+        a.slot_mask = d.slot_mask;
+        a.slot_bc = d.slot_bc;
+        a.slot_tn = (const real*)d.slot_tn;
+        a.slot_adjrow = (const real*)d.slot_adjrow;
+        a.rows_idx = d.rows_idx;
+        a.rows_w = (const real*)d.rows_w;
+        a.knz = d.rows_knz;
+        a.n_rows = d.n_rows;
+        a.gL = (const real*)d.gL;
+        a.gR = (const real*)d.gR;
+        a.fp_of_sp = d.fp_of_sp;
+        a.coord_of_sp = d.coord_of_sp;
+        a.bcp = d.bc_params;
+        for (int v = 0; v < 5; ++v) a.qref[v] = d.ref_state[v];
+        a.mu = mu;
+        a.Pr = Pr;
+        a.Prt = Prt;
+        a.suppress = suppress;
+        a.accumulate = accumulate;
+        int rc;
+        if (d.order == 1)
+            rc = ct == 0 ? afr_visc::launch_viscous_phase<1, 0, real>(a, phase, st)
+                         : afr_visc::launch_viscous_phase<1, 1, real>(a, phase, st);
+        else
+            rc = ct == 0 ? afr_visc::launch_viscous_phase<2, 0, real>(a, phase, st)
+                         : afr_visc::launch_viscous_phase<2, 1, real>(a, phase, st);
+        if (rc) return fail("viscous kernel launch failed");
+    }
+    return 0;
+}
+
+int viscous_phase(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, double mu, double Pr, double Prt,
+                  int suppress, int accumulate, int phase, hipStream_t st) {
+    return d.dtype == AFR_F32 ? run_viscous<float>(d, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st)
+                              : run_viscous<double>(d, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st);
+}
+
+bool viscous_ready(const AfrMeshDesc& d) { return d.adj && d.work_grad && d.work_rvol && d.work_h && d.D[0] && d.D[1]; }
+
+}  // namespace
+
+extern "C" {
+
 int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,
                          double Pr_t, int32_t suppress, void* stream) {
-    (void)mesh; (void)U; (void)mu_t; (void)R; (void)mu; (void)Pr; (void)Pr_t; (void)suppress; (void)stream;
-    return fail("afr_viscous_residual: not built yet");
+    if (!mesh || !U || !R) return fail("afr_viscous_residual: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!viscous_ready(d)) return fail("afr_viscous_residual: mesh has no viscous work buffers");
+    hipStream_t st = (hipStream_t)stream;
+    if (viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
+    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 1, st);
+}
+
+int afr_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr, double Pr_t,
+                 int32_t suppress, void* stream) {
+    if (!mesh || !U || !R) return fail("afr_residual: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!viscous_ready(d)) return fail("afr_residual: mesh has no viscous work buffers");
+    hipStream_t st = (hipStream_t)stream;
+    // volume kernel: prepared state (dQ, umean), gradients, viscous volume term
+    if (viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
+    if (afr_inviscid_from_primitives(mesh, d.work_q, d.work_umean, R, suppress, stream)) return -1;
+    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 1, 1, st);
 }
 
 }  // extern "C"

@@ -1,0 +1,424 @@
+// Viscous (BR1) FR residual kernels for gfx950, one 64-lane wavefront per cell,
+// 4 wavefronts per workgroup sharing the dense operator tables in LDS.
+//
+//   k_visc_volume:  per cell, from the conserved state U:
+//       dQ = primitive(U) - Q_ref, umean = mean_s |U| (outlier reference),
+//       grad (u, v, w, T) = J^-T (D field)                  gradients.py:23-90
+//       G = viscous flux (tau, heat flux), G~ = adj G       flux_kernels.py:72-137
+//       R_vol = sum_m D_m G~_m / det                        viscous_flux.py:113-302
+//       h_ip = mean_s(det)^(1/3)                            flux_kernels.py:141-200
+//   k_visc_iface: per cell and face slot / flux point, the BR1 common flux of
+//       the averaged (Q, grad u, grad T, mu_t) minus the own flux, contracted
+//       with the contravariant face row; boundary faces take the ghost state
+//       with mirrored gradients and the interior-penalty momentum term
+//       (viscous_flux_kernel.py:59-289); + g' distribution / det, outlier
+//       zeroing (troubled_cell.py:339-408), R = or += result.
+#include <hip/hip_runtime.h>
+
+#include "../../include/afr.h"
+#include "afr_physics.hpp"
+#include "afr_viscous.hpp"
+
+using namespace afr;
+
+namespace afr_visc {
+
+constexpr int kWaves = 4;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_fresh(int t) {
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+template <int P, int CT> struct VO {
+    static constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1, NSLOT = CT == 0 ? 5 : 4;
+    static constexpr int NSV = NS * 5;
+};
+
+__device__ __forceinline__ int slot_code(int slot) { return slot < 3 ? slot : (slot == 3 ? 4 : 5); }
+
+// median-certificate outlier zeroing over sR[NS][5] (same rule as the inviscid kernels)
+template <int NS, typename real>
+__device__ void outlier_zeroing(real* sR, const real* umean, int t) {
+    constexpr int NSV = NS * 5, NR = (NSV + 63) / 64;
+    constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
+    constexpr int G = 8, J = (NS + G - 1) / G;
+    const int v = t >> 3, g = t & 7;
+    real mx = 0;
+    if (t < 5 * G) {
+        for (int j = 0; j < J; ++j) {
+            const int s = g + G * j;
+            if (s < NS) {
+                const real x = fabs(sR[s * 5 + v]);
+                mx = x > mx ? x : mx;
+            }
+        }
+    }
+    for (int m = 1; m < G; m <<= 1) {
+        const real o = __shfl_xor(mx, m);
+        mx = o > mx ? o : mx;
+    }
+    const real tt = mx * real(1.0001e-4);
+    int cnt = 0;
+    if (t < 5 * G) {
+        for (int j = 0; j < J; ++j) {
+            const int s = g + G * j;
+            if (s < NS) cnt += fabs(sR[s * 5 + v]) < tt;
+        }
+    }
+    for (int m = 1; m < G; m <<= 1) cnt += __shfl_xor(cnt, m);
+    if (!__any((t < 5 * G) && cnt > k1)) return;
+    real x[NR], f1[NR], f2[NR];
+    for (int k = 0; k < NR; ++k) {
+        const int o = t + 64 * k;
+        x[k] = f1[k] = f2[k] = real(0);
+        if (o < NSV) {
+            const int vv = o % 5;
+            x[k] = fabs(sR[o]);
+            int lt = 0, le = 0;
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const real y = fabs(sR[s2 * 5 + vv]);
+                lt += y < x[k];
+                le += y <= x[k];
+            }
+            f1[k] = (lt <= k1 && k1 < le) ? real(1) : real(0);
+            f2[k] = (lt <= k2 && k2 < le) ? real(1) : real(0);
+        }
+    }
+    real med1[5], med2[5];
+    for (int vv = 0; vv < 5; ++vv) {
+        real a1 = real(-1), a2 = real(-1);
+        for (int k = 0; k < NR; ++k) {
+            const int o = t + 64 * k;
+            if (o < NSV && o % 5 == vv) {
+                if (f1[k] > real(0)) a1 = x[k] > a1 ? x[k] : a1;
+                if (f2[k] > real(0)) a2 = x[k] > a2 ? x[k] : a2;
+            }
+        }
+        for (int m = 1; m < 64; m <<= 1) {
+            const real o1 = __shfl_xor(a1, m), o2 = __shfl_xor(a2, m);
+            a1 = o1 > a1 ? o1 : a1;
+            a2 = o2 > a2 ? o2 : a2;
+        }
+        med1[vv] = a1;
+        med2[vv] = a2;
+    }
+    for (int k = 0; k < NR; ++k) {
+        const int o = t + 64 * k;
+        if (o < NSV) {
+            const int vv = o % 5;
+            real m1v = med1[0], m2v = med2[0];
+            for (int q = 1; q < 5; ++q)
+                if (vv == q) { m1v = med1[q]; m2v = med2[q]; }
+            const real med = (NS % 2) ? m1v : real(0.5) * (m1v + m2v);
+            const real um = real(1e-9) * umean[vv];
+            real ref = med > um ? med : um;
+            ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
+            if (x[k] > real(1e4) * ref) sR[o] = real(0);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ volume kernel
+template <int P, int CT, typename real>
+__global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
+    using O = VO<P, CT>;
+    constexpr int NS = O::NS, NSV = O::NSV;
+    __shared__ real s_D[3 * NS * NS];                       // D[m][s][j]
+    __shared__ real s_w[kWaves][NSV + 4 * NS + 12 * NS + 12 * NS];
+    __shared__ double s_ref[5];
+    const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int t = tid & 63;
+    for (int i = tid; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
+    if (tid < 5) s_ref[tid] = a.qref[tid];
+    __syncthreads();
+    real* const sU = s_w[wid];              // |U| [s][5]
+    real* const sF = sU + NSV;              // (du, dv, dw, dT) [s][4]
+    real* const sG = sF + 4 * NS;           // grad [s][12]
+    real* const sGt = sG + 12 * NS;         // G~ [m][s][4] (v = 1..4)
+    const double q0r = s_ref[0], q0p = s_ref[4];
+    const double T0 = q0p / (rmax(q0r, 1e-10) * kRair);
+
+    const int64_t nw = (int64_t)gridDim.x * kWaves;
+    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
+        const int64_t cell = a.cell_begin + ci;
+        t = lane_fresh(t);
+        // 1. primitives, perturbations, |U|
+        if (t < NS) {
+            const int s = t;
+            double u[5], q[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) u[v] = (double)a.U[(cell * NS + s) * 5 + v];
+            cons2prim(u, q);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                a.dQ[(cell * NS + s) * 5 + v] = (real)(q[v] - s_ref[v]);
+                sU[s * 5 + v] = (real)fabs(u[v]);
+            }
+            const double T = q[4] / (rmax(q[0], 1e-10) * kRair);
+            sF[s * 4 + 0] = (real)(q[1] - s_ref[1]);
+            sF[s * 4 + 1] = (real)(q[2] - s_ref[2]);
+            sF[s * 4 + 2] = (real)(q[3] - s_ref[3]);
+            sF[s * 4 + 3] = (real)(T - T0);
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        if (t < 5) {
+            real acc = 0;
+            for (int s = 0; s < NS; ++s) acc += sU[s * 5 + t];
+            a.umean[cell * 5 + t] = acc / real(NS);
+        } else if (t == 5) {
+            real acc = 0;
+            for (int s = 0; s < NS; ++s) acc += a.det[cell * NS + s];
+            a.h_ip[cell] = (real)cbrt((double)(acc / real(NS)));
+        }
+        // 2. gradients: lane (s, field)
+        for (int o = t; o < 4 * NS; o += 64) {
+            const int s = o >> 2, f = o & 3;
+            real d0 = 0, d1 = 0, d2 = 0;
+            for (int j = 0; j < NS; ++j) {
+                const real x = sF[j * 4 + f];
+                d0 += s_D[(0 * NS + s) * NS + j] * x;
+                d1 += s_D[(1 * NS + s) * NS + j] * x;
+                d2 += s_D[(2 * NS + s) * NS + j] * x;
+            }
+            const real* ad = a.adj + (cell * NS + s) * 9;
+            const real idet = recip(a.det[cell * NS + s]);
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const real g = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
+                sG[s * 12 + f * 3 + n] = g;
+                a.grad[(cell * NS + s) * 12 + f * 3 + n] = g;
+            }
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        // 3. viscous flux at the solution points, contravariant components
+        if (t < NS) {
+            const int s = t;
+            real q[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) q[v] = (real)(s_ref[v] + (double)a.dQ[(cell * NS + s) * 5 + v]);
+            real gv[3][3], gT[3], G[3][5];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) gv[i][j] = sG[s * 12 + i * 3 + j];
+                gT[i] = sG[s * 12 + 9 + i];
+            }
+            const real mut = a.mu_t ? a.mu_t[cell * NS + s] : real(0);
+            viscous_flux(q, gv, gT, (real)a.mu, (real)a.Pr, mut, (real)a.Prt, G);
+            const real* ad = a.adj + (cell * NS + s) * 9;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int v = 1; v < 5; ++v)
+                    sGt[(m * NS + s) * 4 + v - 1] = ad[m * 3 + 0] * G[0][v] + ad[m * 3 + 1] * G[1][v] + ad[m * 3 + 2] * G[2][v];
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        // 4. volume term R_vol = sum_m D_m G~_m / det
+        for (int o = t; o < NSV; o += 64) {
+            const int s = o / 5, v = o % 5;
+            real acc = 0;
+            if (v > 0) {
+                for (int m = 0; m < 3; ++m)
+                    for (int j = 0; j < NS; ++j) acc += s_D[(m * NS + s) * NS + j] * sGt[(m * NS + j) * 4 + v - 1];
+                acc *= recip(a.det[cell * NS + s]);
+            }
+            a.rvol[cell * NSV + o] = acc;
+        }
+        wave_sync();
+    }
+}
+
+// ------------------------------------------------------------------ interface kernel
+template <int P, int CT, int KMAX, typename real>
+__global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
+    using O = VO<P, CT>;
+    constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP, NSLOT = O::NSLOT, NSV = O::NSV;
+    constexpr int NP = 18;  // pack: dQ (5), grad u (9), grad T (3), mu_t (1)
+    __shared__ real s_E[5 * NFP * NS];
+    __shared__ real s_g[2 * N1];
+    __shared__ int s_fp[3 * NS], s_co[3 * NS];
+    __shared__ real s_w[kWaves][NS * NP + 5 * NFP * 5 + NSV];
+    __shared__ double s_ref[5];
+    const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int t = tid & 63;
+    for (int i = tid; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = a.E[i];
+    for (int i = tid; i < 3 * NS; i += 64 * kWaves) {
+        s_fp[i] = a.fp_of_sp[i];
+        s_co[i] = a.coord_of_sp[i];
+    }
+    if (tid < N1) {
+        s_g[tid] = a.gL[tid];
+        s_g[N1 + tid] = a.gR[tid];
+    }
+    if (tid < 5) s_ref[tid] = a.qref[tid];
+    __syncthreads();
+    real* const sP = s_w[wid];           // own pack [s][NP]
+    real* const sJ = sP + NS * NP;       // jumps [slot][fp][5]
+    real* const sR = sJ + 5 * NFP * 5;   // residual [s][5]
+    const real mu = (real)a.mu, Pr = (real)a.Pr, Prt = (real)a.Prt;
+
+    const int64_t nw = (int64_t)gridDim.x * kWaves;
+    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
+        const int64_t cell = a.cell_begin + ci;
+        t = lane_fresh(t);
+        for (int o = t; o < NS * NP; o += 64) {
+            const int s = o / NP, k = o % NP;
+            real x;
+            if (k < 5) x = a.dQ[(cell * NS + s) * 5 + k];
+            else if (k < 17) x = a.grad[(cell * NS + s) * 12 + k - 5];
+            else x = a.mu_t ? a.mu_t[cell * NS + s] : real(0);
+            sP[o] = x;
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        // 1. BR1 jumps: lane = (slot, flux point)
+        if (t < NSLOT * NFP) {
+            const int slot = t / NFP, fp = t % NFP;
+            const int64_t sidx = cell * 5 + slot;
+            const int bc = a.slot_bc[sidx];
+            const int code = slot_code(slot);
+            const real side = (code & 1) ? real(1) : real(-1);
+            real po[NP], pn[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) po[k] = real(0);
+            for (int s = 0; s < NS; ++s) {
+                const real w = s_E[(slot * NFP + fp) * NS + s];
+#pragma unroll
+                for (int k = 0; k < NP; ++k) po[k] += w * sP[s * NP + k];
+            }
+            real Qo[5], Qn[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Qo[v] = (real)(s_ref[v] + (double)po[v]);
+            if (bc == BC_INTERIOR) {
+#pragma unroll
+                for (int k = 0; k < NP; ++k) pn[k] = real(0);
+                const int src = (a.slot_mask[sidx] >> fp) & 1;
+                int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
+                const int nbc = a.slot_nb[sidx * 2 + src];
+                if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
+                const int64_t nb = nbc >= 0 ? nbc : 0;
+                for (int k = 0; k < KMAX && k < a.knz; ++k) {
+                    const int sp = a.rows_idx[(int64_t)row * a.knz + k];
+                    const real w = a.rows_w[(int64_t)row * a.knz + k];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) pn[v] += w * a.dQ[(nb * NS + sp) * 5 + v];
+#pragma unroll
+                    for (int k2 = 0; k2 < 12; ++k2) pn[5 + k2] += w * a.grad[(nb * NS + sp) * 12 + k2];
+                    pn[17] += w * (a.mu_t ? a.mu_t[nb * NS + sp] : real(0));
+                }
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
+            } else {
+                // ghost state (FP64, as in the inviscid path), gradients and mu_t mirrored
+                double qod[5], qgd[5];
+                const double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
+                                      (double)a.slot_tn[sidx * 3 + 2]};
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qod[v] = s_ref[v] + (double)po[v];
+                ghost_state(bc, qod, tn, a.bcp, qgd);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Qn[v] = (real)qgd[v];
+#pragma unroll
+                for (int k = 0; k < NP; ++k) pn[k] = po[k];
+            }
+            real Qa[5], gva[3][3], gTa[3], gvo[3][3], gTo[3];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Qa[v] = real(0.5) * (Qo[v] + Qn[v]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    gvo[i][j] = po[5 + i * 3 + j];
+                    gva[i][j] = real(0.5) * (po[5 + i * 3 + j] + pn[5 + i * 3 + j]);
+                }
+                gTo[i] = po[14 + i];
+                gTa[i] = real(0.5) * (po[14 + i] + pn[14 + i]);
+            }
+            const real mto = po[17], mta = real(0.5) * (po[17] + pn[17]);
+            real Gc[3][5], Go[3][5];
+            viscous_flux(Qa, gva, gTa, mu, Pr, mta, Prt, Gc);
+            viscous_flux(Qo, gvo, gTo, mu, Pr, mto, Prt, Go);
+            const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
+            const real A0 = ar[0], A1 = ar[1], A2 = ar[2];
+            real jmp[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                jmp[v] = A0 * (Gc[0][v] - Go[0][v]) + A1 * (Gc[1][v] - Go[1][v]) + A2 * (Gc[2][v] - Go[2][v]);
+            if (bc != BC_INTERIOR) {
+                const real mag = sqrt(A0 * A0 + A1 * A1 + A2 * A2);
+                real h = a.h_ip[cell];
+                h = h > Tiny<real>::v ? h : Tiny<real>::v;
+                const real scale = real(4.0) * (mu + mto) / h * mag * side;
+#pragma unroll
+                for (int v = 1; v < 4; ++v) jmp[v] -= scale * (Qo[v] - Qn[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sJ[(slot * NFP + fp) * 5 + v] = jmp[v];
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        // 2. distribution, / det, + volume term
+        for (int o = t; o < NSV; o += 64) {
+            const int s = o / 5, v = o % 5;
+            real corr = 0;
+            for (int slot = 0; slot < NSLOT; ++slot) {
+                const int code = slot_code(slot), axis = code >> 1;
+                const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
+                corr += g * sJ[(slot * NFP + s_fp[axis * NS + s]) * 5 + v];
+            }
+            sR[o] = a.rvol[cell * NSV + o] + corr * recip(a.det[cell * NS + s]);
+        }
+        wave_sync();
+        t = lane_fresh(t);
+        if (a.suppress) outlier_zeroing<NS, real>(sR, a.umean + cell * 5, t);
+        wave_sync();
+        for (int o = t; o < NSV; o += 64) {
+            if (a.accumulate)
+                a.R[cell * NSV + o] += sR[o];
+            else
+                a.R[cell * NSV + o] = sR[o];
+        }
+        wave_sync();
+    }
+}
+
+template <int P, int CT, typename real>
+int launch_viscous_phase(const ViscArgs<real>& a, int phase, hipStream_t st) {
+    if (a.n_cells <= 0) return 0;
+    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
+    const int grid = (int)(need < 2048 ? need : 2048);
+    if (phase == 0) {
+        hipLaunchKernelGGL((k_visc_volume<P, CT, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    constexpr int NS = VO<P, CT>::NS, KS = (P + 1) * (P + 1);
+    if (a.knz <= KS)
+        hipLaunchKernelGGL((k_visc_iface<P, CT, KS, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
+    else if (a.knz <= NS)
+        hipLaunchKernelGGL((k_visc_iface<P, CT, NS, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
+    else
+        return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define AFR_VISC_INST(P, CT, T) template int launch_viscous_phase<P, CT, T>(const ViscArgs<T>&, int, hipStream_t);
+AFR_VISC_INST(1, 0, float)
+AFR_VISC_INST(2, 0, float)
+AFR_VISC_INST(1, 1, float)
+AFR_VISC_INST(2, 1, float)
+AFR_VISC_INST(1, 0, double)
+AFR_VISC_INST(2, 0, double)
+AFR_VISC_INST(1, 1, double)
+AFR_VISC_INST(2, 1, double)
+
+}  // namespace afr_visc

@@ -1,0 +1,48 @@
+// Shared declarations of the viscous (BR1) kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afr_visc {
+
+template <typename real>
+struct ViscArgs {
+    int64_t cell_begin, n_cells;   // cells [cell_begin, cell_begin + n_cells) of one type
+    const real* U;                 // (C, NS, 5) conserved
+    const real* mu_t;              // (C, NS) or null
+    real* dQ;                      // (C, NS, 5) out: primitive - Q_ref
+    real* umean;                   // (C, 5) out
+    real* grad;                    // (C, NS, 12) out: grad u (9, [i][j] = du_i/dx_j), grad T (3)
+    real* rvol;                    // (C, NS, 5) out: volume term / det
+    real* h_ip;                    // (C) out: mean(det)^(1/3)
+    real* R;                       // (C, NS, 5) out
+    const real* det;
+    const real* adj;               // (C, NS, 3, 3)
+    const real* D;                 // (3, NS, NS) of the cell type
+    const real* E;                 // (5, NFP, NS) of the cell type, slot order
+    const int* slot_nb;
+    const int* slot_blk;
+    const int* slot_mask;
+    const int* slot_bc;
+    const real* slot_tn;
+    const real* slot_adjrow;
+    const int* rows_idx;
+    const real* rows_w;
+    int knz;
+    int n_rows;
+    const real* gL;
+    const real* gR;
+    const int* fp_of_sp;
+    const int* coord_of_sp;
+    const double* bcp;
+    double qref[5];
+    double mu, Pr, Prt;
+    int suppress;
+    int accumulate;
+};
+
+// phase 0: k_visc_volume; phase 1: k_visc_iface
+template <int P, int CT, typename real>
+int launch_viscous_phase(const ViscArgs<real>& a, int phase, hipStream_t st);
+
+}  // namespace afr_visc

@@ -132,8 +132,10 @@ class DeviceMesh:
         # reference state of the perturbation form: the freestream by default
         self.ref_state = np.asarray(ref_state if ref_state is not None else bcp[BC_FARFIELD], dtype=np.float64)
         self.work_q = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
-        self.work_grad = torch.empty((C, T.ns, 13), dtype=dtype, device=self.device)
+        self.work_grad = torch.empty((C, T.ns, 12), dtype=dtype, device=self.device)
         self.work_umean = torch.empty((C, 5), dtype=dtype, device=self.device)
+        self.work_rvol = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_h = torch.empty((C,), dtype=dtype, device=self.device)
 
         d = _lib.AfrMeshDesc()
         d.abi_version = _lib.ABI_VERSION
@@ -160,6 +162,7 @@ class DeviceMesh:
         for v in range(5):
             d.ref_state[v] = float(self.ref_state[v])
         d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
+        d.work_rvol, d.work_h = p(self.work_rvol), p(self.work_h)
         self._desc = d
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
@@ -285,4 +288,29 @@ class DeviceMesh:
         _lib.check(self.lib.afr_inviscid_residual(self.handle, ctypes.c_void_p(U.data_ptr()),
                                                   ctypes.c_void_p(R.data_ptr()), int(bool(suppress)),
                                                   self._stream()), "afr_inviscid_residual")
+        return R
+
+    def _check_mut(self, mu_t):
+        if mu_t is None:
+            return None
+        if mu_t.dtype != self.dtype or not mu_t.is_contiguous() or tuple(mu_t.shape) != (self.n_cells, self.ns):
+            raise ValueError(f"mu_t must be a contiguous ({self.n_cells}, {self.ns}) tensor of the mesh precision")
+        return ctypes.c_void_p(mu_t.data_ptr())
+
+    def viscous_residual(self, U, mu_t=None, R=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, suppress=True):
+        """Viscous BR1 part of dU/dt (C, ns, 5) of the conserved state U."""
+        self._check_state(U)
+        R = torch.empty_like(U) if R is None else R
+        _lib.check(self.lib.afr_viscous_residual(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                                 ctypes.c_void_p(R.data_ptr()), float(mu), float(Pr), float(Pr_t),
+                                                 int(bool(suppress)), self._stream()), "afr_viscous_residual")
+        return R
+
+    def residual(self, U, mu_t=None, R=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, suppress=True):
+        """Navier-Stokes dU/dt = inviscid + viscous (C, ns, 5)."""
+        self._check_state(U)
+        R = torch.empty_like(U) if R is None else R
+        _lib.check(self.lib.afr_residual(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                         ctypes.c_void_p(R.data_ptr()), float(mu), float(Pr), float(Pr_t),
+                                         int(bool(suppress)), self._stream()), "afr_residual")
         return R

@@ -67,8 +67,10 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *                                      dQ = Q - Q_ref in the storage precision and evaluate
  *                                      point fluxes / Riemann fluxes in FP64
  *   work_q     (C, ns, 5)              scratch for dQ = primitive - Q_ref
- *   work_grad  (C, ns, 13)             scratch for (grad u (9), grad T (3), mu_t)
+ *   work_grad  (C, ns, 12)             scratch for grad u (9, [i][j] = du_i/dx_j) and grad T (3)
  *   work_umean (C, 5)                  scratch for the per-cell mean |U| (outlier reference)
+ *   work_rvol  (C, ns, 5)              scratch for the viscous volume term
+ *   work_h     (C)                     scratch for the interior-penalty length mean(det)^(1/3)
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
@@ -96,6 +98,8 @@ typedef struct AfrMeshDesc {
     void *work_q;
     void *work_grad;
     void *work_umean;
+    void *work_rvol;
+    void *work_h;
 } AfrMeshDesc;
 
 typedef struct AfrMesh AfrMesh;
@@ -160,9 +164,17 @@ int afr_inviscid_from_primitives(const AfrMesh *mesh, const void *dQ, const void
                                  int32_t suppress, void *stream);
 
 /* Viscous BR1 FR residual dU/dt (C, ns, 5); mu_t (C, ns) or NULL.
- * mu, Pr, Pr_t as in the reference (1.8e-5, 0.72, 0.9 by default). */
+ * mu, Pr, Pr_t as in the reference (1.8e-5, 0.72, 0.9 by default).  Uses the
+ * work buffers work_q, work_umean, work_grad, work_rvol and work_h; on return
+ * work_q / work_umean hold the prepared state of U and work_grad its gradients. */
 int afr_viscous_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu,
                          double Pr, double Pr_t, int32_t suppress, void *stream);
+
+/* Full Navier-Stokes FR residual dU/dt = inviscid + viscous (each with its own
+ * outlier zeroing when suppress != 0), as the reference solver sums them
+ * (src/autoflowcfd/core/fr_solver/solver.py:360-420).  mu_t may be NULL. */
+int afr_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu, double Pr,
+                 double Pr_t, int32_t suppress, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,164 @@
+"""Host-side logic: meshes, face slots, sparse rows, locality order, C ABI.
+
+The face-slot tables that the HIP kernels gather through are checked by a NumPy
+emulation of the device algorithm (tests/slot_emulator.py) against the FP64
+oracle's face-major inviscid residual on mixed, prism, tet and periodic meshes.
+"""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import slot_emulator as se
+from cases import freestream, mixed_mesh, smooth_state, tet_box, wall_box
+from autoflowcfd_amd.device import bc_params_table, morton_order, sparse_rows
+from autoflowcfd_amd.face_slots import build_slots
+from autoflowcfd_amd.fr_tables import tables
+from autoflowcfd_amd.geometry import cell_block_metrics
+from autoflowcfd_amd.mesh import hex_box
+from oracle import fr_ops
+from oracle import residual as R
+from oracle.faces import build_flat_faces
+from oracle.geometry import Geometry
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+MESHES = [
+    ("mixed", mixed_mesh, 1), ("mixed", mixed_mesh, 2),
+    ("tetbox", lambda: tet_box(3, seed=1), 2),
+    ("wallbox", lambda: wall_box(seed=2), 1), ("wallbox", lambda: wall_box(seed=2), 2),
+    ("periodic", lambda: tet_box(3, seed=3, displacement=0.0, periodic=True), 2),
+]
+
+
+@pytest.mark.parametrize("name,make,order", MESHES)
+def test_slot_tables_reproduce_oracle_residual(name, make, order):
+    m = make()
+    T = tables(order)
+    ops = fr_ops.generate_operators(order)
+    g = Geometry(m, ops)
+    fl = build_flat_faces(m, ops)
+    Q0 = freestream(0.15)
+    bcp = bc_params_table(Q0)
+    bcd = {c: bcp[c] for c in range(8)}
+    U = smooth_state(g.sps_xyz, Q0, seed=4)
+    ref = R.inviscid_residual(U, g, ops, fl, bcd)
+    geo = cell_block_metrics(m, T, np.float64)
+    sl = build_slots(m, T)
+    em = se.inviscid(m, T, geo, sl, U, bcd, R.euler_flux, R.ausm_up, R.ghost_state, True)
+    tol = 1e-7 if (order == 2 and m.n_tet) else 1e-11
+    assert np.linalg.norm(em - ref) / np.linalg.norm(ref) < tol
+
+
+def test_hex_box_prism_split_counts_and_orientation():
+    m = hex_box(3, 2, 2, 1.0, 1.0, 1.0, split="prism", seed=0)
+    assert m.n_prism == 2 * 3 * 2 * 2 and m.n_tet == 0
+    T = tables(1)
+    geo = cell_block_metrics(m, T, np.float64)
+    assert np.all(geo["det"] > 0)
+    assert np.isclose(geo["volume"].sum(), 1.0, rtol=1e-12)
+    m = hex_box(2, 2, 2, 1.0, 2.0, 1.0, split="tet", seed=1, displacement=0.05)
+    geo = cell_block_metrics(m, tables(2), np.float64)
+    assert m.n_tet == 6 * 8 and np.all(geo["det"] > 0)
+    assert np.isclose(geo["volume"].sum(), 2.0, rtol=1e-10)
+
+
+def test_slots_are_a_partition_of_faces():
+    """Every interior face appears in exactly two slots (one per side), boundary faces in one."""
+    m = wall_box(seed=5)
+    sl = build_slots(m, tables(2), with_geometry=False)
+    interior = sl.bc == -1
+    assert np.all(sl.nb[..., 0][interior] >= 0)
+    # neighbour relation is symmetric
+    pairs = set()
+    for c in range(m.n_cells):
+        for s in np.flatnonzero(interior[c]):
+            pairs.add((c, int(sl.nb[c, s, 0])))
+    assert all((b, a) in pairs for a, b in pairs)
+
+
+def test_sparse_rows_round_trip():
+    rng = np.random.default_rng(0)
+    rows = np.zeros((20, 27))
+    for r in range(20):
+        k = rng.integers(1, 10)
+        rows[r, rng.choice(27, size=k, replace=False)] = rng.normal(size=k)
+    rows[3, 5] = 1e-14  # round-off of a structural zero is dropped
+    nnz, idx, w = sparse_rows(rows)
+    dense = np.zeros_like(rows)
+    for r in range(20):
+        dense[r, idx[r, :nnz[r]]] = w[r, :nnz[r]]
+        assert np.all(w[r, nnz[r]:] == 0)
+    rows[3, 5] = 0.0
+    assert np.array_equal(dense, rows)
+
+
+def test_morton_order_is_a_locality_permutation():
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(size=(5000, 3))
+    perm = morton_order(pts)
+    assert np.array_equal(np.sort(perm), np.arange(5000))
+    step = np.linalg.norm(np.diff(pts[perm], axis=0), axis=1).mean()
+    rand = np.linalg.norm(np.diff(pts, axis=0), axis=1).mean()
+    assert step < 0.25 * rand
+
+
+# ------------------------------------------------------------------ C ABI
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "afr.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\bint\s+(afr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_function():
+    from autoflowcfd_amd import _lib
+    lib = _lib.load()
+    names = _header_functions()
+    assert names, "no functions parsed from include/afr.h"
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, n
+    assert set(_lib.SIGNATURES) == set(names)
+    assert lib.afr_version() == _lib.ABI_VERSION
+
+
+def test_abi_struct_layouts_match_ctypes(tmp_path):
+    from autoflowcfd_amd import _lib
+    src = tmp_path / "layout.c"
+    fields_m = [f[0] for f in _lib.AfrMeshDesc._fields_]
+    fields_g = [f[0] for f in _lib.AfrGeomDesc._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "afr.h"', 'int main(void) {',
+             'printf("%zu %zu\\n", sizeof(AfrMeshDesc), sizeof(AfrGeomDesc));']
+    lines += [f'printf("%zu\\n", offsetof(AfrMeshDesc, {f}));' for f in fields_m]
+    lines += [f'printf("%zu\\n", offsetof(AfrGeomDesc, {f}));' for f in fields_g]
+    lines += ['return 0; }']
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    vals = [int(x) for x in out]
+    assert vals[0] == ctypes.sizeof(_lib.AfrMeshDesc) and vals[1] == ctypes.sizeof(_lib.AfrGeomDesc)
+    off_m = [getattr(_lib.AfrMeshDesc, f).offset for f in fields_m]
+    off_g = [getattr(_lib.AfrGeomDesc, f).offset for f in fields_g]
+    assert vals[2:2 + len(fields_m)] == off_m
+    assert vals[2 + len(fields_m):] == off_g
+
+
+def test_error_paths_without_gpu():
+    from autoflowcfd_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.afr_mesh_create(None, ctypes.byref(h)) != 0
+    buf = ctypes.create_string_buffer(256)
+    lib.afr_last_error(buf, 256)
+    assert b"null" in buf.value
+    d = _lib.AfrMeshDesc()
+    d.abi_version = _lib.ABI_VERSION + 100
+    assert lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(h)) != 0
+    lib.afr_last_error(buf, 256)
+    assert b"ABI" in buf.value

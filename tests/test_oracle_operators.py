@@ -1,0 +1,202 @@
+"""Oracle operator tables pinned by closed forms and library routines.
+
+Each check is chosen so that a plausible slip in the restatement (a Jacobi
+parameter, an exponent of the collapsed weight, a transposed operand, the
+wrong Radau side) fails it:
+  * Jacobi recurrence vs scipy.special.eval_jacobi;
+  * collapsed modal bases: L2-orthogonality under the collapsed-coordinate
+    weights (Karniadakis & Sherwin), which fixes every exponent/parameter;
+  * D, D_fine: exact derivatives of every mode (central differences);
+  * c2f / f2c: exact interpolation on the coarse / over-integration spaces;
+  * E: exact evaluation of every mode at the face Gauss points;
+  * filter: V^-1 F V = diag(exp(-alpha eta^8)), F 1 = 1;
+  * Radau correction: g_L = (-1)^n/2 (P_n - P_{n-1}) (numpy Legendre), g_R(x) = g_L(-x);
+  * distribute mapping: closed form of the tensor index split.
+"""
+
+import numpy as np
+import pytest
+from scipy.special import eval_jacobi
+
+from oracle import fr_ops
+
+ORDERS = (1, 2, 3)
+TYPES = ("prism", "tet")
+
+
+def test_gauss_legendre_is_numpy_leggauss():
+    for n in range(1, 6):
+        x, w = fr_ops.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.allclose(x, xr, atol=1e-15) and np.allclose(w, wr, atol=1e-15)
+
+
+@pytest.mark.parametrize("alpha,beta", [(0, 0), (1, 0), (3, 0), (6, 0), (2, 1)])
+def test_jacobi_matches_scipy(alpha, beta):
+    x = np.linspace(-1, 1, 41)
+    for n in range(6):
+        assert np.allclose(fr_ops.jacobi(x, alpha, beta, n), eval_jacobi(n, alpha, beta, x), atol=1e-12)
+        h = 1e-6
+        fd = (eval_jacobi(n, alpha, beta, x + h) - eval_jacobi(n, alpha, beta, x - h)) / (2 * h)
+        assert np.allclose(fr_ops.djacobi(x, alpha, beta, n), fd, atol=1e-6)
+
+
+def _collapsed_quadrature(m=12):
+    x, w = np.polynomial.legendre.leggauss(m)
+    A, B, C = np.meshgrid(x, x, x, indexing="ij")
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    return A.ravel(), B.ravel(), C.ravel(), W.ravel()
+
+
+@pytest.mark.parametrize("cell_type", TYPES)
+@pytest.mark.parametrize("order", ORDERS)
+def test_modal_basis_orthogonal_under_collapsed_weight(cell_type, order):
+    a, b, c, w = _collapsed_quadrature()
+    V = fr_ops.modal_basis(cell_type, a, b, c, order)[0]
+    if cell_type == "tet":
+        wt = w * ((1 - b) / 2) * ((1 - c) / 2) ** 2
+    else:
+        wt = w * ((1 - b) / 2)
+    M = V.T @ (wt[:, None] * V)
+    off = M - np.diag(np.diag(M))
+    assert np.abs(off).max() < 1e-12 * np.abs(np.diag(M)).max()
+    assert np.all(np.diag(M) > 0)
+
+
+@pytest.mark.parametrize("cell_type", TYPES)
+@pytest.mark.parametrize("order", ORDERS)
+def test_diff_matrices_reproduce_mode_derivatives(cell_type, order):
+    ti = TYPES.index(cell_type)
+    ops = fr_ops.generate_operators(order)
+    p = ops.ref_sps
+    V = fr_ops.modal_basis(cell_type, p[:, 0], p[:, 1], p[:, 2], order)[0]
+    h = 1e-6
+    for m in range(3):
+        dp = np.zeros(3)
+        dp[m] = h
+        Vp = fr_ops.modal_basis(cell_type, *(p + dp).T, order)[0]
+        Vm = fr_ops.modal_basis(cell_type, *(p - dp).T, order)[0]
+        fd = (Vp - Vm) / (2 * h)
+        scale = max(np.abs(fd).max(), 1.0)
+        assert np.abs(ops.D[ti, :, :, m] @ V - fd).max() < 1e-6 * scale
+    assert np.abs(ops.D[ti].sum(axis=1)).max() < 1e-9 * np.abs(ops.D[ti]).max()  # D 1 = 0
+
+
+@pytest.mark.parametrize("cell_type", TYPES)
+@pytest.mark.parametrize("order", (1, 2))
+def test_overintegration_operators_exact(cell_type, order):
+    ti = TYPES.index(cell_type)
+    ops = fr_ops.generate_operators(order)
+    over = min(2 * order, fr_ops.OVERINTEGRATION_MAX_ORDER)
+    sp, fi = ops.ref_sps, ops.ref_fine
+    Vc = fr_ops.modal_basis(cell_type, *sp.T, order)[0]
+    Vf = fr_ops.modal_basis(cell_type, *fi.T, order)[0]
+    assert np.abs(ops.c2f[ti] @ Vc - Vf).max() < 1e-9 * np.abs(Vf).max()
+    Wf = fr_ops.modal_basis(cell_type, *fi.T, over)[0]
+    Wc = fr_ops.modal_basis(cell_type, *sp.T, over)[0]
+    assert np.abs(ops.f2c[ti] @ Wf - Wc).max() < 1e-7 * np.abs(Wc).max()
+    assert np.abs(ops.f2c[ti] @ ops.c2f[ti] - np.eye(ops.n_sps)).max() < 1e-8
+    # D_fine: exact derivatives on the over-integration space
+    h = 1e-6
+    for m in range(3):
+        dp = np.zeros(3)
+        dp[m] = h
+        fd = (fr_ops.modal_basis(cell_type, *(fi + dp).T, over)[0] -
+              fr_ops.modal_basis(cell_type, *(fi - dp).T, over)[0]) / (2 * h)
+        assert np.abs(ops.D_fine[ti, :, :, m] @ Wf - fd).max() < 1e-5 * max(np.abs(fd).max(), 1.0)
+
+
+@pytest.mark.parametrize("cell_type", TYPES)
+@pytest.mark.parametrize("order", ORDERS)
+def test_boundary_extrapolation_exact_on_face_points(cell_type, order):
+    ti = TYPES.index(cell_type)
+    ops = fr_ops.generate_operators(order)
+    n1 = order + 1
+    x, _ = np.polynomial.legendre.leggauss(n1)
+    V = fr_ops.modal_basis(cell_type, *ops.ref_sps.T, order)[0]
+    for axis in range(3):
+        others = [d for d in range(3) if d != axis]
+        for si, side in enumerate((-1.0, 1.0)):
+            pts = np.zeros((n1 * n1, 3))
+            pts[:, axis] = side
+            pts[:, others[0]] = np.repeat(x, n1)   # outer
+            pts[:, others[1]] = np.tile(x, n1)     # inner
+            Vf = fr_ops.modal_basis(cell_type, *pts.T, order)[0]
+            tol = 1e-8 if order < 3 else 1e-6  # P3 tet Vandermonde conditioning ~1e9
+            assert np.abs(ops.E[ti, axis, si] @ V - Vf).max() < tol * max(np.abs(Vf).max(), 1.0)
+
+
+@pytest.mark.parametrize("cell_type", TYPES)
+@pytest.mark.parametrize("order", ORDERS)
+def test_modal_filter_spectrum(cell_type, order):
+    ti = TYPES.index(cell_type)
+    ops = fr_ops.generate_operators(order)
+    n1 = order + 1
+    V = fr_ops.modal_basis(cell_type, *ops.ref_sps.T, order)[0]
+    alpha = -np.log(np.finfo(np.float64).eps)
+    sig = np.array([np.exp(-alpha * (max(i, j, k) / order) ** 8)
+                    for i in range(n1) for j in range(n1) for k in range(n1)])
+    S = np.linalg.solve(V, ops.filt[ti] @ V)
+    assert np.abs(S - np.diag(sig)).max() < 1e-7
+    assert np.abs(ops.filt[ti] @ np.ones(ops.n_sps) - 1.0).max() < (1e-9 if order < 3 else 1e-7)
+
+
+@pytest.mark.parametrize("order", (0, 1, 2, 3))
+def test_radau_correction_closed_form(order):
+    n = order + 1
+    x, _ = np.polynomial.legendre.leggauss(n)
+    gL, gR = fr_ops.correction_derivatives(n)
+    L = np.polynomial.legendre.Legendre
+    radau_left = (-1) ** n / 2.0 * (L.basis(n) - L.basis(n - 1))
+    assert np.allclose(gL, radau_left.deriv()(x), atol=1e-10)
+    assert abs(radau_left(-1.0) - 1.0) < 1e-12 and abs(radau_left(1.0)) < 1e-12
+    assert np.allclose(gR, -radau_left.deriv()(-x), atol=1e-10)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_distribute_mapping_closed_form(order):
+    ops = fr_ops.generate_operators(order)
+    n1 = order + 1
+    for s in range(n1 ** 3):
+        ijk = (s // (n1 * n1), (s // n1) % n1, s % n1)
+        for axis in range(3):
+            o = [ijk[d] for d in range(3) if d != axis]
+            assert ops.coord_of_sp[axis, s] == ijk[axis]
+            assert ops.fp_of_sp[axis, s] == o[0] * n1 + o[1]
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_product_tables_equal_oracle(order):
+    """The shipped FP64 table generator (autoflowcfd_amd.fr_tables) against the oracle."""
+    from autoflowcfd_amd.fr_tables import tables
+    t = tables(order)
+    o = fr_ops.generate_operators(order)
+    tol = 1e-9 if order < 3 else 1e-6
+    for name, a, b in (("D", t.D, o.D), ("E", t.E, o.E), ("c2f", t.c2f, o.c2f), ("Dfine", t.Dfine, o.D_fine),
+                       ("f2c", t.f2c, o.f2c), ("filt", t.filt, o.filt), ("gL", t.gL, o.g_left),
+                       ("gR", t.gR, o.g_right)):
+        assert np.abs(a - b).max() <= tol * max(np.abs(b).max(), 1.0), name
+    assert (t.fp_of_sp == o.fp_of_sp).all() and (t.coord_of_sp == o.coord_of_sp).all()
+    Op = np.stack([np.concatenate([o.f2c[ct] @ o.D_fine[ct, :, :, m] for m in range(3)], axis=1) for ct in (0, 1)])
+    # the folded f2c D_fine of the tet reaches 4e5 at P2: its round-off is 1e-9 relative
+    assert np.abs(t.Op - Op).max() <= 10 * tol * np.abs(Op).max()
+
+
+@pytest.mark.parametrize("order", (1, 2))
+def test_prism_kronecker_factors_rebuild_operators(order):
+    """The factored prism operators used by the prism kernels rebuild the dense tables."""
+    from autoflowcfd_amd.fr_tables import prism_factors, tables
+    F = prism_factors(order)
+    t = tables(order)
+    n1 = order + 1
+    assert np.abs(np.kron(F.c2f_ab, F.c2f_c) - t.c2f[0]).max() < 1e-12
+    Op = np.concatenate([np.kron(F.A_a, F.f2c_c), np.kron(F.A_b, F.f2c_c), np.kron(F.f2c_ab, F.B_c)], axis=1)
+    assert np.abs(Op - t.Op[0]).max() < 1e-10 * np.abs(t.Op[0]).max()
+    I = np.eye(n1)
+    for si, (axis, side) in enumerate(((0, 0), (0, 1), (1, 0))):
+        assert np.abs(np.kron(F.E_ab[si], I) - t.E[0, axis, side]).max() < 1e-12
+    for side in (0, 1):
+        assert np.abs(np.kron(np.eye(n1 * n1), F.e_c[side][None, :]) - t.E[0, 2, side]).max() < 1e-12
+    assert np.abs(np.kron(F.D_ab[0], I) - t.D[0, :, :, 0]).max() < 1e-11
+    assert np.abs(np.kron(F.D_ab[1], I) - t.D[0, :, :, 1]).max() < 1e-11
+    assert np.abs(np.kron(np.eye(n1 * n1), F.D_c) - t.D[0, :, :, 2]).max() < 1e-11

@@ -1,0 +1,134 @@
+"""Oracle point physics pinned by closed-form properties and the reference's own tests.
+
+AUSM+up: consistency F*(q,q,n) = F(q).n (tests/unit/test_fr_residual_inviscid.py:74-98),
+exact upwinding of supersonic pairs (kernels.py:20-155 branch |M| >= 1), Galilean
+sign symmetry; ghost states: tests/unit/test_fr_ghost_state.py; viscous flux:
+Stokes hypothesis (trace-free stress), rigid rotation is stress free, Fourier heat
+flux k = cp (mu/Pr + mu_t/Pr_t); outlier zeroing: tests/unit/test_troubled_cell.py.
+"""
+
+import numpy as np
+
+from autoflowcfd_amd.mesh import (BC_FARFIELD, BC_INLET, BC_OUTLET, BC_SLIP_WALL, BC_SYMMETRY, BC_WALL)
+from oracle import residual as R
+
+
+def _rand_state(rng, n=1):
+    q = np.stack([rng.uniform(0.5, 2.0, n), rng.uniform(-50, 50, n), rng.uniform(-50, 50, n),
+                  rng.uniform(-50, 50, n), rng.uniform(5e4, 2e5, n)], axis=-1)
+    nv = rng.normal(size=(n, 3))
+    return q, nv / np.linalg.norm(nv, axis=-1, keepdims=True)
+
+
+def test_ausm_consistency_random_states():
+    rng = np.random.default_rng(42)
+    q, n = _rand_state(rng, 200)
+    fa = R.ausm_up(q, q, n)
+    fe = np.einsum("ki,kiv->kv", n, R.euler_flux(q))
+    assert np.max(np.abs(fa - fe) / (np.abs(fe) + 1e-6)) < 1e-9
+
+
+def test_ausm_mass_flux_uniform_state():
+    q = np.array([[1.225, 30.0, 5.0, -3.0, 101325.0]])
+    n = np.array([[0.6, 0.8, 0.0]])
+    un = q[0, 1:4] @ n[0]
+    assert abs(R.ausm_up(q, q, n)[0, 0] - q[0, 0] * un) < 1e-6 * abs(q[0, 0] * un)
+
+
+def test_ausm_supersonic_branches():
+    """|M| >= 1 on both sides along +n: M+ = M_L, M- = 0, P+ = 1, P- = 0, no pressure diffusion,
+    so the interface pressure is p_L and the upwind state is the left one (kernels.py:20-155 uses the
+    mean acoustic impedance 0.5 (rho_L a_L + rho_R a_R) in the mass flux)."""
+    rng = np.random.default_rng(1)
+    qL, n = _rand_state(rng, 50)
+    qR, _ = _rand_state(rng, 50)
+    for q in (qL, qR):
+        a = np.sqrt(1.4 * q[:, 4] / q[:, 0])
+        q[:, 1:4] = 2.5 * a[:, None] * n
+    f = R.ausm_up(qL, qR, n)
+    aL, aR = np.sqrt(1.4 * qL[:, 4] / qL[:, 0]), np.sqrt(1.4 * qR[:, 4] / qR[:, 0])
+    mass = 0.5 * (qL[:, 0] * aL + qR[:, 0] * aR) * 2.5
+    assert np.allclose(f[:, 0], mass, rtol=1e-12)
+    assert np.allclose(f[:, 1:4], mass[:, None] * qL[:, 1:4] + qL[:, 4:5] * n, rtol=1e-12)
+    hL = 3.5 * qL[:, 4] / qL[:, 0] + 0.5 * np.sum(qL[:, 1:4] ** 2, axis=1)
+    assert np.allclose(f[:, 4], mass * hL, rtol=1e-12)
+
+
+def test_ausm_mirror_symmetry():
+    """F*(qR, qL, -n) = -F*(qL, qR, n) for the scalar components (conservation across a face)."""
+    rng = np.random.default_rng(5)
+    qL, n = _rand_state(rng, 100)
+    qR, _ = _rand_state(rng, 100)
+    f1 = R.ausm_up(qL, qR, n)
+    f2 = R.ausm_up(qR, qL, -n)
+    assert np.max(np.abs(f1[:, 0] + f2[:, 0]) / (np.abs(f1[:, 0]) + 1e-3)) < 1e-10
+    assert np.max(np.abs(f1[:, 4] + f2[:, 4]) / (np.abs(f1[:, 4]) + 1e-3)) < 1e-10
+
+
+def _ghost(code, q, n, params=None):
+    bcp = {BC_FARFIELD: np.array([1.225, 30.0, 0.0, 0.0, 101325.0]),
+           BC_INLET: np.array([1.2, -10.0, 0.0, 0.0, 1e5]), BC_OUTLET: np.array([0, 0, 0, 0, 101325.0])}
+    if params:
+        bcp.update(params)
+    return R.ghost_state(np.array([code]), q[None, None, :], n[None, None, :], bcp)[0, 0]
+
+
+def test_ghost_states():
+    qi = np.array([1.2, 30.0, 5.0, -2.0, 1e5])
+    ny = np.array([0.0, 1.0, 0.0])
+    g = _ghost(BC_WALL, qi, ny)
+    assert np.allclose(0.5 * (qi[1:4] + g[1:4]), 0.0, atol=1e-10)
+    for code in (BC_SLIP_WALL, BC_SYMMETRY):
+        g = _ghost(code, qi, ny)
+        avg = 0.5 * (qi[1:4] + g[1:4])
+        assert abs(avg @ ny) < 1e-10 and np.isclose(avg[0], qi[1]) and np.isclose(avg[2], qi[3])
+    assert np.allclose(_ghost(BC_FARFIELD, qi, ny), [1.225, 30.0, 0.0, 0.0, 101325.0])
+    nx = np.array([1.0, 0.0, 0.0])
+    qin = np.array([1.0, -5.0, 0.0, 0.0, 9e4])
+    qout = np.array([1.0, 5.0, 0.0, 0.0, 9e4])
+    assert np.allclose(_ghost(BC_INLET, qin, nx), [1.2, -10.0, 0.0, 0.0, 1e5])
+    assert np.allclose(_ghost(BC_INLET, qout, nx), qout)
+    g = _ghost(BC_OUTLET, qout, nx)
+    assert np.isclose(g[4], 101325.0) and np.isclose(g[0], qout[0])
+    assert np.isclose(_ghost(BC_OUTLET, qin, nx)[4], qin[4])
+
+
+def test_viscous_flux_properties():
+    rng = np.random.default_rng(7)
+    q = np.array([1.2, 30.0, -4.0, 2.0, 1e5])
+    mu, Pr, mut, Prt = 1.8e-5, 0.72, 3e-4, 0.9
+    gv = rng.normal(size=(3, 3)) * 100.0
+    gT = rng.normal(size=3) * 10.0
+    G = R.viscous_flux(q[None], gv[None], gT[None], mu, Pr, np.array([mut]), Prt)[0]
+    tau = G[:, 1:4]
+    assert np.allclose(tau, tau.T, rtol=1e-12, atol=1e-12)
+    assert abs(np.trace(tau)) < 1e-12 * np.abs(tau).max()
+    S = 0.5 * (gv + gv.T)
+    dev = S - np.trace(gv) / 3.0 * np.eye(3)
+    assert np.allclose(tau, 2.0 * (mu + mut) * dev, rtol=1e-12, atol=1e-14)
+    cp = 1.4 * 287.0 / 0.4
+    k = cp * (mu / Pr + mut / Prt)
+    assert np.allclose(G[:, 4], q[1:4] @ tau - k * gT, rtol=1e-12)
+    assert np.allclose(G[:, 0], 0.0)
+    W = np.array([[0.0, 3.0, -1.0], [-3.0, 0.0, 2.0], [1.0, -2.0, 0.0]])  # rigid rotation
+    G0 = R.viscous_flux(q[None], W[None], np.zeros((1, 3)), mu, Pr, np.array([mut]), Prt)[0]
+    assert np.abs(G0).max() < 1e-15
+
+
+def test_outlier_zeroing_matches_median_rule():
+    rng = np.random.default_rng(2)
+    res = rng.standard_normal((50, 8, 5)) * 1e-3
+    assert np.array_equal(R.suppress_outliers(res, np.ones((50, 8, 5))), res)
+    res = rng.standard_normal((4, 8, 5)) * 1e-3
+    res[0, 0, 1] = 1e5
+    out = R.suppress_outliers(res, np.ones((4, 8, 5)))
+    assert out[0, 0, 1] == 0.0
+    mask = np.ones_like(res, dtype=bool)
+    mask[0, 0, 1] = False
+    assert np.array_equal(out[mask], res[mask])
+    # the threshold is 1e4 x median over the solution points (numpy median, even count = mean of middle two)
+    res = np.ones((1, 4, 5))
+    res[0, :, 0] = [1.0, 3.0, 2.0, 2.5e4]   # median 2.5 -> threshold 2.5e4: 2.5e4 kept
+    assert R.suppress_outliers(res, np.ones((1, 4, 5)))[0, 3, 0] == 2.5e4
+    res[0, 3, 0] = 2.5001e4
+    assert R.suppress_outliers(res, np.ones((1, 4, 5)))[0, 3, 0] == 0.0
