@@ -25,6 +25,8 @@ class AfrMeshDesc(ctypes.Structure):
         ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
         ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
         ("work_rvol", c_vp), ("work_h", c_vp),
+        ("metric_flux_scale", c_vp), ("volume", c_vp),
+        ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
     ]
 
 
@@ -56,6 +58,12 @@ SIGNATURES = {
                                             ctypes.c_double, ctypes.c_int32, c_vp]),
     "afr_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int32, c_vp]),
+    "afr_local_time_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
+    "afr_rk_stage": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, c_vp]),
+    "afr_residual_norm_f64": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
+    "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int32, c_vp, c_vp]),
 }
 
 _lib = None

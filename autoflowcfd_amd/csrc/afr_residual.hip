@@ -25,6 +25,7 @@
 #include "afr_physics.hpp"
 #include "afr_prism.hpp"
 #include "afr_viscous.hpp"
+#include "afr_step.hpp"
 
 using namespace afr;
 
@@ -606,6 +607,89 @@ int afr_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, 
     if (viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
     if (afr_inviscid_from_primitives(mesh, d.work_q, d.work_umean, R, suppress, stream)) return -1;
     return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 1, 1, st);
+}
+
+namespace {
+
+bool step_ready(const AfrMeshDesc& d) {
+    return d.metric_flux_scale && d.volume && d.work_r && d.work_s1 && d.work_s2 && d.work_dt && d.work_part &&
+           d.filt[0] && d.filt[1];
+}
+
+int ns_of(const AfrMeshDesc& d) { return (d.order + 1) * (d.order + 1) * (d.order + 1); }
+
+}  // namespace
+
+int afr_local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t, void* dt, double mu, void* stream) {
+    if (!mesh || !U || !dt) return fail("afr_local_time_step: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!d.metric_flux_scale || !d.volume) return fail("afr_local_time_step: mesh has no metric_flux_scale/volume");
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = d.dtype == AFR_F32
+                       ? afr_step::launch_dt_local<float>((const float*)U, (const float*)mu_t, (const float*)d.det,
+                                                          (const float*)d.metric_flux_scale, (const float*)d.volume,
+                                                          (float*)dt, d.n_cells, ns_of(d), mu, d.order, st)
+                       : afr_step::launch_dt_local<double>((const double*)U, (const double*)mu_t, (const double*)d.det,
+                                                           (const double*)d.metric_flux_scale,
+                                                           (const double*)d.volume, (double*)dt, d.n_cells, ns_of(d),
+                                                           mu, d.order, st);
+    return rc ? fail("k_dt_local launch failed") : 0;
+}
+
+int afr_rk_stage(const AfrMesh* mesh, const void* U0, const void* Ua, const void* R, const void* dt, void* U_out,
+                 double c0, double c1, double c2, void* stream) {
+    if (!mesh || !U0 || !R || !dt || !U_out || (c1 != 0.0 && !Ua)) return fail("afr_rk_stage: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!d.filt[0] || !d.filt[1]) return fail("afr_rk_stage: mesh has no filter tables");
+    hipStream_t st = (hipStream_t)stream;
+    const void* ua = Ua ? Ua : U0;
+    const int rc = d.dtype == AFR_F32
+                       ? afr_step::launch_rk_stage<float>(ns_of(d), (const float*)U0, (const float*)ua, (const float*)R,
+                                                          (const float*)dt, (float*)U_out, (const float*)d.filt[0],
+                                                          (const float*)d.filt[1], d.n_cells, d.n_prism, c0, c1, c2,
+                                                          1.0, st)
+                       : afr_step::launch_rk_stage<double>(ns_of(d), (const double*)U0, (const double*)ua,
+                                                           (const double*)R, (const double*)dt, (double*)U_out,
+                                                           (const double*)d.filt[0], (const double*)d.filt[1],
+                                                           d.n_cells, d.n_prism, c0, c1, c2, 1.0, st);
+    return rc ? fail("k_rk_stage launch failed") : 0;
+}
+
+int afr_residual_norm_f64(const AfrMesh* mesh, const void* R, int32_t n_vars_total, double* out, void* stream) {
+    if (!mesh || !R || !out) return fail("afr_residual_norm_f64: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!d.work_part) return fail("afr_residual_norm_f64: mesh has no work_part buffer");
+    if (n_vars_total < 5) return fail("afr_residual_norm_f64: n_vars_total < 5");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = d.n_cells * ns_of(d) * 5;
+    const double n_total = (double)d.n_cells * ns_of(d) * n_vars_total;
+    const int rc = d.dtype == AFR_F32
+                       ? afr_step::launch_norm<float>((const float*)R, n, n_total, d.work_part, afr_step::kNormParts,
+                                                      out, st)
+                       : afr_step::launch_norm<double>((const double*)R, n, n_total, d.work_part,
+                                                       afr_step::kNormParts, out, st);
+    return rc ? fail("residual norm launch failed") : 0;
+}
+
+int afr_ssp_rk3_step(const AfrMesh* mesh, void* U, const void* mu_t, double mu, double Pr, double Pr_t,
+                     int32_t n_vars_total, double* norm, void* stream) {
+    if (!mesh || !U || !norm) return fail("afr_ssp_rk3_step: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!step_ready(d)) return fail("afr_ssp_rk3_step: mesh has no pseudo-step work buffers");
+    const bool visc = mu != 0.0 || mu_t != nullptr;
+    auto rhs = [&](const void* X, void* R) -> int {
+        return visc ? afr_residual(mesh, X, mu_t, R, mu, Pr, Pr_t, 1, stream)
+                    : afr_inviscid_residual(mesh, X, R, 1, stream);
+    };
+    if (afr_local_time_step(mesh, U, mu_t, d.work_dt, mu, stream)) return -1;
+    if (rhs(U, d.work_r)) return -1;
+    if (afr_residual_norm_f64(mesh, d.work_r, n_vars_total, norm, stream)) return -1;
+    // Shu-Osher SSP-RK3: U1 = U0 + dt L0; U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L1; U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L2
+    if (afr_rk_stage(mesh, U, nullptr, d.work_r, d.work_dt, d.work_s1, 1.0, 0.0, 1.0, stream)) return -1;
+    if (rhs(d.work_s1, d.work_r)) return -1;
+    if (afr_rk_stage(mesh, U, d.work_s1, d.work_r, d.work_dt, d.work_s2, 0.75, 0.25, 0.25, stream)) return -1;
+    if (rhs(d.work_s2, d.work_r)) return -1;
+    return afr_rk_stage(mesh, U, d.work_s2, d.work_r, d.work_dt, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, stream);
 }
 
 }  // extern "C"

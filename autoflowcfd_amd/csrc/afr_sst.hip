@@ -1,0 +1,477 @@
+// SST k-omega turbulence step on the device (once per pseudo-step, before the
+// mean-flow stages), FP64 arithmetic on storage-precision fields.
+//
+//   k_sst_cell    per cell: strain rate from the gradient of the conserved momentum,
+//                 clipped grad k / grad omega, F1, F2, nu_t (viscosity-ratio cap),
+//                 production / destruction / cross diffusion with mu = 1.8e-5
+//                 (src/autoflowcfd/core/turbulence/sst.py:21-330,
+//                  src/autoflowcfd/core/fr_solver/turbulence.py:178-330);
+//                 transport volume terms -D.(adj rho u phi)/det, -D.(adj Gamma grad phi)/det
+//                 and the face pack (src/autoflowcfd/core/turbulence/transport.py:122-360)
+//   k_sst_face    per face record and flux point: upwind convection and BR1-type
+//                 diffusion corrections from the owner side, Neumann ghost on records
+//                 without neighbour sources, Dirichlet-zero k on WALL records
+//                 (src/autoflowcfd/core/turbulence/transport_kernel.py:103-170)
+//   k_sst_update  per cell: record corrections gathered with the owner's / neighbour's
+//                 flux-point maps (transport_kernel.py:200-260), / rho, outlier zeroing,
+//                 point-implicit source damping and explicit update with the local time
+//                 step, floors (src/autoflowcfd/core/turbulence/sst.py:335-479)
+//   k_wall_distance  nearest wall node per solution point, FP64 brute force
+//                 (src/autoflowcfd/core/fr_solver/turbulence.py:112-121 uses a KD-tree)
+#include <hip/hip_runtime.h>
+
+#include "../../include/afr.h"
+#include "afr_physics.hpp"
+#include "afr_sst.hpp"
+
+using namespace afr;
+
+namespace afr_sst {
+
+constexpr int kWaves = 4;
+constexpr double SK1 = 0.85, SK2 = 1.0, SW1 = 0.5, SW2 = 0.856, B1 = 0.075, B2 = 0.0828, A1 = 0.31, KAP = 0.41,
+                 BSTAR = 0.09;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int slot_code(int slot) { return slot < 3 ? slot : (slot == 3 ? 4 : 5); }
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double finite_or(double x, double alt) { return isfinite(x) ? x : alt; }
+
+__device__ double blend_F1(double k, double w, double d, double nu, double rho, double CD) {
+    w = dmax(w, 1e-10);
+    k = dmax(k, 1e-10);
+    d = dmax(d, 1e-10);
+    const double t1 = sqrt(k) / (BSTAR * w * d);
+    const double t2 = 500.0 * nu / (d * d * w);
+    const double t3 = 4.0 * rho * SW2 * k / (dmax(CD, 1e-10) * d * d);
+    double a = dmin(dmax(t1, t2), t3);
+    a = isfinite(a) ? a : 1e75;
+    a = dmin(a, 1e75);
+    const double a2 = a * a;
+    return tanh(a2 * a2);
+}
+
+__device__ double blend_F2(double k, double w, double d, double nu) {
+    w = dmax(w, 1e-10);
+    k = dmax(k, 1e-10);
+    d = dmax(d, 1e-10);
+    double a = dmax(2.0 * sqrt(k) / (BSTAR * w * d), 500.0 * nu / (d * d * w));
+    a = isfinite(a) ? a : 1e150;
+    a = dmin(a, 1e150);
+    return tanh(a * a);
+}
+
+template <int P, int CT> struct SO {
+    static constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1, NSLOT = CT == 0 ? 5 : 4;
+};
+
+// ------------------------------------------------------------------ per-cell sources and volume terms
+template <int P, int CT, typename real>
+__global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
+    using O = SO<P, CT>;
+    constexpr int NS = O::NS;
+    constexpr int NF = 12;  // per-SP doubles: rho, u, v, w, m1, m2, m3, k, w, nu_t, Gk, Gw
+    __shared__ real s_D[3 * NS * NS];
+    __shared__ double s_w[kWaves][NS * NF + NS * 15 + 3 * NS * 4];
+    for (int i = threadIdx.x; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
+    __syncthreads();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
+    double* sv = s_w[wid];          // [s][NF]
+    double* sg = sv + NS * NF;      // [s][15]: grad m (9), grad k (3), grad w (3)
+    double* sF = sg + NS * 15;      // [m][s][4]: adj (rho u k, rho u w, Gk grad k, Gw grad w) components
+    const double mu = a.mu_molecular;
+    const int64_t nw = (int64_t)gridDim.x * kWaves;
+    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
+        const int64_t cell = a.cell_begin + ci;
+        if (t < NS) {
+            const int64_t i = cell * NS + t;
+            double u5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) u5[v] = (double)a.U[i * 5 + v];
+            const double rho = dmax(u5[0], 1e-10);
+            sv[t * NF + 0] = rho;
+            sv[t * NF + 1] = u5[1] / rho;
+            sv[t * NF + 2] = u5[2] / rho;
+            sv[t * NF + 3] = u5[3] / rho;
+            sv[t * NF + 4] = u5[1];
+            sv[t * NF + 5] = u5[2];
+            sv[t * NF + 6] = u5[3];
+            sv[t * NF + 7] = (double)a.k[i];
+            sv[t * NF + 8] = (double)a.w[i];
+        }
+        wave_sync();
+        // gradients of m1, m2, m3, k, omega: lane (s, field)
+        for (int o = threadIdx.x & 63; o < 5 * NS; o += 64) {
+            const int s = o / 5, f = o % 5;
+            const int src = f < 3 ? 4 + f : 7 + (f - 3);
+            double d0 = 0, d1 = 0, d2 = 0;
+            for (int j = 0; j < NS; ++j) {
+                const double x = sv[j * NF + src];
+                d0 += (double)s_D[(0 * NS + s) * NS + j] * x;
+                d1 += (double)s_D[(1 * NS + s) * NS + j] * x;
+                d2 += (double)s_D[(2 * NS + s) * NS + j] * x;
+            }
+            const real* ad = a.adj + (cell * NS + s) * 9;
+            const double idet = 1.0 / (double)a.det[cell * NS + s];
+            const int base = f < 3 ? f * 3 : 9 + (f - 3) * 3;
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                sg[s * 15 + base + n] =
+                    ((double)ad[0 * 3 + n] * d0 + (double)ad[1 * 3 + n] * d1 + (double)ad[2 * 3 + n] * d2) * idet;
+        }
+        wave_sync();
+        if (t < NS) {
+            const int s = t;
+            const int64_t i = cell * NS + s;
+            const double rho = sv[s * NF + 0], k = sv[s * NF + 7], w = sv[s * NF + 8];
+            const double d = (double)a.d_wall[i];
+            // strain rate of the momentum gradient
+            double S2 = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double sij = 0.5 * (sg[s * 15 + p * 3 + q] + sg[s * 15 + q * 3 + p]);
+                    S2 += sij * sij;
+                }
+            const double S = sqrt(2.0 * S2);
+            // clipped gradients of k and omega
+            double gk[3], gw[3];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                gk[n] = sg[s * 15 + 9 + n];
+                gw[n] = sg[s * 15 + 12 + n];
+            }
+            const double mk = sqrt(gk[0] * gk[0] + gk[1] * gk[1] + gk[2] * gk[2]);
+            const double mw = sqrt(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
+            if (mk > 1e6) {
+                const double f = dmin(dmax(1e6 / dmax(mk, 1e-10), 0.0), 1.0);
+                gk[0] *= f; gk[1] *= f; gk[2] *= f;
+            }
+            if (mw > 1e6) {
+                const double f = dmin(dmax(1e6 / dmax(mw, 1e-10), 0.0), 1.0);
+                gw[0] *= f; gw[1] *= f; gw[2] *= f;
+            }
+            const double gdot = gk[0] * gw[0] + gk[1] * gw[1] + gk[2] * gw[2];
+            // sources (mu = 1.8e-5 as the reference's source terms)
+            const double musrc = 1.8e-5;
+            const double nu = musrc / dmax(rho, 1e-10);
+            const double ks = dmin(k, 1e40), wsf = dmax(dmin(w, 1e40), 1e-10);
+            const double CD = dmax(2.0 * rho * SW2 / wsf * gdot, 1e-10);
+            const double F1 = blend_F1(ks, wsf, d, nu, rho, CD);
+            const double F2 = blend_F2(ks, wsf, d, nu);
+            const double beta = F1 * B1 + (1.0 - F1) * B2;
+            double nut;
+            {
+                const double kk = dmax(ks, 1e-10), ww = dmax(wsf, 1e-10), SS = dmax(S, 1e-10);
+                nut = A1 * kk / dmax(A1 * ww, F2 * SS);
+                nut = dmin(nut, 1.0e5 * musrc / dmax(rho, 1e-10));
+                nut = isfinite(nut) ? nut : 0.0;
+            }
+            const double Pk = dmin(nut * rho * S * S, 10.0 * BSTAR * rho * ks * wsf);
+            double Sk = Pk - rho * BSTAR * ks * wsf;
+            const double g1 = B1 / BSTAR - SW1 * KAP * KAP / sqrt(BSTAR);
+            const double g2 = B2 / BSTAR - SW2 * KAP * KAP / sqrt(BSTAR);
+            const double gam = F1 * g1 + (1.0 - F1) * g2;
+            double Sw = rho * gam * S * S - rho * beta * wsf * wsf + 2.0 * rho * (1.0 - F1) * SW2 / wsf * gdot;
+            Sk = finite_or(Sk, 0.0);
+            Sw = finite_or(Sw, 0.0);
+            const double ir = 1.0 / dmax(rho, 1e-10);
+            a.src[i * 3 + 0] = (real)(Sk * ir);
+            a.src[i * 3 + 1] = (real)(Sw * ir);
+            a.src[i * 3 + 2] = (real)beta;
+            a.nu_t[i] = (real)nut;
+            a.mu_t[i] = (real)(rho * nut);
+            // diffusivities with the transport's own F1 (molecular mu, raw k / omega)
+            const double wsaf = dmax(w, 1e-10);
+            const double CDt = dmax(2.0 * rho * SW2 / wsaf * gdot, 1e-10);
+            const double F1t = blend_F1(k, w, d, mu / dmax(rho, 1e-10), rho, CDt);
+            const double rnut = rho * nut;
+            const double Gk = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
+            const double Gw = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;
+            sv[s * NF + 10] = Gk;
+            sv[s * NF + 11] = Gw;
+            real* pk = a.pack + i * kPack;
+            pk[0] = (real)rho;
+            pk[1] = (real)sv[s * NF + 1];
+            pk[2] = (real)sv[s * NF + 2];
+            pk[3] = (real)sv[s * NF + 3];
+            pk[4] = (real)k;
+            pk[5] = (real)w;
+            pk[6] = (real)Gk;
+            pk[7] = (real)Gw;
+            // contravariant transport fluxes (raw gradients in the diffusion)
+            const real* ad = a.adj + i * 9;
+            const double ru[3] = {rho * sv[s * NF + 1], rho * sv[s * NF + 2], rho * sv[s * NF + 3]};
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const double am[3] = {(double)ad[m * 3 + 0], (double)ad[m * 3 + 1], (double)ad[m * 3 + 2]};
+                const double fr = am[0] * ru[0] + am[1] * ru[1] + am[2] * ru[2];
+                sF[(m * NS + s) * 4 + 0] = fr * k;
+                sF[(m * NS + s) * 4 + 1] = fr * w;
+                sF[(m * NS + s) * 4 + 2] =
+                    Gk * (am[0] * sg[s * 15 + 9] + am[1] * sg[s * 15 + 10] + am[2] * sg[s * 15 + 11]);
+                sF[(m * NS + s) * 4 + 3] =
+                    Gw * (am[0] * sg[s * 15 + 12] + am[1] * sg[s * 15 + 13] + am[2] * sg[s * 15 + 14]);
+            }
+        }
+        wave_sync();
+        // volume terms: -(D.F_conv + D.F_diff)/det for k and omega
+        for (int o = threadIdx.x & 63; o < 2 * NS; o += 64) {
+            const int s = o >> 1, f = o & 1;
+            double acc = 0.0;
+            for (int m = 0; m < 3; ++m)
+                for (int j = 0; j < NS; ++j)
+                    acc += (double)s_D[(m * NS + s) * NS + j] * (sF[(m * NS + j) * 4 + f] + sF[(m * NS + j) * 4 + 2 + f]);
+            a.vol[(cell * NS + s) * 2 + f] = (real)(-acc / (double)a.det[cell * NS + s]);
+        }
+        wave_sync();
+    }
+}
+
+// ------------------------------------------------------------------ per face record corrections
+template <int P, typename real>
+__global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
+    constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1;
+    __shared__ real s_E[2][5 * NFP * NS];
+    for (int i = threadIdx.x; i < 5 * NFP * NS; i += 256) {
+        s_E[0][i] = a.E0[i];
+        s_E[1][i] = a.E1[i];
+    }
+    __syncthreads();
+    const int64_t n = a.n_records * NFP;
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
+        const int64_t r = g / NFP;
+        const int fp = (int)(g % NFP);
+        const int64_t o = a.rec_owner[r];
+        const int slot = a.rec_oslot[r];
+        const int flags = a.rec_flags[r];
+        const real* E = s_E[o < a.n_prism ? 0 : 1] + (slot * NFP + fp) * NS;
+        double po[kPack], pn[kPack];
+#pragma unroll
+        for (int q = 0; q < kPack; ++q) po[q] = 0.0;
+        for (int s = 0; s < NS; ++s) {
+            const double e = (double)E[s];
+            const real* pk = a.pack + (o * NS + s) * kPack;
+#pragma unroll
+            for (int q = 0; q < kPack; ++q) po[q] += e * (double)pk[q];
+        }
+        if (flags & 1) {
+            // neighbour values at the owner's flux points through the owner's face slot
+            const int64_t sidx = o * 5 + slot;
+            const int src = (a.slot_mask[sidx] >> fp) & 1;
+            int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
+            const int nbc = a.slot_nb[sidx * 2 + src];
+            if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;
+            const int64_t nb = nbc >= 0 ? nbc : 0;
+#pragma unroll
+            for (int q = 0; q < kPack; ++q) pn[q] = 0.0;
+            for (int kk = 0; kk < a.knz; ++kk) {
+                const int sp = a.rows_idx[(int64_t)row * a.knz + kk];
+                const double wgt = (double)a.rows_w[(int64_t)row * a.knz + kk];
+                const real* pk = a.pack + (nb * NS + sp) * kPack;
+#pragma unroll
+                for (int q = 0; q < kPack; ++q) pn[q] += wgt * (double)pk[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kPack; ++q) pn[q] = po[q];
+            if (flags & 2) pn[4] = -po[4];  // k = 0 on no-slip walls: mirrored ghost
+        }
+        const double n0 = (double)a.rec_tn[r * 3 + 0], n1 = (double)a.rec_tn[r * 3 + 1], n2 = (double)a.rec_tn[r * 3 + 2];
+        const double mass = po[0] * (po[1] * n0 + po[2] * n1 + po[3] * n2);
+        double c[2];
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            const double phio = po[4 + f], phin = pn[4 + f];
+            const double up = mass >= 0.0 ? phio : phin;
+            const double gface = 0.5 * (po[6 + f] + pn[6 + f]);
+            c[f] = mass * (up - phio) - gface * (0.5 * (phio + phin) - phio);
+        }
+        a.crec[g * 2 + 0] = (real)c[0];
+        a.crec[g * 2 + 1] = (real)c[1];
+    }
+}
+
+// ------------------------------------------------------------------ gather, outlier zeroing, update
+template <int P, int CT, typename real>
+__global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
+    using O = SO<P, CT>;
+    constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP;
+    __shared__ double s_t[kWaves][2 * NS];
+    __shared__ double s_fld[kWaves][2 * NS];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
+    double* st = s_t[wid];
+    double* sf = s_fld[wid];
+    const int64_t nw = (int64_t)gridDim.x * kWaves;
+    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
+        const int64_t cell = a.cell_begin + ci;
+        if (t < 2 * NS) {
+            const int s = t >> 1, f = t & 1;
+            const int64_t i = cell * NS + s;
+            const double det = (double)a.det[i];
+            double acc = 0.0;
+            for (int q = 0; q < kInc; ++q) {
+                const int e = a.inc[cell * kInc + q];
+                if (e < 0) continue;
+                const int64_t r = e >> 3;
+                const int role = (e & 7) >= 6 ? 1 : 0;
+                const int code = (e & 7) - 6 * role;
+                const int axis = code >> 1;
+                const double g = (double)((code & 1) ? a.gR : a.gL)[a.coord_of_sp[axis * NS + s]];
+                const double cv = (double)a.crec[(r * NFP + a.fp_of_sp[axis * NS + s]) * 2 + f];
+                acc += (role == 0 ? -1.0 : 1.0) * g * cv / det;
+            }
+            const double rho = dmax((double)a.U[i * 5], 1e-10);
+            st[t] = ((double)a.vol[i * 2 + f] + acc) / rho;
+            sf[t] = fabs((double)(f == 0 ? a.k[i] : a.w[i]));
+        }
+        wave_sync();
+        // outlier zeroing per field (troubled_cell.py:339-408, n_vars = 1)
+        double tv = 0.0;
+        if (t < 2 * NS) {
+            const int f = t & 1;
+            const double x = fabs(st[t]);
+            double m1 = 0.0, m2 = 0.0, um = 0.0;
+            constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const double y = fabs(st[s2 * 2 + f]);
+                int lt = 0, le = 0;
+                for (int s3 = 0; s3 < NS; ++s3) {
+                    const double z = fabs(st[s3 * 2 + f]);
+                    lt += z < y;
+                    le += z <= y;
+                }
+                if (lt <= k1 && k1 < le) m1 = y;
+                if (lt <= k2 && k2 < le) m2 = y;
+                um += sf[s2 * 2 + f];
+            }
+            const double med = (NS % 2) ? m1 : 0.5 * (m1 + m2);
+            double ref = dmax(med, 1e-9 * um / NS);
+            ref = dmax(ref, 1e-300);
+            tv = x > 1e4 * ref ? 0.0 : st[t];
+            tv = isfinite(tv) ? tv : 0.0;
+        }
+        wave_sync();
+        if (t < 2 * NS) st[t] = tv;
+        wave_sync();
+        if (t < NS) {
+            const int s = t;
+            const int64_t i = cell * NS + s;
+            const double dt = (double)a.dt[i];
+            const double k = (double)a.k[i], w = (double)a.w[i];
+            const double wsafe = dmax(w, 1e-10);
+            const double beta = (double)a.src[i * 3 + 2];
+            double dk = (double)a.src[i * 3 + 0] / (1.0 + dt * BSTAR * wsafe);
+            double dw = (double)a.src[i * 3 + 1] / (1.0 + dt * beta * wsafe);
+            dk = finite_or(dk, 0.0) + st[s * 2 + 0];
+            dw = finite_or(dw, 0.0) + st[s * 2 + 1];
+            dk = finite_or(dk, 0.0);
+            dw = finite_or(dw, 0.0);
+            double kn = k + dt * dk, wn = w + dt * dw;
+            kn = isfinite(kn) ? kn : 1e-12;
+            wn = isfinite(wn) ? wn : 1e-12;
+            a.k[i] = (real)dmax(kn, 1e-12);
+            a.w[i] = (real)dmax(wn, 1e-12);
+        }
+        wave_sync();
+    }
+}
+
+// nearest wall node (FP64 brute force, wall nodes staged through LDS)
+template <typename real>
+__global__ void __launch_bounds__(256) k_wall_distance(const double* __restrict__ xyz, int64_t n,
+                                                       const double* __restrict__ wall, int n_wall, real* out) {
+    __shared__ double s_w[256 * 3];
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += (int64_t)gridDim.x * 256) {
+        const int64_t i = base + threadIdx.x;
+        double x = 0, y = 0, z = 0;
+        if (i < n) {
+            x = xyz[i * 3 + 0];
+            y = xyz[i * 3 + 1];
+            z = xyz[i * 3 + 2];
+        }
+        double best = 1e300;
+        for (int w0 = 0; w0 < n_wall; w0 += 256) {
+            __syncthreads();
+            const int m = n_wall - w0 < 256 ? n_wall - w0 : 256;
+            if (threadIdx.x < m) {
+                s_w[threadIdx.x * 3 + 0] = wall[(int64_t)(w0 + threadIdx.x) * 3 + 0];
+                s_w[threadIdx.x * 3 + 1] = wall[(int64_t)(w0 + threadIdx.x) * 3 + 1];
+                s_w[threadIdx.x * 3 + 2] = wall[(int64_t)(w0 + threadIdx.x) * 3 + 2];
+            }
+            __syncthreads();
+            for (int j = 0; j < m; ++j) {
+                const double dx = x - s_w[j * 3 + 0], dy = y - s_w[j * 3 + 1], dz = z - s_w[j * 3 + 2];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                best = d2 < best ? d2 : best;
+            }
+        }
+        if (i < n) out[i] = (real)sqrt(best);
+    }
+}
+
+template <int P, int CT, typename real>
+int launch_sst_cell(const SstArgs<real>& a, hipStream_t st) {
+    if (a.n_cells <= 0) return 0;
+    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
+                       st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int P, typename real>
+int launch_sst_face(const SstArgs<real>& a, hipStream_t st) {
+    if (a.n_records <= 0) return 0;
+    const int64_t n = a.n_records * (P + 1) * (P + 1);
+    const int64_t need = (n + 255) / 256;
+    hipLaunchKernelGGL((k_sst_face<P, real>), dim3((unsigned)(need < 8192 ? need : 8192)), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int P, int CT, typename real>
+int launch_sst_update(const SstArgs<real>& a, hipStream_t st) {
+    if (a.n_cells <= 0) return 0;
+    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL((k_sst_update<P, CT, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
+                       st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wall_distance(const double* sp_xyz, int64_t n_pts, const double* wall_xyz, int n_wall, void* d_out,
+                         int dtype, hipStream_t st) {
+    const int64_t need = (n_pts + 255) / 256;
+    const dim3 grid((unsigned)(need < 16384 ? (need > 0 ? need : 1) : 16384));
+    if (dtype == AFR_F32)
+        hipLaunchKernelGGL((k_wall_distance<float>), grid, dim3(256), 0, st, sp_xyz, n_pts, wall_xyz, n_wall,
+                           (float*)d_out);
+    else
+        hipLaunchKernelGGL((k_wall_distance<double>), grid, dim3(256), 0, st, sp_xyz, n_pts, wall_xyz, n_wall,
+                           (double*)d_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define AFR_SST_INST(P, CT, T)                                                  \
+    template int launch_sst_cell<P, CT, T>(const SstArgs<T>&, hipStream_t);     \
+    template int launch_sst_update<P, CT, T>(const SstArgs<T>&, hipStream_t);
+AFR_SST_INST(1, 0, float)
+AFR_SST_INST(2, 0, float)
+AFR_SST_INST(1, 1, float)
+AFR_SST_INST(2, 1, float)
+AFR_SST_INST(1, 0, double)
+AFR_SST_INST(2, 0, double)
+AFR_SST_INST(1, 1, double)
+AFR_SST_INST(2, 1, double)
+template int launch_sst_face<1, float>(const SstArgs<float>&, hipStream_t);
+template int launch_sst_face<2, float>(const SstArgs<float>&, hipStream_t);
+template int launch_sst_face<1, double>(const SstArgs<double>&, hipStream_t);
+template int launch_sst_face<2, double>(const SstArgs<double>&, hipStream_t);
+
+}  // namespace afr_sst

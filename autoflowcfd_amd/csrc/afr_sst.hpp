@@ -1,0 +1,60 @@
+// Shared declarations of the SST k-omega kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afr_sst {
+
+constexpr int kPack = 8;  // face pack: rho, u, v, w, k, omega, Gamma_k, Gamma_omega
+constexpr int kInc = 8;   // face-record incidences per cell (prism: 3 quads x 2 + 2 triangles)
+
+template <typename real>
+struct SstArgs {
+    int64_t cell_begin, n_cells;  // cells of one type
+    int64_t n_records;
+    const real* U;                // (C, NS, 5)
+    real* k;                      // (C, NS) in/out
+    real* w;                      // (C, NS) in/out
+    const real* d_wall;           // (C, NS)
+    const real* dt;               // (C, NS)
+    real* nu_t;                   // (C, NS) out
+    real* mu_t;                   // (C, NS) out: rho(U) nu_t
+    real* pack;                   // (C, NS, kPack) scratch
+    real* vol;                    // (C, NS, 2) scratch: volume transport terms of k, omega
+    real* src;                    // (C, NS, 3) scratch: Sk/rho, Somega/rho, beta blend
+    real* crec;                   // (F, NFP, 2) scratch: record corrections
+    const real* det;
+    const real* adj;              // (C, NS, 3, 3)
+    const real* D;                // (3, NS, NS) of the cell type
+    const real* E0;               // (5, NFP, NS) prism, slot order
+    const real* E1;               // (5, NFP, NS) tet, slot order
+    const int* slot_nb;
+    const int* slot_blk;
+    const int* slot_mask;
+    const int* rows_idx;
+    const real* rows_w;
+    int knz, n_rows;
+    int64_t n_prism;
+    const int* rec_owner;         // (F)
+    const int* rec_oslot;         // (F) owner slot index
+    const int* rec_flags;         // (F) bit 0: owner-primary interior record (neighbour sources), bit 1: WALL
+    const real* rec_tn;           // (F, 3) record unit normal
+    const int* inc;               // (C, kInc) record id * 8 + role * 6 + face code  (-1: none)
+    const real* gL;
+    const real* gR;
+    const int* fp_of_sp;
+    const int* coord_of_sp;
+    double mu_molecular;
+};
+
+template <int P, int CT, typename real>
+int launch_sst_cell(const SstArgs<real>& a, hipStream_t st);
+template <int P, typename real>
+int launch_sst_face(const SstArgs<real>& a, hipStream_t st);
+template <int P, int CT, typename real>
+int launch_sst_update(const SstArgs<real>& a, hipStream_t st);
+
+int launch_wall_distance(const double* sp_xyz, int64_t n_pts, const double* wall_xyz, int n_wall, void* d_out,
+                         int dtype, hipStream_t st);
+
+}  // namespace afr_sst

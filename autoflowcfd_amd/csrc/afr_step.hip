@@ -1,0 +1,206 @@
+// One steady pseudo-step on the device: local time step, SSP-RK3 stages with
+// positivity projection and modal filter, FP64 residual norm.
+//
+//   k_dt_local     src/autoflowcfd/core/fr_solver/cfl.py:12-147 (primitives with the
+//                  floors of src/autoflowcfd/core/fr_solver/state.py:76-97)
+//   k_rk_stage     src/autoflowcfd/core/time_integration/base.py:250-335 (Shu-Osher
+//                  stage), positivity.py:13-45, src/autoflowcfd/core/fr_solver/filter.py:29-43
+//   k_sq_partial / k_sum_partials
+//                  src/autoflowcfd/core/fr_solver/state.py:99-107 (RMS, FP64 accumulation,
+//                  fixed summation order: deterministic)
+#include <hip/hip_runtime.h>
+
+#include "../../include/afr.h"
+#include "afr_physics.hpp"
+#include "afr_step.hpp"
+
+using namespace afr;
+
+namespace afr_step {
+
+constexpr int kThreads = 256;
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ U, const real* __restrict__ mu_t,
+                                                       const real* __restrict__ det, const real* __restrict__ mfs,
+                                                       const real* __restrict__ vol, real* __restrict__ dt,
+                                                       int64_t n_cells, int ns, double mu, int order) {
+    const int64_t n = n_cells * ns;
+    const double cfl = 0.1, fa = 1.0 / (2 * order + 1), fv = fa * fa;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / ns;
+        double u5[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) u5[v] = (double)U[i * 5 + v];
+        const double rho = u5[0] > 1e-10 ? u5[0] : 1e-10;
+        const double ir = 1.0 / rho;
+        const double u = u5[1] * ir, v = u5[2] * ir, w = u5[3] * ir;
+        const double ke = 0.5 * (u * u + v * v + w * w);
+        double p = (u5[4] - rho * ke) * 0.4;
+        p = p > 1.0 ? p : 1.0;
+        double a2 = 1.4 * p / rho;
+        a2 = a2 > 1e-10 ? a2 : 1e-10;
+        double ws = sqrt(u * u + v * v + w * w) + sqrt(a2);
+        ws = ws > 1e-10 ? ws : 1e-10;
+        const double h = cbrt(fabs((double)vol[c]));
+        const double dta = cfl * fa * h / ws;
+        double me = mu + (mu_t ? (double)mu_t[i] : 0.0);
+        me = me > 1e-30 ? me : 1e-30;
+        const double dtv = 0.25 * cfl * fv * rho * h * h / me;
+        double den = (double)mfs[i] * ws;
+        den = den > 1e-300 ? den : 1e-300;
+        const double dtg = cfl * fabs((double)det[i]) / den;
+        double d = dta < dtv ? dta : dtv;
+        d = d < dtg ? d : dtg;
+        dt[i] = (real)d;
+    }
+}
+
+// U_out = c0 U0 + c1 Ua + c2 dt R, then positivity (rho >= 1e-6, |u| <= 1e4,
+// p >= p_floor) per solution point and the modal filter per cell.  One
+// wavefront per cell; Ua may alias U_out, U0 may alias U_out (each cell is
+// read completely before it is written).
+template <int NS, typename real>
+__global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const real* Ua, const real* __restrict__ R,
+                                                       const real* __restrict__ dt, real* U_out,
+                                                       const real* __restrict__ filt0, const real* __restrict__ filt1,
+                                                       int64_t n_cells, int64_t n_prism, double c0, double c1,
+                                                       double c2, double p_floor) {
+    constexpr int W = kThreads / 64, NSV = NS * 5;
+    __shared__ real s_f[2][NS * NS];
+    __shared__ real s_x[W][NSV];
+    for (int i = threadIdx.x; i < NS * NS; i += kThreads) {
+        s_f[0][i] = filt0[i];
+        s_f[1][i] = filt1[i];
+    }
+    __syncthreads();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
+    real* x = s_x[wid];
+    for (int64_t cell = (int64_t)blockIdx.x * W + wid; cell < n_cells; cell += (int64_t)gridDim.x * W) {
+        if (t < NS) {
+            const int64_t i = cell * NS + t;
+            const double d = (double)dt[i];
+            double q[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
+                if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
+                q[v] = (double)(real)val;  // stage value in the storage precision
+            }
+            const double rho = q[0] > 1e-6 ? q[0] : 1e-6;
+            double vx = q[1] / rho, vy = q[2] / rho, vz = q[3] / rho;
+            const double vm = sqrt(vx * vx + vy * vy + vz * vz);
+            double m1 = q[1], m2 = q[2], m3 = q[3];
+            if (vm > 1e4) {
+                const double f = 1e4 / vm;
+                vx *= f; vy *= f; vz *= f;
+                m1 = rho * vx; m2 = rho * vy; m3 = rho * vz;
+            }
+            const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+            double E = q[4];
+            if (0.4 * (E - ke) < p_floor) E = p_floor / 0.4 + ke;
+            x[t * 5 + 0] = (real)rho;
+            x[t * 5 + 1] = (real)m1;
+            x[t * 5 + 2] = (real)m2;
+            x[t * 5 + 3] = (real)m3;
+            x[t * 5 + 4] = (real)E;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const real* F = s_f[cell < n_prism ? 0 : 1];
+        for (int o = t; o < NSV; o += 64) {
+            const int s = o / 5, v = o % 5;
+            real acc = 0;
+            for (int j = 0; j < NS; ++j) acc += F[s * NS + j] * x[j * 5 + v];
+            U_out[cell * NSV + o] = acc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// partial sums of x^2 in FP64, one per workgroup
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_sq_partial(const real* __restrict__ x, int64_t n, double* part) {
+    __shared__ double s[kThreads];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double v = (double)x[i];
+        acc += v * v;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = kThreads / 2; m > 0; m >>= 1) {
+        if (threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(kThreads) k_sum_partials(const double* part, int n_part, double n_total, double* out) {
+    __shared__ double s[kThreads];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_part; i += kThreads) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = kThreads / 2; m > 0; m >>= 1) {
+        if (threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = n_total > 0 ? sqrt(s[0] / n_total) : 0.0;
+}
+
+static int grid_for(int64_t n, int per) {
+    int64_t b = (n + per - 1) / per;
+    return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+template <typename real>
+int launch_dt_local(const real* U, const real* mu_t, const real* det, const real* mfs, const real* vol, real* dt,
+                    int64_t n_cells, int ns, double mu, int order, hipStream_t st) {
+    hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U, mu_t,
+                       det, mfs, vol, dt, n_cells, ns, mu, order);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename real>
+int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const real* dt, real* U_out,
+                    const real* filt0, const real* filt1, int64_t n_cells, int64_t n_prism, double c0, double c1,
+                    double c2, double p_floor, hipStream_t st) {
+    const int grid = grid_for(n_cells, kThreads / 64);
+    if (ns == 8)
+        hipLaunchKernelGGL((k_rk_stage<8, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
+                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+    else if (ns == 27)
+        hipLaunchKernelGGL((k_rk_stage<27, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
+                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+    else
+        return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename real>
+int launch_norm(const real* x, int64_t n, double n_total, double* part, int n_part_max, double* out, hipStream_t st) {
+    int g = grid_for(n, kThreads * 8);
+    g = g < n_part_max ? g : n_part_max;
+    hipLaunchKernelGGL((k_sq_partial<real>), dim3(g), dim3(kThreads), 0, st, x, n, part);
+    if (hipGetLastError() != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, st, (const double*)part, g, n_total, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template int launch_dt_local<float>(const float*, const float*, const float*, const float*, const float*, float*,
+                                    int64_t, int, double, int, hipStream_t);
+template int launch_dt_local<double>(const double*, const double*, const double*, const double*, const double*,
+                                     double*, int64_t, int, double, int, hipStream_t);
+template int launch_rk_stage<float>(int, const float*, const float*, const float*, const float*, float*, const float*,
+                                    const float*, int64_t, int64_t, double, double, double, double, hipStream_t);
+template int launch_rk_stage<double>(int, const double*, const double*, const double*, const double*, double*,
+                                     const double*, const double*, int64_t, int64_t, double, double, double, double,
+                                     hipStream_t);
+template int launch_norm<float>(const float*, int64_t, double, double*, int, double*, hipStream_t);
+template int launch_norm<double>(const double*, int64_t, double, double*, int, double*, hipStream_t);
+
+}  // namespace afr_step

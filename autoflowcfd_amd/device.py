@@ -136,6 +136,13 @@ class DeviceMesh:
         self.work_umean = torch.empty((C, 5), dtype=dtype, device=self.device)
         self.work_rvol = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
         self.work_h = torch.empty((C,), dtype=dtype, device=self.device)
+        self.volume_dev = torch.as_tensor(self.volume, dtype=dtype).to(self.device)
+        self.work_r = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_s1 = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_s2 = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_dt = torch.empty((C, T.ns), dtype=dtype, device=self.device)
+        self.work_part = torch.zeros((4096,), dtype=torch.float64, device=self.device)
+        self.norm_dev = torch.zeros((1,), dtype=torch.float64, device=self.device)
 
         d = _lib.AfrMeshDesc()
         d.abi_version = _lib.ABI_VERSION
@@ -163,6 +170,9 @@ class DeviceMesh:
             d.ref_state[v] = float(self.ref_state[v])
         d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
         d.work_rvol, d.work_h = p(self.work_rvol), p(self.work_h)
+        d.metric_flux_scale, d.volume = p(self.metric_flux_scale), p(self.volume_dev)
+        d.work_r, d.work_s1, d.work_s2 = p(self.work_r), p(self.work_s1), p(self.work_s2)
+        d.work_dt, d.work_part = p(self.work_dt), p(self.work_part)
         self._desc = d
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
@@ -314,3 +324,37 @@ class DeviceMesh:
                                          ctypes.c_void_p(R.data_ptr()), float(mu), float(Pr), float(Pr_t),
                                          int(bool(suppress)), self._stream()), "afr_residual")
         return R
+
+    def local_time_step(self, U, mu_t=None, mu=1.8e-5, dt=None):
+        """Local time step (C, ns) of the conserved state U."""
+        self._check_state(U)
+        dt = torch.empty((self.n_cells, self.ns), dtype=self.dtype, device=self.device) if dt is None else dt
+        _lib.check(self.lib.afr_local_time_step(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                                ctypes.c_void_p(dt.data_ptr()), float(mu), self._stream()),
+                   "afr_local_time_step")
+        return dt
+
+    def rk_stage(self, U0, Ua, R, dt, U_out, c0, c1, c2):
+        """U_out = c0 U0 + c1 Ua + c2 dt R, positivity, modal filter."""
+        ua = ctypes.c_void_p(Ua.data_ptr()) if Ua is not None else None
+        _lib.check(self.lib.afr_rk_stage(self.handle, ctypes.c_void_p(U0.data_ptr()), ua, ctypes.c_void_p(R.data_ptr()),
+                                         ctypes.c_void_p(dt.data_ptr()), ctypes.c_void_p(U_out.data_ptr()),
+                                         float(c0), float(c1), float(c2), self._stream()), "afr_rk_stage")
+        return U_out
+
+    def residual_norm(self, R, n_vars_total=5, out=None):
+        """FP64 RMS norm of R as a (1,) device tensor."""
+        out = self.norm_dev if out is None else out
+        _lib.check(self.lib.afr_residual_norm_f64(self.handle, ctypes.c_void_p(R.data_ptr()), int(n_vars_total),
+                                                  ctypes.c_void_p(out.data_ptr()), self._stream()),
+                   "afr_residual_norm_f64")
+        return out
+
+    def ssp_rk3_step(self, U, mu_t=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, n_vars_total=5, norm=None):
+        """One steady pseudo-step of U in place; returns the (1,) FP64 device tensor of the RMS norm of R(U)."""
+        self._check_state(U)
+        norm = self.norm_dev if norm is None else norm
+        _lib.check(self.lib.afr_ssp_rk3_step(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                             float(mu), float(Pr), float(Pr_t), int(n_vars_total),
+                                             ctypes.c_void_p(norm.data_ptr()), self._stream()), "afr_ssp_rk3_step")
+        return norm

@@ -71,6 +71,11 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   work_umean (C, 5)                  scratch for the per-cell mean |U| (outlier reference)
  *   work_rvol  (C, ns, 5)              scratch for the viscous volume term
  *   work_h     (C)                     scratch for the interior-penalty length mean(det)^(1/3)
+ *   metric_flux_scale (C, ns)          sum_m |adj(J)[m, :]| (geometric time-step limit)
+ *   volume     (C)                     cell volumes (GL quadrature)
+ *   work_r, work_s1, work_s2 (C, ns, 5) pseudo-step scratch: residual, stage states
+ *   work_dt    (C, ns)                 local time step
+ *   work_part  (4096) FP64             partial sums of the residual norm
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
@@ -100,6 +105,9 @@ typedef struct AfrMeshDesc {
     void *work_umean;
     void *work_rvol;
     void *work_h;
+    const void *metric_flux_scale, *volume;
+    void *work_r, *work_s1, *work_s2, *work_dt;
+    double *work_part;
 } AfrMeshDesc;
 
 typedef struct AfrMesh AfrMesh;
@@ -175,6 +183,31 @@ int afr_viscous_residual(const AfrMesh *mesh, const void *U, const void *mu_t, v
  * (src/autoflowcfd/core/fr_solver/solver.py:360-420).  mu_t may be NULL. */
 int afr_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu, double Pr,
                  double Pr_t, int32_t suppress, void *stream);
+
+/* Local time step dt (C, ns) of the conserved state U (C, ns, 5):
+ * min(advective, viscous, geometric) limits with CFL 0.1 and the order factors
+ * 1/(2P+1), 1/(2P+1)^2 (src/autoflowcfd/core/fr_solver/cfl.py:12-147); mu_t may be NULL. */
+int afr_local_time_step(const AfrMesh *mesh, const void *U, const void *mu_t, void *dt, double mu, void *stream);
+
+/* One Shu-Osher stage U_out = c0 U0 + c1 Ua + c2 dt R followed by the positivity
+ * projection (p_floor 1) and the modal filter of the first 5 variables
+ * (src/autoflowcfd/core/time_integration/base.py:250-335). U_out may alias U0 or Ua. */
+int afr_rk_stage(const AfrMesh *mesh, const void *U0, const void *Ua, const void *R, const void *dt, void *U_out,
+                 double c0, double c1, double c2, void *stream);
+
+/* RMS norm sqrt(sum R^2 / (C ns n_vars_total)) accumulated in FP64 into out[0]
+ * (a device pointer; src/autoflowcfd/core/fr_solver/state.py:99-107).  n_vars_total
+ * counts the turbulence components of the reference state (7 with SST), whose
+ * residual entries are zero. */
+int afr_residual_norm_f64(const AfrMesh *mesh, const void *R, int32_t n_vars_total, double *out, void *stream);
+
+/* One steady SSP-RK3 pseudo-step of the Navier-Stokes state U (C, ns, 5), in place:
+ * dt_local, R0 = residual(U), three stages with positivity and filter, and the RMS
+ * norm of R0 into norm[0] (device pointer), as src/autoflowcfd/core/fr_solver/step.py:15-221.
+ * mu_t (C, ns) is frozen over the step (may be NULL); mu == 0 with mu_t == NULL skips
+ * the viscous terms.  Uses work_r, work_s1, work_s2, work_dt, work_part. */
+int afr_ssp_rk3_step(const AfrMesh *mesh, void *U, const void *mu_t, double mu, double Pr, double Pr_t,
+                     int32_t n_vars_total, double *norm, void *stream);
 
 #ifdef __cplusplus
 }

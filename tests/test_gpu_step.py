@@ -1,0 +1,74 @@
+"""GPU parity: local time step, one SSP-RK3 pseudo-step (positivity, filter) and the
+FP64 residual norm vs the FP64 oracle (oracle/time_step.py).
+
+Tolerances: FP64 device path 1e-10 relative (1e-7 with P2 tets) for the state
+increment U_new - U and the norm, 1e-12 for dt.  FP32 path: dt 1e-6, norm 1e-5;
+the FP32 state carries 6e-8 relative rounding, which is ~1e-4 of an increment of
+relative size 1e-3, so the updated state is checked to 1e-6 of |U| and the
+increment to 1e-3 of its own norm.
+"""
+
+import numpy as np
+import pytest
+
+from cases import freestream, mixed_mesh, smooth_state, tet_box, wall_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(mesh, order, dtype, amp=1e-2, seed=0):
+    import torch
+    from autoflowcfd_amd.device import DeviceMesh, bc_params_table
+    from oracle import fr_ops
+    from oracle.faces import build_flat_faces
+    from oracle.geometry import Geometry
+    Q0 = freestream(0.15)
+    bcp = bc_params_table(Q0)
+    ops = fr_ops.generate_operators(order)
+    g = Geometry(mesh, ops)
+    fl = build_flat_faces(mesh, ops)
+    U = smooth_state(g.sps_xyz, Q0, amp=amp, seed=seed)
+    if dtype == torch.float32:
+        U = U.astype(np.float32).astype(np.float64)
+    dm = DeviceMesh(mesh, order, dtype=dtype, bc_params=bcp)
+    return dm, ops, g, fl, {c: bcp[c] for c in range(8)}, U
+
+
+CASES = [("wallbox", lambda: wall_box(seed=2), 1), ("wallbox", lambda: wall_box(seed=2), 2),
+         ("mixed", mixed_mesh, 2), ("tetbox", lambda: tet_box(3, seed=1), 1)]
+
+
+@pytest.mark.parametrize("name,make,order", CASES)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_pseudo_step_matches_oracle(name, make, order, prec):
+    import torch
+    from oracle.time_step import local_time_step, ns_step, update_primitives
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    print(f"[gpu] ssp-rk3 step {prec} {name} P{order}", flush=True)
+    mesh = make()
+    dm, ops, g, fl, bcd, U = _setup(mesh, order, dtype)
+    mu = 1.8e-5
+    mut = mu * np.random.default_rng(3).uniform(0, 20, size=U.shape[:2])
+    if dtype == torch.float32:
+        mut = mut.astype(np.float32).astype(np.float64)
+    dt_ref = local_time_step(update_primitives(U), g, order, mu, mut)
+    U_ref, norm_ref, _ = ns_step(U, g, ops, fl, bcd, order, mu=mu, mu_t=mut)
+    Ud = torch.as_tensor(U, dtype=dtype, device="cuda").contiguous()
+    mtd = torch.as_tensor(mut, dtype=dtype, device="cuda").contiguous()
+    dt = dm.local_time_step(Ud, mtd, mu=mu).double().cpu().numpy()
+    norm = float(dm.ssp_rk3_step(Ud, mtd, mu=mu).item())
+    U_new = Ud.double().cpu().numpy()
+    tight = prec == "f64"
+    tol = (1e-7 if (mesh.n_tet and order == 2) else 1e-10) if tight else 1e-5
+    e_dt = np.abs(dt - dt_ref).max() / np.abs(dt_ref).max()
+    dU_ref = U_ref - U
+    e_u = np.linalg.norm((U_new - U) - dU_ref) / np.linalg.norm(dU_ref)
+    e_n = abs(norm - norm_ref) / norm_ref
+    e_s = np.linalg.norm(U_new - U_ref) / np.linalg.norm(U_ref)
+    assert e_dt < (1e-12 if tight else 1e-6), f"dt {e_dt:.3e}"
+    assert e_n < tol, f"norm {e_n:.3e}"
+    if tight:
+        assert e_u < tol, f"state increment {e_u:.3e}"
+    else:
+        assert e_s < 1e-6, f"state {e_s:.3e}"
+        assert e_u < 1e-3, f"state increment {e_u:.3e}"
