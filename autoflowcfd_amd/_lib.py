@@ -27,6 +27,11 @@ class AfrMeshDesc(ctypes.Structure):
         ("work_rvol", c_vp), ("work_h", c_vp),
         ("metric_flux_scale", c_vp), ("volume", c_vp),
         ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
+        ("n_records", ctypes.c_int64), ("rec_owner", c_vp), ("rec_oslot", c_vp), ("rec_flags", c_vp),
+        ("rec_tn", c_vp), ("cell_inc", c_vp),
+        ("work_pack", c_vp), ("work_tvol", c_vp), ("work_tsrc", c_vp), ("work_crec", c_vp),
+        ("n_wall_rec", ctypes.c_int64), ("wrec_owner", c_vp), ("wrec_oslot", c_vp), ("wrec_geom", c_vp),
+        ("w1d", c_vp), ("work_fpart", c_vp),
     ]
 
 
@@ -39,6 +44,7 @@ class AfrGeomDesc(ctypes.Structure):
         ("slot_tn64", c_vp),
         ("det", c_vp), ("adj", c_vp), ("adj_fine", c_vp), ("gcl", c_vp), ("metric_flux_scale", c_vp),
         ("slot_adjrow", c_vp), ("volume", c_vp), ("det_mean", c_vp), ("slot_fallback", c_vp), ("bad_det", c_vp),
+        ("sps_xyz", c_vp),
     ]
 
 
@@ -64,6 +70,12 @@ SIGNATURES = {
     "afr_residual_norm_f64": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_vp, c_vp]),
+    "afr_wall_distance": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_vp]),
+    "afr_sst_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
+    "afr_sst_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int32, c_vp, c_vp]),
+    "afr_local_time_step_nut": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
+    "afr_forces_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
 }
 
 _lib = None

@@ -1,0 +1,31 @@
+// Shared declarations of the wall force kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afr_forces {
+
+template <typename real>
+struct ForceArgs {
+    int64_t n_wall, n_prism;
+    const real* U;
+    const real* mu_t;       // may be null
+    const real* det;
+    const real* adj;
+    const real* D0;         // prism (3, NS, NS)
+    const real* D1;         // tet
+    const real* E0;         // prism (5, NFP, NS) slot order
+    const real* E1;         // tet
+    const int* wrec_owner;  // (W)
+    const int* wrec_oslot;  // (W)
+    const double* wrec_geom;  // (W, 4): unit normal, record area
+    const double* w1d;      // (order+1) Gauss-Legendre weights
+    double mu;
+    double* part;           // (W, 6) scratch
+    double* out;            // (6): pressure force, viscous force
+};
+
+template <int P, typename real>
+int launch_wall_forces(const ForceArgs<real>& a, hipStream_t st);
+
+}  // namespace afr_forces

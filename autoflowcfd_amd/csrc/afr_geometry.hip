@@ -57,6 +57,24 @@ __device__ inline void jac_tet(const double* p, double a, double b, double c, do
     }
 }
 
+// physical position of reference point (a, b, c) (collapsed prism / tet maps)
+__device__ inline void map_prism(const double* p, double a, double b, double c, double x[3]) {
+    const double r = (1.0 + a) * (1.0 - b) / 2.0 - 1.0, s = b;
+    const double l1 = -(r + s) / 2.0, l2 = (1.0 + r) / 2.0, l3 = (1.0 + s) / 2.0;
+    for (int d = 0; d < 3; ++d) {
+        const double bot = l1 * p[0 * 3 + d] + l2 * p[1 * 3 + d] + l3 * p[2 * 3 + d];
+        const double top = l1 * p[3 * 3 + d] + l2 * p[4 * 3 + d] + l3 * p[5 * 3 + d];
+        x[d] = 0.5 * (1.0 - c) * bot + 0.5 * (1.0 + c) * top;
+    }
+}
+
+__device__ inline void map_tet(const double* p, double a, double b, double c, double x[3]) {
+    const double t = c, s = (1.0 + b) * (1.0 - c) / 2.0 - 1.0, r = -(1.0 + a) * (s + t) / 2.0 - 1.0;
+    for (int d = 0; d < 3; ++d)
+        x[d] = -(1.0 + r + s + t) / 2.0 * p[d] + (1.0 + r) / 2.0 * p[3 + d] + (1.0 + s) / 2.0 * p[6 + d] +
+               (1.0 + t) / 2.0 * p[9 + d];
+}
+
 // adj[m][i] = det * (J^{-1})[m][i]
 __device__ inline double det_adj(const double J[3][3], double A[9]) {
     A[0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -93,6 +111,7 @@ struct GeoArgs {
     real* adjrow;
     int* fallback;
     int* bad;
+    double* xyz;  // (C, NS, 3) FP64 solution-point positions, may be null
     int nslot;
 };
 
@@ -151,6 +170,12 @@ __global__ void __launch_bounds__(kT) k_metrics(GeoArgs<real> g) {
                 for (int mm = 0; mm < 3; ++mm)
                     m += sqrt(A[mm * 3] * A[mm * 3] + A[mm * 3 + 1] * A[mm * 3 + 1] + A[mm * 3 + 2] * A[mm * 3 + 2]);
                 g.mfs[gc * NS + s] = (real)m;
+                if (g.xyz) {
+                    double X[3];
+                    if (CT == 0) map_prism(s_x + c * 18, x[0], x[1], x[2], X);
+                    else map_tet(s_x + c * 18, x[0], x[1], x[2], X);
+                    for (int d = 0; d < 3; ++d) g.xyz[(gc * NS + s) * 3 + d] = X[d];
+                }
                 for (int k = 0; k < 9; ++k) {
                     s_a[(c * NS + s) * 9 + k] = A[k];
                     g.adj[(gc * NS + s) * 9 + k] = (real)A[k];
@@ -244,6 +269,7 @@ int launch(const AfrGeomDesc* d, hipStream_t st) {
         g.adjrow = (real*)d->slot_adjrow;
         g.fallback = d->slot_fallback;
         g.bad = d->bad_det;
+        g.xyz = d->sps_xyz;
         g.nslot = ct == 0 ? 5 : 4;
         int64_t tiles = (n + O::CPB - 1) / O::CPB;
         int grid = (int)(tiles < 8192 ? tiles : 8192);

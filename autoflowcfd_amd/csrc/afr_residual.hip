@@ -20,12 +20,15 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
 #include "afr_prism.hpp"
 #include "afr_viscous.hpp"
 #include "afr_step.hpp"
+#include "afr_sst.hpp"
+#include "afr_forces.hpp"
 
 using namespace afr;
 
@@ -620,7 +623,8 @@ int ns_of(const AfrMeshDesc& d) { return (d.order + 1) * (d.order + 1) * (d.orde
 
 }  // namespace
 
-int afr_local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t, void* dt, double mu, void* stream) {
+static int local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t, void* dt, double mu, int nut_mode,
+                           void* stream) {
     if (!mesh || !U || !dt) return fail("afr_local_time_step: null argument");
     const AfrMeshDesc& d = mesh->d;
     if (!d.metric_flux_scale || !d.volume) return fail("afr_local_time_step: mesh has no metric_flux_scale/volume");
@@ -628,12 +632,20 @@ int afr_local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t, vo
     const int rc = d.dtype == AFR_F32
                        ? afr_step::launch_dt_local<float>((const float*)U, (const float*)mu_t, (const float*)d.det,
                                                           (const float*)d.metric_flux_scale, (const float*)d.volume,
-                                                          (float*)dt, d.n_cells, ns_of(d), mu, d.order, st)
+                                                          (float*)dt, d.n_cells, ns_of(d), mu, d.order, nut_mode, st)
                        : afr_step::launch_dt_local<double>((const double*)U, (const double*)mu_t, (const double*)d.det,
                                                            (const double*)d.metric_flux_scale,
                                                            (const double*)d.volume, (double*)dt, d.n_cells, ns_of(d),
-                                                           mu, d.order, st);
+                                                           mu, d.order, nut_mode, st);
     return rc ? fail("k_dt_local launch failed") : 0;
+}
+
+int afr_local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t, void* dt, double mu, void* stream) {
+    return local_time_step(mesh, U, mu_t, dt, mu, 0, stream);
+}
+
+int afr_local_time_step_nut(const AfrMesh* mesh, const void* U, const void* nu_t, void* dt, double mu, void* stream) {
+    return local_time_step(mesh, U, nu_t, dt, mu, 1, stream);
 }
 
 int afr_rk_stage(const AfrMesh* mesh, const void* U0, const void* Ua, const void* R, const void* dt, void* U_out,
@@ -671,17 +683,14 @@ int afr_residual_norm_f64(const AfrMesh* mesh, const void* R, int32_t n_vars_tot
     return rc ? fail("residual norm launch failed") : 0;
 }
 
-int afr_ssp_rk3_step(const AfrMesh* mesh, void* U, const void* mu_t, double mu, double Pr, double Pr_t,
-                     int32_t n_vars_total, double* norm, void* stream) {
-    if (!mesh || !U || !norm) return fail("afr_ssp_rk3_step: null argument");
+static int rk3_stages(const AfrMesh* mesh, void* U, const void* mu_t, double mu, double Pr, double Pr_t,
+                      int32_t n_vars_total, double* norm, void* stream) {
     const AfrMeshDesc& d = mesh->d;
-    if (!step_ready(d)) return fail("afr_ssp_rk3_step: mesh has no pseudo-step work buffers");
     const bool visc = mu != 0.0 || mu_t != nullptr;
     auto rhs = [&](const void* X, void* R) -> int {
         return visc ? afr_residual(mesh, X, mu_t, R, mu, Pr, Pr_t, 1, stream)
                     : afr_inviscid_residual(mesh, X, R, 1, stream);
     };
-    if (afr_local_time_step(mesh, U, mu_t, d.work_dt, mu, stream)) return -1;
     if (rhs(U, d.work_r)) return -1;
     if (afr_residual_norm_f64(mesh, d.work_r, n_vars_total, norm, stream)) return -1;
     // Shu-Osher SSP-RK3: U1 = U0 + dt L0; U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L1; U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L2
@@ -690,6 +699,154 @@ int afr_ssp_rk3_step(const AfrMesh* mesh, void* U, const void* mu_t, double mu, 
     if (afr_rk_stage(mesh, U, d.work_s1, d.work_r, d.work_dt, d.work_s2, 0.75, 0.25, 0.25, stream)) return -1;
     if (rhs(d.work_s2, d.work_r)) return -1;
     return afr_rk_stage(mesh, U, d.work_s2, d.work_r, d.work_dt, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, stream);
+}
+
+int afr_ssp_rk3_step(const AfrMesh* mesh, void* U, const void* mu_t, double mu, double Pr, double Pr_t,
+                     int32_t n_vars_total, double* norm, void* stream) {
+    if (!mesh || !U || !norm) return fail("afr_ssp_rk3_step: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!step_ready(d)) return fail("afr_ssp_rk3_step: mesh has no pseudo-step work buffers");
+    if (afr_local_time_step(mesh, U, mu_t, d.work_dt, mu, stream)) return -1;
+    return rk3_stages(mesh, U, mu_t, mu, Pr, Pr_t, n_vars_total, norm, stream);
+}
+
+int afr_sst_rk3_step(const AfrMesh* mesh, void* U, void* k, void* omega, const void* d_wall, void* nu_t, void* mu_t,
+                     double mu, double Pr, double Pr_t, int32_t n_vars_total, double* norm, void* stream) {
+    if (!mesh || !U || !k || !omega || !d_wall || !nu_t || !mu_t || !norm) return fail("afr_sst_rk3_step: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!step_ready(d)) return fail("afr_sst_rk3_step: mesh has no pseudo-step work buffers");
+    if (afr_local_time_step_nut(mesh, U, nu_t, d.work_dt, mu, stream)) return -1;
+    if (afr_sst_step(mesh, U, k, omega, d_wall, d.work_dt, nu_t, mu_t, mu, stream)) return -1;
+    return rk3_stages(mesh, U, mu_t, mu, Pr, Pr_t, n_vars_total, norm, stream);
+}
+
+int afr_wall_distance(const double* sp_xyz, int64_t n_pts, const double* wall_xyz, int32_t n_wall, void* d_out,
+                      int32_t dtype, void* stream) {
+    if (!sp_xyz || !d_out || n_pts < 0 || n_wall < 1 || !wall_xyz) return fail("afr_wall_distance: bad argument");
+    if (dtype != AFR_F32 && dtype != AFR_F64) return fail("afr_wall_distance: bad dtype");
+    return afr_sst::launch_wall_distance(sp_xyz, n_pts, wall_xyz, n_wall, d_out, dtype, (hipStream_t)stream)
+               ? fail("k_wall_distance launch failed")
+               : 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename real>
+int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* dw, const void* dt, void* nut,
+            void* mut, double mu, hipStream_t st) {
+    afr_sst::SstArgs<real> a;
+    a.n_records = d.n_records;
+    a.U = (const real*)U;
+    a.k = (real*)k;
+    a.w = (real*)w;
+    a.d_wall = (const real*)dw;
+    a.dt = (const real*)dt;
+    a.nu_t = (real*)nut;
+    a.mu_t = (real*)mut;
+    a.pack = (real*)d.work_pack;
+    a.vol = (real*)d.work_tvol;
+    a.src = (real*)d.work_tsrc;
+    a.crec = (real*)d.work_crec;
+    a.det = (const real*)d.det;
+    a.adj = (const real*)d.adj;
+    a.E0 = (const real*)d.E[0];
+    a.E1 = (const real*)d.E[1];
+    a.slot_nb = d.slot_nb;
+    a.slot_blk = d.slot_blk;
+    a.slot_mask = d.slot_mask;
+    a.rows_idx = d.rows_idx;
+    a.rows_w = (const real*)d.rows_w;
+    a.knz = d.rows_knz;
+    a.n_rows = d.n_rows;
+    a.n_prism = d.n_prism;
+    a.rec_owner = d.rec_owner;
+    a.rec_oslot = d.rec_oslot;
+    a.rec_flags = d.rec_flags;
+    a.rec_tn = (const real*)d.rec_tn;
+    a.inc = d.cell_inc;
+    a.gL = (const real*)d.gL;
+    a.gR = (const real*)d.gR;
+    a.fp_of_sp = d.fp_of_sp;
+    a.coord_of_sp = d.coord_of_sp;
+    a.mu_molecular = mu;
+    const int P = d.order;
+    for (int phase = 0; phase < 3; ++phase) {
+        if (phase == 1) {
+            const int rc = P == 1 ? afr_sst::launch_sst_face<1, real>(a, st) : afr_sst::launch_sst_face<2, real>(a, st);
+            if (rc) return fail("k_sst_face launch failed");
+            continue;
+        }
+        for (int ct = 0; ct < 2; ++ct) {
+            afr_sst::SstArgs<real> b = a;
+            b.cell_begin = ct == 0 ? 0 : d.n_prism;
+            b.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
+            b.D = (const real*)d.D[ct];
+            if (b.n_cells <= 0) continue;
+            int rc;
+            if (phase == 0)
+                rc = P == 1 ? (ct == 0 ? afr_sst::launch_sst_cell<1, 0, real>(b, st) : afr_sst::launch_sst_cell<1, 1, real>(b, st))
+                            : (ct == 0 ? afr_sst::launch_sst_cell<2, 0, real>(b, st) : afr_sst::launch_sst_cell<2, 1, real>(b, st));
+            else
+                rc = P == 1 ? (ct == 0 ? afr_sst::launch_sst_update<1, 0, real>(b, st)
+                                       : afr_sst::launch_sst_update<1, 1, real>(b, st))
+                            : (ct == 0 ? afr_sst::launch_sst_update<2, 0, real>(b, st)
+                                       : afr_sst::launch_sst_update<2, 1, real>(b, st));
+            if (rc) return fail("SST kernel launch failed");
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const void* d_wall, const void* dt,
+                 void* nu_t, void* mu_t, double mu_molecular, void* stream) {
+    if (!mesh || !U || !k || !omega || !d_wall || !dt || !nu_t || !mu_t) return fail("afr_sst_step: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (d.n_records <= 0 || !d.rec_owner || !d.rec_oslot || !d.rec_flags || !d.rec_tn || !d.cell_inc ||
+        !d.work_pack || !d.work_tvol || !d.work_tsrc || !d.work_crec || !d.adj)
+        return fail("afr_sst_step: mesh has no SST record tables / work buffers");
+    hipStream_t st = (hipStream_t)stream;
+    return d.dtype == AFR_F32 ? run_sst<float>(d, U, k, omega, d_wall, dt, nu_t, mu_t, mu_molecular, st)
+                              : run_sst<double>(d, U, k, omega, d_wall, dt, nu_t, mu_t, mu_molecular, st);
+}
+
+int afr_forces_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double* out, void* stream) {
+    if (!mesh || !U || !out) return fail("afr_forces_f64: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (d.n_wall_rec < 0 || (d.n_wall_rec > 0 && (!d.wrec_owner || !d.wrec_oslot || !d.wrec_geom || !d.w1d ||
+                                                  !d.work_fpart || !d.adj)))
+        return fail("afr_forces_f64: mesh has no wall-record tables");
+    hipStream_t st = (hipStream_t)stream;
+    auto fill = [&](auto* tag) {
+        using real = std::remove_pointer_t<decltype(tag)>;
+        afr_forces::ForceArgs<real> a;
+        a.n_wall = d.n_wall_rec;
+        a.n_prism = d.n_prism;
+        a.U = (const real*)U;
+        a.mu_t = (const real*)mu_t;
+        a.det = (const real*)d.det;
+        a.adj = (const real*)d.adj;
+        a.D0 = (const real*)d.D[0];
+        a.D1 = (const real*)d.D[1];
+        a.E0 = (const real*)d.E[0];
+        a.E1 = (const real*)d.E[1];
+        a.wrec_owner = d.wrec_owner;
+        a.wrec_oslot = d.wrec_oslot;
+        a.wrec_geom = d.wrec_geom;
+        a.w1d = d.w1d;
+        a.mu = mu;
+        a.part = d.work_fpart;
+        a.out = out;
+        return d.order == 1 ? afr_forces::launch_wall_forces<1, real>(a, st)
+                            : afr_forces::launch_wall_forces<2, real>(a, st);
+    };
+    const int rc = d.dtype == AFR_F32 ? fill((float*)nullptr) : fill((double*)nullptr);
+    return rc ? fail("wall force kernel launch failed") : 0;
 }
 
 }  // extern "C"

@@ -321,9 +321,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
             for (int q = 0; q < kInc; ++q) {
                 const int e = a.inc[cell * kInc + q];
                 if (e < 0) continue;
-                const int64_t r = e >> 3;
-                const int role = (e & 7) >= 6 ? 1 : 0;
-                const int code = (e & 7) - 6 * role;
+                const int64_t r = e >> 4;
+                const int role = (e >> 3) & 1;
+                const int code = e & 7;
                 const int axis = code >> 1;
                 const double g = (double)((code & 1) ? a.gR : a.gL)[a.coord_of_sp[axis * NS + s]];
                 const double cv = (double)a.crec[(r * NFP + a.fp_of_sp[axis * NS + s]) * 2 + f];

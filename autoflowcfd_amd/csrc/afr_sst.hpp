@@ -39,7 +39,7 @@ struct SstArgs {
     const int* rec_oslot;         // (F) owner slot index
     const int* rec_flags;         // (F) bit 0: owner-primary interior record (neighbour sources), bit 1: WALL
     const real* rec_tn;           // (F, 3) record unit normal
-    const int* inc;               // (C, kInc) record id * 8 + role * 6 + face code  (-1: none)
+    const int* inc;               // (C, kInc) record id * 16 + role * 8 + face code  (-1: none)
     const real* gL;
     const real* gR;
     const int* fp_of_sp;

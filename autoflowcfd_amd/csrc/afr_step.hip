@@ -24,7 +24,7 @@ template <typename real>
 __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ U, const real* __restrict__ mu_t,
                                                        const real* __restrict__ det, const real* __restrict__ mfs,
                                                        const real* __restrict__ vol, real* __restrict__ dt,
-                                                       int64_t n_cells, int ns, double mu, int order) {
+                                                       int64_t n_cells, int ns, double mu, int order, int nut_mode) {
     const int64_t n = n_cells * ns;
     const double cfl = 0.1, fa = 1.0 / (2 * order + 1), fv = fa * fa;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
         ws = ws > 1e-10 ? ws : 1e-10;
         const double h = cbrt(fabs((double)vol[c]));
         const double dta = cfl * fa * h / ws;
-        double me = mu + (mu_t ? (double)mu_t[i] : 0.0);
+        // nut_mode: the eddy-viscosity array holds nu_t and mu_t = rho nu_t
+        double me = mu + (mu_t ? (nut_mode ? rho * (double)mu_t[i] : (double)mu_t[i]) : 0.0);
         me = me > 1e-30 ? me : 1e-30;
         const double dtv = 0.25 * cfl * fv * rho * h * h / me;
         double den = (double)mfs[i] * ws;
@@ -159,9 +160,9 @@ static int grid_for(int64_t n, int per) {
 
 template <typename real>
 int launch_dt_local(const real* U, const real* mu_t, const real* det, const real* mfs, const real* vol, real* dt,
-                    int64_t n_cells, int ns, double mu, int order, hipStream_t st) {
+                    int64_t n_cells, int ns, double mu, int order, int nut_mode, hipStream_t st) {
     hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U, mu_t,
-                       det, mfs, vol, dt, n_cells, ns, mu, order);
+                       det, mfs, vol, dt, n_cells, ns, mu, order, nut_mode);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -192,9 +193,9 @@ int launch_norm(const real* x, int64_t n, double n_total, double* part, int n_pa
 }
 
 template int launch_dt_local<float>(const float*, const float*, const float*, const float*, const float*, float*,
-                                    int64_t, int, double, int, hipStream_t);
+                                    int64_t, int, double, int, int, hipStream_t);
 template int launch_dt_local<double>(const double*, const double*, const double*, const double*, const double*,
-                                     double*, int64_t, int, double, int, hipStream_t);
+                                     double*, int64_t, int, double, int, int, hipStream_t);
 template int launch_rk_stage<float>(int, const float*, const float*, const float*, const float*, float*, const float*,
                                     const float*, int64_t, int64_t, double, double, double, double, hipStream_t);
 template int launch_rk_stage<double>(int, const double*, const double*, const double*, const double*, double*,

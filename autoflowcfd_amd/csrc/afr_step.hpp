@@ -9,7 +9,7 @@ constexpr int kNormParts = 4096;  // capacity of the FP64 partial-sum buffer
 
 template <typename real>
 int launch_dt_local(const real* U, const real* mu_t, const real* det, const real* mfs, const real* vol, real* dt,
-                    int64_t n_cells, int ns, double mu, int order, hipStream_t st);
+                    int64_t n_cells, int ns, double mu, int order, int nut_mode, hipStream_t st);
 
 template <typename real>
 int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const real* dt, real* U_out,

@@ -69,9 +69,12 @@ class DeviceMesh:
     """One mesh at one order on one GPU."""
 
     def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, ref_state=None,
-                 slots=None, metrics="gpu"):
-        """metrics: "gpu" (FP64 precompute on the device) or "host" (NumPy FP64, small meshes)."""
+                 slots=None, metrics="gpu", turbulence=None):
+        """metrics: "gpu" (FP64 precompute on the device) or "host" (NumPy FP64, small meshes).
+        turbulence: None or "SST" (face-record tables, wall distance and SST work buffers)."""
         self.lib = _lib.load()
+        self.turbulence = turbulence
+        self.sps_xyz = None
         self.mesh, self.order, self.dtype, self.device = mesh, order, dtype, torch.device(device)
         T = tables(order)
         self.T = T
@@ -121,6 +124,14 @@ class DeviceMesh:
             self.det_mean = np.asarray(geo["det_mean"], dtype=np.float64)
             self.slot_fb = devi(sl.fallback)
             self.slot_adjrow = dev(sl.adjrow)
+            if turbulence is not None:
+                from .geometry import map_ref
+                xyz = np.zeros((C, T.ns, 3))
+                if mesh.n_prism:
+                    xyz[:mesh.n_prism] = map_ref(True, T.sps, mesh.nodes[mesh.prism_conn])
+                if mesh.n_tet:
+                    xyz[mesh.n_prism:] = map_ref(False, T.sps, mesh.nodes[mesh.tet_conn])
+                self.sps_xyz = torch.as_tensor(xyz).to(self.device)
         self.c2f = [dev(T.c2f[ct]) for ct in (0, 1)]
         self.op = [dev(T.Op[ct]) for ct in (0, 1)]
         self.D = [dev(np.transpose(T.D[ct], (2, 0, 1))) for ct in (0, 1)]
@@ -144,6 +155,11 @@ class DeviceMesh:
         self.work_part = torch.zeros((4096,), dtype=torch.float64, device=self.device)
         self.norm_dev = torch.zeros((1,), dtype=torch.float64, device=self.device)
 
+        self._build_wall_records(dev, devi)
+        if turbulence is not None:
+            if turbulence != "SST":
+                raise ValueError(f"unsupported turbulence model {turbulence!r}")
+            self._build_sst(dev, devi)
         d = _lib.AfrMeshDesc()
         d.abi_version = _lib.ABI_VERSION
         d.order = order
@@ -173,10 +189,151 @@ class DeviceMesh:
         d.metric_flux_scale, d.volume = p(self.metric_flux_scale), p(self.volume_dev)
         d.work_r, d.work_s1, d.work_s2 = p(self.work_r), p(self.work_s1), p(self.work_s2)
         d.work_dt, d.work_part = p(self.work_dt), p(self.work_part)
+        d.n_wall_rec = self.n_wall_rec
+        if self.n_wall_rec:
+            d.wrec_owner, d.wrec_oslot = p(self.wrec_owner), p(self.wrec_oslot)
+            d.wrec_geom, d.w1d, d.work_fpart = p(self.wrec_geom), p(self.w1d), p(self.work_fpart)
+        if turbulence is not None:
+            d.n_records = self.n_records
+            d.rec_owner, d.rec_oslot, d.rec_flags = p(self.rec_owner), p(self.rec_oslot), p(self.rec_flags)
+            d.rec_tn, d.cell_inc = p(self.rec_tn), p(self.cell_inc)
+            d.work_pack, d.work_tvol = p(self.work_pack), p(self.work_tvol)
+            d.work_tsrc, d.work_crec = p(self.work_tsrc), p(self.work_crec)
         self._desc = d
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
         self.handle = handle
+
+    @staticmethod
+    def owner_primary(mesh):
+        """First record (lowest index) of each prism quad on its owner side; every triangle face is primary
+        (src/autoflowcfd/fr/face_flux_points_merge.py:101-110)."""
+        fr = mesh.faces
+        F = fr.n_faces
+        quad = (fr.owner < mesh.n_prism) & np.isin(fr.owner_code, (0, 1, 2))
+        primary = np.ones(F, dtype=bool)
+        qi = np.flatnonzero(quad)
+        if len(qi):
+            key = fr.owner[qi] * 8 + fr.owner_code[qi]
+            order = np.lexsort((qi, key))
+            ks = key[order]
+            first = np.ones(len(ks), dtype=bool)
+            first[1:] = ks[1:] != ks[:-1]
+            primary[qi[order[~first]]] = False
+        return primary
+
+    def _build_wall_records(self, dev, devi):
+        """Owner-primary WALL records of the force integral (fr_coefficients.py:108-112)."""
+        from .mesh import BC_WALL
+        fr = self.mesh.faces
+        sel = np.flatnonzero((fr.bc == BC_WALL) & self.owner_primary(self.mesh))
+        self.n_wall_rec = len(sel)
+        if not len(sel):
+            return
+        code_to_slot = np.full(6, -1, dtype=np.int64)
+        for si, code in enumerate(SLOT_CODES[0]):
+            code_to_slot[code] = si
+        self.wrec_owner = devi(fr.owner[sel])
+        self.wrec_oslot = devi(code_to_slot[fr.owner_code[sel]])
+        geom = np.concatenate([fr.normal[sel], fr.area[sel, None]], axis=1)
+        self.wrec_geom = torch.as_tensor(np.ascontiguousarray(geom, dtype=np.float64)).to(self.device)
+        self.w1d = torch.as_tensor(np.polynomial.legendre.leggauss(self.order + 1)[1]).to(self.device)
+        self.work_fpart = torch.zeros((len(sel), 6), dtype=torch.float64, device=self.device)
+        self.forces_dev = torch.zeros((6,), dtype=torch.float64, device=self.device)
+
+    def wall_forces(self, U, mu_t=None, mu=1.8e-5):
+        """(6,) FP64 device tensor: pressure force (3), viscous force (3) on the WALL boundary."""
+        if not self.n_wall_rec:
+            raise ValueError("mesh has no WALL boundary records")
+        self._check_state(U)
+        _lib.check(self.lib.afr_forces_f64(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                           float(mu), ctypes.c_void_p(self.forces_dev.data_ptr()), self._stream()),
+                   "afr_forces_f64")
+        return self.forces_dev
+
+    def _build_sst(self, dev, devi):
+        """Face-record tables of the SST transport and the wall distance of every solution point.
+
+        Records follow the mesh's triangle face records; the owner-primary flag marks the
+        first record (lowest index) of each prism quad on its owner side, the records whose
+        neighbour values the reference assembles from neighbour sources
+        (src/autoflowcfd/fr/face_flux_points_merge.py:101-110); every other record takes a
+        Neumann ghost in the transport (src/autoflowcfd/core/turbulence/transport_kernel.py:103-170).
+        """
+        from .mesh import BC_WALL
+        mesh, T, C = self.mesh, self.T, self.n_cells
+        fr = mesh.faces
+        F = fr.n_faces
+        bnd = fr.neighbor < 0
+        primary = self.owner_primary(mesh)
+        flags = ((~bnd & primary).astype(np.int32)) | ((fr.bc == BC_WALL).astype(np.int32) << 1)
+        code_to_slot = np.full(6, -1, dtype=np.int64)
+        for si, code in enumerate(SLOT_CODES[0]):
+            code_to_slot[code] = si
+        oslot = code_to_slot[fr.owner_code]
+        if np.any(oslot < 0):
+            raise ValueError("face record with an unknown owner face code")
+        inc = np.full((C, 8), -1, dtype=np.int64)
+        fill = np.zeros(C, dtype=np.int64)
+        rid = np.arange(F, dtype=np.int64)
+        entries = [(fr.owner, rid * 16 + fr.owner_code)]
+        nb = np.flatnonzero(~bnd)
+        entries.append((fr.neighbor[nb], nb * 16 + 8 + fr.neighbor_code[nb]))
+        for cells, codes in entries:
+            for c, e in zip(cells, codes):
+                if fill[c] >= 8:
+                    raise ValueError("more than 8 face records on one cell")
+                inc[c, fill[c]] = e
+                fill[c] += 1
+        if F * 16 >= 2 ** 31:
+            raise ValueError("too many face records for the int32 incidence encoding")
+        self.n_records = F
+        self.rec_owner, self.rec_oslot, self.rec_flags = devi(fr.owner), devi(oslot), devi(flags)
+        self.rec_tn = dev(fr.normal)
+        self.cell_inc = devi(inc)
+        dt = self.dtype
+        self.work_pack = torch.empty((C, T.ns, 8), dtype=dt, device=self.device)
+        self.work_tvol = torch.empty((C, T.ns, 2), dtype=dt, device=self.device)
+        self.work_tsrc = torch.empty((C, T.ns, 3), dtype=dt, device=self.device)
+        self.work_crec = torch.empty((F, T.nfp, 2), dtype=dt, device=self.device)
+        # wall distance: nearest WALL node of every solution point
+        self.d_wall = torch.empty((C, T.ns), dtype=dt, device=self.device)
+        wall = np.ascontiguousarray(mesh.nodes[mesh.wall_nodes], dtype=np.float64) if mesh.wall_nodes is not None \
+            else np.zeros((0, 3))
+        if len(wall) == 0:
+            raise ValueError("SST needs WALL boundary nodes for the wall distance")
+        wall_d = torch.as_tensor(wall).to(self.device)
+        xyz = self.sps_xyz.contiguous()
+        _lib.check(self.lib.afr_wall_distance(ctypes.c_void_p(xyz.data_ptr()), C * T.ns,
+                                              ctypes.c_void_p(wall_d.data_ptr()), len(wall),
+                                              ctypes.c_void_p(self.d_wall.data_ptr()),
+                                              0 if dt == torch.float32 else 1, self._stream()), "afr_wall_distance")
+        torch.cuda.synchronize(self.device)
+
+    def sst_fields(self, k0, omega0):
+        """Uniform k, omega (C, ns) and zero nu_t / mu_t of the SST model."""
+        shape = (self.n_cells, self.ns)
+        k = torch.full(shape, float(k0), dtype=self.dtype, device=self.device)
+        w = torch.full(shape, float(omega0), dtype=self.dtype, device=self.device)
+        return k, w, torch.zeros(shape, dtype=self.dtype, device=self.device), \
+            torch.zeros(shape, dtype=self.dtype, device=self.device)
+
+    def sst_step(self, U, k, w, dt, nu_t, mu_t, mu=1.8e-5):
+        _lib.check(self.lib.afr_sst_step(self.handle, ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(k.data_ptr()),
+                                         ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(self.d_wall.data_ptr()),
+                                         ctypes.c_void_p(dt.data_ptr()), ctypes.c_void_p(nu_t.data_ptr()),
+                                         ctypes.c_void_p(mu_t.data_ptr()), float(mu), self._stream()), "afr_sst_step")
+
+    def sst_rk3_step(self, U, k, w, nu_t, mu_t, mu=1.8e-5, Pr=0.72, Pr_t=0.9, n_vars_total=7, norm=None):
+        """One SST pseudo-step (k, omega, nu_t, mu_t and U updated in place); returns the (1,) FP64 norm."""
+        self._check_state(U)
+        norm = self.norm_dev if norm is None else norm
+        _lib.check(self.lib.afr_sst_rk3_step(self.handle, ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(k.data_ptr()),
+                                             ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(self.d_wall.data_ptr()),
+                                             ctypes.c_void_p(nu_t.data_ptr()), ctypes.c_void_p(mu_t.data_ptr()),
+                                             float(mu), float(Pr), float(Pr_t), int(n_vars_total),
+                                             ctypes.c_void_p(norm.data_ptr()), self._stream()), "afr_sst_rk3_step")
+        return norm
 
     def _gpu_metrics(self, sl):
         """FP64 metric precompute on the device (afr_metrics)."""
@@ -220,6 +377,9 @@ class DeviceMesh:
         g.det, g.adj, g.adj_fine, g.gcl = p(self.det), p(self.adj), p(self.adj_fine), p(self.gcl)
         g.metric_flux_scale, g.slot_adjrow = p(self.metric_flux_scale), p(self.slot_adjrow)
         g.volume, g.det_mean, g.slot_fallback, g.bad_det = p(vol), p(dmean), p(self.slot_fb), p(bad)
+        if self.turbulence is not None:
+            self.sps_xyz = torch.empty((C, T.ns, 3), dtype=f64, device=dv)
+            g.sps_xyz = p(self.sps_xyz)
         _lib.check(self.lib.afr_metrics(ctypes.byref(g), self._stream()), "afr_metrics")
         torch.cuda.synchronize(dv)
         if int(bad.item()) != 0:

@@ -76,6 +76,23 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   work_r, work_s1, work_s2 (C, ns, 5) pseudo-step scratch: residual, stage states
  *   work_dt    (C, ns)                 local time step
  *   work_part  (4096) FP64             partial sums of the residual norm
+ * SST k-omega (optional, NULL / 0 when the mesh carries no turbulence model):
+ *   n_records                          triangle face records (reference face order)
+ *   rec_owner  (F) int32               owner cell of each record
+ *   rec_oslot  (F) int32               owner face-slot index of the record's face
+ *   rec_flags  (F) int32               bit 0: owner-primary interior record (has neighbour
+ *                                      sources), bit 1: WALL boundary record
+ *   rec_tn     (F, 3)                  record unit normal (owner -> neighbour / outward)
+ *   cell_inc   (C, 8) int32            face-record incidences of each cell:
+ *                                      record * 16 + role * 8 + cube-face code
+ *                                      (role 0 owner, 1 neighbour), -1 unused
+ *   work_pack  (C, ns, 8), work_tvol (C, ns, 2), work_tsrc (C, ns, 3), work_crec (F, nfp, 2)
+ * Wall forces (optional):
+ *   n_wall_rec                         owner-primary WALL face records
+ *   wrec_owner, wrec_oslot (W) int32   owner cell and owner face slot
+ *   wrec_geom  (W, 4) FP64             record unit normal, record area
+ *   w1d        (order+1) FP64          Gauss-Legendre weights
+ *   work_fpart (W, 6) FP64             per-record force partial sums
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
@@ -108,6 +125,15 @@ typedef struct AfrMeshDesc {
     const void *metric_flux_scale, *volume;
     void *work_r, *work_s1, *work_s2, *work_dt;
     double *work_part;
+    int64_t n_records;
+    const int32_t *rec_owner, *rec_oslot, *rec_flags;
+    const void *rec_tn;
+    const int32_t *cell_inc;
+    void *work_pack, *work_tvol, *work_tsrc, *work_crec;
+    int64_t n_wall_rec;
+    const int32_t *wrec_owner, *wrec_oslot;
+    const double *wrec_geom, *w1d;
+    double *work_fpart;
 } AfrMeshDesc;
 
 typedef struct AfrMesh AfrMesh;
@@ -119,7 +145,8 @@ typedef struct AfrMesh AfrMesh;
  * slot_tn64 (C,5,3) FP64.  Outputs in the storage precision `dtype`: det,
  * adj, adj_fine, gcl, metric_flux_scale, slot_adjrow (layouts as in
  * AfrMeshDesc); FP64 volume (C) and det_mean (C); int32 slot_fallback (C,5);
- * bad_det (1 int32, set non-zero if any det(J) <= 0; caller zeroes it). */
+ * bad_det (1 int32, set non-zero if any det(J) <= 0; caller zeroes it);
+ * sps_xyz (C, ns, 3) FP64 solution-point positions, optional (NULL: not written). */
 typedef struct AfrGeomDesc {
     int32_t abi_version;
     int32_t order;
@@ -136,6 +163,7 @@ typedef struct AfrGeomDesc {
     double *volume, *det_mean;
     int32_t *slot_fallback;
     int32_t *bad_det;
+    double *sps_xyz;
 } AfrGeomDesc;
 
 /* Library ABI version (AFR_ABI_VERSION). */
@@ -189,6 +217,9 @@ int afr_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, 
  * 1/(2P+1), 1/(2P+1)^2 (src/autoflowcfd/core/fr_solver/cfl.py:12-147); mu_t may be NULL. */
 int afr_local_time_step(const AfrMesh *mesh, const void *U, const void *mu_t, void *dt, double mu, void *stream);
 
+/* As afr_local_time_step with the eddy viscosity given as nu_t: mu_t = rho(U) nu_t. */
+int afr_local_time_step_nut(const AfrMesh *mesh, const void *U, const void *nu_t, void *dt, double mu, void *stream);
+
 /* One Shu-Osher stage U_out = c0 U0 + c1 Ua + c2 dt R followed by the positivity
  * projection (p_floor 1) and the modal filter of the first 5 variables
  * (src/autoflowcfd/core/time_integration/base.py:250-335). U_out may alias U0 or Ua. */
@@ -208,6 +239,34 @@ int afr_residual_norm_f64(const AfrMesh *mesh, const void *R, int32_t n_vars_tot
  * the viscous terms.  Uses work_r, work_s1, work_s2, work_dt, work_part. */
 int afr_ssp_rk3_step(const AfrMesh *mesh, void *U, const void *mu_t, double mu, double Pr, double Pr_t,
                      int32_t n_vars_total, double *norm, void *stream);
+
+/* Distance from each of n_pts FP64 points (n_pts, 3) to the nearest of n_wall FP64
+ * wall-node positions (n_wall, 3), written in precision dtype to d_out (n_pts)
+ * (src/autoflowcfd/core/fr_solver/turbulence.py:112-121: KD-tree to wall nodes). */
+int afr_wall_distance(const double *sp_xyz, int64_t n_pts, const double *wall_xyz, int32_t n_wall, void *d_out,
+                      int32_t dtype, void *stream);
+
+/* SST k-omega step of the turbulence fields k, omega (C, ns) in place with the
+ * local time step dt (C, ns), evaluated at the conserved state U: sources with
+ * F1/F2 blending and eddy viscosity, record-based transport, point-implicit update
+ * and floors (src/autoflowcfd/core/fr_solver/turbulence.py:178-330,
+ * src/autoflowcfd/core/turbulence/{sst,transport,transport_kernel}.py).  Writes
+ * nu_t (C, ns) and mu_t = rho(U) nu_t (C, ns), the eddy viscosity of the step. */
+int afr_sst_step(const AfrMesh *mesh, const void *U, void *k, void *omega, const void *d_wall, const void *dt,
+                 void *nu_t, void *mu_t, double mu_molecular, void *stream);
+
+/* One steady pseudo-step with the SST model, as src/autoflowcfd/core/fr_solver/step.py:15-221:
+ * dt_local from U with mu_t = rho(U) nu_t of the previous step (nu_t in/out, zero
+ * initially), the SST step (k, omega, nu_t, mu_t updated), then the SSP-RK3 stages of U
+ * with mu_t frozen; norm[0] (device) = RMS of R(U) over C ns n_vars_total entries. */
+int afr_sst_rk3_step(const AfrMesh *mesh, void *U, void *k, void *omega, const void *d_wall, void *nu_t, void *mu_t,
+                     double mu, double Pr, double Pr_t, int32_t n_vars_total, double *norm, void *stream);
+
+/* FP64 force on the WALL boundary of the state U (C, ns, 5) with eddy viscosity mu_t
+ * (C, ns, may be NULL): out[0..2] pressure force sum p n w, out[3..5] viscous force
+ * -sum (tau . n) w (device pointer, 6 doubles), as
+ * src/autoflowcfd/postprocess/fr_coefficients.py:27-152.  Cd = F_x/(q A), Cl = F_z/(q A). */
+int afr_forces_f64(const AfrMesh *mesh, const void *U, const void *mu_t, double mu, double *out, void *stream);
 
 #ifdef __cplusplus
 }

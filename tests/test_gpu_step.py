@@ -72,3 +72,24 @@ def test_pseudo_step_matches_oracle(name, make, order, prec):
     else:
         assert e_s < 1e-6, f"state {e_s:.3e}"
         assert e_u < 1e-3, f"state increment {e_u:.3e}"
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_wall_forces_match_oracle(prec, order):
+    import torch
+    from oracle.forces import wall_forces
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    print(f"[gpu] wall forces {prec} P{order}", flush=True)
+    mesh = wall_box(seed=2)
+    dm, ops, g, fl, bcd, U = _setup(mesh, order, dtype)
+    mut = 1.8e-5 * np.random.default_rng(9).uniform(0, 20, size=U.shape[:2])
+    if dtype == torch.float32:
+        mut = mut.astype(np.float32).astype(np.float64)
+    fp, fv = wall_forces(U, g, ops, fl, 1.8e-5, mu_t=mut)
+    Ud = torch.as_tensor(U, dtype=dtype, device="cuda").contiguous()
+    mtd = torch.as_tensor(mut, dtype=dtype, device="cuda").contiguous()
+    out = dm.wall_forces(Ud, mtd).cpu().numpy()
+    tol = 1e-12 if prec == "f64" else 1e-6
+    assert np.abs(out[:3] - fp).max() <= tol * np.abs(fp).max()
+    assert np.abs(out[3:] - fv).max() <= (1e-10 if prec == "f64" else 1e-4) * np.abs(fv).max()
