@@ -2,7 +2,8 @@
 FP64 residual norm vs the FP64 oracle (oracle/time_step.py).
 
 Tolerances: FP64 device path 1e-10 relative (1e-7 with P2 tets) for the state
-increment U_new - U and the norm, 1e-12 for dt.  FP32 path: dt 1e-6, norm 1e-5;
+increment U_new - U and the norm, 1e-12 for dt.  FP32 path: dt 1e-6, norm 1e-5
+(1e-4 with P2 tets, whose operators amplify FP32 rounding);
 the FP32 state carries 6e-8 relative rounding, which is ~1e-4 of an increment of
 relative size 1e-3, so the updated state is checked to 1e-6 of |U| and the
 increment to 1e-3 of its own norm.
@@ -59,7 +60,9 @@ def test_pseudo_step_matches_oracle(name, make, order, prec):
     norm = float(dm.ssp_rk3_step(Ud, mtd, mu=mu).item())
     U_new = Ud.double().cpu().numpy()
     tight = prec == "f64"
-    tol = (1e-7 if (mesh.n_tet and order == 2) else 1e-10) if tight else 1e-5
+    # collapsed-basis P2 tet operators reach 4e5: 1e-7 in FP64, 1e-4 in FP32 (measured 2.7e-5 on the norm)
+    p2tet = bool(mesh.n_tet) and order == 2
+    tol = (1e-7 if p2tet else 1e-10) if tight else (1e-4 if p2tet else 1e-5)
     e_dt = np.abs(dt - dt_ref).max() / np.abs(dt_ref).max()
     dU_ref = U_ref - U
     e_u = np.linalg.norm((U_new - U) - dU_ref) / np.linalg.norm(dU_ref)
