@@ -64,6 +64,10 @@ SIGNATURES = {
                                             ctypes.c_double, ctypes.c_int32, c_vp]),
     "afr_residual": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int32, c_vp]),
+    "afr_viscous_volume": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          c_vp]),
+    "afr_viscous_interface": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int32, ctypes.c_int32, c_vp]),
     "afr_local_time_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
     "afr_rk_stage": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, c_vp]),

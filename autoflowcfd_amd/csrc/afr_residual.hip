@@ -600,6 +600,22 @@ int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, v
     return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 1, st);
 }
 
+int afr_viscous_volume(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double Pr, double Pr_t,
+                       void* stream) {
+    if (!mesh || !U) return fail("afr_viscous_volume: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!viscous_ready(d)) return fail("afr_viscous_volume: mesh has no viscous work buffers");
+    return viscous_phase(d, U, mu_t, nullptr, mu, Pr, Pr_t, 0, 0, 0, (hipStream_t)stream);
+}
+
+int afr_viscous_interface(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,
+                          double Pr_t, int32_t suppress, int32_t accumulate, void* stream) {
+    if (!mesh || !U || !R) return fail("afr_viscous_interface: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!viscous_ready(d)) return fail("afr_viscous_interface: mesh has no viscous work buffers");
+    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, accumulate, 1, (hipStream_t)stream);
+}
+
 int afr_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr, double Pr_t,
                  int32_t suppress, void* stream) {
     if (!mesh || !U || !R) return fail("afr_residual: null argument");

@@ -476,6 +476,30 @@ class DeviceMesh:
                                                  int(bool(suppress)), self._stream()), "afr_viscous_residual")
         return R
 
+    def viscous_volume(self, U, mu_t=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9):
+        """Prepared state, gradients and viscous volume term of U into the work buffers."""
+        self._check_state(U)
+        _lib.check(self.lib.afr_viscous_volume(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                               float(mu), float(Pr), float(Pr_t), self._stream()),
+                   "afr_viscous_volume")
+
+    def viscous_interface(self, U, mu_t, R, mu=1.8e-5, Pr=0.72, Pr_t=0.9, suppress=True, accumulate=True):
+        """BR1 interface correction + volume term into R (added when accumulate)."""
+        _lib.check(self.lib.afr_viscous_interface(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                                  ctypes.c_void_p(R.data_ptr()), float(mu), float(Pr), float(Pr_t),
+                                                  int(bool(suppress)), int(bool(accumulate)), self._stream()),
+                   "afr_viscous_interface")
+        return R
+
+    def local_time_step_nut(self, U, nu_t, dt=None, mu=1.8e-5):
+        """Local time step with mu_t = rho(U) nu_t."""
+        self._check_state(U)
+        dt = torch.empty((self.n_cells, self.ns), dtype=self.dtype, device=self.device) if dt is None else dt
+        _lib.check(self.lib.afr_local_time_step_nut(self.handle, ctypes.c_void_p(U.data_ptr()),
+                                                    ctypes.c_void_p(nu_t.data_ptr()), ctypes.c_void_p(dt.data_ptr()),
+                                                    float(mu), self._stream()), "afr_local_time_step_nut")
+        return dt
+
     def residual(self, U, mu_t=None, R=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, suppress=True):
         """Navier-Stokes dU/dt = inviscid + viscous (C, ns, 5)."""
         self._check_state(U)

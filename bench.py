@@ -1,10 +1,13 @@
-"""Benchmark: FR residual evaluation on the graded wall-bounded prism box (one MI355X).
+"""Benchmark: P2 SST RANS residual on the graded wall-bounded prism box (one MI355X).
 
 Workload (default): 82x82x82 hex box on [0,4]x[0,1]x[0,1] (length unit 0.2879 m),
 y-graded toward the no-slip wall with ratio 1.1, split into 2 prisms per hex
-(1,102,736 prisms), P2 (27 solution points per cell), FP32 storage, Mach 0.15
-freestream with a seeded smooth perturbation.  One step = one residual
-evaluation of the state resident in HBM.
+(1,102,736 prisms), P2 (27 solution points per cell), FP32 storage, Mach 0.15,
+Re 1e6 per unit length, k and omega from 1% intensity and viscosity ratio 10,
+seeded smooth perturbation.  One step = one full residual evaluation of the state
+resident in HBM: local time step, SST sources + transport (with the k/omega
+update), inviscid + viscous mean-flow residual.  Full SSP-RK3 pseudo-steps are
+timed afterwards and reported beside it.
 
 Prints one JSON line.  --gpus N with N > 1 exits with a message: this build runs
 on one GPU.
@@ -22,6 +25,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 L_UNIT = 0.2879          # m: mu = rho U L / Re = 1.8e-5 at M 0.15, Re 1e6 per unit length
+MU = 1.8e-5
+K_INF = 0.3908           # 1.5 (0.01 U)^2, 1% turbulence intensity
+OMEGA_INF = 2660.0       # rho k / (10 mu), viscosity ratio 10
 RHO, P_INF = 1.225, 101325.0
 A_INF = float(np.sqrt(1.4 * P_INF / RHO))
 HBM_PEAK = 8.0e12        # B/s, MI355X HBM3E
@@ -68,42 +74,63 @@ def device_state(mesh, ns, Q0, dtype, seed=0):
 def algorithmic_bytes_inviscid(n_sp, n_fine_pts, n_faces, nfp, w=4):
     """Compulsory traffic of one inviscid residual launch: read dQ, write R (5 vars),
     det(J) per solution point, adj(J) at the fine points, per physical face the
-    flux-point normals and face ids/codes (the SURVEY traffic model, inviscid terms)."""
+    flux-point normals and face ids/codes."""
     return n_sp * (5 * w * 2 + w) + n_fine_pts * 9 * w + n_faces * (nfp * 3 * w + 16)
 
 
-def cpu_baseline(order, n_sample=10, budget_s=20.0):
+def algorithmic_bytes_residual(n_sp, n_fine_pts, n_faces, nfp, n_v=7, w=4, sst=True):
+    """B_res of one full residual evaluation: read U, write R (n_v vars), det + adj(J) at the
+    solution points, adj(J) at the fine points, wall distance (SST), face normals and ids."""
+    return (n_sp * n_v * w * 2 + n_sp * 10 * w + n_fine_pts * 9 * w + (n_sp * w if sst else 0)
+            + n_faces * (nfp * 3 * w + 16))
+
+
+def cpu_baseline(order, n_sample=6, budget_s=20.0):
     """FP64 oracle (NumPy restatement of the reference CPU path), single thread, on a
-    bounded sample: an n_sample^3 hex version of the same graded wall box."""
+    bounded sample: an n_sample^3 hex version of the same graded wall box, one full
+    SST RANS residual evaluation (turbulence sources + transport + inviscid + viscous)."""
     from threadpoolctl import threadpool_limits
-    sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle import fr_ops
+    from oracle import fr_ops, sst
     from oracle.faces import build_flat_faces
     from oracle.geometry import Geometry
-    from oracle.residual import inviscid_residual
+    from oracle.residual import BC_WALL, inviscid_residual, viscous_residual
+    from oracle.time_step import local_time_step, update_primitives
+    from autoflowcfd_amd.device import bc_params_table
     mesh = wall_box_mesh(n_sample, seed=1)
     ops = fr_ops.generate_operators(order)
     g = Geometry(mesh, ops)
     fl = build_flat_faces(mesh, ops)
     Q0 = freestream()
-    from autoflowcfd_amd.device import bc_params_table
     bcp = bc_params_table(Q0)
+    bcd = {c: bcp[c] for c in range(8)}
     U = cons(np.broadcast_to(Q0, g.sps_xyz.shape[:-1] + (5,)).copy())
     U *= 1.0 + 1e-3 * np.sin(g.sps_xyz[..., :1] / L_UNIT)
+    d = sst.wall_distance(g.sps_xyz, mesh.nodes[mesh.wall_nodes])
+    k = np.full(U.shape[:2], K_INF)
+    w = np.full(U.shape[:2], OMEGA_INF)
+    wall = fl.bc == BC_WALL
+
+    def residual():
+        Q = update_primitives(U)
+        dt = local_time_step(Q, g, order, MU)
+        _, _, nut, _, _ = sst.turbulence_step(U, k, w, g, ops, fl, d, dt, MU, wall)
+        mut = Q[..., 0] * nut
+        return inviscid_residual(U, g, ops, fl, bcd) + viscous_residual(U, g, ops, fl, bcd, MU, mu_t=mut)
+
     with threadpool_limits(limits=1):
-        inviscid_residual(U, g, ops, fl, {c: bcp[c] for c in range(8)})
+        residual()
         times = []
         t_end = time.perf_counter() + budget_s
         while time.perf_counter() < t_end or not times:
             t0 = time.perf_counter()
-            inviscid_residual(U, g, ops, fl, {c: bcp[c] for c in range(8)})
+            residual()
             times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     n_sp = mesh.n_cells * ops.n_sps
-    return {"value": n_sp * 5 / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy oracle inviscid residual, {n_sample}^3-hex graded wall box "
-                      f"({mesh.n_cells} prisms, P{order}), median of {len(times)} evaluations, "
-                      f"{t:.3f} s each"}
+    return {"value": n_sp * 7 / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle, one P{order} SST RANS residual (turbulence sources + transport + "
+                      f"inviscid + viscous), {n_sample}^3-hex graded wall box ({mesh.n_cells} prisms), "
+                      f"median of {len(times)} evaluations, {t:.3f} s each, 1 thread"}
 
 
 def traffic_from_profiles(name):
@@ -121,8 +148,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=82, help="hex cells per direction")
     ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--rk-steps", type=int, default=5, help="full SSP-RK3 pseudo-steps timed after the residuals")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=10)
+    ap.add_argument("--cpu-sample-n", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -137,46 +165,72 @@ def main():
     log(f"[bench] mesh {mesh.n_cells} prisms, {mesh.faces.n_faces} face records ({time.time() - t0:.1f}s)")
     Q0 = freestream()
     t0 = time.time()
-    dm = DeviceMesh(mesh, args.order, dtype=torch.float32, bc_params=bc_params_table(Q0), metrics="gpu")
+    dm = DeviceMesh(mesh, args.order, dtype=torch.float32, bc_params=bc_params_table(Q0), metrics="gpu",
+                    turbulence="SST")
     torch.cuda.synchronize()
     log(f"[bench] device set-up {time.time() - t0:.1f}s, {dm.slots.n_blocks} row blocks")
     U = device_state(mesh, dm.ns, Q0, torch.float32)
+    k, w, nut, mut = dm.sst_fields(K_INF, OMEGA_INF)
     R = torch.empty_like(U)
-    dQ, umean = dm.work_q, dm.work_umean
+    dt = dm.work_dt
+
+    def residual_step():
+        # one full P2 SST RANS residual evaluation of the resident state: local time step,
+        # SST sources + transport (+ the point-implicit k/omega update), mean-flow residual
+        dm.local_time_step_nut(U, nut, dt)
+        dm.sst_step(U, k, w, dt, nut, mut, mu=MU)
+        dm.residual(U, mut, R, mu=MU)
 
     for _ in range(args.warmup):
-        dm.prepare_state(U, dQ, umean)
-        dm.inviscid_from_primitives(dQ, umean, R)
+        residual_step()
     torch.cuda.synchronize()
     log("[bench] warm-up done")
 
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    n_ev = 6
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record()
-        dm.prepare_state(U, dQ, umean)
-        ev[k][1].record()
-        dm.inviscid_from_primitives(dQ, umean, R)
-        ev[k][2].record()
+    for i in range(args.steps):
+        e = ev[i]
+        e[0].record()
+        dm.local_time_step_nut(U, nut, dt)
+        e[1].record()
+        dm.sst_step(U, k, w, dt, nut, mut, mu=MU)
+        e[2].record()
+        dm.viscous_volume(U, mut, mu=MU)
+        e[3].record()
+        dm.inviscid_from_primitives(dm.work_q, dm.work_umean, R)
+        e[4].record()
+        dm.viscous_interface(U, mut, R, mu=MU)
+        e[5].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    t_prim = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])) * 1e-3
-    t_inv = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)])) * 1e-3
+    ph = np.array([[ev[i][j].elapsed_time(ev[i][j + 1]) for j in range(n_ev - 1)] for i in range(args.steps)]).mean(0)
     ms_per_step = wall / args.steps * 1e3
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (prepare {t_prim * 1e3:.3f} ms, "
-        f"inviscid {t_inv * 1e3:.3f} ms)")
+    names = ["dt_local", "sst", "visc_volume", "inviscid", "visc_interface"]
+    log("[bench] residual: %.3f ms/step (%s)" % (ms_per_step, ", ".join(f"{n} {t:.3f}" for n, t in zip(names, ph))))
     if not bool(torch.isfinite(R).all()):
         raise RuntimeError("non-finite residual")
 
+    # full SSP-RK3 pseudo-steps (3 mean-flow residuals + SST + stages + FP64 norm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.rk_steps):
+        dm.sst_rk3_step(U, k, w, nut, mut, mu=MU)
+    torch.cuda.synchronize()
+    ms_rk = (time.perf_counter() - t0) / max(args.rk_steps, 1) * 1e3
+    norm = float(dm.norm_dev.item())
+    log(f"[bench] SSP-RK3 SST pseudo-step {ms_rk:.3f} ms, residual RMS {norm:.4e}")
+
     n_sp = mesh.n_cells * dm.ns
-    n_phys_faces = int(((~mesh.faces.is_boundary).sum() + mesh.faces.is_boundary.sum()))
-    # physical faces: prism quads are two triangle records
-    is_quad = np.isin(mesh.faces.owner_code, (0, 1, 2)) & (mesh.faces.owner < mesh.n_prism)
-    n_phys_faces = int(n_phys_faces - is_quad.sum() // 2)
-    b_alg = algorithmic_bytes_inviscid(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
-    achieved = b_alg / t_inv
-    value = n_sp * 5 / ms_per_step * 1e3 / 1e9
+    fr = mesh.faces
+    is_quad = np.isin(fr.owner_code, (0, 1, 2)) & (fr.owner < mesh.n_prism)
+    n_phys_faces = int(fr.n_faces - is_quad.sum() // 2)
+    t_inv = ph[3] * 1e-3
+    b_inv = algorithmic_bytes_inviscid(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
+    b_res = algorithmic_bytes_residual(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
+    t_res = ms_per_step * 1e-3
+    value = n_sp * 7 / t_res / 1e9
     line = {
         "metric": "P2 residual GDOF/s + %HBM roofline",
         "value": value,
@@ -190,13 +244,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"inviscid FR residual, {args.n}^3-hex graded wall box split into "
-                               f"{mesh.n_cells} prisms, P{args.order}, {n_sp} solution points x 5 vars",
-                   "cells": mesh.n_cells, "order": args.order, "n_vars": 5},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles("inviscid_c2"),
-                     "kernel": f"k_prism_inviscid<{args.order},float>", "kernel_ms": t_inv * 1e3,
-                     "algorithmic_bytes": b_alg},
+        "config": {"workload": f"P{args.order} SST RANS residual (AUSM+up inviscid + BR1 viscous + SST sources and "
+                               f"transport), {args.n}^3-hex graded wall box split into {mesh.n_cells} prisms, "
+                               f"{n_sp} solution points x 7 variables",
+                   "cells": mesh.n_cells, "order": args.order, "n_vars": 7},
+        "roofline": {"bound": "hbm", "achieved": b_inv / t_inv / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b_inv / t_inv / HBM_PEAK, "traffic": traffic_from_profiles("k_prism_inviscid_c2"),
+                     "kernel": f"k_prism_inviscid<{args.order},{dm.slots.n_blocks and 9},float>",
+                     "kernel_ms": t_inv * 1e3, "algorithmic_bytes": b_inv},
+        "residual_roofline": {"achieved_GBs": b_res / t_res / 1e9, "frac": b_res / t_res / HBM_PEAK,
+                              "algorithmic_bytes": b_res},
+        "phase_ms": {n: float(t) for n, t in zip(names, ph)},
+        "ms_per_rk3_pseudo_step": ms_rk,
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")

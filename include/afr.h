@@ -206,6 +206,17 @@ int afr_inviscid_from_primitives(const AfrMesh *mesh, const void *dQ, const void
 int afr_viscous_residual(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu,
                          double Pr, double Pr_t, int32_t suppress, void *stream);
 
+/* The two phases of afr_viscous_residual, for callers that interleave other work:
+ * afr_viscous_volume writes the prepared state (work_q, work_umean), the gradients
+ * (work_grad), the viscous volume term (work_rvol) and work_h from U;
+ * afr_viscous_interface adds the BR1 interface correction to work_rvol, applies the
+ * outlier zeroing when suppress != 0 and writes (accumulate == 0) or adds
+ * (accumulate != 0) the result to R. */
+int afr_viscous_volume(const AfrMesh *mesh, const void *U, const void *mu_t, double mu, double Pr, double Pr_t,
+                       void *stream);
+int afr_viscous_interface(const AfrMesh *mesh, const void *U, const void *mu_t, void *R, double mu, double Pr,
+                          double Pr_t, int32_t suppress, int32_t accumulate, void *stream);
+
 /* Full Navier-Stokes FR residual dU/dt = inviscid + viscous (each with its own
  * outlier zeroing when suppress != 0), as the reference solver sums them
  * (src/autoflowcfd/core/fr_solver/solver.py:360-420).  mu_t may be NULL. */
