@@ -25,11 +25,12 @@ class AfrMeshDesc(ctypes.Structure):
         ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
         ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
         ("work_rvol", c_vp), ("work_h", c_vp),
+        ("work_vpack", c_vp), ("dsp_w", c_vp * 2), ("esp_w", c_vp * 2), ("dsp_i", c_vp * 2), ("esp_i", c_vp * 2),
         ("metric_flux_scale", c_vp), ("volume", c_vp),
         ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
         ("n_records", ctypes.c_int64), ("rec_owner", c_vp), ("rec_oslot", c_vp), ("rec_flags", c_vp),
         ("rec_tn", c_vp), ("cell_inc", c_vp),
-        ("work_pack", c_vp), ("work_tvol", c_vp), ("work_tsrc", c_vp), ("work_crec", c_vp),
+        ("work_pack", c_vp), ("work_nbp", c_vp), ("work_tvol", c_vp), ("work_tsrc", c_vp), ("work_crec", c_vp),
         ("n_wall_rec", ctypes.c_int64), ("wrec_owner", c_vp), ("wrec_oslot", c_vp), ("wrec_geom", c_vp),
         ("w1d", c_vp), ("work_fpart", c_vp),
     ]
@@ -75,7 +76,7 @@ SIGNATURES = {
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_vp, c_vp]),
     "afr_wall_distance": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_vp]),
-    "afr_sst_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
+    "afr_sst_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
     "afr_sst_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int32, c_vp, c_vp]),
     "afr_local_time_step_nut": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),

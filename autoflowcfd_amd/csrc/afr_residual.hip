@@ -543,8 +543,11 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.R = (real*)R;
         a.det = (const real*)d.det;
         a.adj = (const real*)d.adj;
-        a.D = (const real*)d.D[ct];
-        a.E = (const real*)d.E[ct];
+        a.vpack = (real*)d.work_vpack;
+        a.Dsp_w = (const real*)d.dsp_w[ct];
+        a.Dsp_i = d.dsp_i[ct];
+        a.Esp_w = (const real*)d.esp_w[ct];
+        a.Esp_i = d.esp_i[ct];
         a.slot_nb = d.slot_nb;
         a.slot_blk = d.slot_blk;
         a.slot_mask = d.slot_mask;
@@ -584,7 +587,10 @@ int viscous_phase(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R
                               : run_viscous<double>(d, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st);
 }
 
-bool viscous_ready(const AfrMeshDesc& d) { return d.adj && d.work_grad && d.work_rvol && d.work_h && d.D[0] && d.D[1]; }
+bool viscous_ready(const AfrMeshDesc& d) {
+    return d.adj && d.work_grad && d.work_rvol && d.work_h && d.work_vpack && d.dsp_w[0] && d.dsp_w[1] && d.dsp_i[0] &&
+           d.dsp_i[1] && d.esp_w[0] && d.esp_w[1] && d.esp_i[0] && d.esp_i[1];
+}
 
 }  // namespace
 
@@ -732,7 +738,7 @@ int afr_sst_rk3_step(const AfrMesh* mesh, void* U, void* k, void* omega, const v
     const AfrMeshDesc& d = mesh->d;
     if (!step_ready(d)) return fail("afr_sst_rk3_step: mesh has no pseudo-step work buffers");
     if (afr_local_time_step_nut(mesh, U, nu_t, d.work_dt, mu, stream)) return -1;
-    if (afr_sst_step(mesh, U, k, omega, d_wall, d.work_dt, nu_t, mu_t, mu, stream)) return -1;
+    if (afr_sst_step(mesh, U, k, omega, d_wall, d.work_dt, nu_t, mu_t, nullptr, mu, stream)) return -1;
     return rk3_stages(mesh, U, mu_t, mu, Pr, Pr_t, n_vars_total, norm, stream);
 }
 
@@ -751,7 +757,7 @@ namespace {
 
 template <typename real>
 int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* dw, const void* dt, void* nut,
-            void* mut, double mu, hipStream_t st) {
+            void* mut, void* rates, double mu, hipStream_t st) {
     afr_sst::SstArgs<real> a;
     a.n_records = d.n_records;
     a.U = (const real*)U;
@@ -762,6 +768,8 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
     a.nu_t = (real*)nut;
     a.mu_t = (real*)mut;
     a.pack = (real*)d.work_pack;
+    a.nbp = (real*)d.work_nbp;
+    a.rates = (real*)rates;
     a.vol = (real*)d.work_tvol;
     a.src = (real*)d.work_tsrc;
     a.crec = (real*)d.work_crec;
@@ -820,15 +828,15 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
 extern "C" {
 
 int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const void* d_wall, const void* dt,
-                 void* nu_t, void* mu_t, double mu_molecular, void* stream) {
+                 void* nu_t, void* mu_t, void* rates, double mu_molecular, void* stream) {
     if (!mesh || !U || !k || !omega || !d_wall || !dt || !nu_t || !mu_t) return fail("afr_sst_step: null argument");
     const AfrMeshDesc& d = mesh->d;
     if (d.n_records <= 0 || !d.rec_owner || !d.rec_oslot || !d.rec_flags || !d.rec_tn || !d.cell_inc ||
-        !d.work_pack || !d.work_tvol || !d.work_tsrc || !d.work_crec || !d.adj)
+        !d.work_pack || !d.work_nbp || !d.work_tvol || !d.work_tsrc || !d.work_crec || !d.adj)
         return fail("afr_sst_step: mesh has no SST record tables / work buffers");
     hipStream_t st = (hipStream_t)stream;
-    return d.dtype == AFR_F32 ? run_sst<float>(d, U, k, omega, d_wall, dt, nu_t, mu_t, mu_molecular, st)
-                              : run_sst<double>(d, U, k, omega, d_wall, dt, nu_t, mu_t, mu_molecular, st);
+    return d.dtype == AFR_F32 ? run_sst<float>(d, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st)
+                              : run_sst<double>(d, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st);
 }
 
 int afr_forces_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double* out, void* stream) {

@@ -78,9 +78,12 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     constexpr int NS = O::NS;
     constexpr int NF = 12;  // per-SP doubles: rho, u, v, w, m1, m2, m3, k, w, nu_t, Gk, Gw
+    constexpr int NFP = O::NFP, NSLOT = O::NSLOT;
     __shared__ real s_D[3 * NS * NS];
+    __shared__ real s_E[5 * NFP * NS];
     __shared__ double s_w[kWaves][NS * NF + NS * 15 + 3 * NS * 4];
     for (int i = threadIdx.x; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
+    for (int i = threadIdx.x; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = (CT == 0 ? a.E0 : a.E1)[i];
     __syncthreads();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
     double* sv = s_w[wid];          // [s][NF]
@@ -196,17 +199,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             const double rnut = rho * nut;
             const double Gk = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
             const double Gw = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;
+            sv[s * NF + 9] = rho;  // slot 9: rho again, so the face pack is sv[0..3], k, w, Gk, Gw
             sv[s * NF + 10] = Gk;
             sv[s * NF + 11] = Gw;
-            real* pk = a.pack + i * kPack;
-            pk[0] = (real)rho;
-            pk[1] = (real)sv[s * NF + 1];
-            pk[2] = (real)sv[s * NF + 2];
-            pk[3] = (real)sv[s * NF + 3];
-            pk[4] = (real)k;
-            pk[5] = (real)w;
-            pk[6] = (real)Gk;
-            pk[7] = (real)Gw;
+            real* nbp = a.nbp + i * 4;
+            nbp[0] = (real)k;
+            nbp[1] = (real)w;
+            nbp[2] = (real)Gk;
+            nbp[3] = (real)Gw;
             // contravariant transport fluxes (raw gradients in the diffusion)
             const real* ad = a.adj + i * 9;
             const double ru[3] = {rho * sv[s * NF + 1], rho * sv[s * NF + 2], rho * sv[s * NF + 3]};
@@ -223,6 +223,29 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             }
         }
         wave_sync();
+        // owner-side face traces of (rho, u, v, w, k, omega, Gamma_k, Gamma_omega): lane (slot, flux point)
+        if (t < NSLOT * NFP) {
+            const int slot = t / NFP, fp = t % NFP;
+            const real* e = s_E + (slot * NFP + fp) * NS;
+            double tr[kPack];
+#pragma unroll
+            for (int q = 0; q < kPack; ++q) tr[q] = 0.0;
+            for (int j = 0; j < NS; ++j) {
+                const double ej = (double)e[j];
+                const double* x = sv + j * NF;
+                tr[0] += ej * x[0];
+                tr[1] += ej * x[1];
+                tr[2] += ej * x[2];
+                tr[3] += ej * x[3];
+                tr[4] += ej * x[7];
+                tr[5] += ej * x[8];
+                tr[6] += ej * x[10];
+                tr[7] += ej * x[11];
+            }
+            real* o = a.pack + ((cell * 5 + slot) * NFP + fp) * kPack;
+#pragma unroll
+            for (int q = 0; q < kPack; ++q) o[q] = (real)tr[q];
+        }
         // volume terms: -(D.F_conv + D.F_diff)/det for k and omega
         for (int o = threadIdx.x & 63; o < 2 * NS; o += 64) {
             const int s = o >> 1, f = o & 1;
@@ -240,12 +263,6 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
 template <int P, typename real>
 __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
     constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1;
-    __shared__ real s_E[2][5 * NFP * NS];
-    for (int i = threadIdx.x; i < 5 * NFP * NS; i += 256) {
-        s_E[0][i] = a.E0[i];
-        s_E[1][i] = a.E1[i];
-    }
-    __syncthreads();
     const int64_t n = a.n_records * NFP;
     for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
         const int64_t r = g / NFP;
@@ -253,18 +270,12 @@ __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
         const int64_t o = a.rec_owner[r];
         const int slot = a.rec_oslot[r];
         const int flags = a.rec_flags[r];
-        const real* E = s_E[o < a.n_prism ? 0 : 1] + (slot * NFP + fp) * NS;
-        double po[kPack], pn[kPack];
+        double po[kPack], pn[4];
+        const real* tr = a.pack + ((o * 5 + slot) * NFP + fp) * kPack;
 #pragma unroll
-        for (int q = 0; q < kPack; ++q) po[q] = 0.0;
-        for (int s = 0; s < NS; ++s) {
-            const double e = (double)E[s];
-            const real* pk = a.pack + (o * NS + s) * kPack;
-#pragma unroll
-            for (int q = 0; q < kPack; ++q) po[q] += e * (double)pk[q];
-        }
+        for (int q = 0; q < kPack; ++q) po[q] = (double)tr[q];
         if (flags & 1) {
-            // neighbour values at the owner's flux points through the owner's face slot
+            // neighbour k, omega, Gammas at the owner's flux points through the owner's face slot
             const int64_t sidx = o * 5 + slot;
             const int src = (a.slot_mask[sidx] >> fp) & 1;
             int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
@@ -272,27 +283,27 @@ __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
             if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;
             const int64_t nb = nbc >= 0 ? nbc : 0;
 #pragma unroll
-            for (int q = 0; q < kPack; ++q) pn[q] = 0.0;
+            for (int q = 0; q < 4; ++q) pn[q] = 0.0;
             for (int kk = 0; kk < a.knz; ++kk) {
                 const int sp = a.rows_idx[(int64_t)row * a.knz + kk];
                 const double wgt = (double)a.rows_w[(int64_t)row * a.knz + kk];
-                const real* pk = a.pack + (nb * NS + sp) * kPack;
+                const real* pk = a.nbp + (nb * NS + sp) * 4;
 #pragma unroll
-                for (int q = 0; q < kPack; ++q) pn[q] += wgt * (double)pk[q];
+                for (int q = 0; q < 4; ++q) pn[q] += wgt * (double)pk[q];
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < kPack; ++q) pn[q] = po[q];
-            if (flags & 2) pn[4] = -po[4];  // k = 0 on no-slip walls: mirrored ghost
+            for (int q = 0; q < 4; ++q) pn[q] = po[4 + q];
+            if (flags & 2) pn[0] = -po[4];  // k = 0 on no-slip walls: mirrored ghost
         }
         const double n0 = (double)a.rec_tn[r * 3 + 0], n1 = (double)a.rec_tn[r * 3 + 1], n2 = (double)a.rec_tn[r * 3 + 2];
         const double mass = po[0] * (po[1] * n0 + po[2] * n1 + po[3] * n2);
         double c[2];
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-            const double phio = po[4 + f], phin = pn[4 + f];
+            const double phio = po[4 + f], phin = pn[f];
             const double up = mass >= 0.0 ? phio : phin;
-            const double gface = 0.5 * (po[6 + f] + pn[6 + f]);
+            const double gface = 0.5 * (po[6 + f] + pn[2 + f]);
             c[f] = mass * (up - phio) - gface * (0.5 * (phio + phin) - phio);
         }
         a.crec[g * 2 + 0] = (real)c[0];
@@ -375,11 +386,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
             dw = finite_or(dw, 0.0) + st[s * 2 + 1];
             dk = finite_or(dk, 0.0);
             dw = finite_or(dw, 0.0);
-            double kn = k + dt * dk, wn = w + dt * dw;
-            kn = isfinite(kn) ? kn : 1e-12;
-            wn = isfinite(wn) ? wn : 1e-12;
-            a.k[i] = (real)dmax(kn, 1e-12);
-            a.w[i] = (real)dmax(wn, 1e-12);
+            if (a.rates) {
+                // residual evaluation only: dk/dt and domega/dt of the update, fields untouched
+                a.rates[i * 2 + 0] = (real)dk;
+                a.rates[i * 2 + 1] = (real)dw;
+            } else {
+                double kn = k + dt * dk, wn = w + dt * dw;
+                kn = isfinite(kn) ? kn : 1e-12;
+                wn = isfinite(wn) ? wn : 1e-12;
+                a.k[i] = (real)dmax(kn, 1e-12);
+                a.w[i] = (real)dmax(wn, 1e-12);
+            }
         }
         wave_sync();
     }

@@ -19,7 +19,9 @@ struct SstArgs {
     const real* dt;               // (C, NS)
     real* nu_t;                   // (C, NS) out
     real* mu_t;                   // (C, NS) out: rho(U) nu_t
-    real* pack;                   // (C, NS, kPack) scratch
+    real* pack;                   // (C, 5, NFP, kPack) scratch: owner-side face traces of the pack
+    real* nbp;                    // (C, NS, 4) scratch: k, omega, Gamma_k, Gamma_omega at the solution points
+    real* rates;                  // (C, NS, 2) out when non-null: dk/dt, domega/dt instead of the update
     real* vol;                    // (C, NS, 2) scratch: volume transport terms of k, omega
     real* src;                    // (C, NS, 3) scratch: Sk/rho, Somega/rho, beta blend
     real* crec;                   // (F, NFP, 2) scratch: record corrections

@@ -126,16 +126,30 @@ __device__ void outlier_zeroing(real* sR, const real* umean, int t) {
 }
 
 // ------------------------------------------------------------------ volume kernel
+// Row widths of the sparse collapsed-coordinate derivative matrices: d/da couples the
+// N1 points of an a-line, d/db the N1^2 points of an (a,b) plane, d/dc the N1 points of a
+// c-line for the prism and all N1^3 points for the tet.
+template <int P, int CT> struct DW {
+    static constexpr int N1 = P + 1;
+    static constexpr int W0 = N1, W1 = N1 * N1, W2 = CT == 0 ? N1 : N1 * N1 * N1;
+    static constexpr int WM = W2 > W1 ? W2 : W1;
+};
+
 template <int P, int CT, typename real>
 __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
     using O = VO<P, CT>;
-    constexpr int NS = O::NS, NSV = O::NSV;
-    __shared__ real s_D[3 * NS * NS];                       // D[m][s][j]
+    using W = DW<P, CT>;
+    constexpr int NS = O::NS, NSV = O::NSV, WM = W::WM;
+    __shared__ real s_Dw[3 * NS * WM];                       // sparse D: weights [m][s][k]
+    __shared__ int s_Di[3 * NS * WM];                        //           indices
     __shared__ real s_w[kWaves][NSV + 4 * NS + 12 * NS + 12 * NS];
     __shared__ double s_ref[5];
     const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
-    for (int i = tid; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
+    for (int i = tid; i < 3 * NS * WM; i += 64 * kWaves) {
+        s_Dw[i] = a.Dsp_w[i];
+        s_Di[i] = a.Dsp_i[i];
+    }
     if (tid < 5) s_ref[tid] = a.qref[tid];
     __syncthreads();
     real* const sU = s_w[wid];              // |U| [s][5]
@@ -144,23 +158,32 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
     real* const sGt = sG + 12 * NS;         // G~ [m][s][4] (v = 1..4)
     const double q0r = s_ref[0], q0p = s_ref[4];
     const double T0 = q0p / (rmax(q0r, 1e-10) * kRair);
+    constexpr int WD[3] = {W::W0, W::W1, W::W2};
 
     const int64_t nw = (int64_t)gridDim.x * kWaves;
     for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
         const int64_t cell = a.cell_begin + ci;
         t = lane_fresh(t);
         // 1. primitives, perturbations, |U|
+        real mut_s = real(0);
         if (t < NS) {
             const int s = t;
             double u[5], q[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) u[v] = (double)a.U[(cell * NS + s) * 5 + v];
             cons2prim(u, q);
+            real* vp = a.vpack + (cell * NS + s) * kVP;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                a.dQ[(cell * NS + s) * 5 + v] = (real)(q[v] - s_ref[v]);
+                const real dq = (real)(q[v] - s_ref[v]);
+                a.dQ[(cell * NS + s) * 5 + v] = dq;
+                vp[v] = dq;
                 sU[s * 5 + v] = (real)fabs(u[v]);
             }
+            mut_s = a.mu_t ? a.mu_t[cell * NS + s] : real(0);
+            vp[17] = mut_s;
+            vp[18] = real(0);
+            vp[19] = real(0);
             const double T = q[4] / (rmax(q[0], 1e-10) * kRair);
             sF[s * 4 + 0] = (real)(q[1] - s_ref[1]);
             sF[s * 4 + 1] = (real)(q[2] - s_ref[2]);
@@ -178,23 +201,28 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
             for (int s = 0; s < NS; ++s) acc += a.det[cell * NS + s];
             a.h_ip[cell] = (real)cbrt((double)(acc / real(NS)));
         }
-        // 2. gradients: lane (s, field)
+        // 2. gradients: lane (s, field), sparse derivative rows
         for (int o = t; o < 4 * NS; o += 64) {
             const int s = o >> 2, f = o & 3;
-            real d0 = 0, d1 = 0, d2 = 0;
-            for (int j = 0; j < NS; ++j) {
-                const real x = sF[j * 4 + f];
-                d0 += s_D[(0 * NS + s) * NS + j] * x;
-                d1 += s_D[(1 * NS + s) * NS + j] * x;
-                d2 += s_D[(2 * NS + s) * NS + j] * x;
+            real d[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                real acc = 0;
+#pragma unroll
+                for (int k = 0; k < WD[m]; ++k) {
+                    const int idx = (m * NS + s) * WM + k;
+                    acc += s_Dw[idx] * sF[s_Di[idx] * 4 + f];
+                }
+                d[m] = acc;
             }
             const real* ad = a.adj + (cell * NS + s) * 9;
             const real idet = recip(a.det[cell * NS + s]);
+            real* vp = a.vpack + (cell * NS + s) * kVP;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                const real g = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
+                const real g = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
                 sG[s * 12 + f * 3 + n] = g;
-                a.grad[(cell * NS + s) * 12 + f * 3 + n] = g;
+                vp[5 + f * 3 + n] = g;
             }
         }
         wave_sync();
@@ -212,8 +240,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
                 for (int j = 0; j < 3; ++j) gv[i][j] = sG[s * 12 + i * 3 + j];
                 gT[i] = sG[s * 12 + 9 + i];
             }
-            const real mut = a.mu_t ? a.mu_t[cell * NS + s] : real(0);
-            viscous_flux(q, gv, gT, (real)a.mu, (real)a.Pr, mut, (real)a.Prt, G);
+            viscous_flux(q, gv, gT, (real)a.mu, (real)a.Pr, mut_s, (real)a.Prt, G);
             const real* ad = a.adj + (cell * NS + s) * 9;
 #pragma unroll
             for (int m = 0; m < 3; ++m)
@@ -228,8 +255,13 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
             const int s = o / 5, v = o % 5;
             real acc = 0;
             if (v > 0) {
+#pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    for (int j = 0; j < NS; ++j) acc += s_D[(m * NS + s) * NS + j] * sGt[(m * NS + j) * 4 + v - 1];
+#pragma unroll
+                    for (int k = 0; k < WD[m]; ++k) {
+                        const int idx = (m * NS + s) * WM + k;
+                        acc += s_Dw[idx] * sGt[(m * NS + s_Di[idx]) * 4 + v - 1];
+                    }
                 acc *= recip(a.det[cell * NS + s]);
             }
             a.rvol[cell * NSV + o] = acc;
@@ -243,15 +275,20 @@ template <int P, int CT, int KMAX, typename real>
 __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
     using O = VO<P, CT>;
     constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP, NSLOT = O::NSLOT, NSV = O::NSV;
-    constexpr int NP = 18;  // pack: dQ (5), grad u (9), grad T (3), mu_t (1)
-    __shared__ real s_E[5 * NFP * NS];
+    constexpr int KE = CT == 0 ? N1 * N1 : NS;  // non-zeros of a face-trace row
+    constexpr int NP = 18;                      // pack: dQ (5), grad u (9), grad T (3), mu_t (1)
+    __shared__ real s_Ew[5 * NFP * KE];
+    __shared__ int s_Ei[5 * NFP * KE];
     __shared__ real s_g[2 * N1];
     __shared__ int s_fp[3 * NS], s_co[3 * NS];
-    __shared__ real s_w[kWaves][NS * NP + 5 * NFP * 5 + NSV];
+    __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * kVP + 5 * NFP * 5 + NSV];
     __shared__ double s_ref[5];
     const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
-    for (int i = tid; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = a.E[i];
+    for (int i = tid; i < 5 * NFP * KE; i += 64 * kWaves) {
+        s_Ew[i] = a.Esp_w[i];
+        s_Ei[i] = a.Esp_i[i];
+    }
     for (int i = tid; i < 3 * NS; i += 64 * kWaves) {
         s_fp[i] = a.fp_of_sp[i];
         s_co[i] = a.coord_of_sp[i];
@@ -262,8 +299,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
     }
     if (tid < 5) s_ref[tid] = a.qref[tid];
     __syncthreads();
-    real* const sP = s_w[wid];           // own pack [s][NP]
-    real* const sJ = sP + NS * NP;       // jumps [slot][fp][5]
+    real* const sP = s_w[wid];           // own pack [s][kVP]
+    real* const sJ = sP + NS * kVP;      // jumps [slot][fp][5]
     real* const sR = sJ + 5 * NFP * 5;   // residual [s][5]
     const real mu = (real)a.mu, Pr = (real)a.Pr, Prt = (real)a.Prt;
 
@@ -271,14 +308,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
     for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
         const int64_t cell = a.cell_begin + ci;
         t = lane_fresh(t);
-        for (int o = t; o < NS * NP; o += 64) {
-            const int s = o / NP, k = o % NP;
-            real x;
-            if (k < 5) x = a.dQ[(cell * NS + s) * 5 + k];
-            else if (k < 17) x = a.grad[(cell * NS + s) * 12 + k - 5];
-            else x = a.mu_t ? a.mu_t[cell * NS + s] : real(0);
-            sP[o] = x;
-        }
+        for (int o = t; o < NS * kVP; o += 64) sP[o] = a.vpack[cell * NS * kVP + o];
         wave_sync();
         t = lane_fresh(t);
         // 1. BR1 jumps: lane = (slot, flux point)
@@ -291,10 +321,13 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
             real po[NP], pn[NP];
 #pragma unroll
             for (int k = 0; k < NP; ++k) po[k] = real(0);
-            for (int s = 0; s < NS; ++s) {
-                const real w = s_E[(slot * NFP + fp) * NS + s];
 #pragma unroll
-                for (int k = 0; k < NP; ++k) po[k] += w * sP[s * NP + k];
+            for (int k = 0; k < KE; ++k) {
+                const int e = (slot * NFP + fp) * KE + k;
+                const real w = s_Ew[e];
+                const real* x = sP + s_Ei[e] * kVP;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) po[q] += w * x[q];
             }
             real Qo[5], Qn[5];
 #pragma unroll
@@ -307,14 +340,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                 const int nbc = a.slot_nb[sidx * 2 + src];
                 if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
                 const int64_t nb = nbc >= 0 ? nbc : 0;
-                for (int k = 0; k < KMAX && k < a.knz; ++k) {
-                    const int sp = a.rows_idx[(int64_t)row * a.knz + k];
-                    const real w = a.rows_w[(int64_t)row * a.knz + k];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) pn[v] += w * a.dQ[(nb * NS + sp) * 5 + v];
+                for (int k = 0; k < KMAX; ++k) {
+                    if (k < a.knz) {
+                        const int sp = a.rows_idx[(int64_t)row * a.knz + k];
+                        const real w = a.rows_w[(int64_t)row * a.knz + k];
+                        const real* x = a.vpack + (nb * NS + sp) * kVP;
 #pragma unroll
-                    for (int k2 = 0; k2 < 12; ++k2) pn[5 + k2] += w * a.grad[(nb * NS + sp) * 12 + k2];
-                    pn[17] += w * (a.mu_t ? a.mu_t[nb * NS + sp] : real(0));
+                        for (int q = 0; q < NP; ++q) pn[q] += w * x[q];
+                    }
                 }
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
@@ -371,6 +405,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
         for (int o = t; o < NSV; o += 64) {
             const int s = o / 5, v = o % 5;
             real corr = 0;
+#pragma unroll
             for (int slot = 0; slot < NSLOT; ++slot) {
                 const int code = slot_code(slot), axis = code >> 1;
                 const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];

@@ -5,6 +5,8 @@
 
 namespace afr_visc {
 
+constexpr int kVP = 20;  // per-point viscous pack: dQ (5), grad u (9), grad T (3), mu_t (1), pad (2)
+
 template <typename real>
 struct ViscArgs {
     int64_t cell_begin, n_cells;   // cells [cell_begin, cell_begin + n_cells) of one type
@@ -18,8 +20,11 @@ struct ViscArgs {
     real* R;                       // (C, NS, 5) out
     const real* det;
     const real* adj;               // (C, NS, 3, 3)
-    const real* D;                 // (3, NS, NS) of the cell type
-    const real* E;                 // (5, NFP, NS) of the cell type, slot order
+    real* vpack;                   // (C, NS, kVP) out: per-point pack for the interface kernel
+    const real* Dsp_w;             // (3, NS, NS) sparse derivative rows of the cell type: weights
+    const int* Dsp_i;              //                                            indices
+    const real* Esp_w;             // (5, NFP, KE) sparse face-trace rows of the cell type, slot order
+    const int* Esp_i;
     const int* slot_nb;
     const int* slot_blk;
     const int* slot_mask;

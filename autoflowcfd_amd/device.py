@@ -44,6 +44,26 @@ def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
     return P
 
 
+def sparse_fixed(rows, widths, drop=ROW_DROP):
+    """Dense (..., n) rows -> (w, idx) of shape (..., W) with the non-zeros of each row first and zero
+    padding; widths broadcast against the leading shape and bound the non-zeros per row (checked)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    lead, n = rows.shape[:-1], rows.shape[-1]
+    W = int(np.max(widths))
+    keep = np.abs(rows) > drop * max(np.abs(rows).max(), 1.0)
+    lost = np.abs(np.where(keep, 0.0, rows)).max() if rows.size else 0.0
+    if lost > 1e-9:
+        raise ValueError(f"sparse rows drop weight {lost:.2e}")
+    cnt = keep.sum(axis=-1)
+    if np.any(cnt > np.broadcast_to(widths, lead)):
+        raise ValueError("sparse row wider than its kernel width")
+    order = np.argsort(~keep, axis=-1, kind="stable")[..., :W]
+    valid = np.arange(W) < cnt[..., None]
+    idx = np.where(valid, order, 0).astype(np.int32)
+    w = np.where(valid, np.take_along_axis(rows, order, axis=-1), 0.0)
+    return w, idx
+
+
 def _spread3(x):
     """Insert two zero bits between the low 21 bits of x (uint64)."""
     x = x & np.uint64(0x1FFFFF)
@@ -132,6 +152,20 @@ class DeviceMesh:
                 if mesh.n_tet:
                     xyz[mesh.n_prism:] = map_ref(False, T.sps, mesh.nodes[mesh.tet_conn])
                 self.sps_xyz = torch.as_tensor(xyz).to(self.device)
+        # sparse derivative / face-trace rows of the viscous kernels (widths: d/da N1, d/db N1^2,
+        # d/dc N1 (prism) or N1^3 (tet); face rows N1^2 (prism) or N1^3 (tet))
+        n1 = order + 1
+        self.dsp_w, self.dsp_i, self.esp_w, self.esp_i = [], [], [], []
+        for ct in (0, 1):
+            wd = np.array([n1, n1 * n1, n1 if ct == 0 else n1 ** 3])
+            Dm = np.transpose(T.D[ct], (2, 0, 1))  # (3, ns, ns)
+            w_, i_ = sparse_fixed(Dm, wd[:, None])
+            self.dsp_w.append(dev(w_))
+            self.dsp_i.append(devi(i_))
+            ke = n1 * n1 if ct == 0 else n1 ** 3
+            w_, i_ = sparse_fixed(self.E64[ct], ke)
+            self.esp_w.append(dev(w_))
+            self.esp_i.append(devi(i_))
         self.c2f = [dev(T.c2f[ct]) for ct in (0, 1)]
         self.op = [dev(T.Op[ct]) for ct in (0, 1)]
         self.D = [dev(np.transpose(T.D[ct], (2, 0, 1))) for ct in (0, 1)]
@@ -146,6 +180,7 @@ class DeviceMesh:
         self.work_grad = torch.empty((C, T.ns, 12), dtype=dtype, device=self.device)
         self.work_umean = torch.empty((C, 5), dtype=dtype, device=self.device)
         self.work_rvol = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
+        self.work_vpack = torch.empty((C, T.ns, 20), dtype=dtype, device=self.device)
         self.work_h = torch.empty((C,), dtype=dtype, device=self.device)
         self.volume_dev = torch.as_tensor(self.volume, dtype=dtype).to(self.device)
         self.work_r = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
@@ -186,6 +221,10 @@ class DeviceMesh:
             d.ref_state[v] = float(self.ref_state[v])
         d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
         d.work_rvol, d.work_h = p(self.work_rvol), p(self.work_h)
+        d.work_vpack = p(self.work_vpack)
+        for ct in (0, 1):
+            d.dsp_w[ct], d.dsp_i[ct] = p(self.dsp_w[ct]), p(self.dsp_i[ct])
+            d.esp_w[ct], d.esp_i[ct] = p(self.esp_w[ct]), p(self.esp_i[ct])
         d.metric_flux_scale, d.volume = p(self.metric_flux_scale), p(self.volume_dev)
         d.work_r, d.work_s1, d.work_s2 = p(self.work_r), p(self.work_s1), p(self.work_s2)
         d.work_dt, d.work_part = p(self.work_dt), p(self.work_part)
@@ -198,6 +237,7 @@ class DeviceMesh:
             d.rec_owner, d.rec_oslot, d.rec_flags = p(self.rec_owner), p(self.rec_oslot), p(self.rec_flags)
             d.rec_tn, d.cell_inc = p(self.rec_tn), p(self.cell_inc)
             d.work_pack, d.work_tvol = p(self.work_pack), p(self.work_tvol)
+            d.work_nbp = p(self.work_nbp)
             d.work_tsrc, d.work_crec = p(self.work_tsrc), p(self.work_crec)
         self._desc = d
         handle = ctypes.c_void_p()
@@ -292,7 +332,8 @@ class DeviceMesh:
         self.rec_tn = dev(fr.normal)
         self.cell_inc = devi(inc)
         dt = self.dtype
-        self.work_pack = torch.empty((C, T.ns, 8), dtype=dt, device=self.device)
+        self.work_pack = torch.empty((C, 5, T.nfp, 8), dtype=dt, device=self.device)
+        self.work_nbp = torch.empty((C, T.ns, 4), dtype=dt, device=self.device)
         self.work_tvol = torch.empty((C, T.ns, 2), dtype=dt, device=self.device)
         self.work_tsrc = torch.empty((C, T.ns, 3), dtype=dt, device=self.device)
         self.work_crec = torch.empty((F, T.nfp, 2), dtype=dt, device=self.device)
@@ -318,11 +359,14 @@ class DeviceMesh:
         return k, w, torch.zeros(shape, dtype=self.dtype, device=self.device), \
             torch.zeros(shape, dtype=self.dtype, device=self.device)
 
-    def sst_step(self, U, k, w, dt, nu_t, mu_t, mu=1.8e-5):
+    def sst_step(self, U, k, w, dt, nu_t, mu_t, mu=1.8e-5, rates=None):
+        """SST step: k, omega updated in place, or (rates given) dk/dt, domega/dt written to rates (C, ns, 2)."""
+        r = ctypes.c_void_p(rates.data_ptr()) if rates is not None else None
         _lib.check(self.lib.afr_sst_step(self.handle, ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(k.data_ptr()),
                                          ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(self.d_wall.data_ptr()),
                                          ctypes.c_void_p(dt.data_ptr()), ctypes.c_void_p(nu_t.data_ptr()),
-                                         ctypes.c_void_p(mu_t.data_ptr()), float(mu), self._stream()), "afr_sst_step")
+                                         ctypes.c_void_p(mu_t.data_ptr()), r, float(mu), self._stream()),
+                   "afr_sst_step")
 
     def sst_rk3_step(self, U, k, w, nu_t, mu_t, mu=1.8e-5, Pr=0.72, Pr_t=0.9, n_vars_total=7, norm=None):
         """One SST pseudo-step (k, omega, nu_t, mu_t and U updated in place); returns the (1,) FP64 norm."""

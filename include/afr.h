@@ -70,6 +70,10 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   work_grad  (C, ns, 12)             scratch for grad u (9, [i][j] = du_i/dx_j) and grad T (3)
  *   work_umean (C, 5)                  scratch for the per-cell mean |U| (outlier reference)
  *   work_rvol  (C, ns, 5)              scratch for the viscous volume term
+ *   work_vpack (C, ns, 20)             scratch: per-point (dQ, grad u, grad T, mu_t, pad) pack
+ *   dsp_w[t], dsp_i[t] (3, ns, W_t)    sparse rows of D[t] (W = 9 prism / 27 tet at P2; row m of
+ *                                      direction m holds its non-zeros first, zero-padded)
+ *   esp_w[t], esp_i[t] (5, nfp, K_t)   sparse rows of E[t] in slot order (K = nfp prism / ns tet)
  *   work_h     (C)                     scratch for the interior-penalty length mean(det)^(1/3)
  *   metric_flux_scale (C, ns)          sum_m |adj(J)[m, :]| (geometric time-step limit)
  *   volume     (C)                     cell volumes (GL quadrature)
@@ -86,7 +90,8 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   cell_inc   (C, 8) int32            face-record incidences of each cell:
  *                                      record * 16 + role * 8 + cube-face code
  *                                      (role 0 owner, 1 neighbour), -1 unused
- *   work_pack  (C, ns, 8), work_tvol (C, ns, 2), work_tsrc (C, ns, 3), work_crec (F, nfp, 2)
+ *   work_pack  (C, 5, nfp, 8) owner face traces, work_nbp (C, ns, 4), work_tvol (C, ns, 2),
+ *   work_tsrc  (C, ns, 3), work_crec (F, nfp, 2)
  * Wall forces (optional):
  *   n_wall_rec                         owner-primary WALL face records
  *   wrec_owner, wrec_oslot (W) int32   owner cell and owner face slot
@@ -122,6 +127,9 @@ typedef struct AfrMeshDesc {
     void *work_umean;
     void *work_rvol;
     void *work_h;
+    void *work_vpack;
+    const void *dsp_w[2], *esp_w[2];
+    const int32_t *dsp_i[2], *esp_i[2];
     const void *metric_flux_scale, *volume;
     void *work_r, *work_s1, *work_s2, *work_dt;
     double *work_part;
@@ -129,7 +137,7 @@ typedef struct AfrMeshDesc {
     const int32_t *rec_owner, *rec_oslot, *rec_flags;
     const void *rec_tn;
     const int32_t *cell_inc;
-    void *work_pack, *work_tvol, *work_tsrc, *work_crec;
+    void *work_pack, *work_nbp, *work_tvol, *work_tsrc, *work_crec;
     int64_t n_wall_rec;
     const int32_t *wrec_owner, *wrec_oslot;
     const double *wrec_geom, *w1d;
@@ -262,9 +270,11 @@ int afr_wall_distance(const double *sp_xyz, int64_t n_pts, const double *wall_xy
  * F1/F2 blending and eddy viscosity, record-based transport, point-implicit update
  * and floors (src/autoflowcfd/core/fr_solver/turbulence.py:178-330,
  * src/autoflowcfd/core/turbulence/{sst,transport,transport_kernel}.py).  Writes
- * nu_t (C, ns) and mu_t = rho(U) nu_t (C, ns), the eddy viscosity of the step. */
+ * nu_t (C, ns) and mu_t = rho(U) nu_t (C, ns), the eddy viscosity of the step.  With
+ * rates != NULL the fields are left unchanged and rates (C, ns, 2) receives dk/dt and
+ * domega/dt of the update (the turbulence part of a residual evaluation). */
 int afr_sst_step(const AfrMesh *mesh, const void *U, void *k, void *omega, const void *d_wall, const void *dt,
-                 void *nu_t, void *mu_t, double mu_molecular, void *stream);
+                 void *nu_t, void *mu_t, void *rates, double mu_molecular, void *stream);
 
 /* One steady pseudo-step with the SST model, as src/autoflowcfd/core/fr_solver/step.py:15-221:
  * dt_local from U with mu_t = rho(U) nu_t of the previous step (nu_t in/out, zero
