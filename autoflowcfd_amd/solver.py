@@ -1,0 +1,206 @@
+"""GPUFRSolver: the steady FR solver on one MI355X.
+
+Drop-in for the reference's ``GPUFRSolver`` (src/autoflowcfd/core/gpu/gpu_solver.py:36-453,
+src/autoflowcfd/core/gpu/gpu_solver_io.py:13-84): same constructor arguments, ``step(dt)``
+returning the residual RMS, ``solve(...)`` returning the reference's result dict,
+``get_state_cpu`` / ``set_state_from_cpu``, ``save_checkpoint`` / ``load_checkpoint`` and
+``cleanup``.  The arithmetic follows the reference CPU solver (src/autoflowcfd/core/fr_solver/
+step.py:15-221): local time step (cfl.py), SST k-omega step once per pseudo-step, SSP-RK3 mean
+flow with positivity and filter after every stage, FP64 RMS norm over all n_vars components.
+
+All state stays in HBM between steps; every step is one sequence of HIP launches on the
+current torch stream (libafr.so), and the only host synchronisation per step is the FP64
+norm read-back that the ``step`` contract returns.
+"""
+
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .device import DeviceMesh, bc_params_table
+from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
+
+GAMMA = 1.4
+
+
+@dataclass
+class SolverResult:
+    """src/autoflowcfd/core/fr_solver/state.py:13-17."""
+    converged: bool
+    iterations: int
+    final_residual: float
+
+
+def _cons(Q):
+    rho, u, v, w, p = (Q[..., i] for i in range(5))
+    E = p / ((GAMMA - 1.0) * rho) + 0.5 * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, rho * E], axis=-1)
+
+
+class GPUFRSolver:
+    """Steady P1/P2 FR solver (Euler / Navier-Stokes / SST RANS) on one GPU."""
+
+    def __init__(self, mesh, ops=None, order: int = 2, n_vars: int = 5, device_id: int = 0,
+                 time_scheme: str = "ssp_rk3", cfl: float = 1.0, rho_inf: float = 1.225, vel_inf: float = 33.33,
+                 p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
+                 turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
+                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9):
+        """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
+        compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
+        bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
+        or {BC_OUTLET: p}; the default is the freestream along +x (boundary.py:52-145).
+        k_inf / omega_inf: initial turbulence fields (reference defaults 1e-6 and 1.0, sst.py:47-48).
+        """
+        if time_scheme.lower().replace("-", "_") != "ssp_rk3":
+            raise NotImplementedError(f"time_scheme {time_scheme!r}: this backend implements ssp_rk3")
+        if not torch.cuda.is_available():
+            raise RuntimeError("GPUFRSolver needs a visible GPU (HIP device)")
+        self.device = torch.device(f"cuda:{device_id}")
+        torch.cuda.set_device(self.device)
+        self.mesh, self.ops, self.order, self.device_id = mesh, ops, order, device_id
+        self.turb_model_name = str(turb_model).upper()
+        if self.turb_model_name not in ("NONE", "SST"):
+            raise NotImplementedError(f"turbulence model {turb_model!r}: this backend implements NONE and SST")
+        self.n_vars = 7 if self.turb_model_name == "SST" else n_vars
+        self.mu_molecular, self.Pr, self.Pr_t = mu_molecular, Pr, Pr_t
+        self.cfl = cfl
+        self.freestream = {"rho_inf": rho_inf, "vel_inf": vel_inf, "p_inf": p_inf}
+        Q_free = np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf])
+        overrides = dict(boundary_ghost_provider or {})
+        overrides.update(bc_overrides or {})
+        Q_inlet = overrides.get(BC_INLET)
+        p_out = overrides.get(BC_OUTLET)
+        if BC_FARFIELD in overrides:
+            Q_free = np.asarray(overrides[BC_FARFIELD], dtype=np.float64)
+        p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
+        bcp = bc_params_table(Q_free, Q_inlet, p_out)
+        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp,
+                             turbulence="SST" if self.turb_model_name == "SST" else None)
+        C, ns = self.dm.n_cells, self.dm.ns
+        U0 = _cons(np.broadcast_to(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]), (C, ns, 5)))
+        self.U = torch.as_tensor(U0, dtype=dtype).to(self.device).contiguous()
+        if self.turb_model_name == "SST":
+            self.k, self.omega, self.nu_t, self.mu_t = self.dm.sst_fields(k_inf, omega_inf)
+        else:
+            self.k = self.omega = self.nu_t = self.mu_t = None
+        self.residual_history = []
+        self.iteration = 0
+
+    # ------------------------------------------------------------------ stepping
+    def step(self, dt: float = 0.0) -> float:
+        """One steady pseudo-step (dt is ignored: local time stepping); returns the residual RMS."""
+        if self.turb_model_name == "SST":
+            norm = self.dm.sst_rk3_step(self.U, self.k, self.omega, self.nu_t, self.mu_t, mu=self.mu_molecular,
+                                        Pr=self.Pr, Pr_t=self.Pr_t, n_vars_total=self.n_vars)
+        else:
+            norm = self.dm.ssp_rk3_step(self.U, None, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t,
+                                        n_vars_total=self.n_vars)
+        res = float(norm.item())
+        self.residual_history.append(res)
+        self.iteration += 1
+        return res
+
+    def solve(self, max_iter: int = 1000, dt: float = 1e-4, tol: float = 1e-6, output_interval: int = 10):
+        """Steady loop (gpu_solver.py:403-453): stop at res < tol, on a non-finite residual or max_iter."""
+        converged = False
+        final = float("inf")
+        for i in range(max_iter):
+            t0 = time.time()
+            final = self.step(dt)
+            if output_interval and (i == 0 or (i + 1) % output_interval == 0):
+                print(f"GPU Iter {i + 1}: Residual = {final:.6e} | Time/step: {time.time() - t0:.3f}s", flush=True)
+            if final < tol:
+                converged = True
+                break
+            if not np.isfinite(final):
+                break
+        return {"converged": converged, "iterations": self.iteration, "final_residual": final,
+                "residual_history": self.residual_history}
+
+    # ------------------------------------------------------------------ residuals (reference method names)
+    def compute_inviscid_residual_gpu(self, U_trial=None):
+        U = self.U if U_trial is None else U_trial
+        return self.dm.inviscid_residual(U)
+
+    def compute_viscous_residual_gpu(self, U_trial=None, mu_t_field=None):
+        U = self.U if U_trial is None else U_trial
+        return self.dm.viscous_residual(U, mu_t_field, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t)
+
+    # ------------------------------------------------------------------ forces
+    def compute_aerodynamic_coefficients(self, reference_area: float, include_viscous: bool = True):
+        """Cd, Cl, Cs from the FP64 WALL force integral (fr_coefficients.py:27-152): fixed axes,
+        Cd = F_x / (q A), Cl = F_z / (q A), Cs = F_y / (q A), q = rho_inf vel_inf^2 / 2."""
+        mu_t = self.mu_t if self.turb_model_name == "SST" else None
+        F = self.dm.wall_forces(self.U, mu_t, mu=self.mu_molecular).cpu().numpy()
+        f = F[:3] + (F[3:] if include_viscous else 0.0)
+        q = 0.5 * self.freestream["rho_inf"] * self.freestream["vel_inf"] ** 2
+        d = max(q * reference_area, 1e-300)
+        return {"Cd": float(f[0] / d), "Cl": float(f[2] / d), "Cs": float(f[1] / d),
+                "F_pressure": F[:3].tolist(), "F_viscous": F[3:].tolist()}
+
+    # ------------------------------------------------------------------ state I/O (gpu_solver_io.py:13-84)
+    def _primitives(self, U):
+        rho = np.maximum(U[..., 0], 1e-10)
+        u, v, w = U[..., 1] / rho, U[..., 2] / rho, U[..., 3] / rho
+        p = np.maximum((U[..., 4] - rho * 0.5 * (u * u + v * v + w * w)) * 0.4, 1.0)
+        return np.stack([rho, u, v, w, p], axis=-1)
+
+    def get_state_cpu(self):
+        """{'U', 'Q'} as FP64 NumPy arrays (n_cells, n_sps, n_vars); with SST the last two
+        components are rho k, rho omega (U) and k, omega (Q)."""
+        U = self.U.double().cpu().numpy()
+        Q = self._primitives(U)
+        if self.n_vars > 5:
+            k = self.k.double().cpu().numpy() if self.k is not None else np.zeros(U.shape[:2])
+            w = self.omega.double().cpu().numpy() if self.omega is not None else np.zeros(U.shape[:2])
+            U = np.concatenate([U, (Q[..., 0] * k)[..., None], (Q[..., 0] * w)[..., None]], axis=-1)
+            Q = np.concatenate([Q, k[..., None], w[..., None]], axis=-1)
+        return {"U": U, "Q": Q}
+
+    def set_state_from_cpu(self, U_np: np.ndarray):
+        U_np = np.asarray(U_np, dtype=np.float64)
+        if U_np.shape[:2] != tuple(self.U.shape[:2]):
+            raise ValueError(f"state shape {U_np.shape} does not match ({self.U.shape[0]}, {self.U.shape[1]}, ...)")
+        self.U.copy_(torch.as_tensor(np.ascontiguousarray(U_np[..., :5]), dtype=self.U.dtype))
+        if U_np.shape[-1] >= 7 and self.k is not None:
+            rho = np.maximum(U_np[..., 0], 1e-10)
+            self.k.copy_(torch.as_tensor(U_np[..., 5] / rho, dtype=self.k.dtype))
+            self.omega.copy_(torch.as_tensor(U_np[..., 6] / rho, dtype=self.omega.dtype))
+
+    def save_checkpoint(self, path: str):
+        """NumPy .npz with the reference's HDF5 layout (U, Q, k, omega, residual_history + attrs)."""
+        st = self.get_state_cpu()
+        data = {"U": st["U"], "Q": st["Q"], "iteration": np.int64(self.iteration),
+                "n_cells": np.int64(self.dm.n_cells), "n_sps": np.int64(self.dm.ns), "order": np.int64(self.order),
+                "time_scheme": np.array("ssp_rk3"), "cfl": np.float64(self.cfl),
+                "residual_history": np.asarray(self.residual_history, dtype=np.float64)}
+        if self.k is not None:
+            data["k"] = self.k.double().cpu().numpy()
+            data["omega"] = self.omega.double().cpu().numpy()
+            data["nu_t"] = self.nu_t.double().cpu().numpy()
+        d = os.path.dirname(os.path.abspath(path))
+        os.makedirs(d, exist_ok=True)
+        np.savez(path, **data)
+
+    def load_checkpoint(self, path: str):
+        with np.load(path, allow_pickle=False) as f:
+            if int(f["n_cells"]) != self.dm.n_cells or int(f["n_sps"]) != self.dm.ns:
+                raise ValueError("checkpoint does not match this mesh / order")
+            self.U.copy_(torch.as_tensor(np.ascontiguousarray(f["U"][..., :5]), dtype=self.U.dtype))
+            self.iteration = int(f["iteration"])
+            self.residual_history = f["residual_history"].tolist()
+            if self.k is not None and "k" in f:
+                self.k.copy_(torch.as_tensor(f["k"], dtype=self.k.dtype))
+                self.omega.copy_(torch.as_tensor(f["omega"], dtype=self.omega.dtype))
+                if "nu_t" in f:
+                    self.nu_t.copy_(torch.as_tensor(f["nu_t"], dtype=self.nu_t.dtype))
+
+    def cleanup(self):
+        """Release the device buffers of this solver."""
+        for name in ("U", "k", "omega", "nu_t", "mu_t"):
+            setattr(self, name, None)
+        self.dm = None
+        torch.cuda.empty_cache()

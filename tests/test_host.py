@@ -162,3 +162,15 @@ def test_error_paths_without_gpu():
     assert lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(h)) != 0
     lib.afr_last_error(buf, 256)
     assert b"ABI" in buf.value
+
+
+def test_meshio_round_trip(tmp_path):
+    from autoflowcfd_amd.meshio import load_mesh, save_mesh
+    m = wall_box(seed=3)
+    p = str(tmp_path / "m.npz")
+    save_mesh(p, m)
+    m2 = load_mesh(p)
+    assert m2.n_cells == m.n_cells and m2.faces.n_faces == m.faces.n_faces
+    key = lambda fr: {tuple(t): c for t, c in zip(map(tuple, fr.node_ids), fr.bc)}  # noqa: E731
+    assert key(m2.faces) == key(m.faces)
+    assert np.array_equal(np.sort(m2.wall_nodes), np.sort(m.wall_nodes))
