@@ -66,3 +66,23 @@ def test_solver_sst_runs_and_cli():
                         "--reference-area", "1.0"], cwd=ROOT, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert '"Cd"' in r.stdout
+
+
+def test_graph_replay_matches_eager():
+    """A captured pseudo-step, replayed several times with eager work and syncs between, gives
+    the eager trajectory bit for bit."""
+    import torch
+    from autoflowcfd_amd.solver import GPUFRSolver
+    print("[gpu] hipGraph pseudo-step replay", flush=True)
+    mk = lambda: GPUFRSolver(wall_box(seed=6), order=2, vel_inf=51.04, turb_model="SST",  # noqa: E731
+                             k_inf=0.3908, omega_inf=2660.0)
+    a, b = mk(), mk()
+    eager = [a.step() for _ in range(4)]
+    b.capture_graph()            # includes one eager warm-up step
+    replay = [b.residual_history[-1]]
+    for _ in range(3):
+        replay.append(b.step())
+        torch.cuda.synchronize()
+        _ = b.get_state_cpu()    # eager work between replays
+    assert replay == eager, (replay, eager)
+    assert np.array_equal(a.get_state_cpu()["U"], b.get_state_cpu()["U"])
