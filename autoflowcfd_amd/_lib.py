@@ -5,7 +5,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafr.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 c_vp = ctypes.c_void_p
 
@@ -22,7 +22,8 @@ class AfrMeshDesc(ctypes.Structure):
         ("c2f", c_vp * 2), ("op", c_vp * 2), ("E", c_vp * 2), ("D", c_vp * 2), ("filt", c_vp * 2),
         ("gL", c_vp), ("gR", c_vp), ("fp_of_sp", c_vp), ("coord_of_sp", c_vp), ("prism_tab", c_vp),
         ("prism_schedule", c_vp),
-        ("gcl", c_vp), ("bc_params", c_vp), ("ref_state", ctypes.c_double * 5),
+        ("gcl", c_vp), ("bc_params", c_vp), ("cell_case", c_vp), ("n_cases", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32), ("case_prism_off", c_vp), ("case_tet_off", c_vp), ("case_wall_off", c_vp), ("ref_state", ctypes.c_double * 5),
         ("work_q", c_vp), ("work_grad", c_vp), ("work_umean", c_vp),
         ("work_rvol", c_vp), ("work_h", c_vp),
         ("work_vpack", c_vp), ("dsp_w", c_vp * 2), ("esp_w", c_vp * 2), ("dsp_i", c_vp * 2), ("esp_i", c_vp * 2),
@@ -81,6 +82,8 @@ SIGNATURES = {
                                         ctypes.c_double, ctypes.c_int32, c_vp, c_vp]),
     "afr_local_time_step_nut": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
     "afr_forces_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
+    "afr_residual_norm_cases": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
+    "afr_forces_cases_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
 }
 
 _lib = None

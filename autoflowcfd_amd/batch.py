@@ -1,0 +1,199 @@
+"""Batched multi-case form: independent steady cases of one order stepped as one mesh.
+
+The reference solves one case per process (src/autoflowcfd/core/gpu/gpu_solver.py:36-453, one
+``GPUFRSolver`` per mesh).  Small cases leave most of an MI355X idle: a P2 case of 16k hex
+cells is ~130 workgroups per launch against 256 CUs.  ``CaseBatch`` packs many cases into the
+disjoint union of their meshes (``mesh.concat_meshes``): cells are numbered prisms first then
+tets, case by case (CSR-style offsets ``prism_off`` / ``tet_off`` / wall-record offsets), so
+every kernel of a pseudo-step is ONE launch over all cases, and one HIP graph replays a whole
+batched pseudo-step.
+
+What stays per case:
+  - boundary states: a (n_cases, 8, 5) table indexed by the per-cell case id at every ghost
+    state (FARFIELD / INLET / OUTLET of each case's own free stream, e.g. its Mach and yaw);
+  - initial state and SST k, omega of each case's free stream;
+  - wall distance (computed case by case: cases may overlap in space);
+  - residual norm (FP64, reduced case by case in the order of a single-case reduction) and
+    wall forces (per-case FP64 sums over the case's own wall records).
+What is common: order, storage precision, viscosity, Pr, Pr_t and the constant reference state
+of the perturbation form.  With the same reference state, a case stepped alone
+(``GPUFRSolver(..., ref_state=batch.ref_state)``) follows its batched trajectory bit for bit:
+every kernel computes each cell from that cell's own data, so packing changes only where a
+cell lives, never what is computed for it.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .device import DeviceMesh, bc_params_table
+from .face_slots import build_slots, concat_slots
+from .fr_tables import tables
+from .mesh import concat_meshes
+
+GAMMA = 1.4
+
+
+@dataclass
+class Case:
+    """One steady case: a mesh with face records and its free stream (primitive rho, u, v, w, p)."""
+    mesh: object
+    Q_free: np.ndarray
+    k_inf: float = 1e-6
+    omega_inf: float = 1.0
+    reference_area: float = 1.0
+    name: str = ""
+
+
+def cons(Q):
+    """Primitive (..., 5) -> conserved (..., 5)."""
+    Q = np.asarray(Q, dtype=np.float64)
+    rho, u, v, w, p = (Q[..., i] for i in range(5))
+    E = p / ((GAMMA - 1.0) * rho) + 0.5 * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, rho * E], axis=-1)
+
+
+def static_ref_state(cases):
+    """Common reference state of a batch: mean free-stream density and pressure at rest.
+
+    The perturbation form needs one constant state per mesh; leaving the velocity out keeps it
+    equally close to every case whatever its speed and direction."""
+    Q = np.array([np.asarray(c.Q_free, dtype=np.float64) for c in cases])
+    return np.array([Q[:, 0].mean(), 0.0, 0.0, 0.0, Q[:, 4].mean()])
+
+
+def free_stream_coefficients(F, Q_free, reference_area, include_viscous=True):
+    """Cd, Cl, Cs from a (6,) FP64 wall force (fixed axes; src/autoflowcfd/postprocess/
+    fr_coefficients.py:27-152): Cd = F_x / (q A), Cl = F_z / (q A), Cs = F_y / (q A),
+    q = rho |u_inf|^2 / 2."""
+    Q = np.asarray(Q_free, dtype=np.float64)
+    q = 0.5 * Q[0] * float(np.sum(Q[1:4] ** 2))
+    f = F[:3] + (F[3:] if include_viscous else 0.0)
+    d = max(q * reference_area, 1e-300)
+    return {"Cd": float(f[0] / d), "Cl": float(f[2] / d), "Cs": float(f[1] / d),
+            "F_pressure": F[:3].tolist(), "F_viscous": F[3:].tolist()}
+
+
+class CaseBatch:
+    """Many independent steady cases of one order on one GPU, one launch per kernel per stage."""
+
+    def __init__(self, cases, order=2, dtype=torch.float32, turb_model="SST", mu_molecular=1.8e-5, Pr=0.72,
+                 Pr_t=0.9, device_id=0, ref_state=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("CaseBatch needs a visible GPU (HIP device)")
+        if not cases:
+            raise ValueError("empty batch")
+        self.cases = list(cases)
+        self.n_cases = len(self.cases)
+        self.order, self.dtype = order, dtype
+        self.device = torch.device(f"cuda:{device_id}")
+        torch.cuda.set_device(self.device)
+        self.turb = str(turb_model).upper()
+        if self.turb not in ("NONE", "SST"):
+            raise NotImplementedError(f"turbulence model {turb_model!r}: this backend implements NONE and SST")
+        self.n_vars = 7 if self.turb == "SST" else 5
+        self.mu, self.Pr, self.Pr_t = mu_molecular, Pr, Pr_t
+        self.ref_state = static_ref_state(self.cases) if ref_state is None else np.asarray(ref_state, np.float64)
+        mesh, cell_case, po, to, _ = concat_meshes([c.mesh for c in self.cases])
+        self.mesh, self.cell_case, self.prism_off, self.tet_off = mesh, cell_case, po, to
+        bcp = np.stack([bc_params_table(np.asarray(c.Q_free, dtype=np.float64)) for c in self.cases])
+        # slot tables case by case: each cell gets exactly the rows its case alone has
+        T = tables(order)
+        parts = [c.mesh for c in self.cases]
+        slots = concat_slots([build_slots(p, T, with_geometry=False) for p in parts], [p.n_prism for p in parts],
+                             [p.n_tet for p in parts], T.nfp)
+        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=self.ref_state,
+                             slots=slots, turbulence="SST" if self.turb == "SST" else None, cell_case=cell_case)
+        ns = self.dm.ns
+        Ucase = cons(np.array([c.Q_free for c in self.cases], dtype=np.float64))       # (n_cases, 5)
+        U0 = np.ascontiguousarray(np.broadcast_to(Ucase[cell_case][:, None, :], (mesh.n_cells, ns, 5)))
+        self.U = torch.as_tensor(U0, dtype=dtype).to(self.device).contiguous()
+        if self.turb == "SST":
+            kc = np.array([c.k_inf for c in self.cases])[cell_case]
+            wc = np.array([c.omega_inf for c in self.cases])[cell_case]
+            shape = (mesh.n_cells, ns)
+            self.k = torch.as_tensor(np.broadcast_to(kc[:, None], shape).copy(), dtype=dtype).to(self.device)
+            self.omega = torch.as_tensor(np.broadcast_to(wc[:, None], shape).copy(), dtype=dtype).to(self.device)
+            self.nu_t = torch.zeros(shape, dtype=dtype, device=self.device)
+            self.mu_t = torch.zeros(shape, dtype=dtype, device=self.device)
+        else:
+            self.k = self.omega = self.nu_t = self.mu_t = None
+        self.norms = torch.zeros((self.n_cases,), dtype=torch.float64, device=self.device)
+        self.forces = torch.zeros((self.n_cases, 6), dtype=torch.float64, device=self.device)
+        self.history = []
+        self.iteration = 0
+        self._graph = None
+
+    # ------------------------------------------------------------------ stepping
+    def _launch_step(self):
+        """Enqueue one batched pseudo-step on the current stream (per-case norms into self.norms)."""
+        if self.turb == "SST":
+            return self.dm.sst_rk3_step(self.U, self.k, self.omega, self.nu_t, self.mu_t, mu=self.mu, Pr=self.Pr,
+                                        Pr_t=self.Pr_t, n_vars_total=self.n_vars, norm=self.norms)
+        return self.dm.ssp_rk3_step(self.U, None, mu=self.mu, Pr=self.Pr, Pr_t=self.Pr_t, n_vars_total=self.n_vars,
+                                    norm=self.norms)
+
+    def capture_graph(self):
+        """Record one batched pseudo-step into a HIP graph (after one eager warm-up step, counted)."""
+        torch.cuda.synchronize(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self._launch_step()
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        self.history.append(self.norms.cpu().numpy().copy())
+        self.iteration += 1
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._launch_step()
+        self._graph = g
+        return g
+
+    def step(self, sync=True):
+        """One pseudo-step of every case; returns the per-case residual norms (host, FP64) when
+        sync, else None (the norms stay in self.norms on the device)."""
+        if self._graph is not None:
+            self._graph.replay()
+        else:
+            self._launch_step()
+        self.iteration += 1
+        if not sync:
+            return None
+        n = self.norms.cpu().numpy().copy()
+        self.history.append(n)
+        return n
+
+    def run(self, n_steps, sync_every=1):
+        """n_steps pseudo-steps; the norms are read back every ``sync_every`` steps."""
+        for i in range(n_steps):
+            self.step(sync=((i + 1) % sync_every == 0) or i == n_steps - 1)
+        return np.array(self.history)
+
+    # ------------------------------------------------------------------ per-case results
+    def case_cells(self, i):
+        """Cell indices of case i in the batched mesh, in the case's own order (prisms, tets)."""
+        P = self.mesh.n_prism
+        return np.concatenate([np.arange(self.prism_off[i], self.prism_off[i + 1]),
+                               P + np.arange(self.tet_off[i], self.tet_off[i + 1])])
+
+    def case_state(self, i):
+        """Conserved state (C_i, ns, 5) of case i and, with SST, k and omega (host, FP64)."""
+        idx = torch.as_tensor(self.case_cells(i), device=self.device)
+        out = {"U": self.U.index_select(0, idx).double().cpu().numpy()}
+        if self.turb == "SST":
+            out["k"] = self.k.index_select(0, idx).double().cpu().numpy()
+            out["omega"] = self.omega.index_select(0, idx).double().cpu().numpy()
+        return out
+
+    def wall_forces(self):
+        """(n_cases, 6) FP64 host array: pressure force, viscous force of every case."""
+        mu_t = self.mu_t if self.turb == "SST" else None
+        self.dm.wall_forces_cases(self.U, mu_t, mu=self.mu, out=self.forces)
+        return self.forces.cpu().numpy()
+
+    def coefficients(self, include_viscous=True):
+        F = self.wall_forces()
+        return [free_stream_coefficients(F[i], c.Q_free, c.reference_area, include_viscous)
+                for i, c in enumerate(self.cases)]

@@ -124,18 +124,34 @@ __global__ void __launch_bounds__(256) k_sum_forces(const double* part, int64_t 
     }
 }
 
+int launch_sum_forces(const double* part, int64_t n, double* out, hipStream_t st);
+template <int P, typename real>
+int launch_wall_forces_records(const ForceArgs<real>& a, hipStream_t st);
+
 template <int P, typename real>
 int launch_wall_forces(const ForceArgs<real>& a, hipStream_t st) {
-    if (a.n_wall > 0) {
-        const int64_t need = (a.n_wall + kWaves - 1) / kWaves;
-        hipLaunchKernelGGL((k_wall_forces<P, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
-                           st, a);
-        if (hipGetLastError() != hipSuccess) return -1;
-    }
-    hipLaunchKernelGGL(k_sum_forces, dim3(1), dim3(256), 0, st, (const double*)a.part, a.n_wall, a.out);
+    if (launch_wall_forces_records<P, real>(a, st)) return -1;
+    return launch_sum_forces(a.part, a.n_wall, a.out, st);
+}
+
+int launch_sum_forces(const double* part, int64_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_forces, dim3(1), dim3(256), 0, st, part, n, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int P, typename real>
+int launch_wall_forces_records(const ForceArgs<real>& a, hipStream_t st) {
+    if (a.n_wall <= 0) return 0;
+    const int64_t need = (a.n_wall + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL((k_wall_forces<P, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
+                       st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template int launch_wall_forces_records<1, float>(const ForceArgs<float>&, hipStream_t);
+template int launch_wall_forces_records<2, float>(const ForceArgs<float>&, hipStream_t);
+template int launch_wall_forces_records<1, double>(const ForceArgs<double>&, hipStream_t);
+template int launch_wall_forces_records<2, double>(const ForceArgs<double>&, hipStream_t);
 template int launch_wall_forces<1, float>(const ForceArgs<float>&, hipStream_t);
 template int launch_wall_forces<2, float>(const ForceArgs<float>&, hipStream_t);
 template int launch_wall_forces<1, double>(const ForceArgs<double>&, hipStream_t);

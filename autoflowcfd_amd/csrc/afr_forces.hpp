@@ -28,4 +28,9 @@ struct ForceArgs {
 template <int P, typename real>
 int launch_wall_forces(const ForceArgs<real>& a, hipStream_t st);
 
+// per-record force partials only (a.part), and the fixed-order sum of n records of part into out[6]
+template <int P, typename real>
+int launch_wall_forces_records(const ForceArgs<real>& a, hipStream_t st);
+int launch_sum_forces(const double* part, int64_t n, double* out, hipStream_t st);
+
 }  // namespace afr_forces

@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
 #pragma unroll
                 for (int v = 0; v < 5; ++v) qn[v] = s_ref[v] + (double)dqn[v];
             } else {
-                ghost_state(bc, qo, tn, a.bcp, qn);
+                ghost_state(bc, qo, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[cell] : 0), qn);
             }
             const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
             const double A0 = ar[0], A1 = ar[1], A2 = ar[2];

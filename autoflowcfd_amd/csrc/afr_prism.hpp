@@ -61,6 +61,7 @@ struct PrismArgs {
     const int* fp_of_sp;
     const int* coord_of_sp;
     const double* bcp;
+    const int* cell_case;  // null: one case
     double qref[5];
     double fref[15];
     int suppress;

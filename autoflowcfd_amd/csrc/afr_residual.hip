@@ -16,6 +16,7 @@
 // src/autoflowcfd/core/fr_residual/inviscid_kernel.py:87-284,
 // src/autoflowcfd/core/fr_operators/troubled_cell.py:339-408.
 #include <hip/hip_runtime.h>
+#include <vector>
 
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,8 @@ void afr_set_error(const std::string& msg) { g_err = msg; }
 
 struct AfrMesh {
     AfrMeshDesc d;
+    // host copies of the case offsets of a batched mesh (cell_case set)
+    std::vector<int64_t> case_prism_off, case_tet_off, case_wall_off;
 };
 
 namespace {
@@ -100,7 +103,8 @@ struct Args {
     const int* fp_of_sp;
     const int* coord_of_sp;
     const real* gcl;      // (C, NS, 3) constant-state divergence
-    const double* bcp;    // (8, 5) per-BC reference states
+    const double* bcp;    // (8, 5) per-BC reference states ((n_cases, 8, 5) with cell_case)
+    const int* cell_case; // null: one case
     double qref[5];       // primitive reference state
     double fref[15];      // F(qref)[i][v]
     int suppress;
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) qn[v] = a.qref[v] + (double)dqn[v];
                 } else {
-                    ghost_state(bc, qo, tn, a.bcp, qn);
+                    ghost_state(bc, qo, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[gc] : 0), qn);
                 }
                 const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
                 double A0 = ar[0], A1 = ar[1], A2 = ar[2];
@@ -373,6 +377,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         p.fp_of_sp = d.fp_of_sp;
         p.coord_of_sp = d.coord_of_sp;
         p.bcp = d.bc_params;
+        p.cell_case = d.cell_case;
         for (int v = 0; v < 5; ++v) p.qref[v] = d.ref_state[v];
         for (int k = 0; k < 15; ++k) p.fref[k] = fref[k];
         p.suppress = suppress;
@@ -408,6 +413,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         a.coord_of_sp = d.coord_of_sp;
         a.gcl = (const real*)d.gcl;
         a.bcp = d.bc_params;
+        a.cell_case = d.cell_case;
         for (int v = 0; v < 5; ++v) a.qref[v] = d.ref_state[v];
         for (int k = 0; k < 15; ++k) a.fref[k] = fref[k];
         a.suppress = suppress;
@@ -456,8 +462,31 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
         !desc->work_umean || !desc->prism_tab || !desc->rows_nnz || !desc->rows_idx || !desc->rows_w ||
         (desc->n_prism > 0 && !desc->prism_schedule))
         return fail("afr_mesh_create: missing device array");
+    std::vector<int64_t> po, to, wo;
+    if (desc->cell_case) {
+        const int nc = desc->n_cases;
+        if (nc < 1) return fail("afr_mesh_create: cell_case set with n_cases < 1");
+        if (!desc->case_prism_off || !desc->case_tet_off) return fail("afr_mesh_create: batched mesh without case offsets");
+        po.assign(desc->case_prism_off, desc->case_prism_off + nc + 1);
+        to.assign(desc->case_tet_off, desc->case_tet_off + nc + 1);
+        if (po[0] != 0 || to[0] != 0 || po[nc] != desc->n_prism || to[nc] != desc->n_cells - desc->n_prism)
+            return fail("afr_mesh_create: case offsets must cover the prisms and the tets");
+        if (desc->case_wall_off) {
+            wo.assign(desc->case_wall_off, desc->case_wall_off + nc + 1);
+            if (wo[0] != 0 || wo[nc] != desc->n_wall_rec)
+                return fail("afr_mesh_create: case wall offsets must cover the wall records");
+        }
+        for (int i = 0; i < nc; ++i)
+            if (po[i + 1] < po[i] || to[i + 1] < to[i] || (!wo.empty() && wo[i + 1] < wo[i]))
+                return fail("afr_mesh_create: case offsets not ascending");
+    }
     AfrMesh* m = new AfrMesh;
     m->d = *desc;
+    m->case_prism_off = std::move(po);
+    m->case_tet_off = std::move(to);
+    m->case_wall_off = std::move(wo);
+    // the host arrays are the caller's: only the copies are kept
+    m->d.case_prism_off = m->d.case_tet_off = m->d.case_wall_off = nullptr;
     *out = m;
     return 0;
 }
@@ -563,6 +592,7 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.fp_of_sp = d.fp_of_sp;
         a.coord_of_sp = d.coord_of_sp;
         a.bcp = d.bc_params;
+        a.cell_case = d.cell_case;
         for (int v = 0; v < 5; ++v) a.qref[v] = d.ref_state[v];
         a.mu = mu;
         a.Pr = Pr;
@@ -705,6 +735,31 @@ int afr_residual_norm_f64(const AfrMesh* mesh, const void* R, int32_t n_vars_tot
     return rc ? fail("residual norm launch failed") : 0;
 }
 
+int afr_residual_norm_cases(const AfrMesh* mesh, const void* R, int32_t n_vars_total, double* out, void* stream) {
+    if (!mesh || !R || !out) return fail("afr_residual_norm_cases: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!d.work_part) return fail("afr_residual_norm_cases: mesh has no work_part buffer");
+    if (n_vars_total < 5) return fail("afr_residual_norm_cases: n_vars_total < 5");
+    if (mesh->case_prism_off.empty()) return afr_residual_norm_f64(mesh, R, n_vars_total, out, stream);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t cv = (int64_t)ns_of(d) * 5;
+    const auto& po = mesh->case_prism_off;
+    const auto& to = mesh->case_tet_off;
+    for (int i = 0; i < d.n_cases; ++i) {
+        // the work_part buffer is reused case after case (stream order)
+        const int64_t a0 = po[i] * cv, n0 = (po[i + 1] - po[i]) * cv;
+        const int64_t a1 = (d.n_prism + to[i]) * cv, n1 = (to[i + 1] - to[i]) * cv;
+        const double n_total = (double)((n0 + n1) / 5) * n_vars_total;
+        const int rc = d.dtype == AFR_F32
+                           ? afr_step::launch_norm2<float>((const float*)R, a0, n0, a1, n1, n_total, d.work_part,
+                                                           afr_step::kNormParts, out + i, st)
+                           : afr_step::launch_norm2<double>((const double*)R, a0, n0, a1, n1, n_total, d.work_part,
+                                                            afr_step::kNormParts, out + i, st);
+        if (rc) return fail("residual norm launch failed");
+    }
+    return 0;
+}
+
 static int rk3_stages(const AfrMesh* mesh, void* U, const void* mu_t, double mu, double Pr, double Pr_t,
                       int32_t n_vars_total, double* norm, void* stream) {
     const AfrMeshDesc& d = mesh->d;
@@ -714,7 +769,7 @@ static int rk3_stages(const AfrMesh* mesh, void* U, const void* mu_t, double mu,
                     : afr_inviscid_residual(mesh, X, R, 1, stream);
     };
     if (rhs(U, d.work_r)) return -1;
-    if (afr_residual_norm_f64(mesh, d.work_r, n_vars_total, norm, stream)) return -1;
+    if (afr_residual_norm_cases(mesh, d.work_r, n_vars_total, norm, stream)) return -1;
     // Shu-Osher SSP-RK3: U1 = U0 + dt L0; U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L1; U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L2
     if (afr_rk_stage(mesh, U, nullptr, d.work_r, d.work_dt, d.work_s1, 1.0, 0.0, 1.0, stream)) return -1;
     if (rhs(d.work_s1, d.work_r)) return -1;
@@ -839,8 +894,14 @@ int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const
                               : run_sst<double>(d, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st);
 }
 
-int afr_forces_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double* out, void* stream) {
-    if (!mesh || !U || !out) return fail("afr_forces_f64: null argument");
+// wall_off == nullptr: one sum over every record into out[6]; else per case i the records
+// [wall_off[i], wall_off[i+1]) into out[6 i .. 6 i + 5]
+static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, bool per_case, double* out,
+                       void* stream) {
+    const int64_t* wall_off = per_case ? mesh->case_wall_off.data() : nullptr;
+    const int n_cases = mesh->d.n_cases;
+    if (per_case && mesh->case_wall_off.empty())
+        return fail("afr_forces_cases_f64: mesh has no per-case wall offsets");
     const AfrMeshDesc& d = mesh->d;
     if (d.n_wall_rec < 0 || (d.n_wall_rec > 0 && (!d.wrec_owner || !d.wrec_oslot || !d.wrec_geom || !d.w1d ||
                                                   !d.work_fpart || !d.adj)))
@@ -866,11 +927,30 @@ int afr_forces_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double 
         a.mu = mu;
         a.part = d.work_fpart;
         a.out = out;
-        return d.order == 1 ? afr_forces::launch_wall_forces<1, real>(a, st)
-                            : afr_forces::launch_wall_forces<2, real>(a, st);
+        if (!wall_off)
+            return d.order == 1 ? afr_forces::launch_wall_forces<1, real>(a, st)
+                                : afr_forces::launch_wall_forces<2, real>(a, st);
+        if (d.order == 1 ? afr_forces::launch_wall_forces_records<1, real>(a, st)
+                         : afr_forces::launch_wall_forces_records<2, real>(a, st))
+            return -1;
+        for (int i = 0; i < n_cases; ++i)
+            if (afr_forces::launch_sum_forces(d.work_fpart + wall_off[i] * 6, wall_off[i + 1] - wall_off[i],
+                                              out + 6 * i, st))
+                return -1;
+        return 0;
     };
     const int rc = d.dtype == AFR_F32 ? fill((float*)nullptr) : fill((double*)nullptr);
     return rc ? fail("wall force kernel launch failed") : 0;
+}
+
+int afr_forces_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double* out, void* stream) {
+    if (!mesh || !U || !out) return fail("afr_forces_f64: null argument");
+    return wall_forces(mesh, U, mu_t, mu, false, out, stream);
+}
+
+int afr_forces_cases_f64(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double* out, void* stream) {
+    if (!mesh || !U || !out) return fail("afr_forces_cases_f64: null argument");
+    return wall_forces(mesh, U, mu_t, mu, true, out, stream);
 }
 
 }  // extern "C"

@@ -122,13 +122,17 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
     }
 }
 
-// partial sums of x^2 in FP64, one per workgroup
+// partial sums of x^2 in FP64, one per workgroup, over the logical array x[a0, a0 + n0) followed
+// by x[a1, a1 + n1) (two ranges: the prism and the tet cells of one case of a batched mesh; the
+// same partition and order as one contiguous array of n0 + n1 values)
 template <typename real>
-__global__ void __launch_bounds__(kThreads) k_sq_partial(const real* __restrict__ x, int64_t n, double* part) {
+__global__ void __launch_bounds__(kThreads) k_sq_partial(const real* __restrict__ x, int64_t a0, int64_t n0,
+                                                         int64_t a1, int64_t n1, double* part) {
     __shared__ double s[kThreads];
     double acc = 0.0;
+    const int64_t n = n0 + n1;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const double v = (double)x[i];
+        const double v = (double)x[i < n0 ? a0 + i : a1 + (i - n0)];
         acc += v * v;
     }
     s[threadIdx.x] = acc;
@@ -184,9 +188,15 @@ int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const
 
 template <typename real>
 int launch_norm(const real* x, int64_t n, double n_total, double* part, int n_part_max, double* out, hipStream_t st) {
-    int g = grid_for(n, kThreads * 8);
+    return launch_norm2(x, 0, n, 0, 0, n_total, part, n_part_max, out, st);
+}
+
+template <typename real>
+int launch_norm2(const real* x, int64_t a0, int64_t n0, int64_t a1, int64_t n1, double n_total, double* part,
+                 int n_part_max, double* out, hipStream_t st) {
+    int g = grid_for(n0 + n1, kThreads * 8);
     g = g < n_part_max ? g : n_part_max;
-    hipLaunchKernelGGL((k_sq_partial<real>), dim3(g), dim3(kThreads), 0, st, x, n, part);
+    hipLaunchKernelGGL((k_sq_partial<real>), dim3(g), dim3(kThreads), 0, st, x, a0, n0, a1, n1, part);
     if (hipGetLastError() != hipSuccess) return -1;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, st, (const double*)part, g, n_total, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -201,6 +211,10 @@ template int launch_rk_stage<float>(int, const float*, const float*, const float
 template int launch_rk_stage<double>(int, const double*, const double*, const double*, const double*, double*,
                                      const double*, const double*, int64_t, int64_t, double, double, double, double,
                                      hipStream_t);
+template int launch_norm2<float>(const float*, int64_t, int64_t, int64_t, int64_t, double, double*, int, double*,
+                                 hipStream_t);
+template int launch_norm2<double>(const double*, int64_t, int64_t, int64_t, int64_t, double, double*, int, double*,
+                                  hipStream_t);
 template int launch_norm<float>(const float*, int64_t, double, double*, int, double*, hipStream_t);
 template int launch_norm<double>(const double*, int64_t, double, double*, int, double*, hipStream_t);
 

@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                                       (double)a.slot_tn[sidx * 3 + 2]};
 #pragma unroll
                 for (int v = 0; v < 5; ++v) qod[v] = s_ref[v] + (double)po[v];
-                ghost_state(bc, qod, tn, a.bcp, qgd);
+                ghost_state(bc, qod, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[cell] : 0), qgd);
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Qn[v] = (real)qgd[v];
 #pragma unroll

@@ -40,6 +40,7 @@ struct ViscArgs {
     const int* fp_of_sp;
     const int* coord_of_sp;
     const double* bcp;
+    const int* cell_case;          // null: one case
     double qref[5];
     double mu, Pr, Prt;
     int suppress;

@@ -89,9 +89,12 @@ class DeviceMesh:
     """One mesh at one order on one GPU."""
 
     def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, ref_state=None,
-                 slots=None, metrics="gpu", turbulence=None):
+                 slots=None, metrics="gpu", turbulence=None, cell_case=None):
         """metrics: "gpu" (FP64 precompute on the device) or "host" (NumPy FP64, small meshes).
-        turbulence: None or "SST" (face-record tables, wall distance and SST work buffers)."""
+        turbulence: None or "SST" (face-record tables, wall distance and SST work buffers).
+        cell_case: None, or the case index of every cell of a disjoint union of cases
+        (mesh.concat_meshes); bc_params is then (n_cases, 8, 5) and the wall distance is taken
+        case by case (mesh.meta["wall_off"] splits mesh.wall_nodes)."""
         self.lib = _lib.load()
         self.turbulence = turbulence
         self.sps_xyz = None
@@ -173,8 +176,22 @@ class DeviceMesh:
         self.gL, self.gR = dev(T.gL), dev(T.gR)
         self.fp_of_sp, self.coord_of_sp = devi(T.fp_of_sp), devi(T.coord_of_sp)
         bcp = np.zeros((8, 5)) if bc_params is None else np.asarray(bc_params, dtype=np.float64)
+        self.cell_case = None
+        self.n_cases = 1
+        if cell_case is not None:
+            cc = np.asarray(cell_case)
+            self.n_cases = int(bcp.shape[0]) if bcp.ndim == 3 else 1
+            if bcp.ndim != 3 or bcp.shape[1:] != (8, 5):
+                raise ValueError("batched mesh needs bc_params of shape (n_cases, 8, 5)")
+            if cc.shape != (C,) or (C and (cc.min() < 0 or cc.max() >= self.n_cases)):
+                raise ValueError("cell_case must give a case in [0, n_cases) for every cell")
+            self.cell_case = devi(cc)
+        elif bcp.shape != (8, 5):
+            raise ValueError("bc_params must be (8, 5)")
         self.bc_params = dev(bcp, torch.float64)
         # reference state of the perturbation form: the freestream by default
+        if ref_state is None and bcp.ndim == 3:
+            raise ValueError("a batched mesh needs an explicit common ref_state")
         self.ref_state = np.asarray(ref_state if ref_state is not None else bcp[BC_FARFIELD], dtype=np.float64)
         self.work_q = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
         self.work_grad = torch.empty((C, T.ns, 12), dtype=dtype, device=self.device)
@@ -188,7 +205,7 @@ class DeviceMesh:
         self.work_s2 = torch.empty((C, T.ns, 5), dtype=dtype, device=self.device)
         self.work_dt = torch.empty((C, T.ns), dtype=dtype, device=self.device)
         self.work_part = torch.zeros((4096,), dtype=torch.float64, device=self.device)
-        self.norm_dev = torch.zeros((1,), dtype=torch.float64, device=self.device)
+        self.norm_dev = torch.zeros((self.n_cases,), dtype=torch.float64, device=self.device)
 
         self._build_wall_records(dev, devi)
         if turbulence is not None:
@@ -217,6 +234,16 @@ class DeviceMesh:
         d.gL, d.gR = p(self.gL), p(self.gR)
         d.fp_of_sp, d.coord_of_sp = p(self.fp_of_sp), p(self.coord_of_sp)
         d.gcl, d.bc_params = p(self.gcl), p(self.bc_params)
+        if self.cell_case is not None:
+            # host copies of the case offsets: prisms, tets (from the mesh) and wall records
+            po = np.ascontiguousarray(mesh.meta["prism_off"], dtype=np.int64)
+            to = np.ascontiguousarray(mesh.meta["tet_off"], dtype=np.int64)
+            self.case_prism_off, self.case_tet_off = po, to
+            d.cell_case, d.n_cases = p(self.cell_case), self.n_cases
+            d.case_prism_off = po.ctypes.data_as(ctypes.c_void_p)
+            d.case_tet_off = to.ctypes.data_as(ctypes.c_void_p)
+            if self.n_wall_rec:
+                d.case_wall_off = self.case_wall_off.ctypes.data_as(ctypes.c_void_p)
         for v in range(5):
             d.ref_state[v] = float(self.ref_state[v])
         d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
@@ -273,6 +300,12 @@ class DeviceMesh:
         code_to_slot = np.full(6, -1, dtype=np.int64)
         for si, code in enumerate(SLOT_CODES[0]):
             code_to_slot[code] = si
+        if self.cell_case is not None:
+            cc = self.cell_case.cpu().numpy()
+            cnt = np.bincount(cc[fr.owner[sel]], minlength=self.n_cases)
+            if np.any(np.diff(cc[fr.owner[sel]]) < 0):
+                raise ValueError("wall records of a batched mesh must be grouped case by case")
+            self.case_wall_off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
         self.wrec_owner = devi(fr.owner[sel])
         self.wrec_oslot = devi(code_to_slot[fr.owner_code[sel]])
         geom = np.concatenate([fr.normal[sel], fr.area[sel, None]], axis=1)
@@ -290,6 +323,21 @@ class DeviceMesh:
                                            float(mu), ctypes.c_void_p(self.forces_dev.data_ptr()), self._stream()),
                    "afr_forces_f64")
         return self.forces_dev
+
+    def wall_forces_cases(self, U, mu_t=None, mu=1.8e-5, out=None):
+        """(n_cases, 6) FP64 device tensor of a batched mesh: per-case pressure and viscous force."""
+        if self.cell_case is None:
+            raise ValueError("wall_forces_cases needs a batched mesh (cell_case)")
+        if not self.n_wall_rec:
+            raise ValueError("mesh has no WALL boundary records")
+        self._check_state(U)
+        out = torch.zeros((self.n_cases, 6), dtype=torch.float64, device=self.device) if out is None else out
+        if out.shape != (self.n_cases, 6) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (n_cases, 6) float64 tensor")
+        _lib.check(self.lib.afr_forces_cases_f64(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                                 float(mu), ctypes.c_void_p(out.data_ptr()), self._stream()),
+                   "afr_forces_cases_f64")
+        return out
 
     def _build_sst(self, dev, devi):
         """Face-record tables of the SST transport and the wall distance of every solution point.
@@ -345,10 +393,26 @@ class DeviceMesh:
             raise ValueError("SST needs WALL boundary nodes for the wall distance")
         wall_d = torch.as_tensor(wall).to(self.device)
         xyz = self.sps_xyz.contiguous()
-        _lib.check(self.lib.afr_wall_distance(ctypes.c_void_p(xyz.data_ptr()), C * T.ns,
-                                              ctypes.c_void_p(wall_d.data_ptr()), len(wall),
-                                              ctypes.c_void_p(self.d_wall.data_ptr()),
-                                              0 if dt == torch.float32 else 1, self._stream()), "afr_wall_distance")
+        P = mesh.n_prism
+        if self.cell_case is None:
+            pieces = [(0, C, 0, len(wall))]
+        else:
+            # case by case: (cell range, wall-node range) for the prisms and for the tets of each case
+            po, to, wo = (np.asarray(mesh.meta[k], dtype=np.int64) for k in ("prism_off", "tet_off", "wall_off"))
+            pieces = []
+            for i in range(len(po) - 1):
+                if wo[i + 1] == wo[i] and (po[i + 1] > po[i] or to[i + 1] > to[i]):
+                    raise ValueError(f"case {i} has no WALL boundary nodes for the wall distance")
+                pieces.append((po[i], po[i + 1], wo[i], wo[i + 1]))
+                pieces.append((P + to[i], P + to[i + 1], wo[i], wo[i + 1]))
+        for c0, c1, w0, w1 in pieces:
+            if c1 <= c0:
+                continue
+            _lib.check(self.lib.afr_wall_distance(ctypes.c_void_p(xyz[c0:c1].data_ptr()), int(c1 - c0) * T.ns,
+                                                  ctypes.c_void_p(wall_d[w0:w1].data_ptr()), int(w1 - w0),
+                                                  ctypes.c_void_p(self.d_wall[c0:c1].data_ptr()),
+                                                  0 if dt == torch.float32 else 1, self._stream()),
+                       "afr_wall_distance")
         torch.cuda.synchronize(self.device)
 
     def sst_fields(self, k0, omega0):
@@ -372,6 +436,7 @@ class DeviceMesh:
         """One SST pseudo-step (k, omega, nu_t, mu_t and U updated in place); returns the (1,) FP64 norm."""
         self._check_state(U)
         norm = self.norm_dev if norm is None else norm
+        self._check_norm(norm)
         _lib.check(self.lib.afr_sst_rk3_step(self.handle, ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(k.data_ptr()),
                                              ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(self.d_wall.data_ptr()),
                                              ctypes.c_void_p(nu_t.data_ptr()), ctypes.c_void_p(mu_t.data_ptr()),
@@ -570,6 +635,19 @@ class DeviceMesh:
                                          float(c0), float(c1), float(c2), self._stream()), "afr_rk_stage")
         return U_out
 
+    def residual_norm_cases(self, R, n_vars_total=5, out=None):
+        """Per-case FP64 RMS norms of R as an (n_cases,) device tensor (batched mesh)."""
+        out = self.norm_dev if out is None else out
+        self._check_norm(out)
+        _lib.check(self.lib.afr_residual_norm_cases(self.handle, ctypes.c_void_p(R.data_ptr()), int(n_vars_total),
+                                                    ctypes.c_void_p(out.data_ptr()), self._stream()),
+                   "afr_residual_norm_cases")
+        return out
+
+    def _check_norm(self, out):
+        if out.dtype != torch.float64 or out.numel() < self.n_cases or out.device != self.device:
+            raise ValueError(f"norm output must be a float64 device tensor of >= {self.n_cases} values")
+
     def residual_norm(self, R, n_vars_total=5, out=None):
         """FP64 RMS norm of R as a (1,) device tensor."""
         out = self.norm_dev if out is None else out
@@ -582,6 +660,7 @@ class DeviceMesh:
         """One steady pseudo-step of U in place; returns the (1,) FP64 device tensor of the RMS norm of R(U)."""
         self._check_state(U)
         norm = self.norm_dev if norm is None else norm
+        self._check_norm(norm)
         _lib.check(self.lib.afr_ssp_rk3_step(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
                                              float(mu), float(Pr), float(Pr_t), int(n_vars_total),
                                              ctypes.c_void_p(norm.data_ptr()), self._stream()), "afr_ssp_rk3_step")

@@ -369,3 +369,45 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slo
                 fallback[g, si] = np.sum(fb.astype(np.int64) << np.arange(nfp), axis=1).astype(np.int32)
     return Slots(nb=nb, blk=blk, mask=mask, bc=bc, tn=tn, adjrow=adjrow, fallback=fallback, rows=rows,
                  n_blocks=n_blocks, keyed_slots=keyed_slots, split_slots=len(sc))
+
+
+def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
+    """Slot tables of a disjoint union of meshes (mesh.concat_meshes numbering: all prisms case
+    by case, then all tets case by case) from each case's own tables.
+
+    Built per case, every row, normal and flag of a cell is the one its case alone has, bit
+    for bit (a union built in one pass would locate flux points in batches of another
+    composition, which moves interpolation weights by round-off)."""
+    P = np.asarray(n_prism_list, dtype=np.int64)
+    Tn = np.asarray(n_tet_list, dtype=np.int64)
+    p_off = np.concatenate([[0], np.cumsum(P)])
+    t_off = np.concatenate([[0], np.cumsum(Tn)])
+    P_tot = int(p_off[-1])
+    C = P_tot + int(t_off[-1])
+    s0 = slot_list[0]
+
+    def alloc(a, fill):
+        return None if a is None else np.full((C,) + a.shape[1:], fill, dtype=a.dtype)
+
+    out = {k: alloc(getattr(s0, k), f) for k, f in (("nb", -1), ("blk", 0), ("mask", 0), ("bc", -2),
+                                                     ("tn", 0.0), ("fallback", 0), ("adjrow", 0.0))}
+    rows, blk_off = [], 0
+    for i, sl in enumerate(slot_list):
+        Pi = int(P[i])
+        g = np.concatenate([p_off[i] + np.arange(Pi), P_tot + t_off[i] + np.arange(int(Tn[i]))])
+        if len(g) != sl.nb.shape[0]:
+            raise ValueError(f"case {i}: slot tables do not match its cell counts")
+        if sl.rows.shape[0] % nfp:
+            raise ValueError(f"case {i}: rows are not whole blocks of {nfp}")
+        nb = sl.nb.astype(np.int64)
+        nb_g = np.where(nb < Pi, p_off[i] + nb, P_tot + t_off[i] + (nb - Pi))
+        out["nb"][g] = np.where(nb < 0, -1, nb_g).astype(np.int32)
+        out["blk"][g] = sl.blk + blk_off
+        for k in ("mask", "bc", "tn", "fallback", "adjrow"):
+            if out[k] is not None:
+                out[k][g] = getattr(sl, k)
+        rows.append(sl.rows)
+        blk_off += sl.rows.shape[0] // nfp
+    return Slots(nb=out["nb"], blk=out["blk"], mask=out["mask"], bc=out["bc"], tn=out["tn"], adjrow=out["adjrow"],
+                 fallback=out["fallback"], rows=np.concatenate(rows), n_blocks=blk_off,
+                 keyed_slots=sum(s.keyed_slots for s in slot_list), split_slots=sum(s.split_slots for s in slot_list))

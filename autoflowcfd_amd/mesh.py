@@ -402,18 +402,16 @@ def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_
     return mesh.faces
 
 
-def concat_meshes(meshes, gap=None):
+def concat_meshes(meshes):
     """Disjoint union of meshes as one mesh (the batched multi-case form).
 
     Cells are renumbered prisms first (case by case) then tets (case by case); face records
-    are concatenated case by case with owner / neighbour remapped; case i is translated along
-    x by a gap larger than every case's extent so that geometric queries (wall distance,
-    locality order) never mix cases.  Returns (mesh, cell_case (C,), prism_off (n+1,),
-    tet_off (n+1,), rec_off (n+1,)).
+    are concatenated case by case with owner / neighbour remapped; node coordinates are kept
+    exactly (cases may overlap in space: no geometric query mixes cases, the wall distance is
+    taken case by case over mesh.meta["wall_off"]).  Returns (mesh, cell_case (C,), prism_off
+    (n+1,), tet_off (n+1,), rec_off (n+1,)); the offsets are also in mesh.meta.
     """
     n = len(meshes)
-    ext = max(float(np.ptp(m.nodes, axis=0).max()) for m in meshes)
-    gap = 2.0 * ext + 1.0 if gap is None else gap
     P = np.array([m.n_prism for m in meshes], dtype=np.int64)
     T = np.array([m.n_tet for m in meshes], dtype=np.int64)
     N = np.array([len(m.nodes) for m in meshes], dtype=np.int64)
@@ -425,12 +423,10 @@ def concat_meshes(meshes, gap=None):
     fields = {k: [] for k in ("owner", "neighbor", "owner_code", "neighbor_code", "normal", "area", "center",
                               "node_ids", "translation", "bc")}
     wall = []
+    wall_off = [0]
     rec_off = [0]
-    x0 = 0.0
     for i, m in enumerate(meshes):
-        shift = np.array([x0 - m.nodes[:, 0].min(), 0.0, 0.0])
-        x0 += gap
-        nodes.append(m.nodes + shift)
+        nodes.append(m.nodes)
         prisms.append(m.prism_conn + n_off[i])
         tets.append(m.tet_conn + n_off[i])
 
@@ -446,17 +442,20 @@ def concat_meshes(meshes, gap=None):
         fields["neighbor_code"].append(fr.neighbor_code)
         fields["normal"].append(fr.normal)
         fields["area"].append(fr.area)
-        fields["center"].append(fr.center + shift)
+        fields["center"].append(fr.center)
         fields["node_ids"].append(fr.node_ids + n_off[i])
         fields["translation"].append(fr.translation)
         fields["bc"].append(fr.bc)
         rec_off.append(rec_off[-1] + fr.n_faces)
         if m.wall_nodes is not None and len(m.wall_nodes):
             wall.append(m.wall_nodes + n_off[i])
+        wall_off.append(wall_off[-1] + (0 if m.wall_nodes is None else len(m.wall_nodes)))
     out = Mesh(nodes=np.concatenate(nodes), prism_conn=np.concatenate(prisms).reshape(-1, 6),
                tet_conn=np.concatenate(tets).reshape(-1, 4))
     out.faces = FaceRecords(**{k: np.concatenate(v) for k, v in fields.items()})
     out.wall_nodes = np.concatenate(wall) if wall else np.zeros(0, np.int64)
     cell_case = np.concatenate([np.repeat(np.arange(n), P), np.repeat(np.arange(n), T)]).astype(np.int32)
-    out.meta["n_cases"] = n
-    return out, cell_case, p_off, t_off, np.asarray(rec_off, dtype=np.int64)
+    rec_off = np.asarray(rec_off, dtype=np.int64)
+    out.meta.update(n_cases=n, prism_off=p_off, tet_off=t_off, rec_off=rec_off,
+                    wall_off=np.asarray(wall_off, dtype=np.int64))
+    return out, cell_case, p_off, t_off, rec_off

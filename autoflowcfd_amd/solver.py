@@ -47,12 +47,15 @@ class GPUFRSolver:
                  time_scheme: str = "ssp_rk3", cfl: float = 1.0, rho_inf: float = 1.225, vel_inf: float = 33.33,
                  p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
                  turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
-                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9):
+                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None):
         """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
         compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
         bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
         or {BC_OUTLET: p}; the default is the freestream along +x (boundary.py:52-145).
         k_inf / omega_inf: initial turbulence fields (reference defaults 1e-6 and 1.0, sst.py:47-48).
+        ref_state: constant primitive state of the perturbation form (default: the FARFIELD state);
+        a case stepped alone with the ref_state of a batch reproduces its batched trajectory bit
+        for bit (batch.CaseBatch).
         """
         if time_scheme.lower().replace("-", "_") != "ssp_rk3":
             raise NotImplementedError(f"time_scheme {time_scheme!r}: this backend implements ssp_rk3")
@@ -77,7 +80,7 @@ class GPUFRSolver:
             Q_free = np.asarray(overrides[BC_FARFIELD], dtype=np.float64)
         p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
         bcp = bc_params_table(Q_free, Q_inlet, p_out)
-        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp,
+        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=ref_state,
                              turbulence="SST" if self.turb_model_name == "SST" else None)
         C, ns = self.dm.n_cells, self.dm.ns
         U0 = _cons(np.broadcast_to(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]), (C, ns, 5)))

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define AFR_ABI_VERSION 2
+#define AFR_ABI_VERSION 3
 
 enum { AFR_F32 = 0, AFR_F64 = 1 };
 
@@ -62,7 +62,15 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *              A_a = f2c_ab D_a, A_b = f2c_ab D_b, B_c = f2c_c D_c, f2c_c,
  *              f2c_ab, E_ab (3 quad slots), e_c (2 c slots), D_ab, D_c
  *   gcl        (C, ns, 3)              constant-state divergence sum_{m,j} Op[s][m,j] adj_fine[j][m][i]
- *   bc_params  (8, 5) FP64             per BC code reference state (always double)
+ *   bc_params  (8, 5) FP64             per BC code reference state (always double);
+ *                                      (n_cases, 8, 5) when cell_case is set
+ *   cell_case  (C) int32, may be NULL  case index of every cell (batched multi-case meshes:
+ *                                      disjoint cases with their own boundary states); the
+ *                                      prisms of case i are [case_prism_off[i], case_prism_off[i+1]),
+ *                                      its tets n_prism + [case_tet_off[i], case_tet_off[i+1]) and
+ *                                      its wall records [case_wall_off[i], case_wall_off[i+1]).
+ *                                      On such a mesh the pseudo-step entry points write one
+ *                                      norm per case (norm: n_cases doubles).
  *   ref_state  (5) FP64, host values   primitive reference state Q_ref; the kernels carry
  *                                      dQ = Q - Q_ref in the storage precision and evaluate
  *                                      point fluxes / Riemann fluxes in FP64
@@ -121,6 +129,12 @@ typedef struct AfrMeshDesc {
     const int32_t *prism_schedule;
     const void *gcl;
     const double *bc_params;
+    const int32_t *cell_case;
+    int32_t n_cases;
+    int32_t pad2_;
+    const int64_t *case_prism_off; /* HOST (n_cases + 1), copied at afr_mesh_create */
+    const int64_t *case_tet_off;   /* HOST (n_cases + 1) */
+    const int64_t *case_wall_off;  /* HOST (n_cases + 1) over the wall records; NULL without walls */
     double ref_state[5];
     void *work_q;
     void *work_grad;
@@ -253,7 +267,8 @@ int afr_residual_norm_f64(const AfrMesh *mesh, const void *R, int32_t n_vars_tot
 
 /* One steady SSP-RK3 pseudo-step of the Navier-Stokes state U (C, ns, 5), in place:
  * dt_local, R0 = residual(U), three stages with positivity and filter, and the RMS
- * norm of R0 into norm[0] (device pointer), as src/autoflowcfd/core/fr_solver/step.py:15-221.
+ * norm of R0 into norm[0] (device pointer; on a batched mesh norm[i] per case, as
+ * afr_residual_norm_cases), as src/autoflowcfd/core/fr_solver/step.py:15-221.
  * mu_t (C, ns) is frozen over the step (may be NULL); mu == 0 with mu_t == NULL skips
  * the viscous terms.  Uses work_r, work_s1, work_s2, work_dt, work_part. */
 int afr_ssp_rk3_step(const AfrMesh *mesh, void *U, const void *mu_t, double mu, double Pr, double Pr_t,
@@ -279,9 +294,22 @@ int afr_sst_step(const AfrMesh *mesh, const void *U, void *k, void *omega, const
 /* One steady pseudo-step with the SST model, as src/autoflowcfd/core/fr_solver/step.py:15-221:
  * dt_local from U with mu_t = rho(U) nu_t of the previous step (nu_t in/out, zero
  * initially), the SST step (k, omega, nu_t, mu_t updated), then the SSP-RK3 stages of U
- * with mu_t frozen; norm[0] (device) = RMS of R(U) over C ns n_vars_total entries. */
+ * with mu_t frozen; norm[0] (device) = RMS of R(U) over C ns n_vars_total entries (on a batched
+ * mesh norm[i] per case). */
 int afr_sst_rk3_step(const AfrMesh *mesh, void *U, void *k, void *omega, const void *d_wall, void *nu_t, void *mu_t,
                      double mu, double Pr, double Pr_t, int32_t n_vars_total, double *norm, void *stream);
+
+/* Per-case RMS norms of a batched mesh (cell_case set): out[i] = sqrt(sum over the cells of
+ * case i of R^2 / (cells_i ns n_vars_total)); out: n_cases doubles (device).  Each case is
+ * reduced in the same order as afr_residual_norm_f64 over that case alone: bit-identical values.
+ * Per-case form of src/autoflowcfd/core/fr_solver/state.py:99-107. */
+int afr_residual_norm_cases(const AfrMesh *mesh, const void *R, int32_t n_vars_total, double *out, void *stream);
+
+/* Per-case wall forces of a batched mesh: out (n_cases, 6) doubles (device), bit-identical to
+ * afr_forces_f64 of each case alone.  Per-case form of
+ * src/autoflowcfd/postprocess/fr_coefficients.py:108-112. */
+int afr_forces_cases_f64(const AfrMesh *mesh, const void *U, const void *mu_t, double mu, double *out,
+                         void *stream);
 
 /* FP64 force on the WALL boundary of the state U (C, ns, 5) with eddy viscosity mu_t
  * (C, ns, may be NULL): out[0..2] pressure force sum p n w, out[3..5] viscous force
