@@ -645,7 +645,7 @@ class DeviceMesh:
         return out
 
     def _check_norm(self, out):
-        if out.dtype != torch.float64 or out.numel() < self.n_cases or out.device != self.device:
+        if out.dtype != torch.float64 or out.numel() < self.n_cases or not out.is_cuda:
             raise ValueError(f"norm output must be a float64 device tensor of >= {self.n_cases} values")
 
     def residual_norm(self, R, n_vars_total=5, out=None):

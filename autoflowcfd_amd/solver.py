@@ -101,11 +101,12 @@ class GPUFRSolver:
         return self.dm.ssp_rk3_step(self.U, None, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t,
                                     n_vars_total=self.n_vars)
 
-    def capture_graph(self):
+    def capture_graph(self, capture_error_mode: str = "global"):
         """Record one pseudo-step into a HIP graph; later steps replay it (one launch per step).
 
         The pseudo-step has no host synchronisation, no allocation and no memset (the FP64 norm
-        partials are fully overwritten), so it captures as is; buffers are the solver's own."""
+        partials are fully overwritten), so it captures as is; buffers are the solver's own.
+        capture_error_mode "thread_local" lets other host threads keep launching meanwhile."""
         torch.cuda.synchronize(self.device)
         side = torch.cuda.Stream(self.device)
         side.wait_stream(torch.cuda.current_stream(self.device))
@@ -117,7 +118,7 @@ class GPUFRSolver:
         self.residual_history.append(float(self.dm.norm_dev.item()))
         self.iteration += 1
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with torch.cuda.graph(g, capture_error_mode=capture_error_mode):
             self._graph_norm = self._launch_step()
         self._graph = g
         return g

@@ -141,6 +141,41 @@ def traffic_from_profiles(name):
     return None
 
 
+def served_leg(args):
+    """Served stream vs the plain one-at-a-time GPU path on the same seeded requests: solves/h,
+    p50 / p99 latency, and the largest Cd difference between the two (same-result check)."""
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, latency_stats, solve_alone, synthetic_requests
+    reqs = synthetic_requests(args.serve_requests, seed=args.serve_seed, hex_range=(2e3, 2e4), rate=args.serve_rate,
+                              max_iter=args.serve_max_iter, rel_tol=1e-6)
+    n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
+    log(f"[bench] served leg: {len(reqs)} requests, {n_sp} solution points in total")
+    # warm the code paths once (library load, first graph capture) outside both timings
+    SolveServer(dtype=torch.float32, max_active=2).run(reqs[:1])
+    solve_alone(reqs[:1], dtype=torch.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = solve_alone(reqs, dtype=torch.float32)
+    log(f"[bench] plain path done in {time.perf_counter() - t0:.1f}s")
+    srv = SolveServer(dtype=torch.float32, max_active=args.serve_active)
+    t0 = time.perf_counter()
+    served = srv.run(reqs)
+    log(f"[bench] served path done in {time.perf_counter() - t0:.1f}s (peak {srv.peak_active} resident solves)")
+    sp, ss = latency_stats(plain), latency_stats(served)
+    dcd = max(abs(a.coefficients.get("Cd", 0.0) - b.coefficients.get("Cd", 0.0)) for a, b in zip(served, plain))
+    same = all(a.residual_history == b.residual_history for a, b in zip(served, plain))
+    return {
+        "workload": f"{len(reqs)} steady laminar NS solves, graded wall boxes of 2e3-2e4 hex (log-uniform) "
+                    f"split into prisms, P1/P2, Mach 0.05-0.3, yaw 0-10 deg, Poisson arrivals "
+                    f"{args.serve_rate}/s, stop at res/res0 < 1e-6 or {args.serve_max_iter} pseudo-steps",
+        "solves_per_hour": ss["solves_per_hour"], "p50_latency_s": ss["p50_s"], "p99_latency_s": ss["p99_s"],
+        "plain_solves_per_hour": sp["solves_per_hour"], "plain_p50_latency_s": sp["p50_s"],
+        "plain_p99_latency_s": sp["p99_s"], "speedup_vs_plain": ss["solves_per_hour"] / sp["solves_per_hour"],
+        "max_abs_dCd_vs_plain": dcd, "histories_identical": same, "peak_resident_solves": srv.peak_active,
+        "iterations": [r.iterations for r in served],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +186,12 @@ def main():
     ap.add_argument("--rk-steps", type=int, default=2, help="full SSP-RK3 pseudo-steps timed after the residuals")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=6)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-max-iter", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
+    ap.add_argument("--serve-seed", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -233,7 +274,7 @@ def main():
     t_res = ms_per_step * 1e-3
     value = n_sp * 7 / t_res / 1e9
     line = {
-        "metric": "P2 residual GDOF/s + %HBM roofline",
+        "metric": "P2 residual GDOF/s + %HBM roofline; served solves/h & p99 latency vs plain",
         "value": value,
         "unit": "GDOF/s",
         "n_gpus": 1,
@@ -251,13 +292,17 @@ def main():
                    "cells": mesh.n_cells, "order": args.order, "n_vars": 7},
         "roofline": {"bound": "hbm", "achieved": b_inv / t_inv / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": b_inv / t_inv / HBM_PEAK, "traffic": traffic_from_profiles("k_prism_inviscid_c2"),
-                     "kernel": f"k_prism_inviscid<{args.order},{dm.slots.n_blocks and 9},float>",
+                     "kernel": f"k_prism_inviscid<{args.order},{dm.rows_knz},float>",
                      "kernel_ms": t_inv * 1e3, "algorithmic_bytes": b_inv},
         "residual_roofline": {"achieved_GBs": b_res / t_res / 1e9, "frac": b_res / t_res / HBM_PEAK,
                               "algorithmic_bytes": b_res},
         "phase_ms": {n: float(t) for n, t in zip(names, ph)},
         "ms_per_rk3_pseudo_step": ms_rk,
     }
+    if not args.no_serve:
+        del U, R, rates, k, w, nut, mut, dm
+        torch.cuda.empty_cache()
+        line["served"] = served_leg(args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(args.order, n_sample=args.cpu_sample_n)

@@ -1,0 +1,88 @@
+"""GPU: served steady solves (continuous batching) against the plain one-at-a-time path.
+
+Each served solve runs the same pseudo-steps as the plain loop and retires by the same rule
+after every step, so its residual history, iteration count and force coefficients must equal
+the plain path's BIT FOR BIT, however the requests interleave on the device.
+"""
+
+import numpy as np
+import pytest
+
+from cases import freestream, wall_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(max_iter=25, rel_tol=0.3, stagger=True):
+    from autoflowcfd_amd.batch import Case
+    from autoflowcfd_amd.serve import SolveRequest
+    spec = [((4, 3, 2), 2, 0.15, 0.0, 0.00), ((3, 2, 2), 1, 0.25, 7.0, 0.00), ((5, 3, 3), 2, 0.08, 3.0, 0.02),
+            ((3, 3, 2), 1, 0.20, 9.0, 0.05), ((4, 2, 2), 2, 0.30, 5.0, 0.05)]
+    out = []
+    for j, (n, order, mach, yaw, t) in enumerate(spec):
+        Q = freestream(mach, yaw)
+        U = float(np.linalg.norm(Q[1:4]))
+        k = 1.5 * (0.01 * U) ** 2
+        case = Case(mesh=wall_box(n=n, seed=30 + j), Q_free=Q, k_inf=k, omega_inf=Q[0] * k / (10 * 1.8e-5))
+        out.append(SolveRequest(rid=j, case=case, order=order, arrival=t if stagger else 0.0,
+                                max_iter=max(max_iter - 3 * j, 2), rel_tol=rel_tol))
+    return out
+
+
+def _same(served, plain):
+    assert [r.rid for r in served] == [r.rid for r in plain]
+    for a, b in zip(served, plain):
+        assert a.iterations == b.iterations, (a.rid, a.iterations, b.iterations)
+        assert a.residual_history == b.residual_history, a.rid
+        assert a.converged == b.converged
+        assert a.coefficients["Cd"] == b.coefficients["Cd"], (a.rid, a.coefficients, b.coefficients)
+        assert a.coefficients["Cl"] == b.coefficients["Cl"]
+
+
+def test_served_equals_plain_and_overlaps():
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, latency_stats, solve_alone
+    print("[gpu] SolveServer NS mixed P1/P2 vs plain", flush=True)
+    reqs = _requests()
+    plain = solve_alone(reqs, dtype=torch.float32)
+    srv = SolveServer(dtype=torch.float32, max_active=4)
+    served = srv.run(reqs)
+    _same(served, plain)
+    st = latency_stats(served)
+    assert st["p99_s"] >= st["p50_s"] > 0
+    assert any(r.converged for r in served) or all(r.iterations == q.max_iter for r, q in zip(served, reqs))
+
+
+def test_long_served_solves_are_resident_together():
+    """Solves long enough to outlast the set-up of the next request step together on the device."""
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, solve_alone
+    print("[gpu] SolveServer overlap (300-step solves)", flush=True)
+    reqs = _requests(max_iter=300, rel_tol=1e-12, stagger=False)[:4]
+    plain = solve_alone(reqs, dtype=torch.float32)
+    srv = SolveServer(dtype=torch.float32, max_active=4)
+    served = srv.run(reqs)
+    _same(served, plain)
+    assert srv.peak_active >= 2
+
+
+def test_served_memory_budget_admits_one_at_a_time():
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, solve_alone
+    print("[gpu] SolveServer with a one-solve HBM budget", flush=True)
+    reqs = _requests(max_iter=15)[:3]
+    plain = solve_alone(reqs, dtype=torch.float32)
+    srv = SolveServer(dtype=torch.float32, max_active=4, hbm_budget=1)   # nothing fits beside a running solve
+    served = srv.run(reqs)
+    _same(served, plain)
+    assert srv.peak_active == 1
+
+
+def test_served_sst_equals_plain():
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, solve_alone
+    print("[gpu] SolveServer SST vs plain", flush=True)
+    reqs = _requests(max_iter=8)[:3]        # few steps: the SST transport grows k, omega over many
+    plain = solve_alone(reqs, dtype=torch.float32, turb_model="SST")
+    served = SolveServer(dtype=torch.float32, turb_model="SST", max_active=3).run(reqs)
+    _same(served, plain)
