@@ -345,9 +345,26 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
             sf[t] = fabs((double)(f == 0 ? a.k[i] : a.w[i]));
         }
         wave_sync();
-        // outlier zeroing per field (troubled_cell.py:339-408, n_vars = 1)
+        // outlier zeroing per field (troubled_cell.py:339-408, n_vars = 1).  Certificate first:
+        // when at most (NS-1)/2 values of a field lie below 1.0001e-4 of its largest |value|, the
+        // median is at least that, so no value exceeds 1e4 x max(median, floor) and nothing is
+        // zeroed; only otherwise is the median selected exactly.
         double tv = 0.0;
+        bool need = false;
         if (t < 2 * NS) {
+            const int f = t & 1;
+            double mx = 0.0;
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const double y = fabs(st[s2 * 2 + f]);
+                mx = y > mx ? y : mx;
+            }
+            const double thr = mx * 1.0001e-4;
+            int cnt = 0;
+            for (int s2 = 0; s2 < NS; ++s2) cnt += fabs(st[s2 * 2 + f]) < thr;
+            need = cnt > (NS - 1) / 2;
+            tv = isfinite(st[t]) ? st[t] : 0.0;
+        }
+        if (__any(need) && t < 2 * NS) {
             const int f = t & 1;
             const double x = fabs(st[t]);
             double m1 = 0.0, m2 = 0.0, um = 0.0;
