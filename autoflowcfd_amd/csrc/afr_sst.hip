@@ -68,37 +68,85 @@ __device__ double blend_F2(double k, double w, double d, double nu) {
     return tanh(a * a);
 }
 
+// Blend functions in FP32 (FP32 storage path): tanh of the same clipped arguments; an infinite
+// or non-finite argument saturates tanh at 1 exactly as the FP64 form's 1e75 / 1e150 caps do.
+__device__ float blend_F1_f(float k, float w, float d, float nu, float rho, float CD) {
+    w = fmaxf(w, 1e-10f);
+    k = fmaxf(k, 1e-10f);
+    d = fmaxf(d, 1e-10f);
+    const float t1 = sqrtf(k) / (float(BSTAR) * w * d);
+    const float t2 = 500.0f * nu / (d * d * w);
+    const float t3 = 4.0f * rho * float(SW2) * k / (fmaxf(CD, 1e-10f) * d * d);
+    float a = fminf(fmaxf(t1, t2), t3);
+    a = isfinite(a) ? a : 1e30f;
+    a = fminf(a, 1e30f);
+    const float a2 = a * a;
+    return tanhf(a2 * a2);
+}
+
+__device__ float blend_F2_f(float k, float w, float d, float nu) {
+    w = fmaxf(w, 1e-10f);
+    k = fmaxf(k, 1e-10f);
+    d = fmaxf(d, 1e-10f);
+    float a = fmaxf(2.0f * sqrtf(k) / (float(BSTAR) * w * d), 500.0f * nu / (d * d * w));
+    a = isfinite(a) ? a : 1e30f;
+    a = fminf(a, 1e30f);
+    return tanhf(a * a);
+}
+
+template <typename real> struct Blend;
+template <> struct Blend<double> {
+    static __device__ double F1(double k, double w, double d, double nu, double rho, double CD) {
+        return blend_F1(k, w, d, nu, rho, CD);
+    }
+    static __device__ double F2(double k, double w, double d, double nu) { return blend_F2(k, w, d, nu); }
+};
+template <> struct Blend<float> {
+    static __device__ double F1(double k, double w, double d, double nu, double rho, double CD) {
+        return (double)blend_F1_f((float)dmin(k, 1e38), (float)dmin(w, 1e38), (float)dmin(d, 1e38),
+                                  (float)nu, (float)rho, (float)dmin(CD, 1e38));
+    }
+    static __device__ double F2(double k, double w, double d, double nu) {
+        return (double)blend_F2_f((float)dmin(k, 1e38), (float)dmin(w, 1e38), (float)dmin(d, 1e38), (float)nu);
+    }
+};
+
 template <int P, int CT> struct SO {
     static constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1, NSLOT = CT == 0 ? 5 : 4;
 };
 
 // ------------------------------------------------------------------ per-cell sources and volume terms
+// One wave per cell.  The contractions (gradients D x, face traces E x, volume divergence D F)
+// run in the storage precision on per-cell perturbations x - x_0 (x_0: the value at the cell's
+// first solution point; D rows sum to zero and E rows to one, so D x = D (x - x_0) and
+// E x = x_0 + E (x - x_0)), which keeps FP32 contractions as accurate as FP64 ones on FP32
+// data.  The SST source terms are FP64 (the blend functions FP32 on the FP32 path).
 template <int P, int CT, typename real>
 __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     constexpr int NS = O::NS;
-    constexpr int NF = 12;  // per-SP doubles: rho, u, v, w, m1, m2, m3, k, w, nu_t, Gk, Gw
+    constexpr int NF = 12;  // per-SP: rho, u, v, w, m1, m2, m3, k, w, (unused), Gk, Gw
     constexpr int NFP = O::NFP, NSLOT = O::NSLOT;
     __shared__ real s_D[3 * NS * NS];
     __shared__ real s_E[5 * NFP * NS];
-    __shared__ double s_w[kWaves][NS * NF + NS * 15 + 3 * NS * 4];
+    __shared__ real s_w[kWaves][NS * NF + NS * 15 + 3 * NS * 4];
     for (int i = threadIdx.x; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
     for (int i = threadIdx.x; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = (CT == 0 ? a.E0 : a.E1)[i];
     __syncthreads();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
-    double* sv = s_w[wid];          // [s][NF]
-    double* sg = sv + NS * NF;      // [s][15]: grad m (9), grad k (3), grad w (3)
-    double* sF = sg + NS * 15;      // [m][s][4]: adj (rho u k, rho u w, Gk grad k, Gw grad w) components
+    real* sv = s_w[wid];          // [s][NF]
+    real* sg = sv + NS * NF;      // [s][15]: grad m (9), grad k (3), grad w (3)
+    real* sF = sg + NS * 15;      // [m][s][4]: contravariant fluxes rho u k, rho u w, Gk grad k, Gw grad w
     const double mu = a.mu_molecular;
     const int64_t nw = (int64_t)gridDim.x * kWaves;
     for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
         const int64_t cell = a.cell_begin + ci;
         if (t < NS) {
             const int64_t i = cell * NS + t;
-            double u5[5];
+            real u5[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) u5[v] = (double)a.U[i * 5 + v];
-            const double rho = dmax(u5[0], 1e-10);
+            for (int v = 0; v < 5; ++v) u5[v] = a.U[i * 5 + v];
+            const real rho = u5[0] > real(1e-10) ? u5[0] : real(1e-10);
             sv[t * NF + 0] = rho;
             sv[t * NF + 1] = u5[1] / rho;
             sv[t * NF + 2] = u5[2] / rho;
@@ -106,34 +154,33 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             sv[t * NF + 4] = u5[1];
             sv[t * NF + 5] = u5[2];
             sv[t * NF + 6] = u5[3];
-            sv[t * NF + 7] = (double)a.k[i];
-            sv[t * NF + 8] = (double)a.w[i];
+            sv[t * NF + 7] = a.k[i];
+            sv[t * NF + 8] = a.w[i];
         }
         wave_sync();
-        // gradients of m1, m2, m3, k, omega: lane (s, field)
-        for (int o = threadIdx.x & 63; o < 5 * NS; o += 64) {
+        // gradients of m1, m2, m3, k, omega: lane (s, field), perturbations about SP 0
+        for (int o = t; o < 5 * NS; o += 64) {
             const int s = o / 5, f = o % 5;
             const int src = f < 3 ? 4 + f : 7 + (f - 3);
-            double d0 = 0, d1 = 0, d2 = 0;
+            const real x0 = sv[src];
+            real d0 = 0, d1 = 0, d2 = 0;
             for (int j = 0; j < NS; ++j) {
-                const double x = sv[j * NF + src];
-                d0 += (double)s_D[(0 * NS + s) * NS + j] * x;
-                d1 += (double)s_D[(1 * NS + s) * NS + j] * x;
-                d2 += (double)s_D[(2 * NS + s) * NS + j] * x;
+                const real x = sv[j * NF + src] - x0;
+                d0 += s_D[(0 * NS + s) * NS + j] * x;
+                d1 += s_D[(1 * NS + s) * NS + j] * x;
+                d2 += s_D[(2 * NS + s) * NS + j] * x;
             }
             const real* ad = a.adj + (cell * NS + s) * 9;
-            const double idet = 1.0 / (double)a.det[cell * NS + s];
+            const real idet = real(1) / a.det[cell * NS + s];
             const int base = f < 3 ? f * 3 : 9 + (f - 3) * 3;
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-                sg[s * 15 + base + n] =
-                    ((double)ad[0 * 3 + n] * d0 + (double)ad[1 * 3 + n] * d1 + (double)ad[2 * 3 + n] * d2) * idet;
+            for (int n = 0; n < 3; ++n) sg[s * 15 + base + n] = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
         }
         wave_sync();
         if (t < NS) {
             const int s = t;
             const int64_t i = cell * NS + s;
-            const double rho = sv[s * NF + 0], k = sv[s * NF + 7], w = sv[s * NF + 8];
+            const double rho = (double)sv[s * NF + 0], k = (double)sv[s * NF + 7], w = (double)sv[s * NF + 8];
             const double d = (double)a.d_wall[i];
             // strain rate of the momentum gradient
             double S2 = 0.0;
@@ -141,7 +188,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             for (int p = 0; p < 3; ++p)
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double sij = 0.5 * (sg[s * 15 + p * 3 + q] + sg[s * 15 + q * 3 + p]);
+                    const double sij = 0.5 * ((double)sg[s * 15 + p * 3 + q] + (double)sg[s * 15 + q * 3 + p]);
                     S2 += sij * sij;
                 }
             const double S = sqrt(2.0 * S2);
@@ -149,8 +196,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             double gk[3], gw[3];
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                gk[n] = sg[s * 15 + 9 + n];
-                gw[n] = sg[s * 15 + 12 + n];
+                gk[n] = (double)sg[s * 15 + 9 + n];
+                gw[n] = (double)sg[s * 15 + 12 + n];
             }
             const double mk = sqrt(gk[0] * gk[0] + gk[1] * gk[1] + gk[2] * gk[2]);
             const double mw = sqrt(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
@@ -168,8 +215,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             const double nu = musrc / dmax(rho, 1e-10);
             const double ks = dmin(k, 1e40), wsf = dmax(dmin(w, 1e40), 1e-10);
             const double CD = dmax(2.0 * rho * SW2 / wsf * gdot, 1e-10);
-            const double F1 = blend_F1(ks, wsf, d, nu, rho, CD);
-            const double F2 = blend_F2(ks, wsf, d, nu);
+            const double F1 = Blend<real>::F1(ks, wsf, d, nu, rho, CD);
+            const double F2 = Blend<real>::F2(ks, wsf, d, nu);
             const double beta = F1 * B1 + (1.0 - F1) * B2;
             double nut;
             {
@@ -195,13 +242,12 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             // diffusivities with the transport's own F1 (molecular mu, raw k / omega)
             const double wsaf = dmax(w, 1e-10);
             const double CDt = dmax(2.0 * rho * SW2 / wsaf * gdot, 1e-10);
-            const double F1t = blend_F1(k, w, d, mu / dmax(rho, 1e-10), rho, CDt);
+            const double F1t = Blend<real>::F1(k, w, d, mu / dmax(rho, 1e-10), rho, CDt);
             const double rnut = rho * nut;
             const double Gk = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
             const double Gw = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;
-            sv[s * NF + 9] = rho;  // slot 9: rho again, so the face pack is sv[0..3], k, w, Gk, Gw
-            sv[s * NF + 10] = Gk;
-            sv[s * NF + 11] = Gw;
+            sv[s * NF + 10] = (real)Gk;
+            sv[s * NF + 11] = (real)Gw;
             real* nbp = a.nbp + i * 4;
             nbp[0] = (real)k;
             nbp[1] = (real)w;
@@ -209,17 +255,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             nbp[3] = (real)Gw;
             // contravariant transport fluxes (raw gradients in the diffusion)
             const real* ad = a.adj + i * 9;
-            const double ru[3] = {rho * sv[s * NF + 1], rho * sv[s * NF + 2], rho * sv[s * NF + 3]};
+            const real r = sv[s * NF + 0];
+            const real ru[3] = {r * sv[s * NF + 1], r * sv[s * NF + 2], r * sv[s * NF + 3]};
+            const real gkr = (real)Gk, gwr = (real)Gw, kr = sv[s * NF + 7], wr = sv[s * NF + 8];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                const double am[3] = {(double)ad[m * 3 + 0], (double)ad[m * 3 + 1], (double)ad[m * 3 + 2]};
-                const double fr = am[0] * ru[0] + am[1] * ru[1] + am[2] * ru[2];
-                sF[(m * NS + s) * 4 + 0] = fr * k;
-                sF[(m * NS + s) * 4 + 1] = fr * w;
-                sF[(m * NS + s) * 4 + 2] =
-                    Gk * (am[0] * sg[s * 15 + 9] + am[1] * sg[s * 15 + 10] + am[2] * sg[s * 15 + 11]);
-                sF[(m * NS + s) * 4 + 3] =
-                    Gw * (am[0] * sg[s * 15 + 12] + am[1] * sg[s * 15 + 13] + am[2] * sg[s * 15 + 14]);
+                const real am[3] = {ad[m * 3 + 0], ad[m * 3 + 1], ad[m * 3 + 2]};
+                const real fr = am[0] * ru[0] + am[1] * ru[1] + am[2] * ru[2];
+                sF[(m * NS + s) * 4 + 0] = fr * kr;
+                sF[(m * NS + s) * 4 + 1] = fr * wr;
+                sF[(m * NS + s) * 4 + 2] = gkr * (am[0] * sg[s * 15 + 9] + am[1] * sg[s * 15 + 10] + am[2] * sg[s * 15 + 11]);
+                sF[(m * NS + s) * 4 + 3] = gwr * (am[0] * sg[s * 15 + 12] + am[1] * sg[s * 15 + 13] + am[2] * sg[s * 15 + 14]);
             }
         }
         wave_sync();
@@ -227,33 +273,34 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
         if (t < NSLOT * NFP) {
             const int slot = t / NFP, fp = t % NFP;
             const real* e = s_E + (slot * NFP + fp) * NS;
-            double tr[kPack];
+            constexpr int SRC[kPack] = {0, 1, 2, 3, 7, 8, 10, 11};
+            real x0[kPack], tr[kPack];
 #pragma unroll
-            for (int q = 0; q < kPack; ++q) tr[q] = 0.0;
+            for (int q = 0; q < kPack; ++q) {
+                x0[q] = sv[SRC[q]];
+                tr[q] = 0;
+            }
             for (int j = 0; j < NS; ++j) {
-                const double ej = (double)e[j];
-                const double* x = sv + j * NF;
-                tr[0] += ej * x[0];
-                tr[1] += ej * x[1];
-                tr[2] += ej * x[2];
-                tr[3] += ej * x[3];
-                tr[4] += ej * x[7];
-                tr[5] += ej * x[8];
-                tr[6] += ej * x[10];
-                tr[7] += ej * x[11];
+                const real ej = e[j];
+                const real* x = sv + j * NF;
+#pragma unroll
+                for (int q = 0; q < kPack; ++q) tr[q] += ej * (x[SRC[q]] - x0[q]);
             }
             real* o = a.pack + ((cell * 5 + slot) * NFP + fp) * kPack;
 #pragma unroll
-            for (int q = 0; q < kPack; ++q) o[q] = (real)tr[q];
+            for (int q = 0; q < kPack; ++q) o[q] = x0[q] + tr[q];
         }
-        // volume terms: -(D.F_conv + D.F_diff)/det for k and omega
-        for (int o = threadIdx.x & 63; o < 2 * NS; o += 64) {
+        // volume terms: -(D.F_conv + D.F_diff)/det for k and omega, perturbations about SP 0
+        for (int o = t; o < 2 * NS; o += 64) {
             const int s = o >> 1, f = o & 1;
-            double acc = 0.0;
-            for (int m = 0; m < 3; ++m)
+            real acc = 0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const real f0 = sF[(m * NS) * 4 + f] + sF[(m * NS) * 4 + 2 + f];
                 for (int j = 0; j < NS; ++j)
-                    acc += (double)s_D[(m * NS + s) * NS + j] * (sF[(m * NS + j) * 4 + f] + sF[(m * NS + j) * 4 + 2 + f]);
-            a.vol[(cell * NS + s) * 2 + f] = (real)(-acc / (double)a.det[cell * NS + s]);
+                    acc += s_D[(m * NS + s) * NS + j] * (sF[(m * NS + j) * 4 + f] + sF[(m * NS + j) * 4 + 2 + f] - f0);
+            }
+            a.vol[(cell * NS + s) * 2 + f] = -acc / a.det[cell * NS + s];
         }
         wave_sync();
     }
