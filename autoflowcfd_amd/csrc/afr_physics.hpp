@@ -235,4 +235,34 @@ __host__ __device__ __forceinline__ void viscous_flux(const real q[5], const rea
     }
 }
 
+// Per-wave walk over the cells of a persistent grid.  Workgroups are dispatched round-robin
+// over the 8 XCDs (blockIdx % 8); each XCD walks one contiguous range of the cell list so
+// that, with a locality-ordered list (schedule), the neighbour data of consecutive cells is
+// shared in that XCD's L2.  The grid must be a multiple of 8 (or below 8): every list
+// position is then visited exactly once.
+struct CellWalk {
+    int64_t it, hi, step;
+    __device__ __forceinline__ CellWalk(int64_t n, int waves_per_block, int wid) {
+        const int nxcd = gridDim.x >= 8 ? 8 : 1;
+        const int xcd = blockIdx.x % nxcd, lb = blockIdx.x / nxcd;
+        step = (int64_t)(gridDim.x / nxcd) * waves_per_block;
+        it = n * xcd / nxcd + (int64_t)lb * waves_per_block + wid;
+        hi = n * (xcd + 1) / nxcd;
+    }
+};
+
+// cell of list position it: schedule[it] (a permutation of [0, n)) or it itself
+__device__ __forceinline__ int64_t walk_cell(const int* schedule, int64_t it) {
+    return schedule ? (int64_t)__builtin_amdgcn_readfirstlane(schedule[it]) : it;
+}
+
+// persistent grid of at most cap workgroups for n cells at wpb waves per workgroup, a
+// multiple of 8 when at least 8
+inline int walk_grid(int64_t n, int wpb, int cap) {
+    int64_t need = (n + wpb - 1) / wpb;
+    int64_t g = need < cap ? need : cap;
+    g = g >= 8 ? g / 8 * 8 : g;
+    return (int)(g > 0 ? g : 1);
+}
+
 }  // namespace afr

@@ -485,13 +485,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> 
                 int sp[KMAX];
                 real rw[KMAX];
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) {
-                    sp[k] = 0;
-                    rw[k] = real(0);
-                    if (k < a.knz) {
-                        sp[k] = a.rows_idx[(int64_t)row * a.knz + k];
-                        rw[k] = a.rows_w[(int64_t)row * a.knz + k];
-                    }
+                for (int k = 0; k < KMAX; ++k) {  // rows are KMAX wide (zero-padded)
+                    sp[k] = a.rows_idx[(int64_t)row * KMAX + k];
+                    rw[k] = a.rows_w[(int64_t)row * KMAX + k];
                 }
                 real dqn[5] = {0, 0, 0, 0, 0};
                 if (src == 0) {  // primary source: staged in LDS
@@ -632,9 +628,9 @@ int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
     const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64 * kWaves);
     // interpolation rows: Kronecker-sparse (prism sources: P2 <= 9, P1 <= 4 non-zeros) or dense (tet sources)
     constexpr int KS = (P + 1) * (P + 1), NS = PK<P>::NS;
-    if (a.knz <= KS)
+    if (a.knz == KS)
         hipLaunchKernelGGL((k_prism_inviscid<P, KS, real>), grid, block, 0, st, a);
-    else if (a.knz <= NS)
+    else if (a.knz == NS)
         hipLaunchKernelGGL((k_prism_inviscid<P, NS, real>), grid, block, 0, st, a);
     else
         return -1;

@@ -462,6 +462,11 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
         !desc->work_umean || !desc->prism_tab || !desc->rows_nnz || !desc->rows_idx || !desc->rows_w ||
         (desc->n_prism > 0 && !desc->prism_schedule))
         return fail("afr_mesh_create: missing device array");
+    {
+        const int n1 = desc->order + 1;
+        if (desc->rows_knz != n1 * n1 && desc->rows_knz != n1 * n1 * n1)
+            return fail("afr_mesh_create: rows_knz must be (order+1)^2 or (order+1)^3 (zero-padded rows)");
+    }
     std::vector<int64_t> po, to, wo;
     if (desc->cell_case) {
         const int nc = desc->n_cases;
@@ -573,6 +578,9 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.det = (const real*)d.det;
         a.adj = (const real*)d.adj;
         a.vpack = (real*)d.work_vpack;
+        // the interface kernel reads neighbour data: locality order; the volume kernel only
+        // its own cell: storage order (contiguous reads)
+        a.schedule = (ct == 0 && phase != 0) ? d.prism_schedule : nullptr;
         a.Dsp_w = (const real*)d.dsp_w[ct];
         a.Dsp_i = d.dsp_i[ct];
         a.Esp_w = (const real*)d.esp_w[ct];
@@ -862,6 +870,7 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
             b.cell_begin = ct == 0 ? 0 : d.n_prism;
             b.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
             b.D = (const real*)d.D[ct];
+            b.schedule = ct == 0 ? d.prism_schedule : nullptr;
             if (b.n_cells <= 0) continue;
             int rc;
             if (phase == 0)

@@ -138,9 +138,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
     real* sg = sv + NS * NF;      // [s][15]: grad m (9), grad k (3), grad w (3)
     real* sF = sg + NS * 15;      // [m][s][4]: contravariant fluxes rho u k, rho u w, Gk grad k, Gw grad w
     const double mu = a.mu_molecular;
-    const int64_t nw = (int64_t)gridDim.x * kWaves;
-    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
-        const int64_t cell = a.cell_begin + ci;
+    for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
+        const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         if (t < NS) {
             const int64_t i = cell * NS + t;
             real u5[5];
@@ -307,7 +306,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
 }
 
 // ------------------------------------------------------------------ per face record corrections
-template <int P, typename real>
+template <int P, int KMAX, typename real>
 __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
     constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1;
     const int64_t n = a.n_records * NFP;
@@ -331,10 +330,17 @@ __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
             const int64_t nb = nbc >= 0 ? nbc : 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) pn[q] = 0.0;
-            for (int kk = 0; kk < a.knz; ++kk) {
-                const int sp = a.rows_idx[(int64_t)row * a.knz + kk];
-                const double wgt = (double)a.rows_w[(int64_t)row * a.knz + kk];
-                const real* pk = a.nbp + (nb * NS + sp) * 4;
+            int spk[KMAX];
+            real wk[KMAX];
+#pragma unroll
+            for (int kk = 0; kk < KMAX; ++kk) {  // rows are KMAX wide (zero-padded)
+                spk[kk] = a.rows_idx[(int64_t)row * KMAX + kk];
+                wk[kk] = a.rows_w[(int64_t)row * KMAX + kk];
+            }
+#pragma unroll
+            for (int kk = 0; kk < KMAX; ++kk) {
+                const real* pk = a.nbp + (nb * NS + spk[kk]) * 4;
+                const double wgt = (double)wk[kk];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) pn[q] += wgt * (double)pk[q];
             }
@@ -368,24 +374,44 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
     double* st = s_t[wid];
     double* sf = s_fld[wid];
-    const int64_t nw = (int64_t)gridDim.x * kWaves;
-    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
-        const int64_t cell = a.cell_begin + ci;
+    for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
+        const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         if (t < 2 * NS) {
             const int s = t >> 1, f = t & 1;
             const int64_t i = cell * NS + s;
             const double det = (double)a.det[i];
-            double acc = 0.0;
+            // per-lane lookups of the three cube axes, hoisted out of the incidence loop
+            int fps[3];
+            double gl[3], gr[3];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                fps[ax] = a.fp_of_sp[ax * NS + s];
+                const int cs = a.coord_of_sp[ax * NS + s];
+                gl[ax] = (double)a.gL[cs];
+                gr[ax] = (double)a.gR[cs];
+            }
+            int ev[kInc];
+#pragma unroll
+            for (int q = 0; q < kInc; ++q) ev[q] = a.inc[cell * kInc + q];
+            double cv[kInc];
+#pragma unroll
             for (int q = 0; q < kInc; ++q) {
-                const int e = a.inc[cell * kInc + q];
+                const int e = ev[q];
+                const int axis = (e & 7) >> 1;
+                const int fp = axis == 0 ? fps[0] : (axis == 1 ? fps[1] : fps[2]);
+                cv[q] = e < 0 ? 0.0 : (double)a.crec[((int64_t)(e >> 4) * NFP + fp) * 2 + f];
+            }
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < kInc; ++q) {
+                const int e = ev[q];
                 if (e < 0) continue;
-                const int64_t r = e >> 4;
                 const int role = (e >> 3) & 1;
                 const int code = e & 7;
                 const int axis = code >> 1;
-                const double g = (double)((code & 1) ? a.gR : a.gL)[a.coord_of_sp[axis * NS + s]];
-                const double cv = (double)a.crec[(r * NFP + a.fp_of_sp[axis * NS + s]) * 2 + f];
-                acc += (role == 0 ? -1.0 : 1.0) * g * cv / det;
+                const double g = (code & 1) ? (axis == 0 ? gr[0] : (axis == 1 ? gr[1] : gr[2]))
+                                            : (axis == 0 ? gl[0] : (axis == 1 ? gl[1] : gl[2]));
+                acc += (role == 0 ? -1.0 : 1.0) * g * cv[q] / det;
             }
             const double rho = dmax((double)a.U[i * 5], 1e-10);
             st[t] = ((double)a.vol[i * 2 + f] + acc) / rho;
@@ -502,8 +528,7 @@ __global__ void __launch_bounds__(256) k_wall_distance(const double* __restrict_
 template <int P, int CT, typename real>
 int launch_sst_cell(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, 4096)), dim3(64 * kWaves), 0,
                        st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -513,15 +538,21 @@ int launch_sst_face(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_records <= 0) return 0;
     const int64_t n = a.n_records * (P + 1) * (P + 1);
     const int64_t need = (n + 255) / 256;
-    hipLaunchKernelGGL((k_sst_face<P, real>), dim3((unsigned)(need < 8192 ? need : 8192)), dim3(256), 0, st, a);
+    constexpr int KS = (P + 1) * (P + 1), NS = KS * (P + 1);
+    const dim3 grid((unsigned)(need < 8192 ? need : 8192));
+    if (a.knz == KS)
+        hipLaunchKernelGGL((k_sst_face<P, KS, real>), grid, dim3(256), 0, st, a);
+    else if (a.knz == NS)
+        hipLaunchKernelGGL((k_sst_face<P, NS, real>), grid, dim3(256), 0, st, a);
+    else
+        return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int P, int CT, typename real>
 int launch_sst_update(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL((k_sst_update<P, CT, real>), dim3((unsigned)(need < 4096 ? need : 4096)), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_sst_update<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, 4096)), dim3(64 * kWaves), 0,
                        st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -47,6 +47,7 @@ struct SstArgs {
     const int* fp_of_sp;
     const int* coord_of_sp;
     double mu_molecular;
+    const int* schedule;          // null, or the locality order of the cells of this type
 };
 
 template <int P, int CT, typename real>

@@ -160,9 +160,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
     const double T0 = q0p / (rmax(q0r, 1e-10) * kRair);
     constexpr int WD[3] = {W::W0, W::W1, W::W2};
 
-    const int64_t nw = (int64_t)gridDim.x * kWaves;
-    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
-        const int64_t cell = a.cell_begin + ci;
+    for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
+        const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         t = lane_fresh(t);
         // 1. primitives, perturbations, |U|
         real mut_s = real(0);
@@ -304,9 +303,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
     real* const sR = sJ + 5 * NFP * 5;   // residual [s][5]
     const real mu = (real)a.mu, Pr = (real)a.Pr, Prt = (real)a.Prt;
 
-    const int64_t nw = (int64_t)gridDim.x * kWaves;
-    for (int64_t ci = (int64_t)blockIdx.x * kWaves + wid; ci < a.n_cells; ci += nw) {
-        const int64_t cell = a.cell_begin + ci;
+    for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
+        const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         t = lane_fresh(t);
         for (int o = t; o < NS * kVP; o += 64) sP[o] = a.vpack[cell * NS * kVP + o];
         wave_sync();
@@ -340,15 +338,33 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                 const int nbc = a.slot_nb[sidx * 2 + src];
                 if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
                 const int64_t nb = nbc >= 0 ? nbc : 0;
+                int spk[KMAX];
+                real wk[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {  // rows are KMAX wide (zero-padded)
+                    spk[k] = a.rows_idx[(int64_t)row * KMAX + k];
+                    wk[k] = a.rows_w[(int64_t)row * KMAX + k];
+                }
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k) {
-                    if (k < a.knz) {
-                        const int sp = a.rows_idx[(int64_t)row * a.knz + k];
-                        const real w = a.rows_w[(int64_t)row * a.knz + k];
-                        const real* x = a.vpack + (nb * NS + sp) * kVP;
+                    const float4* x = reinterpret_cast<const float4*>(a.vpack + (nb * NS + spk[k]) * kVP);
+                    real xv[20];
+                    if constexpr (sizeof(real) == 4) {
 #pragma unroll
-                        for (int q = 0; q < NP; ++q) pn[q] += w * x[q];
+                        for (int c = 0; c < 5; ++c) {
+                            const float4 f = x[c];
+                            xv[4 * c + 0] = f.x;
+                            xv[4 * c + 1] = f.y;
+                            xv[4 * c + 2] = f.z;
+                            xv[4 * c + 3] = f.w;
+                        }
+                    } else {
+                        const real* xr = a.vpack + (nb * NS + spk[k]) * kVP;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) xv[q] = xr[q];
                     }
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) pn[q] += wk[k] * xv[q];
                 }
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
@@ -430,16 +446,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
 template <int P, int CT, typename real>
 int launch_viscous_phase(const ViscArgs<real>& a, int phase, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
-    const int grid = (int)(need < 2048 ? need : 2048);
+    const int grid = walk_grid(a.n_cells, kWaves, 2048);
     if (phase == 0) {
         hipLaunchKernelGGL((k_visc_volume<P, CT, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
     constexpr int NS = VO<P, CT>::NS, KS = (P + 1) * (P + 1);
-    if (a.knz <= KS)
+    if (a.knz == KS)
         hipLaunchKernelGGL((k_visc_iface<P, CT, KS, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
-    else if (a.knz <= NS)
+    else if (a.knz == NS)
         hipLaunchKernelGGL((k_visc_iface<P, CT, NS, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
     else
         return -1;

@@ -41,6 +41,7 @@ struct ViscArgs {
     const int* coord_of_sp;
     const double* bcp;
     const int* cell_case;          // null: one case
+    const int* schedule;           // null, or the locality order of the cells of this type
     double qref[5];
     double mu, Pr, Prt;
     int suppress;

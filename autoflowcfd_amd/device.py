@@ -120,8 +120,16 @@ class DeviceMesh:
         self.rows = dev(sl.rows)
         # sparse rows plus one trailing all-zero row (index n_rows) for lanes without a source
         nnz, idx, w = sparse_rows(np.concatenate([sl.rows, np.zeros((1, sl.rows.shape[1]))]))
+        # fixed row width: (P+1)^2 when every row fits (prism sources), else (P+1)^3, zero-padded,
+        # so the kernels read a compile-time number of entries per row without a bound check
+        width = (order + 1) ** 2 if idx.shape[1] <= (order + 1) ** 2 else T.ns
+        if idx.shape[1] > width:
+            raise ValueError("interpolation row wider than the solution points")
+        pad = width - idx.shape[1]
+        idx = np.pad(idx, ((0, 0), (0, pad)))
+        w = np.pad(w, ((0, 0), (0, pad)))
         self.rows_nnz, self.rows_idx, self.rows_w = devi(nnz), devi(idx), dev(w)
-        self.rows_knz = int(idx.shape[1])
+        self.rows_knz = int(width)
         self.prism_tab = dev(prism_factors(order).packed())
         # prisms are processed in Morton order of their centroids (L2 reuse of source cells)
         cen = mesh.nodes[mesh.prism_conn].mean(axis=1) if mesh.n_prism else np.zeros((0, 3))
@@ -375,9 +383,21 @@ class DeviceMesh:
                 fill[c] += 1
         if F * 16 >= 2 ** 31:
             raise ValueError("too many face records for the int32 incidence encoding")
+        # records renumbered in the locality order of their owners (prisms: the Morton schedule,
+        # then tets): the face kernel then walks neighbouring cells together; each cell keeps
+        # its incidence list in the same order, only the record ids are renamed
+        rank = np.arange(C, dtype=np.int64)
+        if mesh.n_prism:
+            sched = self.prism_schedule.cpu().numpy().astype(np.int64)
+            rank[sched] = np.arange(mesh.n_prism)
+        perm = np.argsort(rank[fr.owner], kind="stable")
+        new_id = np.empty(F, dtype=np.int64)
+        new_id[perm] = np.arange(F)
+        inc = np.where(inc >= 0, new_id[np.maximum(inc, 0) >> 4] * 16 + (inc & 15), -1)
         self.n_records = F
-        self.rec_owner, self.rec_oslot, self.rec_flags = devi(fr.owner), devi(oslot), devi(flags)
-        self.rec_tn = dev(fr.normal)
+        self.rec_perm = perm
+        self.rec_owner, self.rec_oslot, self.rec_flags = devi(fr.owner[perm]), devi(oslot[perm]), devi(flags[perm])
+        self.rec_tn = dev(fr.normal[perm])
         self.cell_inc = devi(inc)
         dt = self.dtype
         self.work_pack = torch.empty((C, 5, T.nfp, 8), dtype=dt, device=self.device)

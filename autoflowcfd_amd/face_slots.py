@@ -97,6 +97,7 @@ def locate(mesh, cells, codes, targets):
             free[sel] = np.stack([a, b, c], -1)[..., other]
             continue
         x = np.zeros((len(sel), n, 2))
+        polish = False
         for _ in range(60):
             ref = np.empty((len(sel), n, 3))
             ref[..., ax] = sd
@@ -108,8 +109,11 @@ def locate(mesh, cells, codes, targets):
             Jtr = np.einsum("knia,kni->kna", Jf, res)
             dx = -np.linalg.solve(JtJ, Jtr[..., None])[..., 0]
             x += dx
-            if np.max(np.abs(dx)) < 1e-15:
+            # quadratic convergence: once the step is below 1e-12 one more step reaches the
+            # round-off floor (a 1e-15 test on the batch maximum can wait on round-off noise)
+            if polish:
                 break
+            polish = np.max(np.abs(dx)) < 1e-12
         free[sel] = x
     return free
 
@@ -411,3 +415,10 @@ def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
     return Slots(nb=out["nb"], blk=out["blk"], mask=out["mask"], bc=out["bc"], tn=out["tn"], adjrow=out["adjrow"],
                  fallback=out["fallback"], rows=np.concatenate(rows), n_blocks=blk_off,
                  keyed_slots=sum(s.keyed_slots for s in slot_list), split_slots=sum(s.split_slots for s in slot_list))
+
+
+def slots_for(mesh, order):
+    """Slot tables of a mesh at an order without geometry (the device metric precompute fills
+    the rest): the host set-up of one solve, runnable in a worker process."""
+    from .fr_tables import tables
+    return build_slots(mesh, tables(order), with_geometry=False)

@@ -7,8 +7,10 @@ solve resident in HBM until it retires:
 
   * admission: a request is admitted when an active slot is free and its estimated device
     bytes fit the HBM budget (bytes per solution point, measured on the solves already built);
-  * set-up (mesh tables, metrics, first pseudo-step, HIP-graph capture of the pseudo-step)
-    runs on a worker thread with its own stream, overlapping the running solves;
+  * set-up overlaps the running solves: the host face-slot tables of every waiting request
+    are built ahead in worker processes (NumPy, no GPU), and the device part (tables upload,
+    metrics, first pseudo-step, HIP-graph capture of the pseudo-step) runs on a worker
+    thread with its own stream;
   * every active solve has its own stream (default priority); one pseudo-step is one graph
     replay followed by an asynchronous copy of its FP64 norm to pinned host memory and an
     event; the scheduler polls the events and relaunches each solve as soon as its previous
@@ -21,8 +23,9 @@ solve resident in HBM until it retires:
 of the norm after every step, retire, next request.
 """
 
+import multiprocessing
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List
 
@@ -30,6 +33,7 @@ import numpy as np
 import torch
 
 from .batch import Case, cons, free_stream_coefficients
+from .face_slots import slots_for
 from .mesh import BC_FARFIELD, BC_INLET
 
 
@@ -58,6 +62,7 @@ class SolveResult:
     arrival: float
     admitted: float
     finished: float
+    setup_s: float = 0.0          # set-up time once started: tables, metrics, first step, graph capture
 
     @property
     def latency(self):
@@ -74,7 +79,7 @@ def _stop(hist, it, req):
     return it >= req.max_iter, False
 
 
-def build_solver(req, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, ref_state=None):
+def build_solver(req, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, ref_state=None, slots=None):
     """GPUFRSolver of a request: free stream of the case at every FARFIELD / INLET face and as
     the uniform initial state (``ref_state``: the perturbation form's constant state, default
     the free stream)."""
@@ -83,7 +88,7 @@ def build_solver(req, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, ref_sta
     Q = np.asarray(c.Q_free, dtype=np.float64)
     s = GPUFRSolver(c.mesh, order=req.order, rho_inf=Q[0], vel_inf=float(np.linalg.norm(Q[1:4])), p_inf=Q[4],
                     turb_model=turb_model, dtype=dtype, k_inf=c.k_inf, omega_inf=c.omega_inf, mu_molecular=mu,
-                    bc_overrides={BC_FARFIELD: Q, BC_INLET: Q}, ref_state=ref_state)
+                    bc_overrides={BC_FARFIELD: Q, BC_INLET: Q}, ref_state=ref_state, slots=slots)
     U0 = np.broadcast_to(cons(Q), (s.dm.n_cells, s.dm.ns, 5))
     s.U.copy_(torch.as_tensor(np.ascontiguousarray(U0), dtype=dtype))
     return s
@@ -113,15 +118,22 @@ def solve_alone(requests, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, clo
         if use_graph:
             s.capture_graph()
             hist.append(s.residual_history[-1])
-        while not hist or not _stop(hist, len(hist), req)[0]:
+        else:
+            hist.append(s.step())
+        setup = clock() - t0 - start
+        while not _stop(hist, len(hist), req)[0]:
             hist.append(s.step())
         conv = _stop(hist, len(hist), req)[1]
         coef = _coefficients(s, req.case)
         torch.cuda.synchronize()
         out.append(SolveResult(req.rid, req.order, s.dm.n_cells, len(hist), conv, hist[-1], hist, coef,
-                               req.arrival, start, clock() - t0))
+                               req.arrival, start, clock() - t0, setup))
         s.cleanup()
     return out
+
+
+def _noop(_):
+    return None
 
 
 @dataclass
@@ -135,13 +147,14 @@ class _Active:
     nbytes: int
     hist: List[float] = field(default_factory=list)
     in_flight: bool = False
+    setup_s: float = 0.0
 
 
 class SolveServer:
     """Continuous batching of steady solves on one GPU (see the module docstring)."""
 
     def __init__(self, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, max_active=8, hbm_budget=None,
-                 use_graph=True, clock=time.perf_counter):
+                 use_graph=True, clock=time.perf_counter, host_workers=2):
         if not torch.cuda.is_available():
             raise RuntimeError("SolveServer needs a visible GPU (HIP device)")
         self.dtype, self.turb, self.mu = dtype, turb_model, mu
@@ -153,6 +166,26 @@ class SolveServer:
         self.bytes_per_point = {}      # order -> measured device bytes per solution point
         self.peak_active = 0
         self.peak_bytes = 0
+        self.loop_rounds = 0           # scheduler rounds, and those in which nothing was ready
+        self.idle_rounds = 0
+        self.steps_launched = 0
+        # host set-up processes: fresh interpreters (spawn) that import NumPy only, never the GPU
+        self.host_pool = None
+        if host_workers > 0:
+            self.host_pool = ProcessPoolExecutor(max_workers=host_workers,
+                                                 mp_context=multiprocessing.get_context("spawn"))
+            list(self.host_pool.map(_noop, range(host_workers)))     # start the workers now
+
+    def close(self):
+        if self.host_pool is not None:
+            self.host_pool.shutdown(wait=True)
+            self.host_pool = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def estimate_bytes(self, req):
         ns = (req.order + 1) ** 3
@@ -160,12 +193,14 @@ class SolveServer:
         # before any solve of this order is measured: a generous 2 KiB per solution point
         return int(self.bytes_per_point.get(req.order, 2048.0) * n_pts * 1.1)
 
-    def _prepare(self, req):
+    def _prepare(self, req, host_future):
         """Worker thread: build the solve, run its first pseudo-step and capture the step graph."""
+        t_start = self.clock()
+        slots = host_future.result() if host_future is not None else None
         stream = torch.cuda.Stream()
         before = torch.cuda.memory_allocated()
         with torch.cuda.stream(stream):
-            s = build_solver(req, self.dtype, self.turb, self.mu)
+            s = build_solver(req, self.dtype, self.turb, self.mu, slots=slots)
             if self.use_graph:
                 s.capture_graph(capture_error_mode="thread_local")    # warm-up step counted
                 first = s.residual_history[-1]
@@ -173,13 +208,15 @@ class SolveServer:
                 first = s.step()
         torch.cuda.synchronize()
         used = max(torch.cuda.memory_allocated() - before, 1)
-        return s, stream, first, used
+        return s, stream, first, used, self.clock() - t_start
 
     def run(self, requests) -> List[SolveResult]:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         t0 = self.clock()
         now = lambda: self.clock() - t0  # noqa: E731
         waiting, active, results = [], [], []
+        host = {}                # rid -> future of the host slot tables (or None)
+        self._graveyard = []     # retired solvers awaiting release
         i_next = 0
         used_bytes = 0
         prep = None              # (future, request, admitted time, reserved bytes)
@@ -188,25 +225,31 @@ class SolveServer:
             while len(results) < len(reqs):
                 t = now()
                 while i_next < len(reqs) and reqs[i_next].arrival <= t:
-                    waiting.append(reqs[i_next])
+                    r_new = reqs[i_next]
+                    waiting.append(r_new)
+                    # host tables built ahead in the worker processes, off the GPU thread
+                    host[r_new.rid] = (self.host_pool.submit(slots_for, r_new.case.mesh, r_new.order)
+                                       if self.host_pool is not None else None)
                     i_next += 1
+                if prep is None and self._graveyard:
+                    self._graveyard.clear()      # no capture in flight: release retired solvers
                 # admission: one set-up at a time, slots and memory permitting
                 if prep is None and waiting and len(active) < self.max_active:
                     need = self.estimate_bytes(waiting[0])
                     if used_bytes + need <= self.hbm_budget or not active:
                         req = waiting.pop(0)
                         used_bytes += need
-                        prep = (pool.submit(self._prepare, req), req, t, need)
+                        prep = (pool.submit(self._prepare, req, host.pop(req.rid)), req, t, need)
                 if prep is not None and prep[0].done():
                     fut, req, t_adm, need = prep
                     prep = None
-                    s, stream, first, used = fut.result()
+                    s, stream, first, used, setup = fut.result()
                     ns = (req.order + 1) ** 3
                     self.bytes_per_point[req.order] = max(self.bytes_per_point.get(req.order, 0.0),
                                                           used / (s.dm.n_cells * ns))
                     used_bytes += used - need
                     a = _Active(req, s, stream, torch.zeros(1, dtype=torch.float64, pin_memory=True),
-                                torch.cuda.Event(), t_adm, used, hist=[first])
+                                torch.cuda.Event(), t_adm, used, hist=[first], setup_s=setup)
                     stop, conv = _stop(a.hist, 1, req)
                     if stop:
                         results.append(self._retire(a, conv, now))
@@ -231,10 +274,13 @@ class SolveServer:
                             continue
                     self._launch(a)
                     progressed = True
+                self.loop_rounds += 1
                 if not progressed:
+                    self.idle_rounds += 1
                     time.sleep(0.0002)
         finally:
             pool.shutdown(wait=True)
+            self._graveyard.clear()
         return sorted(results, key=lambda r: r.rid)
 
     def _launch(self, a):
@@ -250,6 +296,7 @@ class SolveServer:
             a.event.record(a.stream)
         s.iteration += 1
         a.in_flight = True
+        self.steps_launched += 1
 
     def _retire(self, a, converged, now):
         s = a.solver
@@ -258,11 +305,11 @@ class SolveServer:
         a.stream.synchronize()
         s.residual_history = list(a.hist)
         r = SolveResult(a.req.rid, a.req.order, s.dm.n_cells, len(a.hist), converged, a.hist[-1], list(a.hist), coef,
-                        a.req.arrival, a.admitted, now())
-        for name in ("U", "k", "omega", "nu_t", "mu_t"):
-            setattr(s, name, None)
-        s.dm = None
-        s._graph = None
+                        a.req.arrival, a.admitted, now(), a.setup_s)
+        # the solver (device buffers, HIP graph) is released later by the scheduler, at a point
+        # where no graph capture runs on the set-up thread: destroying a graph during another
+        # thread's capture is not permitted
+        self._graveyard.append(s)
         return r
 
 

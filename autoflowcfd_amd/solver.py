@@ -47,12 +47,13 @@ class GPUFRSolver:
                  time_scheme: str = "ssp_rk3", cfl: float = 1.0, rho_inf: float = 1.225, vel_inf: float = 33.33,
                  p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
                  turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
-                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None):
+                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None, slots=None):
         """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
         compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
         bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
         or {BC_OUTLET: p}; the default is the freestream along +x (boundary.py:52-145).
         k_inf / omega_inf: initial turbulence fields (reference defaults 1e-6 and 1.0, sst.py:47-48).
+        slots: precomputed face-slot tables of the mesh at this order (face_slots.slots_for).
         ref_state: constant primitive state of the perturbation form (default: the FARFIELD state);
         a case stepped alone with the ref_state of a batch reproduces its batched trajectory bit
         for bit (batch.CaseBatch).
@@ -81,7 +82,7 @@ class GPUFRSolver:
         p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
         bcp = bc_params_table(Q_free, Q_inlet, p_out)
         self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=ref_state,
-                             turbulence="SST" if self.turb_model_name == "SST" else None)
+                             slots=slots, turbulence="SST" if self.turb_model_name == "SST" else None)
         C, ns = self.dm.n_cells, self.dm.ns
         U0 = _cons(np.broadcast_to(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]), (C, ns, 5)))
         self.U = torch.as_tensor(U0, dtype=dtype).to(self.device).contiguous()
@@ -91,6 +92,8 @@ class GPUFRSolver:
             self.k = self.omega = self.nu_t = self.mu_t = None
         self.residual_history = []
         self.iteration = 0
+        self._graph = None
+        self._graph_norm = None
 
     # ------------------------------------------------------------------ stepping
     def _launch_step(self):
@@ -125,7 +128,7 @@ class GPUFRSolver:
 
     def step(self, dt: float = 0.0) -> float:
         """One steady pseudo-step (dt is ignored: local time stepping); returns the residual RMS."""
-        if getattr(self, "_graph", None) is not None:
+        if self._graph is not None:
             self._graph.replay()
             norm = self._graph_norm
         else:

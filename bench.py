@@ -162,6 +162,9 @@ def served_leg(args):
     served = srv.run(reqs)
     log(f"[bench] served path done in {time.perf_counter() - t0:.1f}s (peak {srv.peak_active} resident solves)")
     sp, ss = latency_stats(plain), latency_stats(served)
+    busy = [r.finished - r.admitted - r.setup_s for r in plain]
+    log(f"[bench] plain: mean set-up {np.mean([r.setup_s for r in plain]):.3f}s, mean stepping {np.mean(busy):.3f}s "
+        f"per solve; served: mean set-up {np.mean([r.setup_s for r in served]):.3f}s")
     dcd = max(abs(a.coefficients.get("Cd", 0.0) - b.coefficients.get("Cd", 0.0)) for a, b in zip(served, plain))
     same = all(a.residual_history == b.residual_history for a, b in zip(served, plain))
     return {
@@ -187,9 +190,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=6)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-max-iter", type=int, default=200)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
     ap.add_argument("--serve-seed", type=int, default=1)
     args = ap.parse_args()
@@ -201,6 +205,9 @@ def main():
     import torch
     from autoflowcfd_amd.device import DeviceMesh, bc_params_table
 
+    if args.serve_only:
+        print(json.dumps(served_leg(args)), flush=True)
+        return
     t0 = time.time()
     mesh = wall_box_mesh(args.n)
     log(f"[bench] mesh {mesh.n_cells} prisms, {mesh.faces.n_faces} face records ({time.time() - t0:.1f}s)")
