@@ -85,6 +85,20 @@ def algorithmic_bytes_residual(n_sp, n_fine_pts, n_faces, nfp, n_v=7, w=4, sst=T
             + n_faces * (nfp * 3 * w + 16))
 
 
+def algorithmic_bytes_visc_volume(n_sp, n_cells, w=4):
+    """Compulsory traffic of the viscous volume launch: read U (5), mu_t, det, adj (9) per
+    solution point; write dQ (5), the 18 used values of the interface pack, the volume
+    residual (5) per point, and the outlier field mean (5) and h per cell."""
+    return n_sp * (5 + 1 + 1 + 9 + 5 + 18 + 5) * w + n_cells * 6 * w
+
+
+def algorithmic_bytes_visc_iface(n_sp, n_cells, nfp, w=4):
+    """Compulsory traffic of the viscous interface launch: read the 18 pack values, the volume
+    residual (5) and det per point, read-modify-write R (5); per cell the five face slots'
+    extrapolated adjugate rows (nfp x 3), normals (3), six slot ints, h and the field mean (5)."""
+    return n_sp * (18 + 5 + 1 + 10) * w + n_cells * (5 * (nfp * 3 + 3) * w + 5 * 6 * 4 + 6 * w)
+
+
 def cpu_baseline(order, n_sample=6, budget_s=20.0):
     """FP64 oracle (NumPy restatement of the reference CPU path), single thread, on a
     bounded sample: an n_sample^3 hex version of the same graded wall box, one full
@@ -133,11 +147,14 @@ def cpu_baseline(order, n_sample=6, budget_s=20.0):
                       f"median of {len(times)} evaluations, {t:.3f} s each, 1 thread"}
 
 
-def traffic_from_profiles(name):
-    p = os.path.join(HERE, "profiles", f"{name}.json")
+def traffic_from_profiles(kernel, workload="wallbox82_p2_sst_fp32"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy's), or None."""
+    p = os.path.join(HERE, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            k = json.load(f).get("kernels", {}).get(kernel)
+        return None if k is None else k.get("bytes_per_launch")
     return None
 
 
@@ -275,8 +292,21 @@ def main():
     fr = mesh.faces
     is_quad = np.isin(fr.owner_code, (0, 1, 2)) & (fr.owner < mesh.n_prism)
     n_phys_faces = int(fr.n_faces - is_quad.sum() // 2)
-    t_inv = ph[3] * 1e-3
     b_inv = algorithmic_bytes_inviscid(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
+    # per-kernel rooflines of the three single-kernel phases (prism-only mesh: one launch each);
+    # the dominant one (longest launch) is the headline roofline
+    P, knz = args.order, dm.rows_knz
+    kr = {}
+    for kname, phase, b in ((f"afr_prism::k_prism_inviscid<{P}, {knz}, float>", "inviscid", b_inv),
+                            (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
+                             algorithmic_bytes_visc_volume(n_sp, mesh.n_cells)),
+                            (f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>", "visc_interface",
+                             algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp))):
+        t = float(ph[names.index(phase)]) * 1e-3
+        kr[kname] = {"bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b / t / HBM_PEAK, "traffic": traffic_from_profiles(kname), "kernel_ms": t * 1e3,
+                     "algorithmic_bytes": b}
+    dominant = max(kr, key=lambda k: kr[k]["kernel_ms"])
     b_res = algorithmic_bytes_residual(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
     t_res = ms_per_step * 1e-3
     value = n_sp * 7 / t_res / 1e9
@@ -297,10 +327,8 @@ def main():
                                f"transport), {args.n}^3-hex graded wall box split into {mesh.n_cells} prisms, "
                                f"{n_sp} solution points x 7 variables",
                    "cells": mesh.n_cells, "order": args.order, "n_vars": 7},
-        "roofline": {"bound": "hbm", "achieved": b_inv / t_inv / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b_inv / t_inv / HBM_PEAK, "traffic": traffic_from_profiles("k_prism_inviscid_c2"),
-                     "kernel": f"k_prism_inviscid<{args.order},{dm.rows_knz},float>",
-                     "kernel_ms": t_inv * 1e3, "algorithmic_bytes": b_inv},
+        "roofline": dict(kr[dominant], kernel=dominant),
+        "kernel_rooflines": kr,
         "residual_roofline": {"achieved_GBs": b_res / t_res / 1e9, "frac": b_res / t_res / HBM_PEAK,
                               "algorithmic_bytes": b_res},
         "phase_ms": {n: float(t) for n, t in zip(names, ph)},
