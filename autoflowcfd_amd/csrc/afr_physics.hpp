@@ -235,6 +235,31 @@ __host__ __device__ __forceinline__ void viscous_flux(const real q[5], const rea
     }
 }
 
+// A . G for the viscous flux G of viscous_flux, contracted with a direction A (3): the five
+// components A_i G[i][v], computed from tau . A (tau is symmetric) without forming G.
+template <typename real>
+__host__ __device__ __forceinline__ void viscous_flux_dot(const real q[5], const real gv[3][3], const real gT[3],
+                                                         real mu, real Pr, real mut, real Prt, const real A[3],
+                                                         real out[5]) {
+    const real mt = mu + mut;
+    const real lam = real(-2.0 / 3.0) * mt * (gv[0][0] + gv[1][1] + gv[2][2]);
+    real tA[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        real acc = lam * A[j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc += A[i] * (mt * (gv[i][j] + gv[j][i]));
+        tA[j] = acc;
+    }
+    const real cp = real(kGamma * kRair / (kGamma - 1.0));
+    const real k = mu * cp / Pr + mut * cp / Prt;
+    out[0] = real(0);
+    out[1] = tA[0];
+    out[2] = tA[1];
+    out[3] = tA[2];
+    out[4] = q[1] * tA[0] + q[2] * tA[1] + q[3] * tA[2] - k * (A[0] * gT[0] + A[1] * gT[1] + A[2] * gT[2]);
+}
+
 // Per-wave walk over the cells of a persistent grid.  Workgroups are dispatched round-robin
 // over the 8 XCDs (blockIdx % 8); each XCD walks one contiguous range of the cell list so
 // that, with a locality-ordered list (schedule), the neighbour data of consecutive cells is

@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
             real po[NP], pn[NP];
 #pragma unroll
             for (int k = 0; k < NP; ++k) po[k] = real(0);
-#pragma unroll
+#pragma unroll 3
             for (int k = 0; k < KE; ++k) {
                 const int e = (slot * NFP + fp) * KE + k;
                 const real w = s_Ew[e];
@@ -345,7 +345,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                     spk[k] = a.rows_idx[(int64_t)row * KMAX + k];
                     wk[k] = a.rows_w[(int64_t)row * KMAX + k];
                 }
-#pragma unroll
+                // three neighbour points in flight at a time: enough loads to cover the latency
+                // without the register cost (and lost occupancy) of all KMAX at once
+#pragma unroll 3
                 for (int k = 0; k < KMAX; ++k) {
                     const float4* x = reinterpret_cast<const float4*>(a.vpack + (nb * NS + spk[k]) * kVP);
                     real xv[20];
@@ -395,15 +397,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                 gTa[i] = real(0.5) * (po[14 + i] + pn[14 + i]);
             }
             const real mto = po[17], mta = real(0.5) * (po[17] + pn[17]);
-            real Gc[3][5], Go[3][5];
-            viscous_flux(Qa, gva, gTa, mu, Pr, mta, Prt, Gc);
-            viscous_flux(Qo, gvo, gTo, mu, Pr, mto, Prt, Go);
             const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
             const real A0 = ar[0], A1 = ar[1], A2 = ar[2];
-            real jmp[5];
+            const real Av[3] = {A0, A1, A2};
+            // A . (G(common) - G(own)) from the contracted fluxes
+            real fc[5], fo[5], jmp[5];
+            viscous_flux_dot(Qa, gva, gTa, mu, Pr, mta, Prt, Av, fc);
+            viscous_flux_dot(Qo, gvo, gTo, mu, Pr, mto, Prt, Av, fo);
 #pragma unroll
-            for (int v = 0; v < 5; ++v)
-                jmp[v] = A0 * (Gc[0][v] - Go[0][v]) + A1 * (Gc[1][v] - Go[1][v]) + A2 * (Gc[2][v] - Go[2][v]);
+            for (int v = 0; v < 5; ++v) jmp[v] = fc[v] - fo[v];
             if (bc != BC_INTERIOR) {
                 const real mag = sqrt(A0 * A0 + A1 * A1 + A2 * A2);
                 real h = a.h_ip[cell];
