@@ -76,6 +76,8 @@ SIGNATURES = {
     "afr_residual_norm_f64": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_vp, c_vp]),
+    "afr_ssp_rk3_step_dt": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int32, c_vp, c_vp]),
     "afr_wall_distance": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_vp]),
     "afr_sst_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_vp]),
     "afr_sst_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,

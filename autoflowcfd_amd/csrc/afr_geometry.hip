@@ -293,11 +293,12 @@ extern "C" int afr_metrics(const AfrGeomDesc* d, void* stream) {
         afr_set_error("afr_metrics: bad descriptor");
         return -1;
     }
-    if (d->order < 1 || d->order > 2 || d->n_cells <= 0 || d->n_prism < 0 || d->n_prism > d->n_cells) {
+    if (d->order < 1 || d->order > 3 || d->n_cells <= 0 || d->n_prism < 0 || d->n_prism > d->n_cells) {
         afr_set_error("afr_metrics: bad order or cell counts");
         return -1;
     }
     hipStream_t st = (hipStream_t)stream;
-    if (d->dtype == AFR_F64) return d->order == 1 ? launch<1, double>(d, st) : launch<2, double>(d, st);
-    return d->order == 1 ? launch<1, float>(d, st) : launch<2, float>(d, st);
+    if (d->dtype == AFR_F64)
+        return d->order == 1 ? launch<1, double>(d, st) : d->order == 2 ? launch<2, double>(d, st) : launch<3, double>(d, st);
+    return d->order == 1 ? launch<1, float>(d, st) : d->order == 2 ? launch<2, float>(d, st) : launch<3, float>(d, st);
 }

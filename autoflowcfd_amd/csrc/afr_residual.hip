@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <vector>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -155,9 +156,15 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
     using O = Ord<P>;
     constexpr int NS = O::NS, NF = O::NF, NFP = O::NFP, CPB = O::CPB, N1 = O::N1;
     constexpr int NSLOT = Cell<CT>::NSLOT;
-    __shared__ real s_c2f[NF * NS];
-    __shared__ real s_op[NS * 3 * NF];
-    __shared__ real s_E[NSLOT * NFP * NS];
+    // P3 operator tables (64 x 64 and larger) do not fit LDS beside the tiles: they are read
+    // from global memory (L1/L2 resident) instead
+    constexpr bool OPS_LDS = P <= 2;
+    __shared__ real s_c2f_l[OPS_LDS ? NF * NS : 1];
+    __shared__ real s_op_l[OPS_LDS ? NS * 3 * NF : 1];
+    __shared__ real s_E_l[OPS_LDS ? NSLOT * NFP * NS : 1];
+    const real* const s_c2f = OPS_LDS ? s_c2f_l : a.c2f;
+    const real* const s_op = OPS_LDS ? s_op_l : a.op;
+    const real* const s_E = OPS_LDS ? s_E_l : a.E;
     __shared__ real s_Q[CPB * NS * 5];
     __shared__ real s_Ft[CPB * 3 * NF * 5];
     __shared__ real s_J[CPB * NSLOT * NFP * 5];
@@ -167,9 +174,11 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
     __shared__ int s_co[3 * NS];
 
     const int t = threadIdx.x;
-    for (int i = t; i < NF * NS; i += kThreads) s_c2f[i] = a.c2f[i];
-    for (int i = t; i < NS * 3 * NF; i += kThreads) s_op[i] = a.op[i];
-    for (int i = t; i < NSLOT * NFP * NS; i += kThreads) s_E[i] = a.E[i];
+    if constexpr (OPS_LDS) {
+        for (int i = t; i < NF * NS; i += kThreads) s_c2f_l[i] = a.c2f[i];
+        for (int i = t; i < NS * 3 * NF; i += kThreads) s_op_l[i] = a.op[i];
+        for (int i = t; i < NSLOT * NFP * NS; i += kThreads) s_E_l[i] = a.E[i];
+    }
     for (int i = t; i < 3 * NS; i += kThreads) {
         s_fp[i] = a.fp_of_sp[i];
         s_co[i] = a.coord_of_sp[i];
@@ -348,7 +357,9 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         for (int i = 0; i < 3; ++i)
             for (int v = 0; v < 5; ++v) fref[i * 5 + v] = F[i][v];
     }
-    if (d.n_prism > 0) {
+    if constexpr (P >= 3) {
+        if (d.n_prism > 0) return fail("order 3 is supported on tet meshes only");
+    } else if (d.n_prism > 0) {
         // prisms: Kronecker-factored wave-per-cell kernel
         afr_prism::PrismArgs<real> p;
         p.n_cells = d.n_prism;
@@ -452,7 +463,11 @@ int afr_last_error(char* buf, size_t n) {
 int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
     if (!desc || !out) return fail("afr_mesh_create: null argument");
     if (desc->abi_version != AFR_ABI_VERSION) return fail("afr_mesh_create: ABI version mismatch");
-    if (desc->order < 1 || desc->order > 2) return fail("afr_mesh_create: order must be 1 or 2");
+    if (desc->order < 1 || desc->order > 3) return fail("afr_mesh_create: order must be 1, 2 or 3");
+    if (desc->order == 3 && desc->n_prism > 0) return fail("afr_mesh_create: order 3 is supported on tet meshes only");
+    // the collapsed tet operators reach 6e7 at P3: FP32 rounding of the state alone is amplified
+    // into O(1e-2) relative residual errors, so P3 runs in FP64 storage and arithmetic
+    if (desc->order == 3 && desc->dtype != AFR_F64) return fail("afr_mesh_create: order 3 requires FP64 (dtype 1)");
     if (desc->dtype != AFR_F32 && desc->dtype != AFR_F64) return fail("afr_mesh_create: bad dtype");
     if (desc->n_cells <= 0 || desc->n_prism < 0 || desc->n_prism > desc->n_cells)
         return fail("afr_mesh_create: bad cell counts");
@@ -518,14 +533,16 @@ int afr_prepare_state(const AfrMesh* mesh, const void* U, void* dQ, void* umean,
         auto* u = (const float*)U;
         auto* q = (float*)dQ;
         auto* m = (float*)umean;
-        rc = ns == 8 ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, st)
-                     : afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, st)
+                        : afr_prism::launch_prim_umean<64, float>(u, q, m, d.n_cells, d.ref_state, st);
     } else {
         auto* u = (const double*)U;
         auto* q = (double*)dQ;
         auto* m = (double*)umean;
-        rc = ns == 8 ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, st)
-                     : afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, st)
+                        : afr_prism::launch_prim_umean<64, double>(u, q, m, d.n_cells, d.ref_state, st);
     }
     return rc ? fail("k_prim_umean launch failed") : 0;
 }
@@ -538,13 +555,15 @@ int afr_inviscid_from_primitives(const AfrMesh* mesh, const void* dQ, const void
     if (d.dtype == AFR_F32) {
         auto* q = (const float*)dQ;
         auto* m = (const float*)umean;
-        return d.order == 1 ? launch_inviscid<1, float>(d, q, m, (float*)R, suppress, st)
-                            : launch_inviscid<2, float>(d, q, m, (float*)R, suppress, st);
+        return d.order == 1   ? launch_inviscid<1, float>(d, q, m, (float*)R, suppress, st)
+               : d.order == 2 ? launch_inviscid<2, float>(d, q, m, (float*)R, suppress, st)
+                              : launch_inviscid<3, float>(d, q, m, (float*)R, suppress, st);
     }
     auto* q = (const double*)dQ;
     auto* m = (const double*)umean;
-    return d.order == 1 ? launch_inviscid<1, double>(d, q, m, (double*)R, suppress, st)
-                        : launch_inviscid<2, double>(d, q, m, (double*)R, suppress, st);
+    return d.order == 1   ? launch_inviscid<1, double>(d, q, m, (double*)R, suppress, st)
+           : d.order == 2 ? launch_inviscid<2, double>(d, q, m, (double*)R, suppress, st)
+                          : launch_inviscid<3, double>(d, q, m, (double*)R, suppress, st);
 }
 
 int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t suppress, void* stream) {
@@ -608,7 +627,9 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.suppress = suppress;
         a.accumulate = accumulate;
         int rc;
-        if (d.order == 1)
+        if (d.order == 3)
+            rc = ct == 0 ? -1 : afr_visc::launch_viscous_phase<3, 1, real>(a, phase, st);
+        else if (d.order == 1)
             rc = ct == 0 ? afr_visc::launch_viscous_phase<1, 0, real>(a, phase, st)
                          : afr_visc::launch_viscous_phase<1, 1, real>(a, phase, st);
         else
@@ -795,6 +816,19 @@ int afr_ssp_rk3_step(const AfrMesh* mesh, void* U, const void* mu_t, double mu, 
     return rk3_stages(mesh, U, mu_t, mu, Pr, Pr_t, n_vars_total, norm, stream);
 }
 
+int afr_ssp_rk3_step_dt(const AfrMesh* mesh, void* U, const void* mu_t, double dt, double mu, double Pr, double Pr_t,
+                        int32_t n_vars_total, double* norm, void* stream) {
+    if (!mesh || !U || !norm) return fail("afr_ssp_rk3_step_dt: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (!step_ready(d)) return fail("afr_ssp_rk3_step_dt: mesh has no pseudo-step work buffers");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail("afr_ssp_rk3_step_dt: dt must be positive and finite");
+    const int64_t n = d.n_cells * ns_of(d);
+    const int rc = d.dtype == AFR_F32 ? afr_step::launch_fill<float>((float*)d.work_dt, n, dt, (hipStream_t)stream)
+                                      : afr_step::launch_fill<double>((double*)d.work_dt, n, dt, (hipStream_t)stream);
+    if (rc) return fail("k_fill launch failed");
+    return rk3_stages(mesh, U, mu_t, mu, Pr, Pr_t, n_vars_total, norm, stream);
+}
+
 int afr_sst_rk3_step(const AfrMesh* mesh, void* U, void* k, void* omega, const void* d_wall, void* nu_t, void* mu_t,
                      double mu, double Pr, double Pr_t, int32_t n_vars_total, double* norm, void* stream) {
     if (!mesh || !U || !k || !omega || !d_wall || !nu_t || !mu_t || !norm) return fail("afr_sst_rk3_step: null argument");
@@ -821,6 +855,7 @@ namespace {
 template <typename real>
 int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* dw, const void* dt, void* nut,
             void* mut, void* rates, double mu, hipStream_t st) {
+    if (d.order > 2) return fail("SST is supported at orders 1 and 2");
     afr_sst::SstArgs<real> a;
     a.n_records = d.n_records;
     a.U = (const real*)U;
@@ -907,6 +942,7 @@ int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const
 // [wall_off[i], wall_off[i+1]) into out[6 i .. 6 i + 5]
 static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, bool per_case, double* out,
                        void* stream) {
+    if (mesh->d.order > 2) return fail("afr_forces_f64: wall forces are supported at orders 1 and 2");
     const int64_t* wall_off = per_case ? mesh->case_wall_off.data() : nullptr;
     const int n_cases = mesh->d.n_cases;
     if (per_case && mesh->case_wall_off.empty())

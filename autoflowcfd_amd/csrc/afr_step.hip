@@ -181,10 +181,27 @@ int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const
     else if (ns == 27)
         hipLaunchKernelGGL((k_rk_stage<27, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
                            filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+    else if (ns == 64)
+        hipLaunchKernelGGL((k_rk_stage<64, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
+                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
     else
         return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_fill(real* __restrict__ x, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        x[i] = (real)v;
+}
+
+template <typename real>
+int launch_fill(real* x, int64_t n, double v, hipStream_t st) {
+    hipLaunchKernelGGL((k_fill<real>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, st, x, n, v);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_fill<float>(float*, int64_t, double, hipStream_t);
+template int launch_fill<double>(double*, int64_t, double, hipStream_t);
 
 template <typename real>
 int launch_norm(const real* x, int64_t n, double n_total, double* part, int n_part_max, double* out, hipStream_t st) {

@@ -19,6 +19,10 @@ int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const
 template <typename real>
 int launch_norm(const real* x, int64_t n, double n_total, double* part, int n_part_max, double* out, hipStream_t st);
 
+// x[0, n) = v (a fill kernel, not a memset: safe inside captured graphs)
+template <typename real>
+int launch_fill(real* x, int64_t n, double v, hipStream_t st);
+
 // norm of the logical array x[a0, a0 + n0) ++ x[a1, a1 + n1) (bit-identical to launch_norm of the
 // same values stored contiguously)
 template <typename real>

@@ -129,9 +129,12 @@ __device__ void outlier_zeroing(real* sR, const real* umean, int t) {
 // Row widths of the sparse collapsed-coordinate derivative matrices: d/da couples the
 // N1 points of an a-line, d/db the N1^2 points of an (a,b) plane, d/dc the N1 points of a
 // c-line for the prism and all N1^3 points for the tet.
+// non-zeros per derivative row: prism (N1, N1^2, N1), tet (N1, N1^2, N1^3); P3 tet rows are
+// kept dense (the collapsed-basis round-off entries there are not negligible)
 template <int P, int CT> struct DW {
-    static constexpr int N1 = P + 1;
-    static constexpr int W0 = N1, W1 = N1 * N1, W2 = CT == 0 ? N1 : N1 * N1 * N1;
+    static constexpr int N1 = P + 1, NS = N1 * N1 * N1;
+    static constexpr bool DENSE = CT == 1 && P >= 3;
+    static constexpr int W0 = DENSE ? NS : N1, W1 = DENSE ? NS : N1 * N1, W2 = CT == 0 ? N1 : NS;
     static constexpr int WM = W2 > W1 ? W2 : W1;
 };
 
@@ -140,15 +143,21 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
     using O = VO<P, CT>;
     using W = DW<P, CT>;
     constexpr int NS = O::NS, NSV = O::NSV, WM = W::WM;
-    __shared__ real s_Dw[3 * NS * WM];                       // sparse D: weights [m][s][k]
-    __shared__ int s_Di[3 * NS * WM];                        //           indices
+    // sparse D in LDS unless it is too large (P3 tets: 3 x 64 x 64 entries): then global memory
+    constexpr bool D_LDS = 3 * NS * WM * (sizeof(real) + 4) <= 64 * 1024;
+    __shared__ real s_Dw_l[D_LDS ? 3 * NS * WM : 1];         // sparse D: weights [m][s][k]
+    __shared__ int s_Di_l[D_LDS ? 3 * NS * WM : 1];          //           indices
+    const real* const s_Dw = D_LDS ? s_Dw_l : a.Dsp_w;
+    const int* const s_Di = D_LDS ? s_Di_l : a.Dsp_i;
     __shared__ real s_w[kWaves][NSV + 4 * NS + 12 * NS + 12 * NS];
     __shared__ double s_ref[5];
     const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
-    for (int i = tid; i < 3 * NS * WM; i += 64 * kWaves) {
-        s_Dw[i] = a.Dsp_w[i];
-        s_Di[i] = a.Dsp_i[i];
+    if constexpr (D_LDS) {
+        for (int i = tid; i < 3 * NS * WM; i += 64 * kWaves) {
+            s_Dw_l[i] = a.Dsp_w[i];
+            s_Di_l[i] = a.Dsp_i[i];
+        }
     }
     if (tid < 5) s_ref[tid] = a.qref[tid];
     __syncthreads();
@@ -338,18 +347,16 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                 const int nbc = a.slot_nb[sidx * 2 + src];
                 if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
                 const int64_t nb = nbc >= 0 ? nbc : 0;
-                int spk[KMAX];
-                real wk[KMAX];
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k) {  // rows are KMAX wide (zero-padded)
-                    spk[k] = a.rows_idx[(int64_t)row * KMAX + k];
-                    wk[k] = a.rows_w[(int64_t)row * KMAX + k];
-                }
-                // three neighbour points in flight at a time: enough loads to cover the latency
-                // without the register cost (and lost occupancy) of all KMAX at once
+                // rows are KMAX wide (zero-padded); three neighbour points in flight at a time:
+                // enough loads to cover the latency without the register cost (and lost
+                // occupancy) of all KMAX at once
+                const int* const ri = a.rows_idx + (int64_t)row * KMAX;
+                const real* const rwp = a.rows_w + (int64_t)row * KMAX;
 #pragma unroll 3
                 for (int k = 0; k < KMAX; ++k) {
-                    const float4* x = reinterpret_cast<const float4*>(a.vpack + (nb * NS + spk[k]) * kVP);
+                    const int spk = ri[k];
+                    const real wk = rwp[k];
+                    const float4* x = reinterpret_cast<const float4*>(a.vpack + (nb * NS + spk) * kVP);
                     real xv[20];
                     if constexpr (sizeof(real) == 4) {
 #pragma unroll
@@ -361,12 +368,12 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                             xv[4 * c + 3] = f.w;
                         }
                     } else {
-                        const real* xr = a.vpack + (nb * NS + spk[k]) * kVP;
+                        const real* xr = a.vpack + (nb * NS + spk) * kVP;
 #pragma unroll
                         for (int q = 0; q < NP; ++q) xv[q] = xr[q];
                     }
 #pragma unroll
-                    for (int q = 0; q < NP; ++q) pn[q] += wk[k] * xv[q];
+                    for (int q = 0; q < NP; ++q) pn[q] += wk * xv[q];
                 }
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
@@ -472,5 +479,7 @@ AFR_VISC_INST(1, 0, double)
 AFR_VISC_INST(2, 0, double)
 AFR_VISC_INST(1, 1, double)
 AFR_VISC_INST(2, 1, double)
+AFR_VISC_INST(3, 1, float)
+AFR_VISC_INST(3, 1, double)
 
 }  // namespace afr_visc

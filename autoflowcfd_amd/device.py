@@ -50,6 +50,10 @@ def sparse_fixed(rows, widths, drop=ROW_DROP):
     rows = np.asarray(rows, dtype=np.float64)
     lead, n = rows.shape[:-1], rows.shape[-1]
     W = int(np.max(widths))
+    if W >= n and np.all(np.broadcast_to(widths, lead) >= n):
+        # full width: the rows stay dense (nothing dropped)
+        idx = np.broadcast_to(np.arange(n, dtype=np.int32), rows.shape).copy()
+        return rows.copy(), idx
     keep = np.abs(rows) > drop * max(np.abs(rows).max(), 1.0)
     lost = np.abs(np.where(keep, 0.0, rows)).max() if rows.size else 0.0
     if lost > 1e-9:
@@ -169,6 +173,8 @@ class DeviceMesh:
         self.dsp_w, self.dsp_i, self.esp_w, self.esp_i = [], [], [], []
         for ct in (0, 1):
             wd = np.array([n1, n1 * n1, n1 if ct == 0 else n1 ** 3])
+            if ct == 1 and order >= 3:
+                wd = np.array([T.ns, T.ns, T.ns])   # P3 tet rows stay dense (DW<P,CT> in the kernel)
             Dm = np.transpose(T.D[ct], (2, 0, 1))  # (3, ns, ns)
             w_, i_ = sparse_fixed(Dm, wd[:, None])
             self.dsp_w.append(dev(w_))
@@ -675,6 +681,18 @@ class DeviceMesh:
                                                   ctypes.c_void_p(out.data_ptr()), self._stream()),
                    "afr_residual_norm_f64")
         return out
+
+    def ssp_rk3_step_dt(self, U, dt, mu_t=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, n_vars_total=5, norm=None):
+        """One SSP-RK3 step of U in place with the same dt at every solution point; returns the
+        (1,) FP64 device norm of R(U)."""
+        self._check_state(U)
+        norm = self.norm_dev if norm is None else norm
+        self._check_norm(norm)
+        _lib.check(self.lib.afr_ssp_rk3_step_dt(self.handle, ctypes.c_void_p(U.data_ptr()), self._check_mut(mu_t),
+                                                float(dt), float(mu), float(Pr), float(Pr_t), int(n_vars_total),
+                                                ctypes.c_void_p(norm.data_ptr()), self._stream()),
+                   "afr_ssp_rk3_step_dt")
+        return norm
 
     def ssp_rk3_step(self, U, mu_t=None, mu=1.8e-5, Pr=0.72, Pr_t=0.9, n_vars_total=5, norm=None):
         """One steady pseudo-step of U in place; returns the (1,) FP64 device tensor of the RMS norm of R(U)."""

@@ -47,13 +47,16 @@ class GPUFRSolver:
                  time_scheme: str = "ssp_rk3", cfl: float = 1.0, rho_inf: float = 1.225, vel_inf: float = 33.33,
                  p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
                  turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
-                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None, slots=None):
+                 bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None, slots=None,
+                 global_time_step: bool = False):
         """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
         compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
         bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
         or {BC_OUTLET: p}; the default is the freestream along +x (boundary.py:52-145).
         k_inf / omega_inf: initial turbulence fields (reference defaults 1e-6 and 1.0, sst.py:47-48).
         slots: precomputed face-slot tables of the mesh at this order (face_slots.slots_for).
+        global_time_step: step(dt) advances every solution point by the given dt (the global-dt
+        runs of tests/validation/test_tgv.py:70-92) instead of the local time step.
         ref_state: constant primitive state of the perturbation form (default: the FARFIELD state);
         a case stepped alone with the ref_state of a batch reproduces its batched trajectory bit
         for bit (batch.CaseBatch).
@@ -94,6 +97,9 @@ class GPUFRSolver:
         self.iteration = 0
         self._graph = None
         self._graph_norm = None
+        self.global_time_step = bool(global_time_step)
+        if self.global_time_step and self.turb_model_name == "SST":
+            raise NotImplementedError("global_time_step is implemented for the laminar path")
 
     # ------------------------------------------------------------------ stepping
     def _launch_step(self):
@@ -127,7 +133,15 @@ class GPUFRSolver:
         return g
 
     def step(self, dt: float = 0.0) -> float:
-        """One steady pseudo-step (dt is ignored: local time stepping); returns the residual RMS."""
+        """One pseudo-step; returns the residual RMS.  dt is used only with global_time_step
+        (every point advances by dt); otherwise the local time step is used."""
+        if self.global_time_step:
+            norm = self.dm.ssp_rk3_step_dt(self.U, dt, None, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t,
+                                           n_vars_total=self.n_vars)
+            res = float(norm.item())
+            self.residual_history.append(res)
+            self.iteration += 1
+            return res
         if self._graph is not None:
             self._graph.replay()
             norm = self._graph_norm

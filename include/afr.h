@@ -274,6 +274,12 @@ int afr_residual_norm_f64(const AfrMesh *mesh, const void *R, int32_t n_vars_tot
 int afr_ssp_rk3_step(const AfrMesh *mesh, void *U, const void *mu_t, double mu, double Pr, double Pr_t,
                      int32_t n_vars_total, double *norm, void *stream);
 
+/* afr_ssp_rk3_step with one time step dt for every solution point instead of dt_local (the
+ * global-dt stepping of the reference's unsteady validation runs,
+ * tests/validation/test_tgv.py:70-92 with src/autoflowcfd/core/fr_solver/step.py:15-221). */
+int afr_ssp_rk3_step_dt(const AfrMesh *mesh, void *U, const void *mu_t, double dt, double mu, double Pr,
+                        double Pr_t, int32_t n_vars_total, double *norm, void *stream);
+
 /* Distance from each of n_pts FP64 points (n_pts, 3) to the nearest of n_wall FP64
  * wall-node positions (n_wall, 3), written in precision dtype to d_out (n_pts)
  * (src/autoflowcfd/core/fr_solver/turbulence.py:112-121: KD-tree to wall nodes). */
