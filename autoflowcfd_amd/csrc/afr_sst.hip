@@ -125,18 +125,25 @@ template <int P, int CT, typename real>
 __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     constexpr int NS = O::NS;
-    constexpr int NF = 12;  // per-SP: rho, u, v, w, m1, m2, m3, k, w, (unused), Gk, Gw
+    // per-SP record (12): rho, u, v, w, k, omega, Gk, Gw (the face-trace fields, contiguous),
+    // m1, m2, m3 (the momentum gradient fields), pad
+    constexpr int NF = 12, G_ = 16;  // gradient record: grad m (9), grad k (3), grad omega (3), pad
     constexpr int NFP = O::NFP, NSLOT = O::NSLOT;
-    __shared__ real s_D[3 * NS * NS];
+    // D interleaved as [s][j][4] (the three reference directions and a pad): one 16-byte read
+    // per (s, j) instead of three
+    __shared__ __attribute__((aligned(16))) real s_D[NS * NS * 4];
     __shared__ real s_E[5 * NFP * NS];
-    __shared__ real s_w[kWaves][NS * NF + NS * 15 + 3 * NS * 4];
-    for (int i = threadIdx.x; i < 3 * NS * NS; i += 64 * kWaves) s_D[i] = a.D[i];
+    __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * NF + NS * G_ + NS * 8];
+    for (int i = threadIdx.x; i < NS * NS * 4; i += 64 * kWaves) {
+        const int m = i & 3, sj = i >> 2;
+        s_D[i] = m < 3 ? a.D[m * NS * NS + sj] : real(0);
+    }
     for (int i = threadIdx.x; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = (CT == 0 ? a.E0 : a.E1)[i];
     __syncthreads();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
     real* sv = s_w[wid];          // [s][NF]
-    real* sg = sv + NS * NF;      // [s][15]: grad m (9), grad k (3), grad w (3)
-    real* sF = sg + NS * 15;      // [m][s][4]: contravariant fluxes rho u k, rho u w, Gk grad k, Gw grad w
+    real* sg = sv + NS * NF;      // [s][G_]
+    real* sF = sg + NS * G_;      // [j][f][4]: contravariant convective + diffusive flux of k (f=0), omega (f=1), m = 0..2
     const double mu = a.mu_molecular;
     for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
         const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
@@ -146,40 +153,44 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) u5[v] = a.U[i * 5 + v];
             const real rho = u5[0] > real(1e-10) ? u5[0] : real(1e-10);
-            sv[t * NF + 0] = rho;
-            sv[t * NF + 1] = u5[1] / rho;
-            sv[t * NF + 2] = u5[2] / rho;
-            sv[t * NF + 3] = u5[3] / rho;
-            sv[t * NF + 4] = u5[1];
-            sv[t * NF + 5] = u5[2];
-            sv[t * NF + 6] = u5[3];
-            sv[t * NF + 7] = a.k[i];
-            sv[t * NF + 8] = a.w[i];
+            real* r = sv + t * NF;
+            r[0] = rho;
+            r[1] = u5[1] / rho;
+            r[2] = u5[2] / rho;
+            r[3] = u5[3] / rho;
+            r[4] = a.k[i];
+            r[5] = a.w[i];
+            r[8] = u5[1];
+            r[9] = u5[2];
+            r[10] = u5[3];
         }
         wave_sync();
         // gradients of m1, m2, m3, k, omega: lane (s, field), perturbations about SP 0
         for (int o = t; o < 5 * NS; o += 64) {
             const int s = o / 5, f = o % 5;
-            const int src = f < 3 ? 4 + f : 7 + (f - 3);
+            const int src = f < 3 ? 8 + f : 4 + (f - 3);
             const real x0 = sv[src];
             real d0 = 0, d1 = 0, d2 = 0;
+            const real* Dr = s_D + s * NS * 4;
             for (int j = 0; j < NS; ++j) {
                 const real x = sv[j * NF + src] - x0;
-                d0 += s_D[(0 * NS + s) * NS + j] * x;
-                d1 += s_D[(1 * NS + s) * NS + j] * x;
-                d2 += s_D[(2 * NS + s) * NS + j] * x;
+                d0 += Dr[j * 4 + 0] * x;
+                d1 += Dr[j * 4 + 1] * x;
+                d2 += Dr[j * 4 + 2] * x;
             }
             const real* ad = a.adj + (cell * NS + s) * 9;
             const real idet = real(1) / a.det[cell * NS + s];
             const int base = f < 3 ? f * 3 : 9 + (f - 3) * 3;
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sg[s * 15 + base + n] = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
+            for (int n = 0; n < 3; ++n) sg[s * G_ + base + n] = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
         }
         wave_sync();
         if (t < NS) {
             const int s = t;
             const int64_t i = cell * NS + s;
-            const double rho = (double)sv[s * NF + 0], k = (double)sv[s * NF + 7], w = (double)sv[s * NF + 8];
+            const real* r = sv + s * NF;
+            const real* gs = sg + s * G_;
+            const double rho = (double)r[0], k = (double)r[4], w = (double)r[5];
             const double d = (double)a.d_wall[i];
             // strain rate of the momentum gradient
             double S2 = 0.0;
@@ -187,7 +198,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             for (int p = 0; p < 3; ++p)
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double sij = 0.5 * ((double)sg[s * 15 + p * 3 + q] + (double)sg[s * 15 + q * 3 + p]);
+                    const double sij = 0.5 * ((double)gs[p * 3 + q] + (double)gs[q * 3 + p]);
                     S2 += sij * sij;
                 }
             const double S = sqrt(2.0 * S2);
@@ -195,8 +206,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             double gk[3], gw[3];
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                gk[n] = (double)sg[s * 15 + 9 + n];
-                gw[n] = (double)sg[s * 15 + 12 + n];
+                gk[n] = (double)gs[9 + n];
+                gw[n] = (double)gs[12 + n];
             }
             const double mk = sqrt(gk[0] * gk[0] + gk[1] * gk[1] + gk[2] * gk[2]);
             const double mw = sqrt(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
@@ -245,26 +256,27 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
             const double rnut = rho * nut;
             const double Gk = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
             const double Gw = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;
-            sv[s * NF + 10] = (real)Gk;
-            sv[s * NF + 11] = (real)Gw;
+            real* rw = sv + s * NF;
+            rw[6] = (real)Gk;
+            rw[7] = (real)Gw;
             real* nbp = a.nbp + i * 4;
             nbp[0] = (real)k;
             nbp[1] = (real)w;
             nbp[2] = (real)Gk;
             nbp[3] = (real)Gw;
-            // contravariant transport fluxes (raw gradients in the diffusion)
+            // contravariant transport fluxes (raw gradients in the diffusion), convective +
+            // diffusive summed per field: sF[s][f][m]
             const real* ad = a.adj + i * 9;
-            const real r = sv[s * NF + 0];
-            const real ru[3] = {r * sv[s * NF + 1], r * sv[s * NF + 2], r * sv[s * NF + 3]};
-            const real gkr = (real)Gk, gwr = (real)Gw, kr = sv[s * NF + 7], wr = sv[s * NF + 8];
+            const real rr = r[0];
+            const real ru[3] = {rr * r[1], rr * r[2], rr * r[3]};
+            const real gkr = (real)Gk, gwr = (real)Gw, kr = r[4], wr = r[5];
+            real* fo = sF + s * 8;
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const real am[3] = {ad[m * 3 + 0], ad[m * 3 + 1], ad[m * 3 + 2]};
                 const real fr = am[0] * ru[0] + am[1] * ru[1] + am[2] * ru[2];
-                sF[(m * NS + s) * 4 + 0] = fr * kr;
-                sF[(m * NS + s) * 4 + 1] = fr * wr;
-                sF[(m * NS + s) * 4 + 2] = gkr * (am[0] * sg[s * 15 + 9] + am[1] * sg[s * 15 + 10] + am[2] * sg[s * 15 + 11]);
-                sF[(m * NS + s) * 4 + 3] = gwr * (am[0] * sg[s * 15 + 12] + am[1] * sg[s * 15 + 13] + am[2] * sg[s * 15 + 14]);
+                fo[0 * 4 + m] = fr * kr + gkr * (am[0] * gs[9] + am[1] * gs[10] + am[2] * gs[11]);
+                fo[1 * 4 + m] = fr * wr + gwr * (am[0] * gs[12] + am[1] * gs[13] + am[2] * gs[14]);
             }
         }
         wave_sync();
@@ -272,32 +284,31 @@ __global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
         if (t < NSLOT * NFP) {
             const int slot = t / NFP, fp = t % NFP;
             const real* e = s_E + (slot * NFP + fp) * NS;
-            constexpr int SRC[kPack] = {0, 1, 2, 3, 7, 8, 10, 11};
             real x0[kPack], tr[kPack];
 #pragma unroll
             for (int q = 0; q < kPack; ++q) {
-                x0[q] = sv[SRC[q]];
+                x0[q] = sv[q];
                 tr[q] = 0;
             }
             for (int j = 0; j < NS; ++j) {
                 const real ej = e[j];
                 const real* x = sv + j * NF;
 #pragma unroll
-                for (int q = 0; q < kPack; ++q) tr[q] += ej * (x[SRC[q]] - x0[q]);
+                for (int q = 0; q < kPack; ++q) tr[q] += ej * (x[q] - x0[q]);
             }
             real* o = a.pack + ((cell * 5 + slot) * NFP + fp) * kPack;
 #pragma unroll
             for (int q = 0; q < kPack; ++q) o[q] = x0[q] + tr[q];
         }
-        // volume terms: -(D.F_conv + D.F_diff)/det for k and omega, perturbations about SP 0
+        // volume terms: -(D.F)/det for k and omega, perturbations about SP 0
         for (int o = t; o < 2 * NS; o += 64) {
             const int s = o >> 1, f = o & 1;
+            const real* Dr = s_D + s * NS * 4;
+            const real f00 = sF[f * 4 + 0], f01 = sF[f * 4 + 1], f02 = sF[f * 4 + 2];
             real acc = 0;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const real f0 = sF[(m * NS) * 4 + f] + sF[(m * NS) * 4 + 2 + f];
-                for (int j = 0; j < NS; ++j)
-                    acc += s_D[(m * NS + s) * NS + j] * (sF[(m * NS + j) * 4 + f] + sF[(m * NS + j) * 4 + 2 + f] - f0);
+            for (int j = 0; j < NS; ++j) {
+                const real* fj = sF + j * 8 + f * 4;
+                acc += Dr[j * 4 + 0] * (fj[0] - f00) + Dr[j * 4 + 1] * (fj[1] - f01) + Dr[j * 4 + 2] * (fj[2] - f02);
             }
             a.vol[(cell * NS + s) * 2 + f] = -acc / a.det[cell * NS + s];
         }
