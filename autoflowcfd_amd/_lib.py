@@ -92,13 +92,15 @@ _lib = None
 
 
 def load():
-    """Load libafr.so; raises if it has not been built (there is no fallback path)."""
+    """Load libafr.so (or the build named by AFR_LIBRARY); raises if it has not been built
+    (there is no fallback path)."""
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m autoflowcfd_amd.build`")
-    lib = ctypes.CDLL(LIB_PATH)
+    path = os.environ.get("AFR_LIBRARY", LIB_PATH)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m autoflowcfd_amd.build`")
+    lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

@@ -21,6 +21,15 @@
 
 using namespace afr;
 
+// unroll factors of the interface kernel's own-trace and neighbour gathers (register budget
+// against loads in flight)
+#ifndef AFR_VISC_TRACE_UNROLL
+#define AFR_VISC_TRACE_UNROLL 3
+#endif
+#ifndef AFR_VISC_GATHER_UNROLL
+#define AFR_VISC_GATHER_UNROLL 3
+#endif
+
 namespace afr_visc {
 
 constexpr int kWaves = 4;
@@ -328,7 +337,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
             real po[NP], pn[NP];
 #pragma unroll
             for (int k = 0; k < NP; ++k) po[k] = real(0);
-#pragma unroll 3
+#pragma unroll AFR_VISC_TRACE_UNROLL
             for (int k = 0; k < KE; ++k) {
                 const int e = (slot * NFP + fp) * KE + k;
                 const real w = s_Ew[e];
@@ -352,7 +361,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
                 // occupancy) of all KMAX at once
                 const int* const ri = a.rows_idx + (int64_t)row * KMAX;
                 const real* const rwp = a.rows_w + (int64_t)row * KMAX;
-#pragma unroll 3
+#pragma unroll AFR_VISC_GATHER_UNROLL
                 for (int k = 0; k < KMAX; ++k) {
                     const int spk = ri[k];
                     const real wk = rwp[k];
