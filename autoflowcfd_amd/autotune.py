@@ -1,0 +1,103 @@
+"""Autotuner of the persistent grid sizes of the residual kernels on one mesh.
+
+Every residual kernel walks its cells with a persistent grid; how many workgroups that grid
+has trades latency hiding against tail effects and L2 sharing, and the best value depends on
+the kernel, the order, the precision and the mesh size.  ``autotune`` times each kernel's phase
+with HIP events on the launch stream for a set of candidate grid sizes (results do not depend on
+the grid: only the time does) and leaves the fastest one set on the mesh
+(``afr_mesh_set_grid``).  Choices are cached per (device, order, dtype, kernel, size bucket) in
+memory and, when ``cache_path`` is given, in a JSON file.
+"""
+
+import json
+import math
+import os
+
+import torch
+
+from . import _lib
+
+KERNELS = {"prism_inviscid": 0, "tet_inviscid": 1, "visc_volume": 2, "visc_iface": 3, "sst_cell": 4,
+           "sst_face": 5, "sst_update": 6}
+CANDIDATES = (0, 256, 512, 1024, 2048, 4096, 8192, 16384)   # workgroups; 0 = built-in default
+
+_memo = {}
+
+
+def _key(dm, name):
+    bucket = int(round(math.log2(max(dm.n_cells, 1))))
+    dev = torch.cuda.get_device_name(dm.device)
+    return f"{dev}|P{dm.order}|{str(dm.dtype).split('.')[-1]}|{name}|2^{bucket}"
+
+
+def set_grid(dm, name, cap):
+    _lib.check(dm.lib.afr_mesh_set_grid(dm.handle, KERNELS[name], int(cap)), "afr_mesh_set_grid")
+
+
+def get_grid(dm, name):
+    return int(dm.lib.afr_mesh_get_grid(dm.handle, KERNELS[name]))
+
+
+def _time(fn, reps):
+    st = torch.cuda.current_stream()
+    fn()                                         # warm-up at this setting
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(st)
+    for _ in range(reps):
+        fn()
+    ev[1].record(st)
+    ev[1].synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CANDIDATES, cache_path=None):
+    """Pick the grid size of every residual kernel present on this mesh.
+
+    U: the state to time on (device, resident); mu_t: eddy viscosity or None; sst_fields:
+    (k, omega, nu_t, mu_t) to tune the SST kernels (None skips them).  Returns
+    {kernel: {"grid": chosen, "ms": {candidate: time}}}; the choice stays set on the mesh."""
+    cache = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+    R = torch.empty_like(U)
+    phases = {}
+    if dm.mesh.n_prism:
+        phases["prism_inviscid"] = lambda: dm.inviscid_from_primitives(dm.work_q, dm.work_umean, R)
+    if dm.mesh.n_tet:
+        phases["tet_inviscid"] = lambda: dm.inviscid_from_primitives(dm.work_q, dm.work_umean, R)
+    phases["visc_volume"] = lambda: dm.viscous_volume(U, mu_t, mu=mu)
+    phases["visc_iface"] = lambda: dm.viscous_interface(U, mu_t, R, mu=mu)
+    if sst_fields is not None and dm.turbulence == "SST":
+        k, w, nut, mut = sst_fields
+        rates = torch.empty((dm.n_cells, dm.ns, 2), dtype=dm.dtype, device=dm.device)
+        dt = dm.work_dt
+        dm.local_time_step_nut(U, nut, dt, mu=mu)
+        sst = lambda: dm.sst_step(U, k, w, dt, nut, mut, mu=mu, rates=rates)  # noqa: E731
+        for name in ("sst_cell", "sst_face", "sst_update"):
+            phases[name] = sst                   # the SST call times its three kernels; one varies
+    dm.viscous_volume(U, mu_t, mu=mu)            # dQ / field means for the inviscid phase
+    out = {}
+    for name, fn in phases.items():
+        key = _key(dm, name)
+        if key in _memo or key in cache:
+            cap = _memo.get(key, cache.get(key))
+            set_grid(dm, name, cap)
+            out[name] = {"grid": cap, "ms": None, "cached": True}
+            continue
+        times = {}
+        for cap in candidates:
+            set_grid(dm, name, cap)
+            times[cap] = _time(fn, reps)
+        best = min(times, key=times.get)
+        set_grid(dm, name, best)
+        _memo[key] = best
+        cache[key] = best
+        out[name] = {"grid": best, "ms": times}
+    dm.viscous_volume(U, mu_t, mu=mu)
+    torch.cuda.synchronize(dm.device)
+    if cache_path:
+        os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1)
+    return out

@@ -623,7 +623,8 @@ template <int P, typename real>
 int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
     // persistent grid: a multiple of the 8 XCDs, up to 4 workgroups per CU of 256 CUs
     const int64_t need = (a.n_cells + kWaves - 1) / kWaves;
-    int64_t g = need < 1024 ? need : 1024;
+    const int64_t cap = a.grid_cap > 0 ? a.grid_cap : 1024;
+    int64_t g = need < cap ? need : cap;
     g = g >= 8 ? g / 8 * 8 : g;
     const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64 * kWaves);
     // interpolation rows: Kronecker-sparse (prism sources: P2 <= 9, P1 <= 4 non-zeros) or dense (tet sources)

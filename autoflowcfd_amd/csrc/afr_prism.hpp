@@ -37,6 +37,7 @@ template <typename real>
 struct PrismArgs {
     int64_t n_cells;  // prisms occupy cells [0, n_cells)
     const int* schedule;  // (n_cells) processing order of the prisms (locality order)
+    int grid_cap = 0;     // persistent grid size (workgroups); 0: built-in default
     const real* dQ;
     const real* umean;
     real* R;

@@ -342,8 +342,9 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
     }
 }
 
-int grid_for(int64_t tiles) {
-    int64_t g = tiles < 4096 ? tiles : 4096;
+int grid_for(int64_t tiles, int cap = 0) {
+    const int64_t c = cap > 0 ? cap : 4096;
+    int64_t g = tiles < c ? tiles : c;
     return (int)(g > 0 ? g : 1);
 }
 
@@ -364,6 +365,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         afr_prism::PrismArgs<real> p;
         p.n_cells = d.n_prism;
         p.schedule = d.prism_schedule;
+        p.grid_cap = d.grid_cap[AFR_K_PRISM_INVISCID];
         p.dQ = Q;
         p.umean = umean;
         p.R = R;
@@ -429,7 +431,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         for (int k = 0; k < 15; ++k) a.fref[k] = fref[k];
         a.suppress = suppress;
         a.n_rows = d.n_rows;
-        const int grid = grid_for((n + O::CPB - 1) / O::CPB);
+        const int grid = grid_for((n + O::CPB - 1) / O::CPB, d.grid_cap[AFR_K_TET_INVISCID]);
         hipLaunchKernelGGL((k_inviscid<P, 1, real>), dim3(grid), dim3(kThreads), 0, st, a);
         HIP_TRY(hipGetLastError());
     }
@@ -507,8 +509,23 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
     m->case_wall_off = std::move(wo);
     // the host arrays are the caller's: only the copies are kept
     m->d.case_prism_off = m->d.case_tet_off = m->d.case_wall_off = nullptr;
+    for (int k = 0; k < 8; ++k) m->d.grid_cap[k] = 0;
     *out = m;
     return 0;
+}
+
+int afr_mesh_set_grid(AfrMesh* mesh, int32_t kernel, int32_t grid_cap) {
+    if (!mesh) return fail("afr_mesh_set_grid: null mesh");
+    if (kernel < 0 || kernel > AFR_K_SST_UPDATE) return fail("afr_mesh_set_grid: unknown kernel id");
+    if (grid_cap != 0 && (grid_cap < 8 || grid_cap > 65536))
+        return fail("afr_mesh_set_grid: grid_cap must be 0 or in [8, 65536]");
+    mesh->d.grid_cap[kernel] = grid_cap / 8 * 8;
+    return 0;
+}
+
+int afr_mesh_get_grid(const AfrMesh* mesh, int32_t kernel) {
+    if (!mesh || kernel < 0 || kernel > AFR_K_SST_UPDATE) return fail("afr_mesh_get_grid: bad argument");
+    return mesh->d.grid_cap[kernel];
 }
 
 int afr_mesh_destroy(AfrMesh* mesh) {
@@ -600,6 +617,7 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         // the interface kernel reads neighbour data: locality order; the volume kernel only
         // its own cell: storage order (contiguous reads)
         a.schedule = (ct == 0 && phase != 0) ? d.prism_schedule : nullptr;
+        a.grid_cap = d.grid_cap[phase == 0 ? AFR_K_VISC_VOLUME : AFR_K_VISC_IFACE];
         a.Dsp_w = (const real*)d.dsp_w[ct];
         a.Dsp_i = d.dsp_i[ct];
         a.Esp_w = (const real*)d.esp_w[ct];
@@ -893,6 +911,7 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
     a.fp_of_sp = d.fp_of_sp;
     a.coord_of_sp = d.coord_of_sp;
     a.mu_molecular = mu;
+    a.grid_cap_face = d.grid_cap[AFR_K_SST_FACE];
     const int P = d.order;
     for (int phase = 0; phase < 3; ++phase) {
         if (phase == 1) {
@@ -906,6 +925,7 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
             b.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
             b.D = (const real*)d.D[ct];
             b.schedule = ct == 0 ? d.prism_schedule : nullptr;
+            b.grid_cap = d.grid_cap[phase == 0 ? AFR_K_SST_CELL : AFR_K_SST_UPDATE];
             if (b.n_cells <= 0) continue;
             int rc;
             if (phase == 0)

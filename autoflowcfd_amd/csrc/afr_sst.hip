@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(256) k_wall_distance(const double* __restrict_
 template <int P, int CT, typename real>
 int launch_sst_cell(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, 4096)), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)), dim3(64 * kWaves), 0,
                        st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -550,7 +550,8 @@ int launch_sst_face(const SstArgs<real>& a, hipStream_t st) {
     const int64_t n = a.n_records * (P + 1) * (P + 1);
     const int64_t need = (n + 255) / 256;
     constexpr int KS = (P + 1) * (P + 1), NS = KS * (P + 1);
-    const dim3 grid((unsigned)(need < 8192 ? need : 8192));
+    const int64_t cap = a.grid_cap_face > 0 ? a.grid_cap_face : 8192;
+    const dim3 grid((unsigned)(need < cap ? need : cap));
     if (a.knz == KS)
         hipLaunchKernelGGL((k_sst_face<P, KS, real>), grid, dim3(256), 0, st, a);
     else if (a.knz == NS)
@@ -563,7 +564,7 @@ int launch_sst_face(const SstArgs<real>& a, hipStream_t st) {
 template <int P, int CT, typename real>
 int launch_sst_update(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    hipLaunchKernelGGL((k_sst_update<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, 4096)), dim3(64 * kWaves), 0,
+    hipLaunchKernelGGL((k_sst_update<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)), dim3(64 * kWaves), 0,
                        st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

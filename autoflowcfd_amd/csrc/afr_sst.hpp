@@ -48,6 +48,8 @@ struct SstArgs {
     const int* coord_of_sp;
     double mu_molecular;
     const int* schedule;          // null, or the locality order of the cells of this type
+    int grid_cap = 0;     // persistent grid size (workgroups); 0: built-in default
+    int grid_cap_face = 0;  // same for the face-record kernel
 };
 
 template <int P, int CT, typename real>

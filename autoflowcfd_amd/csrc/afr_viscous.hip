@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
 template <int P, int CT, typename real>
 int launch_viscous_phase(const ViscArgs<real>& a, int phase, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    const int grid = walk_grid(a.n_cells, kWaves, 2048);
+    const int grid = walk_grid(a.n_cells, kWaves, a.grid_cap > 0 ? a.grid_cap : 2048);
     if (phase == 0) {
         hipLaunchKernelGGL((k_visc_volume<P, CT, real>), dim3(grid), dim3(64 * kWaves), 0, st, a);
         return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -42,6 +42,7 @@ struct ViscArgs {
     const double* bcp;
     const int* cell_case;          // null: one case
     const int* schedule;           // null, or the locality order of the cells of this type
+    int grid_cap = 0;     // persistent grid size (workgroups); 0: built-in default
     double qref[5];
     double mu, Pr, Prt;
     int suppress;

@@ -207,6 +207,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=6)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in persistent grid sizes")
     ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-max-iter", type=int, default=200)
@@ -251,6 +252,14 @@ def main():
         residual_step()
     torch.cuda.synchronize()
     log("[bench] warm-up done")
+    tuned = None
+    if not args.no_autotune:
+        from autoflowcfd_amd.autotune import autotune
+        tuned = autotune(dm, U, mut, mu=MU, sst_fields=(k, w, nut, mut))
+        log("[bench] autotuned grids: " + ", ".join(f"{n} {v['grid']}" for n, v in tuned.items()))
+        for _ in range(args.warmup):
+            residual_step()
+        torch.cuda.synchronize()
 
     n_ev = 6
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)] for _ in range(args.steps)]
@@ -333,6 +342,7 @@ def main():
                               "algorithmic_bytes": b_res},
         "phase_ms": {n: float(t) for n, t in zip(names, ph)},
         "ms_per_rk3_pseudo_step": ms_rk,
+        "autotuned_grids": None if tuned is None else {n: v["grid"] for n, v in tuned.items()},
     }
     if not args.no_serve:
         del U, R, rates, k, w, nut, mut, dm

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define AFR_ABI_VERSION 3
+#define AFR_ABI_VERSION 4
 
 enum { AFR_F32 = 0, AFR_F64 = 1 };
 
@@ -156,7 +156,21 @@ typedef struct AfrMeshDesc {
     const int32_t *wrec_owner, *wrec_oslot;
     const double *wrec_geom, *w1d;
     double *work_fpart;
+    /* persistent-grid sizes (workgroups) per kernel, indexed by AfrKernel; 0 = built-in default.
+     * Set by afr_mesh_set_grid (the autotuner), not read from the caller's descriptor. */
+    int32_t grid_cap[8];
 } AfrMeshDesc;
+
+/* Kernels whose persistent grid size is tunable (afr_mesh_set_grid). */
+enum AfrKernel {
+    AFR_K_PRISM_INVISCID = 0,
+    AFR_K_TET_INVISCID = 1,
+    AFR_K_VISC_VOLUME = 2,
+    AFR_K_VISC_IFACE = 3,
+    AFR_K_SST_CELL = 4,
+    AFR_K_SST_FACE = 5,
+    AFR_K_SST_UPDATE = 6
+};
 
 typedef struct AfrMesh AfrMesh;
 
@@ -304,6 +318,13 @@ int afr_sst_step(const AfrMesh *mesh, const void *U, void *k, void *omega, const
  * mesh norm[i] per case). */
 int afr_sst_rk3_step(const AfrMesh *mesh, void *U, void *k, void *omega, const void *d_wall, void *nu_t, void *mu_t,
                      double mu, double Pr, double Pr_t, int32_t n_vars_total, double *norm, void *stream);
+
+/* Persistent grid size of one kernel on this mesh: grid_cap workgroups (0 = the built-in
+ * default; otherwise 8 .. 65536, rounded down to a multiple of the 8 XCDs).  Results do not
+ * depend on it; the autotuner (autoflowcfd_amd/autotune.py) picks it by timing.  No
+ * reference counterpart (the reference launches CuPy kernels with fixed grids). */
+int afr_mesh_set_grid(AfrMesh *mesh, int32_t kernel, int32_t grid_cap);
+int afr_mesh_get_grid(const AfrMesh *mesh, int32_t kernel);
 
 /* Per-case RMS norms of a batched mesh (cell_case set): out[i] = sqrt(sum over the cells of
  * case i of R^2 / (cells_i ns n_vars_total)); out: n_cases doubles (device).  Each case is
