@@ -103,7 +103,7 @@ def _coefficients(solver, case):
 
 
 def solve_alone(requests, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, clock=time.perf_counter,
-                use_graph=False):
+                use_graph=False, ref_state=None):
     """The plain path: requests one at a time in arrival order; each step reads the norm back
     (GPUFRSolver.step, eager launches unless use_graph).  Returns results with arrival / start /
     finish times on one clock."""
@@ -113,7 +113,7 @@ def solve_alone(requests, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, clo
         while clock() - t0 < req.arrival:
             time.sleep(min(0.001, max(req.arrival - (clock() - t0), 0.0)))
         start = clock() - t0
-        s = build_solver(req, dtype, turb_model, mu)
+        s = build_solver(req, dtype, turb_model, mu, ref_state=ref_state)
         hist = []
         if use_graph:
             s.capture_graph()

@@ -196,6 +196,43 @@ def served_leg(args):
     }
 
 
+def batched_leg(args):
+    """Batched multi-case form vs the plain one-at-a-time path on the same seeded cases (fixed
+    step count, set-up included in both): solves/h and the largest Cd difference."""
+    import torch
+    from autoflowcfd_amd.batch import CaseBatch, static_ref_state
+    from autoflowcfd_amd.serve import solve_alone, synthetic_requests
+    reqs = synthetic_requests(args.batch_cases, seed=args.serve_seed + 7, hex_range=(2e3, 2e4), rate=1e9,
+                              max_iter=args.batch_steps, rel_tol=0.0)
+    ref = static_ref_state([r.case for r in reqs])
+    n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
+    log(f"[bench] batched leg: {len(reqs)} cases, {n_sp} solution points, {args.batch_steps} pseudo-steps each")
+    solve_alone(reqs[:1], dtype=torch.float32, ref_state=ref)          # warm the code paths
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = solve_alone(reqs, dtype=torch.float32, ref_state=ref)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cd = {}
+    for order in sorted({r.order for r in reqs}):
+        grp = [r for r in reqs if r.order == order]
+        b = CaseBatch([r.case for r in grp], order=order, dtype=torch.float32, turb_model="NONE", ref_state=ref)
+        b.capture_graph()                                   # one eager warm-up step, counted
+        for _ in range(args.batch_steps - 1):
+            b.step(sync=False)
+        torch.cuda.synchronize()
+        for r, c in zip(grp, b.coefficients()):
+            cd[r.rid] = c["Cd"]
+        del b
+    t_batch = time.perf_counter() - t0
+    log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s")
+    dcd = max(abs(cd[r.rid] - r.coefficients["Cd"]) for r in plain)
+    return {"workload": f"{len(reqs)} laminar NS cases, graded wall boxes of 2e3-2e4 hex split into prisms, P1/P2 "
+                        f"(one batch per order), {args.batch_steps} pseudo-steps each, set-up included",
+            "solves_per_hour": len(reqs) / t_batch * 3600.0, "plain_solves_per_hour": len(reqs) / t_plain * 3600.0,
+            "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +251,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
     ap.add_argument("--serve-seed", type=int, default=1)
+    ap.add_argument("--batch-cases", type=int, default=16)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -224,7 +264,10 @@ def main():
     from autoflowcfd_amd.device import DeviceMesh, bc_params_table
 
     if args.serve_only:
-        print(json.dumps(served_leg(args)), flush=True)
+        out = served_leg(args)
+        if not args.no_batch:
+            out["batched"] = batched_leg(args)
+        print(json.dumps(out), flush=True)
         return
     t0 = time.time()
     mesh = wall_box_mesh(args.n)
@@ -348,6 +391,8 @@ def main():
         del U, R, rates, k, w, nut, mut, dm
         torch.cuda.empty_cache()
         line["served"] = served_leg(args)
+    if not args.no_batch:
+        line["batched"] = batched_leg(args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(args.order, n_sample=args.cpu_sample_n)
