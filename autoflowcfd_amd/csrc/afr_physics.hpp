@@ -9,6 +9,37 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Occupancy targets (waves per SIMD) of the heavy kernels, as build parameters: 0 leaves the
+// register allocation to the compiler; n > 0 asks for at least n waves per SIMD (the compiler
+// then budgets registers for it, spilling if it must).  These latency-bound kernels gain more
+// from waves in flight than they lose to spills: measured on the 82^3 P2 SST residual, the
+// interface kernel 4.41 -> 3.80 ms at 7 waves, the SST update 8 waves best (SST 6.93 -> 6.34 ms);
+// the SST cell, inviscid and viscous volume kernels are bounded by LDS or already at 8.
+#ifndef AFR_WPE_SST_CELL
+#define AFR_WPE_SST_CELL 0
+#endif
+#ifndef AFR_WPE_SST_UPDATE
+#define AFR_WPE_SST_UPDATE 8
+#endif
+#ifndef AFR_WPE_VISC_IFACE
+#define AFR_WPE_VISC_IFACE 7
+#endif
+#ifndef AFR_WPE_VISC_VOLUME
+#define AFR_WPE_VISC_VOLUME 0
+#endif
+#ifndef AFR_WPE_PRISM
+#define AFR_WPE_PRISM 0
+#endif
+#define AFR_WPE_ATTR_0
+#define AFR_WPE_ATTR_N(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
+#define AFR_WPE_SEL(n) AFR_WPE_SEL2(n)
+#define AFR_WPE_SEL2(n) AFR_WPE_ATTR_##n
+#define AFR_WPE_ATTR_4 AFR_WPE_ATTR_N(4)
+#define AFR_WPE_ATTR_5 AFR_WPE_ATTR_N(5)
+#define AFR_WPE_ATTR_6 AFR_WPE_ATTR_N(6)
+#define AFR_WPE_ATTR_7 AFR_WPE_ATTR_N(7)
+#define AFR_WPE_ATTR_8 AFR_WPE_ATTR_N(8)
+
 namespace afr {
 
 template <typename real> struct Tiny;

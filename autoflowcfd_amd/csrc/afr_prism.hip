@@ -248,7 +248,7 @@ __device__ __forceinline__ void outlier_zeroing(real* sR, const real* umean, int
 }
 
 template <int P, int KMAX, typename real>
-__global__ void __launch_bounds__(64 * kWaves) k_prism_inviscid(PrismArgs<real> a) {
+__global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_prism_inviscid(PrismArgs<real> a) {
     using L = PL<P, real>;
     using v4 = typename V4T<real>::type;
     constexpr int N1 = L::N1, NAB = L::NAB, NF1 = L::NF1, NABF = L::NABF, NF = L::NF, NS = L::NS;

@@ -122,7 +122,7 @@ template <int P, int CT> struct SO {
 // E x = x_0 + E (x - x_0)), which keeps FP32 contractions as accurate as FP64 ones on FP32
 // data.  The SST source terms are FP64 (the blend functions FP32 on the FP32 path).
 template <int P, int CT, typename real>
-__global__ void __launch_bounds__(64 * kWaves) k_sst_cell(SstArgs<real> a) {
+__global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     constexpr int NS = O::NS;
     // per-SP record (12): rho, u, v, w, k, omega, Gk, Gw (the face-trace fields, contiguous),
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
 
 // ------------------------------------------------------------------ gather, outlier zeroing, update
 template <int P, int CT, typename real>
-__global__ void __launch_bounds__(64 * kWaves) k_sst_update(SstArgs<real> a) {
+__global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k_sst_update(SstArgs<real> a) {
     using O = SO<P, CT>;
     constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP;
     __shared__ double s_t[kWaves][2 * NS];

@@ -148,7 +148,7 @@ template <int P, int CT> struct DW {
 };
 
 template <int P, int CT, typename real>
-__global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
+__global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) k_visc_volume(ViscArgs<real> a) {
     using O = VO<P, CT>;
     using W = DW<P, CT>;
     constexpr int NS = O::NS, NSV = O::NSV, WM = W::WM;
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_visc_volume(ViscArgs<real> a) {
 
 // ------------------------------------------------------------------ interface kernel
 template <int P, int CT, int KMAX, typename real>
-__global__ void __launch_bounds__(64 * kWaves) k_visc_iface(ViscArgs<real> a) {
+__global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k_visc_iface(ViscArgs<real> a) {
     using O = VO<P, CT>;
     constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP, NSLOT = O::NSLOT, NSV = O::NSV;
     constexpr int KE = CT == 0 ? N1 * N1 : NS;  // non-zeros of a face-trace row
