@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
     __shared__ int s_Di_l[D_LDS ? 3 * NS * WM : 1];          //           indices
     const real* const s_Dw = D_LDS ? s_Dw_l : a.Dsp_w;
     const int* const s_Di = D_LDS ? s_Di_l : a.Dsp_i;
-    __shared__ real s_w[kWaves][NSV + 4 * NS + 12 * NS + 12 * NS];
+    __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * kVP + NSV + 4 * NS + 12 * NS + 12 * NS];
     __shared__ double s_ref[5];
     const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
     }
     if (tid < 5) s_ref[tid] = a.qref[tid];
     __syncthreads();
-    real* const sU = s_w[wid];              // |U| [s][5]
+    real* const sPk = s_w[wid];             // interface pack [s][kVP], written out coalesced
+    real* const sU = sPk + NS * kVP;        // |U| [s][5]
     real* const sF = sU + NSV;              // (du, dv, dw, dT) [s][4]
     real* const sG = sF + 4 * NS;           // grad [s][12]
     real* const sGt = sG + 12 * NS;         // G~ [m][s][4] (v = 1..4)
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
 #pragma unroll
             for (int v = 0; v < 5; ++v) u[v] = (double)a.U[(cell * NS + s) * 5 + v];
             cons2prim(u, q);
-            real* vp = a.vpack + (cell * NS + s) * kVP;
+            real* vp = sPk + s * kVP;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const real dq = (real)(q[v] - s_ref[v]);
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
             }
             const real* ad = a.adj + (cell * NS + s) * 9;
             const real idet = recip(a.det[cell * NS + s]);
-            real* vp = a.vpack + (cell * NS + s) * kVP;
+            real* vp = sPk + s * kVP;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const real g = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
@@ -244,12 +245,20 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
         }
         wave_sync();
         t = lane_fresh(t);
+        // the cell's pack, contiguous in HBM: whole-line stores
+        if constexpr (sizeof(real) == 4 && (NS * kVP) % 4 == 0) {
+            float4* dst = reinterpret_cast<float4*>(a.vpack + cell * NS * kVP);
+            const float4* src = reinterpret_cast<const float4*>(sPk);
+            for (int o = t; o < NS * kVP / 4; o += 64) dst[o] = src[o];
+        } else {
+            for (int o = t; o < NS * kVP; o += 64) a.vpack[cell * NS * kVP + o] = sPk[o];
+        }
         // 3. viscous flux at the solution points, contravariant components
         if (t < NS) {
             const int s = t;
             real q[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) q[v] = (real)(s_ref[v] + (double)a.dQ[(cell * NS + s) * 5 + v]);
+            for (int v = 0; v < 5; ++v) q[v] = (real)(s_ref[v] + (double)sPk[s * kVP + v]);
             real gv[3][3], gT[3], G[3][5];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
