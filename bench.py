@@ -245,6 +245,7 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=6)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in persistent grid sizes")
+    ap.add_argument("--grids", default="", help="explicit grid sizes instead of autotuning, e.g. visc_iface=16384,...")
     ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-max-iter", type=int, default=200)
@@ -296,7 +297,15 @@ def main():
     torch.cuda.synchronize()
     log("[bench] warm-up done")
     tuned = None
-    if not args.no_autotune:
+    if args.grids:
+        from autoflowcfd_amd.autotune import set_grid
+        tuned = {}
+        for item in args.grids.split(","):
+            name, cap = item.split("=")
+            set_grid(dm, name.strip(), int(cap))
+            tuned[name.strip()] = {"grid": int(cap)}
+        log("[bench] grids set: " + args.grids)
+    elif not args.no_autotune:
         from autoflowcfd_amd.autotune import autotune
         tuned = autotune(dm, U, mut, mu=MU, sst_fields=(k, w, nut, mut))
         log("[bench] autotuned grids: " + ", ".join(f"{n} {v['grid']}" for n, v in tuned.items()))
