@@ -924,6 +924,10 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
             b.cell_begin = ct == 0 ? 0 : d.n_prism;
             b.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
             b.D = (const real*)d.D[ct];
+            b.Dsp_w = (const real*)d.dsp_w[ct];
+            b.Dsp_i = d.dsp_i[ct];
+            b.Esp_w = (const real*)d.esp_w[ct];
+            b.Esp_i = d.esp_i[ct];
             b.schedule = ct == 0 ? d.prism_schedule : nullptr;
             b.grid_cap = d.grid_cap[phase == 0 ? AFR_K_SST_CELL : AFR_K_SST_UPDATE];
             if (b.n_cells <= 0) continue;

@@ -121,24 +121,39 @@ template <int P, int CT> struct SO {
 // first solution point; D rows sum to zero and E rows to one, so D x = D (x - x_0) and
 // E x = x_0 + E (x - x_0)), which keeps FP32 contractions as accurate as FP64 ones on FP32
 // data.  The SST source terms are FP64 (the blend functions FP32 on the FP32 path).
+// non-zeros per sparse derivative row (prism: N1, N1^2, N1; tet: N1, N1^2, N1^3) and per
+// face-trace row (prism N1^2, tet N1^3): the widths of the mesh's sparse D / E tables
+template <int P, int CT> struct SW {
+    static constexpr int N1 = P + 1, NS = N1 * N1 * N1;
+    static constexpr int W0 = N1, W1 = N1 * N1, W2 = CT == 0 ? N1 : NS;
+    static constexpr int WM = W2 > W1 ? W2 : W1;
+    static constexpr int KE = CT == 0 ? N1 * N1 : NS;
+};
+
 template <int P, int CT, typename real>
 __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
-    constexpr int NS = O::NS;
+    using W = SW<P, CT>;
+    constexpr int NS = O::NS, WM = W::WM, KE = W::KE;
+    constexpr int WD[3] = {W::W0, W::W1, W::W2};
     // per-SP record (12): rho, u, v, w, k, omega, Gk, Gw (the face-trace fields, contiguous),
     // m1, m2, m3 (the momentum gradient fields), pad
     constexpr int NF = 12, G_ = 16;  // gradient record: grad m (9), grad k (3), grad omega (3), pad
     constexpr int NFP = O::NFP, NSLOT = O::NSLOT;
-    // D interleaved as [s][j][4] (the three reference directions and a pad): one 16-byte read
-    // per (s, j) instead of three
-    __shared__ __attribute__((aligned(16))) real s_D[NS * NS * 4];
-    __shared__ real s_E[5 * NFP * NS];
+    // sparse derivative and face-trace rows (the D / E non-zeros only)
+    __shared__ real s_Dw[3 * NS * WM];
+    __shared__ int s_Di[3 * NS * WM];
+    __shared__ real s_Ew[5 * NFP * KE];
+    __shared__ int s_Ei[5 * NFP * KE];
     __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * NF + NS * G_ + NS * 8];
-    for (int i = threadIdx.x; i < NS * NS * 4; i += 64 * kWaves) {
-        const int m = i & 3, sj = i >> 2;
-        s_D[i] = m < 3 ? a.D[m * NS * NS + sj] : real(0);
+    for (int i = threadIdx.x; i < 3 * NS * WM; i += 64 * kWaves) {
+        s_Dw[i] = a.Dsp_w[i];
+        s_Di[i] = a.Dsp_i[i];
     }
-    for (int i = threadIdx.x; i < 5 * NFP * NS; i += 64 * kWaves) s_E[i] = (CT == 0 ? a.E0 : a.E1)[i];
+    for (int i = threadIdx.x; i < 5 * NFP * KE; i += 64 * kWaves) {
+        s_Ew[i] = a.Esp_w[i];
+        s_Ei[i] = a.Esp_i[i];
+    }
     __syncthreads();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
     real* sv = s_w[wid];          // [s][NF]
@@ -165,24 +180,27 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             r[10] = u5[3];
         }
         wave_sync();
-        // gradients of m1, m2, m3, k, omega: lane (s, field), perturbations about SP 0
+        // gradients of m1, m2, m3, k, omega: lane (s, field), sparse rows, perturbations about SP 0
         for (int o = t; o < 5 * NS; o += 64) {
             const int s = o / 5, f = o % 5;
             const int src = f < 3 ? 8 + f : 4 + (f - 3);
             const real x0 = sv[src];
-            real d0 = 0, d1 = 0, d2 = 0;
-            const real* Dr = s_D + s * NS * 4;
-            for (int j = 0; j < NS; ++j) {
-                const real x = sv[j * NF + src] - x0;
-                d0 += Dr[j * 4 + 0] * x;
-                d1 += Dr[j * 4 + 1] * x;
-                d2 += Dr[j * 4 + 2] * x;
+            real d[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                real acc = 0;
+#pragma unroll
+                for (int k = 0; k < WD[m]; ++k) {
+                    const int idx = (m * NS + s) * WM + k;
+                    acc += s_Dw[idx] * (sv[s_Di[idx] * NF + src] - x0);
+                }
+                d[m] = acc;
             }
             const real* ad = a.adj + (cell * NS + s) * 9;
             const real idet = real(1) / a.det[cell * NS + s];
             const int base = f < 3 ? f * 3 : 9 + (f - 3) * 3;
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sg[s * G_ + base + n] = (ad[0 * 3 + n] * d0 + ad[1 * 3 + n] * d1 + ad[2 * 3 + n] * d2) * idet;
+            for (int n = 0; n < 3; ++n) sg[s * G_ + base + n] = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
         }
         wave_sync();
         if (t < NS) {
@@ -283,16 +301,17 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
         // owner-side face traces of (rho, u, v, w, k, omega, Gamma_k, Gamma_omega): lane (slot, flux point)
         if (t < NSLOT * NFP) {
             const int slot = t / NFP, fp = t % NFP;
-            const real* e = s_E + (slot * NFP + fp) * NS;
             real x0[kPack], tr[kPack];
 #pragma unroll
             for (int q = 0; q < kPack; ++q) {
                 x0[q] = sv[q];
                 tr[q] = 0;
             }
-            for (int j = 0; j < NS; ++j) {
-                const real ej = e[j];
-                const real* x = sv + j * NF;
+#pragma unroll
+            for (int k = 0; k < KE; ++k) {
+                const int e = (slot * NFP + fp) * KE + k;
+                const real ej = s_Ew[e];
+                const real* x = sv + s_Ei[e] * NF;
 #pragma unroll
                 for (int q = 0; q < kPack; ++q) tr[q] += ej * (x[q] - x0[q]);
             }
@@ -303,12 +322,15 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
         // volume terms: -(D.F)/det for k and omega, perturbations about SP 0
         for (int o = t; o < 2 * NS; o += 64) {
             const int s = o >> 1, f = o & 1;
-            const real* Dr = s_D + s * NS * 4;
-            const real f00 = sF[f * 4 + 0], f01 = sF[f * 4 + 1], f02 = sF[f * 4 + 2];
             real acc = 0;
-            for (int j = 0; j < NS; ++j) {
-                const real* fj = sF + j * 8 + f * 4;
-                acc += Dr[j * 4 + 0] * (fj[0] - f00) + Dr[j * 4 + 1] * (fj[1] - f01) + Dr[j * 4 + 2] * (fj[2] - f02);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const real f0 = sF[f * 4 + m];
+#pragma unroll
+                for (int k = 0; k < WD[m]; ++k) {
+                    const int idx = (m * NS + s) * WM + k;
+                    acc += s_Dw[idx] * (sF[s_Di[idx] * 8 + f * 4 + m] - f0);
+                }
             }
             a.vol[(cell * NS + s) * 2 + f] = -acc / a.det[cell * NS + s];
         }

@@ -28,6 +28,10 @@ struct SstArgs {
     const real* det;
     const real* adj;              // (C, NS, 3, 3)
     const real* D;                // (3, NS, NS) of the cell type
+    const real* Dsp_w;            // (3, NS, WM) sparse derivative rows of the cell type: weights
+    const int* Dsp_i;             //                                                     point indices
+    const real* Esp_w;            // (5, NFP, KE) sparse face-trace rows of the cell type (slot order)
+    const int* Esp_i;
     const real* E0;               // (5, NFP, NS) prism, slot order
     const real* E1;               // (5, NFP, NS) tet, slot order
     const int* slot_nb;
