@@ -475,9 +475,13 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
             const double x = fabs(st[t]);
             double m1 = 0.0, m2 = 0.0, um = 0.0;
             constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
+            // rare path (the certificate failed): kept rolled so that it does not set the
+            // register budget of the whole kernel
+#pragma unroll 1
             for (int s2 = 0; s2 < NS; ++s2) {
                 const double y = fabs(st[s2 * 2 + f]);
                 int lt = 0, le = 0;
+#pragma unroll 1
                 for (int s3 = 0; s3 < NS; ++s3) {
                     const double z = fabs(st[s3 * 2 + f]);
                     lt += z < y;

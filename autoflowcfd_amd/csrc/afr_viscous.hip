@@ -91,7 +91,8 @@ __device__ void outlier_zeroing(real* sR, const real* umean, int t) {
             const int vv = o % 5;
             x[k] = fabs(sR[o]);
             int lt = 0, le = 0;
-            for (int s2 = 0; s2 < NS; ++s2) {
+#pragma unroll 1
+            for (int s2 = 0; s2 < NS; ++s2) {  // rare path: rolled, off the register budget
                 const real y = fabs(sR[s2 * 5 + vv]);
                 lt += y < x[k];
                 le += y <= x[k];
