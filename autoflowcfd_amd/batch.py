@@ -30,7 +30,7 @@ import torch
 from .device import DeviceMesh, bc_params_table
 from .face_slots import build_slots, concat_slots
 from .fr_tables import tables
-from .mesh import concat_meshes
+from .mesh import concat_meshes, locality_renumbered
 
 GAMMA = 1.4
 
@@ -95,12 +95,17 @@ class CaseBatch:
         self.n_vars = 7 if self.turb == "SST" else 5
         self.mu, self.Pr, self.Pr_t = mu_molecular, Pr, Pr_t
         self.ref_state = static_ref_state(self.cases) if ref_state is None else np.asarray(ref_state, np.float64)
-        mesh, cell_case, po, to, _ = concat_meshes([c.mesh for c in self.cases])
+        # every case stored in the locality order GPUFRSolver uses for it alone
+        parts, self._new_of_old = [], []
+        for c in self.cases:
+            m, n_o = locality_renumbered(c.mesh)
+            parts.append(m)
+            self._new_of_old.append(n_o)
+        mesh, cell_case, po, to, _ = concat_meshes(parts)
         self.mesh, self.cell_case, self.prism_off, self.tet_off = mesh, cell_case, po, to
         bcp = np.stack([bc_params_table(np.asarray(c.Q_free, dtype=np.float64)) for c in self.cases])
         # slot tables case by case: each cell gets exactly the rows its case alone has
         T = tables(order)
-        parts = [c.mesh for c in self.cases]
         slots = concat_slots([build_slots(p, T, with_geometry=False) for p in parts], [p.n_prism for p in parts],
                              [p.n_tet for p in parts], T.nfp)
         self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=self.ref_state,
@@ -173,13 +178,15 @@ class CaseBatch:
 
     # ------------------------------------------------------------------ per-case results
     def case_cells(self, i):
-        """Cell indices of case i in the batched mesh, in the case's own order (prisms, tets)."""
+        """Cell indices of case i in the batched mesh, in the case mesh's own cell order."""
         P = self.mesh.n_prism
-        return np.concatenate([np.arange(self.prism_off[i], self.prism_off[i + 1]),
-                               P + np.arange(self.tet_off[i], self.tet_off[i + 1])])
+        dev = np.concatenate([np.arange(self.prism_off[i], self.prism_off[i + 1]),
+                              P + np.arange(self.tet_off[i], self.tet_off[i + 1])])
+        return dev[self._new_of_old[i]]
 
     def case_state(self, i):
-        """Conserved state (C_i, ns, 5) of case i and, with SST, k and omega (host, FP64)."""
+        """Conserved state (C_i, ns, 5) of case i and, with SST, k and omega (host, FP64), in the
+        case mesh's own cell order."""
         idx = torch.as_tensor(self.case_cells(i), device=self.device)
         out = {"U": self.U.index_select(0, idx).double().cpu().numpy()}
         if self.turb == "SST":

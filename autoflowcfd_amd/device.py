@@ -14,7 +14,7 @@ from . import _lib
 from .face_slots import NSLOT, SLOT_CODES, build_slots
 from .fr_tables import prism_factors, tables
 from .geometry import cell_block_metrics
-from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
+from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET, locality_renumbered, morton_order  # noqa: F401
 
 GAMMA = 1.4
 ROW_DROP = 1e-11   # cross-interpolation weights below this are round-off of structural zeros
@@ -66,27 +66,6 @@ def sparse_fixed(rows, widths, drop=ROW_DROP):
     idx = np.where(valid, order, 0).astype(np.int32)
     w = np.where(valid, np.take_along_axis(rows, order, axis=-1), 0.0)
     return w, idx
-
-
-def _spread3(x):
-    """Insert two zero bits between the low 21 bits of x (uint64)."""
-    x = x & np.uint64(0x1FFFFF)
-    x = (x | (x << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
-    x = (x | (x << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
-    x = (x | (x << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
-    x = (x | (x << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
-    x = (x | (x << np.uint64(2))) & np.uint64(0x1249249249249249)
-    return x
-
-
-def morton_order(points):
-    """Permutation of the points along a 3-D Morton (Z-order) curve of their bounding box."""
-    if len(points) == 0:
-        return np.zeros(0, dtype=np.int64)
-    lo, hi = points.min(axis=0), points.max(axis=0)
-    q = ((points - lo) / np.maximum(hi - lo, 1e-300) * (2 ** 21 - 1)).astype(np.uint64)
-    key = _spread3(q[:, 0]) | (_spread3(q[:, 1]) << np.uint64(1)) | (_spread3(q[:, 2]) << np.uint64(2))
-    return np.argsort(key, kind="stable")
 
 
 class DeviceMesh:

@@ -417,8 +417,12 @@ def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
                  keyed_slots=sum(s.keyed_slots for s in slot_list), split_slots=sum(s.split_slots for s in slot_list))
 
 
-def slots_for(mesh, order):
+def slots_for(mesh, order, storage="locality"):
     """Slot tables of a mesh at an order without geometry (the device metric precompute fills
-    the rest): the host set-up of one solve, runnable in a worker process."""
+    the rest): the host set-up of one solve, runnable in a worker process.  storage="locality"
+    builds them for the mesh renumbered as GPUFRSolver stores it (mesh.locality_renumbered)."""
     from .fr_tables import tables
+    from .mesh import locality_renumbered
+    if storage == "locality":
+        mesh, _ = locality_renumbered(mesh)
     return build_slots(mesh, tables(order), with_geometry=False)

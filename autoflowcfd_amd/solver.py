@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from .device import DeviceMesh, bc_params_table
+from .device import DeviceMesh, bc_params_table, locality_renumbered
 from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
 
 GAMMA = 1.4
@@ -48,7 +48,7 @@ class GPUFRSolver:
                  p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
                  turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
                  bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None, slots=None,
-                 global_time_step: bool = False):
+                 global_time_step: bool = False, storage: str = "locality"):
         """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
         compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
         bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
@@ -57,6 +57,11 @@ class GPUFRSolver:
         slots: precomputed face-slot tables of the mesh at this order (face_slots.slots_for).
         global_time_step: step(dt) advances every solution point by the given dt (the global-dt
         runs of tests/validation/test_tgv.py:70-92) instead of the local time step.
+        storage: "locality" stores the cells on the device in the Morton order of their centroids
+        (neighbour gathers then share cache lines and pages); "input" keeps the mesh's order.
+        Either way every array this class takes or returns (states, residuals, checkpoints) is
+        in the mesh's own cell order; ``U`` and ``dm`` are the device-order internals.  With
+        storage="locality", ``slots`` must be the tables of the renumbered mesh.
         ref_state: constant primitive state of the perturbation form (default: the FARFIELD state);
         a case stepped alone with the ref_state of a batch reproduces its batched trajectory bit
         for bit (batch.CaseBatch).
@@ -84,7 +89,15 @@ class GPUFRSolver:
             Q_free = np.asarray(overrides[BC_FARFIELD], dtype=np.float64)
         p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
         bcp = bc_params_table(Q_free, Q_inlet, p_out)
-        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=ref_state,
+        if storage == "locality":
+            dev_mesh, new_of_old = locality_renumbered(mesh)
+        elif storage == "input":
+            dev_mesh, new_of_old = mesh, np.arange(mesh.n_cells)
+        else:
+            raise ValueError(f"storage {storage!r}: 'locality' or 'input'")
+        self._new_of_old = torch.as_tensor(new_of_old, dtype=torch.int64, device=self.device)
+        self._old_of_new = torch.as_tensor(np.argsort(new_of_old), dtype=torch.int64, device=self.device)
+        self.dm = DeviceMesh(dev_mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=ref_state,
                              slots=slots, turbulence="SST" if self.turb_model_name == "SST" else None)
         C, ns = self.dm.n_cells, self.dm.ns
         U0 = _cons(np.broadcast_to(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]), (C, ns, 5)))
@@ -170,13 +183,28 @@ class GPUFRSolver:
                 "residual_history": self.residual_history}
 
     # ------------------------------------------------------------------ residuals (reference method names)
+    def cell_volumes(self):
+        """FP64 cell volumes (numpy), mesh cell order."""
+        return self.dm.volume[self._new_of_old.cpu().numpy()]
+
+    def _to_dev(self, x):
+        """Mesh cell order -> device cell order (rows)."""
+        return x.index_select(0, self._old_of_new).contiguous()
+
+    def _to_user(self, x):
+        """Device cell order -> mesh cell order (rows)."""
+        return x.index_select(0, self._new_of_old).contiguous()
+
     def compute_inviscid_residual_gpu(self, U_trial=None):
-        U = self.U if U_trial is None else U_trial
-        return self.dm.inviscid_residual(U)
+        """Inviscid dU/dt (device tensor, mesh cell order) of the state or of U_trial (mesh order)."""
+        U = self.U if U_trial is None else self._to_dev(U_trial)
+        return self._to_user(self.dm.inviscid_residual(U))
 
     def compute_viscous_residual_gpu(self, U_trial=None, mu_t_field=None):
-        U = self.U if U_trial is None else U_trial
-        return self.dm.viscous_residual(U, mu_t_field, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t)
+        """Viscous dU/dt (device tensor, mesh cell order); U_trial and mu_t_field in mesh order."""
+        U = self.U if U_trial is None else self._to_dev(U_trial)
+        mut = None if mu_t_field is None else self._to_dev(mu_t_field)
+        return self._to_user(self.dm.viscous_residual(U, mut, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t))
 
     # ------------------------------------------------------------------ forces
     def compute_aerodynamic_coefficients(self, reference_area: float, include_viscous: bool = True):
@@ -200,11 +228,11 @@ class GPUFRSolver:
     def get_state_cpu(self):
         """{'U', 'Q'} as FP64 NumPy arrays (n_cells, n_sps, n_vars); with SST the last two
         components are rho k, rho omega (U) and k, omega (Q)."""
-        U = self.U.double().cpu().numpy()
+        U = self._to_user(self.U).double().cpu().numpy()
         Q = self._primitives(U)
         if self.n_vars > 5:
-            k = self.k.double().cpu().numpy() if self.k is not None else np.zeros(U.shape[:2])
-            w = self.omega.double().cpu().numpy() if self.omega is not None else np.zeros(U.shape[:2])
+            k = self._to_user(self.k).double().cpu().numpy() if self.k is not None else np.zeros(U.shape[:2])
+            w = self._to_user(self.omega).double().cpu().numpy() if self.omega is not None else np.zeros(U.shape[:2])
             U = np.concatenate([U, (Q[..., 0] * k)[..., None], (Q[..., 0] * w)[..., None]], axis=-1)
             Q = np.concatenate([Q, k[..., None], w[..., None]], axis=-1)
         return {"U": U, "Q": Q}
@@ -213,11 +241,12 @@ class GPUFRSolver:
         U_np = np.asarray(U_np, dtype=np.float64)
         if U_np.shape[:2] != tuple(self.U.shape[:2]):
             raise ValueError(f"state shape {U_np.shape} does not match ({self.U.shape[0]}, {self.U.shape[1]}, ...)")
-        self.U.copy_(torch.as_tensor(np.ascontiguousarray(U_np[..., :5]), dtype=self.U.dtype))
+        dev = lambda a: self._to_dev(torch.as_tensor(np.ascontiguousarray(a), device=self.device))  # noqa: E731
+        self.U.copy_(dev(U_np[..., :5]).to(self.U.dtype))
         if U_np.shape[-1] >= 7 and self.k is not None:
             rho = np.maximum(U_np[..., 0], 1e-10)
-            self.k.copy_(torch.as_tensor(U_np[..., 5] / rho, dtype=self.k.dtype))
-            self.omega.copy_(torch.as_tensor(U_np[..., 6] / rho, dtype=self.omega.dtype))
+            self.k.copy_(dev(U_np[..., 5] / rho).to(self.k.dtype))
+            self.omega.copy_(dev(U_np[..., 6] / rho).to(self.omega.dtype))
 
     def save_checkpoint(self, path: str):
         """NumPy .npz with the reference's HDF5 layout (U, Q, k, omega, residual_history + attrs)."""
@@ -227,9 +256,9 @@ class GPUFRSolver:
                 "time_scheme": np.array("ssp_rk3"), "cfl": np.float64(self.cfl),
                 "residual_history": np.asarray(self.residual_history, dtype=np.float64)}
         if self.k is not None:
-            data["k"] = self.k.double().cpu().numpy()
-            data["omega"] = self.omega.double().cpu().numpy()
-            data["nu_t"] = self.nu_t.double().cpu().numpy()
+            data["k"] = self._to_user(self.k).double().cpu().numpy()
+            data["omega"] = self._to_user(self.omega).double().cpu().numpy()
+            data["nu_t"] = self._to_user(self.nu_t).double().cpu().numpy()
         d = os.path.dirname(os.path.abspath(path))
         os.makedirs(d, exist_ok=True)
         np.savez(path, **data)
@@ -238,14 +267,15 @@ class GPUFRSolver:
         with np.load(path, allow_pickle=False) as f:
             if int(f["n_cells"]) != self.dm.n_cells or int(f["n_sps"]) != self.dm.ns:
                 raise ValueError("checkpoint does not match this mesh / order")
-            self.U.copy_(torch.as_tensor(np.ascontiguousarray(f["U"][..., :5]), dtype=self.U.dtype))
+            dev = lambda a: self._to_dev(torch.as_tensor(np.ascontiguousarray(a), device=self.device))  # noqa: E731
+            self.U.copy_(dev(f["U"][..., :5]).to(self.U.dtype))
             self.iteration = int(f["iteration"])
             self.residual_history = f["residual_history"].tolist()
             if self.k is not None and "k" in f:
-                self.k.copy_(torch.as_tensor(f["k"], dtype=self.k.dtype))
-                self.omega.copy_(torch.as_tensor(f["omega"], dtype=self.omega.dtype))
+                self.k.copy_(dev(f["k"]).to(self.k.dtype))
+                self.omega.copy_(dev(f["omega"]).to(self.omega.dtype))
                 if "nu_t" in f:
-                    self.nu_t.copy_(torch.as_tensor(f["nu_t"], dtype=self.nu_t.dtype))
+                    self.nu_t.copy_(dev(f["nu_t"]).to(self.nu_t.dtype))
 
     def cleanup(self):
         """Release the device buffers of this solver."""

@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=6)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in persistent grid sizes")
+    ap.add_argument("--storage", choices=("locality", "input"), default="locality",
+                    help="device cell storage order: Morton order of the centroids, or the mesh's own order")
     ap.add_argument("--grids", default="", help="explicit grid sizes instead of autotuning, e.g. visc_iface=16384,...")
     ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
@@ -273,6 +275,11 @@ def main():
     t0 = time.time()
     mesh = wall_box_mesh(args.n)
     log(f"[bench] mesh {mesh.n_cells} prisms, {mesh.faces.n_faces} face records ({time.time() - t0:.1f}s)")
+    if args.storage == "locality":
+        # the solver stores cells in the Morton order of their centroids (the input order is random)
+        from autoflowcfd_amd.device import morton_order
+        from autoflowcfd_amd.mesh import renumber_cells
+        mesh, _ = renumber_cells(mesh, morton_order(mesh.nodes[mesh.prism_conn].mean(axis=1)))
     Q0 = freestream()
     t0 = time.time()
     dm = DeviceMesh(mesh, args.order, dtype=torch.float32, bc_params=bc_params_table(Q0), metrics="gpu",

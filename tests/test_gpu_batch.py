@@ -45,10 +45,11 @@ def _run_pair(cases, order, dtype, turb, n_steps, graph):
         hist_a = [s.step() for _ in range(n_steps)]
         assert np.array_equal(hist_b[:, i], np.array(hist_a)), (i, hist_b[:, i], hist_a)
         st = b.case_state(i)
-        assert np.array_equal(st["U"], s.U.double().cpu().numpy()), i
+        alone = s.get_state_cpu()
+        assert np.array_equal(st["U"], alone["U"][..., :5]), i
         if turb == "SST":
-            assert np.array_equal(st["k"], s.k.double().cpu().numpy()), i
-            assert np.array_equal(st["omega"], s.omega.double().cpu().numpy()), i
+            assert np.array_equal(st["k"], alone["Q"][..., 5]), i
+            assert np.array_equal(st["omega"], alone["Q"][..., 6]), i
     return b
 
 

@@ -63,7 +63,7 @@ def _global_dt(solver, volumes):
     h = np.power(np.abs(volumes), 1.0 / 3.0)[:, None]
     CFL = 0.1
     dt = np.minimum(CFL * h / ws, 0.25 * CFL * rho * h ** 2 / MU)
-    local = solver.dm.local_time_step(solver.U, None, mu=MU).double().cpu().numpy()
+    local = solver._to_user(solver.dm.local_time_step(solver.U, None, mu=MU)).double().cpu().numpy()
     torch.cuda.synchronize()
     return float(np.minimum(dt, local).min())
 
@@ -88,11 +88,11 @@ def test_tgv_kinetic_energy_decays_monotonically(prec):
     mesh = _mesh()
     s = _solver(mesh, 2, dtype)
     s.set_state_from_cpu(_tgv_U(_sps_xyz(mesh, 2)))
-    vol = s.dm.volume
+    vol = s.cell_volumes()
     ke = [_kinetic_energy(s)]
     for i in range(N_STEPS):
         s.step(_global_dt(s, vol))
-        U = s.U.double().cpu().numpy()
+        U = s.get_state_cpu()["U"]
         assert np.all(np.isfinite(U)), f"diverged at step {i}"
         ke.append(_kinetic_energy(s))
     tail = ke[1:]
@@ -167,6 +167,6 @@ def test_p3_tet_residual_and_step_match_oracle(prec):
     U_ref = ssp_rk3(U, lambda X: inviscid_residual(X, g, ops, fl, bcd) + viscous_residual(X, g, ops, fl, bcd, MU),
                     np.full(U.shape[:2], dt), ops, 0)
     s.step(dt)
-    U_new = s.U.double().cpu().numpy()
+    U_new = s.get_state_cpu()["U"]
     inc = np.linalg.norm((U_new - U) - (U_ref - U)) / np.linalg.norm(U_ref - U)
     assert inc < 2e-4, f"step increment {inc:.3e}"

@@ -227,3 +227,33 @@ def test_concat_meshes_is_a_disjoint_union():
         Ri = R.inviscid_residual(U[cells], gi, ops, build_flat_faces(p, ops), bcd)
         # FP64 noise (point-location Newton, summation order): 1.5e-12 of max|R| measured
         assert np.abs(Rall[cells] - Ri).max() < 1e-10 * np.abs(Ri).max(), i
+
+
+def test_every_module_imports():
+    """Every product module imports on a machine without a GPU (syntax and import-time errors
+    show here, not first on the GPU)."""
+    import importlib
+    for name in ("autoflowcfd_amd.solver", "autoflowcfd_amd.batch", "autoflowcfd_amd.serve",
+                 "autoflowcfd_amd.autotune", "autoflowcfd_amd.api", "autoflowcfd_amd.meshio",
+                 "autoflowcfd_amd.device", "autoflowcfd_amd.face_slots", "autoflowcfd_amd.mesh",
+                 "autoflowcfd_amd.fr_tables", "autoflowcfd_amd.geometry", "autoflowcfd_amd.build"):
+        importlib.import_module(name)
+
+
+def test_renumbered_mesh_keeps_the_residual():
+    """A mesh stored in another cell order gives the same oracle residual, cell for cell."""
+    from autoflowcfd_amd.mesh import locality_renumbered
+    for make, order in ((lambda: wall_box(seed=2), 2), (mixed_mesh, 1)):
+        m = make()
+        m2, new_of_old = locality_renumbered(m)
+        ops = fr_ops.generate_operators(order)
+        Q0 = freestream(0.15)
+        bcp = bc_params_table(Q0)
+        bcd = {c: bcp[c] for c in range(8)}
+        g = Geometry(m, ops)
+        U = smooth_state(g.sps_xyz, Q0, seed=4)
+        R1 = R.inviscid_residual(U, g, ops, build_flat_faces(m, ops), bcd)
+        U2 = np.empty_like(U)
+        U2[new_of_old] = U
+        R2 = R.inviscid_residual(U2, Geometry(m2, ops), ops, build_flat_faces(m2, ops), bcd)
+        assert np.abs(R2[new_of_old] - R1).max() <= 1e-12 * np.abs(R1).max()
