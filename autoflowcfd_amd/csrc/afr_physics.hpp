@@ -11,10 +11,10 @@
 
 // Occupancy targets (waves per SIMD) of the heavy kernels, as build parameters: 0 leaves the
 // register allocation to the compiler; n > 0 asks for at least n waves per SIMD (the compiler
-// then budgets registers for it, spilling if it must).  These latency-bound kernels gain more
-// from waves in flight than they lose to spills: measured on the 82^3 P2 SST residual, the
-// interface kernel 4.41 -> 3.80 ms at 7 waves, the SST update 8 waves best (SST 6.93 -> 6.34 ms);
-// the SST cell, inviscid and viscous volume kernels are bounded by LDS or already at 8.
+// then budgets registers for it, spilling if it must).  Measured on the 82^3 P2 SST residual:
+// the interface kernel 3.52 (7 waves, 28 B/lane spilled), 3.22 (6 waves, no spill), 3.55 ms
+// (compiler's choice, 5 waves); the SST update 8 waves best (SST 6.93 -> 6.34 ms); the SST cell,
+// inviscid and viscous volume kernels are bounded by LDS or already at 8.
 #ifndef AFR_WPE_SST_CELL
 #define AFR_WPE_SST_CELL 0
 #endif
@@ -22,7 +22,7 @@
 #define AFR_WPE_SST_UPDATE 8
 #endif
 #ifndef AFR_WPE_VISC_IFACE
-#define AFR_WPE_VISC_IFACE 7
+#define AFR_WPE_VISC_IFACE 6
 #endif
 #ifndef AFR_WPE_VISC_VOLUME
 #define AFR_WPE_VISC_VOLUME 0

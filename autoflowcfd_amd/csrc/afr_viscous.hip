@@ -298,9 +298,106 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
 }
 
 // ------------------------------------------------------------------ interface kernel
+// BR1 jump of one slot / flux point from the own trace po and the other side's trace pn (18
+// values each: dQ, grad u, grad T, mu_t); boundary slots take the ghost state, mirrored
+// gradients and mu_t, and the interior-penalty momentum term.
+template <typename real>
+__device__ __forceinline__ void br1_jump(const ViscArgs<real>& a, const double* s_ref, int64_t cell, int64_t sidx,
+                                         int slot, int fp, int nfp, int bc, const real* po, real* pn, real mu, real Pr,
+                                         real Prt, real* jmp) {
+    constexpr int NP = 18;
+    const real side = (slot_code(slot) & 1) ? real(1) : real(-1);
+    real Qo[5], Qn[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Qo[v] = (real)(s_ref[v] + (double)po[v]);
+    if (bc == BC_INTERIOR) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
+    } else {
+        // ghost state (FP64, as in the inviscid path), gradients and mu_t mirrored
+        double qod[5], qgd[5];
+        const double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
+                              (double)a.slot_tn[sidx * 3 + 2]};
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qod[v] = s_ref[v] + (double)po[v];
+        ghost_state(bc, qod, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[cell] : 0), qgd);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Qn[v] = (real)qgd[v];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) pn[k] = po[k];
+    }
+    real Qa[5], gva[3][3], gTa[3], gvo[3][3], gTo[3];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Qa[v] = real(0.5) * (Qo[v] + Qn[v]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            gvo[i][j] = po[5 + i * 3 + j];
+            gva[i][j] = real(0.5) * (po[5 + i * 3 + j] + pn[5 + i * 3 + j]);
+        }
+        gTo[i] = po[14 + i];
+        gTa[i] = real(0.5) * (po[14 + i] + pn[14 + i]);
+    }
+    const real mto = po[17], mta = real(0.5) * (po[17] + pn[17]);
+    const real* ar = a.slot_adjrow + (sidx * nfp + fp) * 3;
+    const real A0 = ar[0], A1 = ar[1], A2 = ar[2];
+    const real Av[3] = {A0, A1, A2};
+    // A . (G(common) - G(own)) from the contracted fluxes
+    real fc[5], fo[5];
+    viscous_flux_dot(Qa, gva, gTa, mu, Pr, mta, Prt, Av, fc);
+    viscous_flux_dot(Qo, gvo, gTo, mu, Pr, mto, Prt, Av, fo);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) jmp[v] = fc[v] - fo[v];
+    if (bc != BC_INTERIOR) {
+        const real mag = sqrt(A0 * A0 + A1 * A1 + A2 * A2);
+        real h = a.h_ip[cell];
+        h = h > Tiny<real>::v ? h : Tiny<real>::v;
+        const real scale = real(4.0) * (mu + mto) / h * mag * side;
+#pragma unroll
+        for (int v = 1; v < 4; ++v) jmp[v] -= scale * (Qo[v] - Qn[v]);
+    }
+}
+
+// acc[q] += w x[q], q < 18, for one pack point (16-byte aligned, global memory or LDS)
+template <typename real>
+__device__ __forceinline__ void add_pack_point(real* acc, real w, const real* x) {
+    if constexpr (sizeof(real) == 4) {
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        real xv[20];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float4 f = x4[c];
+            xv[4 * c + 0] = f.x;
+            xv[4 * c + 1] = f.y;
+            xv[4 * c + 2] = f.z;
+            xv[4 * c + 3] = f.w;
+        }
+#pragma unroll
+        for (int q = 0; q < 18; ++q) acc[q] += w * xv[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 18; ++q) acc[q] += w * x[q];
+    }
+}
+
+// Balanced lanes (prisms, P1/P2): a prism's own face-trace rows have N1 non-zeros except on its
+// b face (slot 2, N1^2), and of its neighbour rows at most one slot is N1^2 wide on conforming
+// prism meshes.  One lane per (slot, flux point) would run every row to the widest row of the
+// wave; instead the main lane of (slot, fp) sums the first N1 terms of its rows and (N1 - 1) x NFP
+// helper lanes sum the further N1-term parts of the wide own row and of one wide neighbour slot,
+// returned through LDS.  Other wide rows (a second wide slot, second-source points) are finished
+// by their main lane.
+template <int P, int CT, int KMAX, typename real> struct IfaceBal {
+    static constexpr int N1 = P + 1, NFP = N1 * N1, NSLOT = CT == 0 ? 5 : 4;
+    static constexpr int NM = NSLOT * NFP, NX = (N1 - 1) * NFP;
+    static constexpr bool ON = CT == 0 && KMAX == N1 * N1 && NM + NX <= 64;
+};
+
 template <int P, int CT, int KMAX, typename real>
 __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k_visc_iface(ViscArgs<real> a) {
     using O = VO<P, CT>;
+    using B = IfaceBal<P, CT, KMAX, real>;
     constexpr int N1 = O::N1, NS = O::NS, NFP = O::NFP, NSLOT = O::NSLOT, NSV = O::NSV;
     constexpr int KE = CT == 0 ? N1 * N1 : NS;  // non-zeros of a face-trace row
     constexpr int NP = 18;                      // pack: dQ (5), grad u (9), grad T (3), mu_t (1)
@@ -338,12 +435,123 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
         wave_sync();
         t = lane_fresh(t);
         // 1. BR1 jumps: lane = (slot, flux point)
-        if (t < NSLOT * NFP) {
+        if constexpr (B::ON) {
+            constexpr int NM = B::NM, NX = B::NX, OWN_WIDE = 2;
+            static_assert((N1 - 1) * NFP * 2 * NP <= NS * kVP + 5 * NFP * 5, "helper traces fit the pack + jump space");
+            const bool is_main = t < NM, is_help = t >= NM && t < NM + NX;
+            const int fp = is_main ? t % NFP : (t - NM) % NFP;
+            const int part = is_main ? 0 : 1 + (t - NM) / NFP;
+            const int slot = is_main ? t / NFP : OWN_WIDE;     // own row of this lane
+            const int64_t sidx = cell * 5 + slot;
+            int bc = BC_INTERIOR, nrow = a.n_rows;
+            int64_t nnb = 0;
+            bool live = false;
+            if (is_main) {
+                bc = a.slot_bc[sidx];
+                if (bc == BC_INTERIOR) {
+                    const int src = (a.slot_mask[sidx] >> fp) & 1;
+                    const int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
+                    const int nbc = a.slot_nb[sidx * 2 + src];
+                    live = nbc >= 0 && row >= 0 && row < a.n_rows;
+                    if (live) {
+                        nrow = row;
+                        nnb = nbc;
+                    }
+                    live = live && src == 0;    // helper parts only for first-source rows
+                }
+            }
+            // the first main lane with a wide first-source neighbour row names the helped slot
+            const bool wide = live && a.rows_w[(int64_t)nrow * KMAX + N1] != real(0);
+            const unsigned long long wm = __ballot(wide);
+            const int wslot = wm ? (int)((__ffsll((long long)wm) - 1) / NFP) : -1;
+            const bool use_help = is_main && live && slot == wslot;
+            if (is_help && wslot >= 0) {
+                const int64_t ws = cell * 5 + wslot;
+                const int src = (a.slot_mask[ws] >> fp) & 1;
+                const int row = a.slot_blk[ws * 2 + src] * NFP + fp;
+                const int nbc = a.slot_nb[ws * 2 + src];
+                live = src == 0 && nbc >= 0 && row >= 0 && row < a.n_rows;
+                if (live) {
+                    nrow = row;
+                    nnb = nbc;
+                }
+            }
+            const bool has_nb = is_main ? (bc == BC_INTERIOR && nrow < a.n_rows) : live;
+            real po[NP], pn[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) po[k] = pn[k] = real(0);
+            if (is_main || is_help) {
+                const int eb = (slot * NFP + fp) * KE + part * N1;
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const real w = s_Ew[eb + j];
+                    const real* x = sP + s_Ei[eb + j] * kVP;
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) po[q] += w * x[q];
+                }
+                if (is_main && slot != OWN_WIDE) {
+#pragma unroll 1
+                    for (int k = N1; k < KE; ++k) {   // rows are non-zeros first
+                        const real w = s_Ew[eb + k];
+                        if (w == real(0)) break;
+                        const real* x = sP + s_Ei[eb + k] * kVP;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) po[q] += w * x[q];
+                    }
+                }
+                if (has_nb) {
+                    const int* const ri = a.rows_idx + (int64_t)nrow * KMAX;
+                    const real* const rw = a.rows_w + (int64_t)nrow * KMAX;
+                    const int k0 = part * N1;
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) add_pack_point(pn, rw[k0 + j], a.vpack + (nnb * NS + ri[k0 + j]) * kVP);
+                    if (is_main && !use_help) {
+#pragma unroll 1
+                        for (int k = N1; k < KMAX; ++k) {
+                            const real w = rw[k];
+                            if (w == real(0)) break;
+                            add_pack_point(pn, w, a.vpack + (nnb * NS + ri[k]) * kVP);
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            // helper parts through LDS (pack and jump space, free until the jumps are written)
+            real* const scr = sP;
+            if (is_help) {
+                real* d = scr + ((part - 1) * NFP + fp) * 2 * NP;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    d[q] = po[q];
+                    d[NP + q] = pn[q];
+                }
+            }
+            wave_sync();
+            if (is_main) {
+                if (slot == OWN_WIDE) {
+#pragma unroll
+                    for (int pp = 1; pp < N1; ++pp)
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) po[q] += scr[((pp - 1) * NFP + fp) * 2 * NP + q];
+                }
+                if (use_help) {
+#pragma unroll
+                    for (int pp = 1; pp < N1; ++pp)
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) pn[q] += scr[((pp - 1) * NFP + fp) * 2 * NP + NP + q];
+                }
+            }
+            wave_sync();
+            if (is_main) {
+                real jmp[5];
+                br1_jump(a, s_ref, cell, sidx, slot, fp, NFP, bc, po, pn, mu, Pr, Prt, jmp);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sJ[(slot * NFP + fp) * 5 + v] = jmp[v];
+            }
+        } else if (t < NSLOT * NFP) {
             const int slot = t / NFP, fp = t % NFP;
             const int64_t sidx = cell * 5 + slot;
             const int bc = a.slot_bc[sidx];
-            const int code = slot_code(slot);
-            const real side = (code & 1) ? real(1) : real(-1);
             real po[NP], pn[NP];
 #pragma unroll
             for (int k = 0; k < NP; ++k) po[k] = real(0);
@@ -355,9 +563,6 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
 #pragma unroll
                 for (int q = 0; q < NP; ++q) po[q] += w * x[q];
             }
-            real Qo[5], Qn[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Qo[v] = (real)(s_ref[v] + (double)po[v]);
             if (bc == BC_INTERIOR) {
 #pragma unroll
                 for (int k = 0; k < NP; ++k) pn[k] = real(0);
@@ -372,74 +577,10 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
                 const int* const ri = a.rows_idx + (int64_t)row * KMAX;
                 const real* const rwp = a.rows_w + (int64_t)row * KMAX;
 #pragma unroll AFR_VISC_GATHER_UNROLL
-                for (int k = 0; k < KMAX; ++k) {
-                    const int spk = ri[k];
-                    const real wk = rwp[k];
-                    const float4* x = reinterpret_cast<const float4*>(a.vpack + (nb * NS + spk) * kVP);
-                    real xv[20];
-                    if constexpr (sizeof(real) == 4) {
-#pragma unroll
-                        for (int c = 0; c < 5; ++c) {
-                            const float4 f = x[c];
-                            xv[4 * c + 0] = f.x;
-                            xv[4 * c + 1] = f.y;
-                            xv[4 * c + 2] = f.z;
-                            xv[4 * c + 3] = f.w;
-                        }
-                    } else {
-                        const real* xr = a.vpack + (nb * NS + spk) * kVP;
-#pragma unroll
-                        for (int q = 0; q < NP; ++q) xv[q] = xr[q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) pn[q] += wk * xv[q];
-                }
-#pragma unroll
-                for (int v = 0; v < 5; ++v) Qn[v] = (real)(s_ref[v] + (double)pn[v]);
-            } else {
-                // ghost state (FP64, as in the inviscid path), gradients and mu_t mirrored
-                double qod[5], qgd[5];
-                const double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
-                                      (double)a.slot_tn[sidx * 3 + 2]};
-#pragma unroll
-                for (int v = 0; v < 5; ++v) qod[v] = s_ref[v] + (double)po[v];
-                ghost_state(bc, qod, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[cell] : 0), qgd);
-#pragma unroll
-                for (int v = 0; v < 5; ++v) Qn[v] = (real)qgd[v];
-#pragma unroll
-                for (int k = 0; k < NP; ++k) pn[k] = po[k];
+                for (int k = 0; k < KMAX; ++k) add_pack_point(pn, rwp[k], a.vpack + (nb * NS + ri[k]) * kVP);
             }
-            real Qa[5], gva[3][3], gTa[3], gvo[3][3], gTo[3];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Qa[v] = real(0.5) * (Qo[v] + Qn[v]);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    gvo[i][j] = po[5 + i * 3 + j];
-                    gva[i][j] = real(0.5) * (po[5 + i * 3 + j] + pn[5 + i * 3 + j]);
-                }
-                gTo[i] = po[14 + i];
-                gTa[i] = real(0.5) * (po[14 + i] + pn[14 + i]);
-            }
-            const real mto = po[17], mta = real(0.5) * (po[17] + pn[17]);
-            const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
-            const real A0 = ar[0], A1 = ar[1], A2 = ar[2];
-            const real Av[3] = {A0, A1, A2};
-            // A . (G(common) - G(own)) from the contracted fluxes
-            real fc[5], fo[5], jmp[5];
-            viscous_flux_dot(Qa, gva, gTa, mu, Pr, mta, Prt, Av, fc);
-            viscous_flux_dot(Qo, gvo, gTo, mu, Pr, mto, Prt, Av, fo);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) jmp[v] = fc[v] - fo[v];
-            if (bc != BC_INTERIOR) {
-                const real mag = sqrt(A0 * A0 + A1 * A1 + A2 * A2);
-                real h = a.h_ip[cell];
-                h = h > Tiny<real>::v ? h : Tiny<real>::v;
-                const real scale = real(4.0) * (mu + mto) / h * mag * side;
-#pragma unroll
-                for (int v = 1; v < 4; ++v) jmp[v] -= scale * (Qo[v] - Qn[v]);
-            }
+            real jmp[5];
+            br1_jump(a, s_ref, cell, sidx, slot, fp, NFP, bc, po, pn, mu, Pr, Prt, jmp);
 #pragma unroll
             for (int v = 0; v < 5; ++v) sJ[(slot * NFP + fp) * 5 + v] = jmp[v];
         }
