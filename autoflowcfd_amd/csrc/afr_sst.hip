@@ -130,22 +130,35 @@ template <int P, int CT> struct SW {
     static constexpr int KE = CT == 0 ? N1 * N1 : NS;
 };
 
+// cells per wavefront of the cell kernel: two half-waves of 32 lanes when a cell has at most
+// 32 solution points, so the per-point phases (FP64 sources) keep 54 of 64 lanes busy at P2
+#ifndef AFR_SST_PAIR
+#define AFR_SST_PAIR 1
+#endif
+template <int P, int CT> struct SstCellPlan {
+    static constexpr int NS = (P + 1) * (P + 1) * (P + 1);
+    static constexpr int CPW = (AFR_SST_PAIR && CT == 0 && NS <= 32) ? 2 : 1;   // cells per wave (prisms)
+    static constexpr int LW = 64 / CPW;                             // lanes per cell
+};
+
 template <int P, int CT, typename real>
 __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     using W = SW<P, CT>;
-    constexpr int NS = O::NS, WM = W::WM, KE = W::KE;
+    using L = SstCellPlan<P, CT>;
+    constexpr int NS = O::NS, WM = W::WM, KE = W::KE, CPW = L::CPW, LW = L::LW;
     constexpr int WD[3] = {W::W0, W::W1, W::W2};
     // per-SP record (12): rho, u, v, w, k, omega, Gk, Gw (the face-trace fields, contiguous),
     // m1, m2, m3 (the momentum gradient fields), pad
     constexpr int NF = 12, G_ = 16;  // gradient record: grad m (9), grad k (3), grad omega (3), pad
     constexpr int NFP = O::NFP, NSLOT = O::NSLOT;
+    constexpr int PER_CELL = NS * NF + NS * G_ + NS * 8;
     // sparse derivative and face-trace rows (the D / E non-zeros only)
     __shared__ real s_Dw[3 * NS * WM];
     __shared__ int s_Di[3 * NS * WM];
     __shared__ real s_Ew[5 * NFP * KE];
     __shared__ int s_Ei[5 * NFP * KE];
-    __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * NF + NS * G_ + NS * 8];
+    __shared__ __attribute__((aligned(16))) real s_w[kWaves][CPW * PER_CELL];
     for (int i = threadIdx.x; i < 3 * NS * WM; i += 64 * kWaves) {
         s_Dw[i] = a.Dsp_w[i];
         s_Di[i] = a.Dsp_i[i];
@@ -156,19 +169,27 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
     }
     __syncthreads();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
-    real* sv = s_w[wid];          // [s][NF]
-    real* sg = sv + NS * NF;      // [s][G_]
-    real* sF = sg + NS * G_;      // [j][f][4]: contravariant convective + diffusive flux of k (f=0), omega (f=1), m = 0..2
+    const int h = t / LW, tl = t % LW;               // cell of the wave, lane within the cell
+    real* sv = s_w[wid] + h * PER_CELL;  // [s][NF]
+    real* sg = sv + NS * NF;             // [s][G_]
+    real* sF = sg + NS * G_;             // [j][f][4]: contravariant convective + diffusive flux of k (f=0), omega (f=1), m = 0..2
     const double mu = a.mu_molecular;
-    for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
-        const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
-        if (t < NS) {
-            const int64_t i = cell * NS + t;
+    const int64_t n_units = (a.n_cells + CPW - 1) / CPW;
+    for (CellWalk cw(n_units, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
+        // list positions CPW it + h; a missing second cell leaves its half-wave idle
+        const int64_t pos0 = cw.it * CPW;
+        const int64_t c0 = walk_cell(a.schedule, pos0);
+        const int64_t c1 = (CPW == 2 && pos0 + 1 < a.n_cells) ? walk_cell(a.schedule, pos0 + 1) : c0;
+        const bool live = h == 0 || pos0 + 1 < a.n_cells;
+        const int64_t cell = a.cell_begin + (h == 0 ? c0 : c1);
+        const int64_t cb = cell * NS;                // first solution point of the cell
+        if (live && tl < NS) {
+            const int64_t i = cb + tl;
             real u5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) u5[v] = a.U[i * 5 + v];
             const real rho = u5[0] > real(1e-10) ? u5[0] : real(1e-10);
-            real* r = sv + t * NF;
+            real* r = sv + tl * NF;
             r[0] = rho;
             r[1] = u5[1] / rho;
             r[2] = u5[2] / rho;
@@ -181,7 +202,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
         }
         wave_sync();
         // gradients of m1, m2, m3, k, omega: lane (s, field), sparse rows, perturbations about SP 0
-        for (int o = t; o < 5 * NS; o += 64) {
+        for (int o = tl; o < 5 * NS; o += LW) {
+            if (!live) break;
             const int s = o / 5, f = o % 5;
             const int src = f < 3 ? 8 + f : 4 + (f - 3);
             const real x0 = sv[src];
@@ -196,16 +218,16 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
                 }
                 d[m] = acc;
             }
-            const real* ad = a.adj + (cell * NS + s) * 9;
-            const real idet = real(1) / a.det[cell * NS + s];
+            const real* ad = a.adj + (cb + s) * 9;
+            const real idet = real(1) / a.det[cb + s];
             const int base = f < 3 ? f * 3 : 9 + (f - 3) * 3;
 #pragma unroll
             for (int n = 0; n < 3; ++n) sg[s * G_ + base + n] = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
         }
         wave_sync();
-        if (t < NS) {
-            const int s = t;
-            const int64_t i = cell * NS + s;
+        if (live && tl < NS) {
+            const int s = tl;
+            const int64_t i = cb + s;
             const real* r = sv + s * NF;
             const real* gs = sg + s * G_;
             const double rho = (double)r[0], k = (double)r[4], w = (double)r[5];
@@ -299,8 +321,9 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
         }
         wave_sync();
         // owner-side face traces of (rho, u, v, w, k, omega, Gamma_k, Gamma_omega): lane (slot, flux point)
-        if (t < NSLOT * NFP) {
-            const int slot = t / NFP, fp = t % NFP;
+        for (int o = tl; o < NSLOT * NFP; o += LW) {
+            if (!live) break;
+            const int slot = o / NFP, fp = o % NFP;
             real x0[kPack], tr[kPack];
 #pragma unroll
             for (int q = 0; q < kPack; ++q) {
@@ -315,12 +338,13 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
 #pragma unroll
                 for (int q = 0; q < kPack; ++q) tr[q] += ej * (x[q] - x0[q]);
             }
-            real* o = a.pack + ((cell * 5 + slot) * NFP + fp) * kPack;
+            real* ov = a.pack + ((cell * 5 + slot) * NFP + fp) * kPack;
 #pragma unroll
-            for (int q = 0; q < kPack; ++q) o[q] = x0[q] + tr[q];
+            for (int q = 0; q < kPack; ++q) ov[q] = x0[q] + tr[q];
         }
         // volume terms: -(D.F)/det for k and omega, perturbations about SP 0
-        for (int o = t; o < 2 * NS; o += 64) {
+        for (int o = tl; o < 2 * NS; o += LW) {
+            if (!live) break;
             const int s = o >> 1, f = o & 1;
             real acc = 0;
 #pragma unroll
@@ -332,7 +356,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
                     acc += s_Dw[idx] * (sF[s_Di[idx] * 8 + f * 4 + m] - f0);
                 }
             }
-            a.vol[(cell * NS + s) * 2 + f] = -acc / a.det[cell * NS + s];
+            a.vol[(cb + s) * 2 + f] = -acc / a.det[cb + s];
         }
         wave_sync();
     }
@@ -565,8 +589,10 @@ __global__ void __launch_bounds__(256) k_wall_distance(const double* __restrict_
 template <int P, int CT, typename real>
 int launch_sst_cell(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(a.n_cells, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)), dim3(64 * kWaves), 0,
-                       st, a);
+    constexpr int CPW = SstCellPlan<P, CT>::CPW;
+    const int64_t units = (a.n_cells + CPW - 1) / CPW;
+    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(units, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)),
+                       dim3(64 * kWaves), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -233,6 +233,99 @@ def batched_leg(args):
             "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd}
 
 
+def tgv_noise(xyz, v0, seed=7, modes=4):
+    """Seeded smooth velocity noise of rms 1e-3 v0: per component a sum of periodic Fourier modes
+    (wavenumbers 1..2, random phases and amplitudes).  Point-wise white noise is not usable at P3:
+    the collapsed-coordinate derivative rows reach 6e7 and turn it into gradients of 1e6."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(xyz.shape)
+    for c in range(3):
+        for _ in range(modes):
+            k = rng.integers(1, 3, 3)
+            ph = rng.uniform(0, 2 * np.pi, 3)
+            amp = rng.standard_normal()
+            out[..., c] += amp * np.sin(k[0] * xyz[..., 0] + ph[0]) * np.sin(k[1] * xyz[..., 1] + ph[1]) * \
+                np.sin(k[2] * xyz[..., 2] + ph[2])
+    return 1e-3 * v0 * out / np.sqrt(modes / 8.0)
+
+
+def tgv_p3_leg(args):
+    """P3 Navier-Stokes Taylor-Green vortex: triply periodic box of side 2 pi, N^3 hex x 6 tets,
+    FP64 (the P3 path is FP64 only), Mach 0.1 (V0 = 0.1 a_inf), Re 1600, the initial field of
+    tests/validation/test_tgv.py:96-112 of the reference plus smooth seeded noise of 1e-3 V0, SSP-RK3
+    with one global dt frozen from the initial state (CFL 0.1 on V^(1/3) with |u| + a, the viscous
+    bound and the minimum local time step: the global dt of test_tgv.py:70-92).  Reports the time
+    per step (3 residual evaluations), GDOF/s per residual evaluation, and the kinetic energy and
+    enstrophy (mean over solution points of 1/2 rho |u|^2 and 1/2 rho |curl u|^2, from the
+    device's own velocity gradients) along the run."""
+    import torch
+    from autoflowcfd_amd.fr_tables import tables
+    from autoflowcfd_amd.geometry import map_ref
+    from autoflowcfd_amd.mesh import hex_box
+    from autoflowcfd_amd.solver import GPUFRSolver
+    n, steps = args.tgv_p3, args.tgv_steps
+    gam, L = 1.4, 2.0 * np.pi
+    a_inf = float(np.sqrt(gam * P_INF / RHO))
+    v0 = 0.1 * a_inf
+    mu = RHO * v0 * 1.0 / 1600.0
+    t0 = time.time()
+    mesh = hex_box(n, n, n, L, L, L, split="tet", periodic=True, permute=False)
+    s = GPUFRSolver(mesh, order=3, rho_inf=RHO, vel_inf=v0, p_inf=P_INF, mu_molecular=mu, dtype=torch.float64,
+                    global_time_step=True)
+    xyz = map_ref(False, tables(3).sps, mesh.nodes[mesh.tet_conn])
+    x, y, z = xyz[..., 0], xyz[..., 1], xyz[..., 2]
+    noise = tgv_noise(xyz, v0)
+    u = v0 * np.sin(x) * np.cos(y) * np.cos(z) + noise[..., 0]
+    v = -v0 * np.cos(x) * np.sin(y) * np.cos(z) + noise[..., 1]
+    w = noise[..., 2]
+    p = P_INF + (RHO * v0 ** 2 / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    U0 = np.stack([np.full_like(u, RHO), RHO * u, RHO * v, RHO * w,
+                   p / (gam - 1.0) + 0.5 * RHO * (u * u + v * v + w * w)], axis=-1)
+    del xyz, x, y, z, noise, u, v, w, p
+    s.set_state_from_cpu(U0)
+    h = np.abs(s.cell_volumes())[:, None] ** (1.0 / 3.0)
+    vel = U0[..., 1:4] / U0[..., :1]
+    pr = (gam - 1.0) * (U0[..., 4] - 0.5 * RHO * np.sum(vel ** 2, axis=-1))
+    vm = np.linalg.norm(vel, axis=-1)
+    dt_glob = np.minimum(0.1 * h / (vm + np.sqrt(gam * pr / RHO)), 0.25 * 0.1 * RHO * h ** 2 / mu).min()
+    dt_loc = float(s.dm.local_time_step(s.U, None, mu=mu).min().item())
+    dt = float(min(dt_glob, dt_loc))
+    del U0, vel, pr, vm, h
+    n_sp = mesh.n_cells * s.dm.ns
+    log(f"[tgv-p3] {mesh.n_cells} tets, {n_sp} solution points, set-up {time.time() - t0:.1f}s, dt {dt:.3e}")
+
+    def diagnostics():
+        U = s.U
+        ke = float((0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0]).mean().item())
+        s.dm.viscous_volume(U, None, mu=mu)
+        g = s.dm.work_vpack[..., 5:14].reshape(U.shape[0], U.shape[1], 3, 3)   # [i][j] = du_i/dx_j
+        om2 = (g[..., 2, 1] - g[..., 1, 2]) ** 2 + (g[..., 0, 2] - g[..., 2, 0]) ** 2 + (g[..., 1, 0] - g[..., 0, 1]) ** 2
+        ens = float((0.5 * U[..., 0] * om2).mean().item())
+        return ke, ens, bool(torch.isfinite(U).all().item())
+
+    hist = [(0,) + diagnostics()]
+    every = max(steps // 10, 1)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_steps = 0.0
+    for i in range(steps):
+        ev0.record()
+        s.step(dt)
+        ev1.record()
+        ev1.synchronize()
+        t_steps += ev0.elapsed_time(ev1)
+        if (i + 1) % every == 0 or i + 1 == steps:
+            hist.append((i + 1,) + diagnostics())
+            log(f"[tgv-p3] step {i + 1}: KE {hist[-1][1]:.6e} enstrophy {hist[-1][2]:.6e} finite {hist[-1][3]}")
+            if not hist[-1][3]:
+                break
+    ms = t_steps / max(hist[-1][0], 1)
+    return {"config": {"workload": f"P3 NS Taylor-Green vortex, Mach 0.1, Re 1600, {n}^3 hex x 6 tets "
+                                   f"({mesh.n_cells} cells, {n_sp} solution points), FP64, global dt"},
+            "steps": hist[-1][0], "dt": dt, "ms_per_step": ms,
+            "gdof_per_s_per_residual": n_sp * 5 * 3 / (ms * 1e-3) / 1e9,
+            "history": [{"step": a, "ke": b, "enstrophy": c, "finite": d} for a, b, c, d in hist]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +350,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=16)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
+    ap.add_argument("--tgv-p3", type=int, default=0,
+                    help="run only the P3 Taylor-Green vortex leg on N^3 hex x 6 tets (prints its JSON)")
+    ap.add_argument("--tgv-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -266,6 +362,9 @@ def main():
     import torch
     from autoflowcfd_amd.device import DeviceMesh, bc_params_table
 
+    if args.tgv_p3:
+        print(json.dumps(tgv_p3_leg(args)), flush=True)
+        return
     if args.serve_only:
         out = served_leg(args)
         if not args.no_batch:

@@ -55,3 +55,38 @@ def test_p1_euler_pulse_ten_steps_matches_oracle():
     inc = np.linalg.norm((U - U0) - (U_ref - U0)) / np.linalg.norm(U_ref - U0)
     assert inc < 1e-9, f"10-step increment {inc:.3e}"
     assert np.allclose(norms, norms_ref, rtol=1e-9, atol=0.0), (norms, norms_ref)
+
+
+def test_p1_euler_pulse_full_configuration_matches_oracle_fixture():
+    """The same configuration at its full size, 16^3 hex x 6 tets (24,576 cells), against the FP64
+    oracle's 10-step result stored by tests/golden/make_config_goldens.py (the oracle takes
+    minutes here): residual RMS before every step to 1e-9, the increment at 2000 seeded solution
+    points and its global norm to 1e-9, per-variable checksums of the final state to 1e-12."""
+    import os
+    import sys
+    import torch
+    from autoflowcfd_amd.device import DeviceMesh, bc_params_table
+    from autoflowcfd_amd.fr_tables import tables
+    from autoflowcfd_amd.geometry import map_ref
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_config_goldens as gold
+    print("[gpu] P1 Euler FP64 pulse, 16^3 hex x 6 tets, 10 SSP-RK3 steps vs fixture", flush=True)
+    ref = np.load(os.path.join(here, "golden", "config_c1_p1_euler_16.npz"))
+    mesh, Q0 = gold.c1_setup()
+    assert mesh.n_tet == int(ref["n_cells"])
+    xyz = map_ref(False, tables(1).sps, mesh.nodes[mesh.tet_conn])
+    U0 = gold.c1_state(xyz, Q0)
+    dm = DeviceMesh(mesh, 1, dtype=torch.float64, bc_params=bc_params_table(Q0))
+    Ud = torch.as_tensor(U0, dtype=torch.float64, device="cuda").contiguous()
+    norms = [float(dm.ssp_rk3_step(Ud, None, mu=0.0).item()) for _ in range(10)]
+    U = Ud.cpu().numpy()
+    assert np.allclose(norms, ref["norms"], rtol=1e-9, atol=0.0), (norms, ref["norms"])
+    ci, si = ref["sample_cell"], ref["sample_sp"]
+    d_gpu, d_ref = U[ci, si] - U0[ci, si], ref["sample_U"] - U0[ci, si]
+    inc = np.linalg.norm(d_gpu - d_ref) / np.linalg.norm(d_ref)
+    assert inc < 1e-9, f"sampled increment {inc:.3e}"
+    g = np.linalg.norm(U - U0)
+    assert abs(g - float(ref["inc_norm"])) <= 1e-9 * float(ref["inc_norm"])
+    assert np.allclose(U.sum(axis=(0, 1)), ref["checksum"], rtol=1e-12, atol=0.0)
+    assert np.allclose((U ** 2).sum(axis=(0, 1)), ref["checksum2"], rtol=1e-12, atol=0.0)
