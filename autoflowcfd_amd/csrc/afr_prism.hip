@@ -67,6 +67,14 @@ __device__ __forceinline__ void st4(real* p, typename V4T<real>::type v) {
 constexpr int r4(int n) { return (n + 3) / 4 * 4; }
 constexpr int r64(int n) { return (n + 63) / 64 * 64; }
 
+// 1: the five slot source cells' dQ are copied into LDS with the own cell's; 0: only the own
+// cell is staged and the interface lanes read the sources from global memory.  0 frees
+// 5 x 540 B of LDS per wave at P2 FP32 (29 instead of 39 KB per workgroup, 5 instead of 4 waves
+// per SIMD): measured 3.08 -> 2.76 ms on the 82^3 P2 mesh.
+#ifndef AFR_PRISM_STAGE_NB
+#define AFR_PRISM_STAGE_NB 0
+#endif
+
 // Padded LDS layout of the prism kernel (units: reals).
 template <int P, typename real> struct PL {
     using K = PK<P>;
@@ -86,7 +94,7 @@ template <int P, typename real> struct PL {
     static constexpr int T_G = T_EC + 2 * N1;                  // [2][N1] gL, gR
     static constexpr int T_N = r4(T_G + 2 * N1);
     // per-wave buffers
-    static constexpr int QN_DW = r64(6 * NSV * WPR);           // own cell + 5 slot sources, raw dwords
+    static constexpr int QN_DW = r64((AFR_PRISM_STAGE_NB ? 6 : 1) * NSV * WPR);  // own cell (+ 5 slot sources), raw dwords
     static constexpr int W_QN = 0;
     static constexpr int W_XY = W_QN + QN_DW / WPR;            // X [NF1][NX] | Y [NAB][5][4]
     static constexpr int XY_N = r4(NF1 * NX > NAB * 20 ? NF1 * NX : NAB * 20);
@@ -100,6 +108,7 @@ template <int P, typename real> struct PL {
 };
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
+
 
 // Copies the states of up to 6 cells (dword offsets base[0..5] into src, each
 // CELL_DW dwords) back to back into dst[0 .. NCH*64) with one global_load_lds
@@ -491,7 +500,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
                     rw[k] = a.rows_w[(int64_t)row * KMAX + k];
                 }
                 real dqn[5] = {0, 0, 0, 0, 0};
-                if (src == 0) {  // primary source: staged in LDS
+                if (AFR_PRISM_STAGE_NB && src == 0) {  // primary source: staged in LDS
                     const real* qs = sQ + (1 + slot) * NSV;
 #pragma unroll
                     for (int k = 0; k < KMAX; ++k)
