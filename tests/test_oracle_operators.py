@@ -200,3 +200,27 @@ def test_prism_kronecker_factors_rebuild_operators(order):
     assert np.abs(np.kron(F.D_ab[0], I) - t.D[0, :, :, 0]).max() < 1e-11
     assert np.abs(np.kron(F.D_ab[1], I) - t.D[0, :, :, 1]).max() < 1e-11
     assert np.abs(np.kron(np.eye(n1 * n1), F.D_c) - t.D[0, :, :, 2]).max() < 1e-11
+
+
+def test_tet_filter_reproduces_reference_stated_norms():
+    """The reference's modal_filter.py:60-71 states, for the P2 collapsed tet basis, a spectral
+    norm of 5.78 for the filter with eta = max(i,j,k)/P (the one it uses) and of 192.9 for a
+    total-degree eta, with white noise amplified 17.5x by the latter and damped to 0.63x by the
+    former (the ratio of the two, 27.8, does not depend on how the noise is normalised).  The
+    same figures from this build's basis pin the tet modal basis and the filter at P2.  At P3 the
+    same construction has a spectral norm near 950: the per-stage filter then amplifies nodal
+    noise, which is why P3 tet runs of the reference's algorithm diverge."""
+    from oracle.fr_ops import FILTER_ALPHA, FILTER_ORDER, modal_basis, modal_filter, tensor_points
+    P, n1d = 2, 3
+    sps = tensor_points(n1d)
+    V = modal_basis("tet", sps[:, 0], sps[:, 1], sps[:, 2], P)[0]
+    sig = np.array([np.exp(-FILTER_ALPHA * ((i + j + k) / (3 * P)) ** (2 * FILTER_ORDER))
+                    for i in range(n1d) for j in range(n1d) for k in range(n1d)])
+    F_total = V @ np.diag(sig) @ np.linalg.inv(V)
+    F_max = modal_filter("tet", P)
+    assert abs(np.linalg.norm(F_max, 2) - 5.78) < 0.005
+    assert abs(np.linalg.norm(F_total, 2) - 192.9) < 0.05
+    x = np.random.default_rng(0).standard_normal((27, 20000))
+    ratio = (F_total @ x).std() / (F_max @ x).std()
+    assert abs(ratio - 17.5 / 0.63) < 0.5, ratio
+    assert np.linalg.norm(modal_filter("tet", 3), 2) > 500.0
