@@ -468,10 +468,10 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
                 const int axis = code >> 1;
                 const double g = (code & 1) ? (axis == 0 ? gr[0] : (axis == 1 ? gr[1] : gr[2]))
                                             : (axis == 0 ? gl[0] : (axis == 1 ? gl[1] : gl[2]));
-                acc += (role == 0 ? -1.0 : 1.0) * g * cv[q] / det;
+                acc += (role == 0 ? -1.0 : 1.0) * g * cv[q];
             }
             const double rho = dmax((double)a.U[i * 5], 1e-10);
-            st[t] = ((double)a.vol[i * 2 + f] + acc) / rho;
+            st[t] = ((double)a.vol[i * 2 + f] + acc / det) / rho;
             sf[t] = fabs((double)(f == 0 ? a.k[i] : a.w[i]));
         }
         wave_sync();

@@ -174,7 +174,7 @@ def served_leg(args):
     t0 = time.perf_counter()
     plain = solve_alone(reqs, dtype=torch.float32)
     log(f"[bench] plain path done in {time.perf_counter() - t0:.1f}s")
-    srv = SolveServer(dtype=torch.float32, max_active=args.serve_active)
+    srv = SolveServer(dtype=torch.float32, max_active=args.serve_active, host_workers=args.serve_host_workers)
     t0 = time.perf_counter()
     served = srv.run(reqs)
     log(f"[bench] served path done in {time.perf_counter() - t0:.1f}s (peak {srv.peak_active} resident solves)")
@@ -347,6 +347,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
     ap.add_argument("--serve-seed", type=int, default=1)
+    ap.add_argument("--serve-host-workers", type=int, default=2, help="processes building host tables")
     ap.add_argument("--batch-cases", type=int, default=16)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
