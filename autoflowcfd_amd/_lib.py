@@ -86,6 +86,10 @@ SIGNATURES = {
     "afr_forces_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
     "afr_mesh_set_grid": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32]),
     "afr_mesh_get_grid": (ctypes.c_int, [c_vp, ctypes.c_int32]),
+    "afr_graph_begin": (ctypes.c_int, [c_vp]),
+    "afr_graph_end": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "afr_graph_launch": (ctypes.c_int, [c_vp, c_vp]),
+    "afr_graph_destroy": (ctypes.c_int, [c_vp]),
     "afr_residual_norm_cases": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
     "afr_forces_cases_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
 }

@@ -533,6 +533,47 @@ int afr_mesh_destroy(AfrMesh* mesh) {
     return 0;
 }
 
+struct AfrGraph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int afr_graph_begin(void* stream) {
+    if (!stream) return fail("afr_graph_begin: capture needs a created stream, not the null stream");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+
+int afr_graph_end(void* stream, AfrGraph** out) {
+    if (!stream || !out) return fail("afr_graph_end: null stream or output");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (r != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail((std::string("afr_graph_end: hipGraphInstantiate: ") + hipGetErrorString(r)).c_str());
+    }
+    *out = new AfrGraph{g, e};
+    return 0;
+}
+
+int afr_graph_launch(AfrGraph* graph, void* stream) {
+    if (!graph) return fail("afr_graph_launch: null graph");
+    HIP_TRY(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return 0;
+}
+
+int afr_graph_destroy(AfrGraph* graph) {
+    if (!graph) return 0;
+    const hipError_t r1 = hipGraphExecDestroy(graph->exec);
+    const hipError_t r2 = hipGraphDestroy(graph->graph);
+    delete graph;
+    if (r1 != hipSuccess || r2 != hipSuccess) return fail("afr_graph_destroy: hipGraph(Exec)Destroy failed");
+    return 0;
+}
+
 int afr_primitives(const void* U, void* Q, int64_t n_points, int32_t dtype, const double* ref, void* stream) {
     if (!U || !Q || n_points < 0) return fail("afr_primitives: bad argument");
     hipStream_t st = (hipStream_t)stream;

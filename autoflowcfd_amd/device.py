@@ -68,6 +68,45 @@ def sparse_fixed(rows, widths, drop=ROW_DROP):
     return w, idx
 
 
+class LibGraph:
+    """A HIP graph of afr_* calls captured and replayed through the C ABI (afr_graph_begin /
+    afr_graph_end / afr_graph_launch), for callers that do not use torch's graphs:
+
+        g = LibGraph(dm.lib)
+        with torch.cuda.stream(s):
+            g.begin(s); dm.ssp_rk3_step(U); g.end(s)     # captured, not run
+        g.launch(s)                                       # one pseudo-step per launch
+
+    Nothing may allocate on the stream while it is captured (the DeviceMesh step calls do not)."""
+
+    def __init__(self, lib):
+        self.lib, self.handle = lib, None
+
+    def begin(self, stream):
+        _lib.check(self.lib.afr_graph_begin(ctypes.c_void_p(stream.cuda_stream)), "afr_graph_begin")
+
+    def end(self, stream):
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.afr_graph_end(ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h)), "afr_graph_end")
+        self.handle = h
+
+    def launch(self, stream):
+        if self.handle is None:
+            raise RuntimeError("LibGraph.launch: nothing captured")
+        _lib.check(self.lib.afr_graph_launch(self.handle, ctypes.c_void_p(stream.cuda_stream)), "afr_graph_launch")
+
+    def close(self):
+        if self.handle is not None:
+            _lib.check(self.lib.afr_graph_destroy(self.handle), "afr_graph_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceMesh:
     """One mesh at one order on one GPU."""
 

@@ -344,6 +344,20 @@ int afr_forces_cases_f64(const AfrMesh *mesh, const void *U, const void *mu_t, d
  * src/autoflowcfd/postprocess/fr_coefficients.py:27-152.  Cd = F_x/(q A), Cl = F_z/(q A). */
 int afr_forces_f64(const AfrMesh *mesh, const void *U, const void *mu_t, double mu, double *out, void *stream);
 
+/* HIP graph of afr_* calls (no reference counterpart: the reference launches every kernel of a
+ * pseudo-step from Python, src/autoflowcfd/core/gpu/gpu_solver.py:step).  afr_graph_begin starts
+ * capturing the work this host thread issues on `stream` (thread-local capture: other threads
+ * may keep launching); afr_graph_end stops and instantiates it; afr_graph_launch replays it on a
+ * stream with the pointers and scalars it was captured with, e.g. one afr_ssp_rk3_step or
+ * afr_sst_rk3_step per launch, one launch per pseudo-step.  The afr_* entry points allocate
+ * nothing, issue no memset or copy and do not synchronise, so each of them can be captured.
+ * `stream` must be a created stream, not the null stream. */
+typedef struct AfrGraph AfrGraph;
+int afr_graph_begin(void *stream);
+int afr_graph_end(void *stream, AfrGraph **out);
+int afr_graph_launch(AfrGraph *graph, void *stream);
+int afr_graph_destroy(AfrGraph *graph);
+
 #ifdef __cplusplus
 }
 #endif

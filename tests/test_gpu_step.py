@@ -117,3 +117,40 @@ def test_grid_sizes_do_not_change_results():
     res = autotune(dm, Ud, mut, reps=1, candidates=(0, 8, 1024))
     assert set(res) >= {"prism_inviscid", "visc_volume", "visc_iface"}
     assert torch.equal(dm.residual(Ud, mut, torch.empty_like(Ud)), ref)
+
+
+def test_c_abi_graph_replays_equal_eager_steps():
+    """One laminar pseudo-step captured through the C ABI (afr_graph_begin / afr_graph_end) and
+    replayed four times, with eager steps of a second copy of the state and host
+    synchronisations between the replays: every replay equals the eager step bit for bit
+    (state and FP64 norm)."""
+    import torch
+    from autoflowcfd_amd.device import LibGraph
+    print("[gpu] C-ABI HIP graph replay vs eager", flush=True)
+    mesh = wall_box(seed=3)
+    dm, ops, g, fl, bcd, U = _setup(mesh, 2, torch.float32)
+    torch.cuda.synchronize()                            # the mesh set-up ran on the default stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        UA = torch.as_tensor(U, dtype=torch.float32, device="cuda").contiguous()
+        UB = UA.clone()
+        nB = torch.zeros(1, dtype=torch.float64, device="cuda")
+        dm.ssp_rk3_step(UA, None, mu=1.8e-5)            # first use outside the graph
+        dm.ssp_rk3_step(UB, None, mu=1.8e-5, norm=nB)
+    s.synchronize()
+    assert torch.equal(UA, UB)
+    graph = LibGraph(dm.lib)
+    with torch.cuda.stream(s):
+        graph.begin(s)
+        dm.ssp_rk3_step(UA, None, mu=1.8e-5)            # captured, not run
+        graph.end(s)
+    s.synchronize()
+    assert torch.equal(UA, UB)
+    for r in range(4):
+        graph.launch(s)
+        with torch.cuda.stream(s):
+            dm.ssp_rk3_step(UB, None, mu=1.8e-5, norm=nB)
+        s.synchronize()
+        assert torch.equal(UA, UB), r
+        assert torch.equal(dm.norm_dev, nB), r
+    graph.close()
