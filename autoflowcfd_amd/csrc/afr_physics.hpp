@@ -84,7 +84,8 @@ template <typename real>
 __host__ __device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {
     const real g1 = real(kGamma - 1.0);
     real rho = rmax(U[0], real(1e-10));
-    real u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, E = U[4] / rho;
+    const real ir = recip(rho);   // one reciprocal instead of four divisions
+    real u = U[1] * ir, v = U[2] * ir, w = U[3] * ir, E = U[4] * ir;
     Q[0] = rho; Q[1] = u; Q[2] = v; Q[3] = w;
     Q[4] = g1 * rho * (E - real(0.5) * (u * u + v * v + w * w));
 }

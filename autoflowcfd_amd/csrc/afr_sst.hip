@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             const double gdot = gk[0] * gw[0] + gk[1] * gw[1] + gk[2] * gw[2];
             // sources (mu = 1.8e-5 as the reference's source terms)
             const double musrc = 1.8e-5;
-            const double nu = musrc / dmax(rho, 1e-10);
+            const double irho = recip(dmax(rho, 1e-10));   // one reciprocal for the four uses
+            const double nu = musrc * irho;
             const double ks = dmin(k, 1e40), wsf = dmax(dmin(w, 1e40), 1e-10);
             const double CD = dmax(2.0 * rho * SW2 / wsf * gdot, 1e-10);
             const double F1 = Blend<real>::F1(ks, wsf, d, nu, rho, CD);
@@ -272,7 +273,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             {
                 const double kk = dmax(ks, 1e-10), ww = dmax(wsf, 1e-10), SS = dmax(S, 1e-10);
                 nut = A1 * kk / dmax(A1 * ww, F2 * SS);
-                nut = dmin(nut, 1.0e5 * musrc / dmax(rho, 1e-10));
+                nut = dmin(nut, 1.0e5 * musrc * irho);
                 nut = isfinite(nut) ? nut : 0.0;
             }
             const double Pk = dmin(nut * rho * S * S, 10.0 * BSTAR * rho * ks * wsf);
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             double Sw = rho * gam * S * S - rho * beta * wsf * wsf + 2.0 * rho * (1.0 - F1) * SW2 / wsf * gdot;
             Sk = finite_or(Sk, 0.0);
             Sw = finite_or(Sw, 0.0);
-            const double ir = 1.0 / dmax(rho, 1e-10);
+            const double ir = irho;
             a.src[i * 3 + 0] = (real)(Sk * ir);
             a.src[i * 3 + 1] = (real)(Sw * ir);
             a.src[i * 3 + 2] = (real)beta;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             // diffusivities with the transport's own F1 (molecular mu, raw k / omega)
             const double wsaf = dmax(w, 1e-10);
             const double CDt = dmax(2.0 * rho * SW2 / wsaf * gdot, 1e-10);
-            const double F1t = Blend<real>::F1(k, w, d, mu / dmax(rho, 1e-10), rho, CDt);
+            const double F1t = Blend<real>::F1(k, w, d, mu * irho, rho, CDt);
             const double rnut = rho * nut;
             const double Gk = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
             const double Gw = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;

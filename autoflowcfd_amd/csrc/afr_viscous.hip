@@ -148,12 +148,14 @@ template <int P, int CT> struct DW {
     static constexpr int WM = W2 > W1 ? W2 : W1;
 };
 
-// LDS per wave of the volume kernel: pack [s][kVP], grad [s][12], G~ [m][s][4]; |U| [s][5] and
-// the gradient fields [s][4] live in the G~ space until G~ is written (measured: 2.15 -> 1.91 ms
-// on the 82^3 P2 mesh from the occupancy this frees; two cells per wave instead measured 2.19 ms)
+// LDS per wave of the volume kernel: pack [s][kVP] (which also holds the gradients) and
+// G~ [m][s][4]; |U| [s][5] and the gradient fields [s][4] live in the G~ space until G~ is
+// written.  Occupancy is bounded by this LDS: measured on the 82^3 P2 mesh 2.15 ms with separate
+// |U|, field and gradient arrays, 1.91 ms with |U| and the fields in the G~ space (two cells per
+// wave instead measured 2.19 ms).
 template <int P, int CT> struct VolPlan {
     static constexpr int NS = (P + 1) * (P + 1) * (P + 1);
-    static constexpr int PER = NS * kVP + 12 * NS + 12 * NS;
+    static constexpr int PER = NS * kVP + 12 * NS;
 };
 
 template <int P, int CT, typename real>
@@ -182,8 +184,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
     if (tid < 5) s_ref[tid] = a.qref[tid];
     __syncthreads();
     real* const sPk = s_w[wid];               // interface pack [s][kVP], written out coalesced
-    real* const sG = sPk + NS * kVP;          // grad [s][12]
-    real* const sGt = sG + 12 * NS;           // G~ [m][s][4] (v = 1..4)
+    real* const sGt = sPk + NS * kVP;         // G~ [m][s][4] (v = 1..4)
     real* const sU = sGt;                     // |U| [s][5]            (until G~ is written)
     real* const sF = sGt + NSV;               // (du, dv, dw, dT) [s][4] (until G~ is written)
     const double q0r = s_ref[0], q0p = s_ref[4];
@@ -248,11 +249,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
             const real idet = recip(a.det[cell * NS + s]);
             real* vp = sPk + s * kVP;
 #pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const real g = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
-                sG[s * 12 + f * 3 + n] = g;
-                vp[5 + f * 3 + n] = g;
-            }
+            for (int n = 0; n < 3; ++n)
+                vp[5 + f * 3 + n] = (ad[0 * 3 + n] * d[0] + ad[1 * 3 + n] * d[1] + ad[2 * 3 + n] * d[2]) * idet;
         }
         wave_sync();
         t = lane_fresh(t);
@@ -275,8 +273,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) gv[i][j] = sG[s * 12 + i * 3 + j];
-                gT[i] = sG[s * 12 + 9 + i];
+                for (int j = 0; j < 3; ++j) gv[i][j] = sPk[s * kVP + 5 + i * 3 + j];
+                gT[i] = sPk[s * kVP + 14 + i];
             }
             viscous_flux(q, gv, gT, (real)a.mu, (real)a.Pr, mut_s, (real)a.Prt, G);
             const real* ad = a.adj + (cell * NS + s) * 9;
