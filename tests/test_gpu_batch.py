@@ -96,3 +96,39 @@ def test_laminar_mixed_batch_graph_equals_alone():
              Case(mesh=tet_box(3, seed=1), Q_free=freestream(0.1, 4.0)),
              Case(mesh=wall_box(n=(3, 2, 2), seed=3), Q_free=freestream(0.3, 9.0))]
     _run_pair(cases, 2, torch.float32, "NONE", 3, graph=True)
+
+
+def test_batched_cd_matches_oracle_per_case():
+    """The batched multi-case configuration's report: per-case Cd / Cl against the FP64 oracle.
+    Three laminar P2 wall boxes of different sizes, Mach and yaw in one FP64 batch, each given its
+    own smooth perturbed state (a skin friction and a pressure field to integrate): the batched
+    per-case wall forces (afr_forces_cases_f64) against the oracle's wall-force integral of each
+    case alone, to 1e-10 of the force scale.  (Pseudo-steps of the batch are tied to the oracle
+    through the bitwise batch-equals-alone tests above and the single-case step tests.)"""
+    import torch
+    from autoflowcfd_amd.batch import Case, CaseBatch, free_stream_coefficients
+    from cases import smooth_state
+    from oracle import fr_ops
+    from oracle.faces import build_flat_faces
+    from oracle.forces import wall_forces
+    from oracle.geometry import Geometry
+    print("[gpu] CaseBatch P2 fp64 per-case Cd vs oracle", flush=True)
+    spec = [((4, 3, 2), 0.15, 0.0), ((3, 2, 2), 0.25, 7.0), ((4, 2, 3), 0.08, 3.0)]
+    cases = [Case(mesh=wall_box(n=n, seed=40 + j), Q_free=freestream(m, y)) for j, (n, m, y) in enumerate(spec)]
+    b = CaseBatch(cases, order=2, dtype=torch.float64, turb_model="NONE")
+    ops = fr_ops.generate_operators(2)
+    states = []
+    for i, c in enumerate(cases):
+        g = Geometry(c.mesh, ops)
+        U = smooth_state(g.sps_xyz, c.Q_free, amp=2e-2, seed=50 + i)
+        states.append((g, build_flat_faces(c.mesh, ops), U))
+        idx = torch.as_tensor(b.case_cells(i), device=b.U.device)
+        b.U[idx] = torch.as_tensor(U, dtype=torch.float64, device=b.U.device)
+    coef = b.coefficients()
+    for i, (c, (g, fl, U)) in enumerate(zip(cases, states)):
+        Fp, Fv = wall_forces(U, g, ops, fl, 1.8e-5)
+        ref = free_stream_coefficients(np.concatenate([Fp, Fv]), c.Q_free, c.reference_area)
+        scale = max(abs(ref["Cd"]), abs(ref["Cl"]), abs(ref["Cs"]))
+        assert scale > 0
+        for key in ("Cd", "Cl", "Cs"):
+            assert abs(coef[i][key] - ref[key]) <= 1e-10 * scale, (i, key, coef[i][key], ref[key])
