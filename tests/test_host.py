@@ -257,3 +257,22 @@ def test_renumbered_mesh_keeps_the_residual():
         U2[new_of_old] = U
         R2 = R.inviscid_residual(U2, Geometry(m2, ops), ops, build_flat_faces(m2, ops), bcd)
         assert np.abs(R2[new_of_old] - R1).max() <= 1e-12 * np.abs(R1).max()
+
+
+def test_graph_entry_points_reject_bad_arguments_without_a_gpu():
+    """The C-ABI graph calls validate their arguments before any HIP call: a null stream, output
+    or graph is an error with a message (CPU-checkable), and destroying a null graph is a no-op."""
+    import ctypes
+    import pytest
+    from autoflowcfd_amd import _lib
+    lib = _lib.load()
+    with pytest.raises(RuntimeError, match="null stream"):
+        _lib.check(lib.afr_graph_begin(None), "afr_graph_begin")
+    with pytest.raises(RuntimeError, match="null stream or output"):
+        _lib.check(lib.afr_graph_end(None, None), "afr_graph_end")
+    with pytest.raises(RuntimeError, match="null graph"):
+        _lib.check(lib.afr_graph_launch(None, None), "afr_graph_launch")
+    assert lib.afr_graph_destroy(None) == 0
+    from autoflowcfd_amd.device import LibGraph
+    with pytest.raises(RuntimeError, match="nothing captured"):
+        LibGraph(lib).launch(None)
