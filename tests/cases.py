@@ -27,6 +27,20 @@ def wall_box(n=(4, 4, 3), L=(4.0, 1.0, 1.0), grading=1.1, seed=0):
     return hex_box(*n, *L, split="prism", grading_y=grading, seed=seed, bc_by_plane=WALL_BOX_BC)
 
 
+def odd_prism_mesh(seed=5):
+    """A graded wall box of 3 x 2 x 2 hex with its last prism removed: 23 prisms (an odd count,
+    so kernels that take two prisms per wavefront meet a lone last prism).  Boundary records on
+    y = 0 are WALL, every other boundary record (the removed prism's faces included) FARFIELD."""
+    from autoflowcfd_amd.mesh import BC_FARFIELD, mesh_from_cells
+    src = wall_box(n=(3, 2, 2), seed=seed)
+    m = mesh_from_cells(src.nodes, src.prism_conn[:-1], default_bc=BC_FARFIELD)
+    fr = m.faces
+    on_wall = fr.is_boundary & (np.abs(fr.center[:, 1]) < 1e-12)
+    fr.bc[on_wall] = BC_WALL
+    m.wall_nodes = np.unique(fr.node_ids[on_wall])
+    return m
+
+
 def tet_box(n=3, seed=0, displacement=0.1, periodic=False):
     return hex_box(n, n, n, split="tet", displacement=displacement, seed=seed, periodic=periodic)
 
