@@ -440,6 +440,20 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
     for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
         const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         t = lane_fresh(t);
+        // balanced lanes: the main lanes' slot tables are loaded with the own pack (independent
+        // addresses, one memory round trip for both)
+        int m_bc = BC_INTERIOR, m_mask = 0, m_blk0 = -1, m_blk1 = -1, m_nb0 = -1, m_nb1 = -1;
+        if constexpr (B::ON) {
+            if (t < B::NM) {
+                const int64_t sidx = cell * 5 + t / NFP;
+                m_bc = a.slot_bc[sidx];
+                m_mask = a.slot_mask[sidx];
+                m_blk0 = a.slot_blk[sidx * 2];
+                m_blk1 = a.slot_blk[sidx * 2 + 1];
+                m_nb0 = a.slot_nb[sidx * 2];
+                m_nb1 = a.slot_nb[sidx * 2 + 1];
+            }
+        }
         for (int o = t; o < NS * kVP; o += 64) sP[o] = a.vpack[cell * NS * kVP + o];
         wave_sync();
         t = lane_fresh(t);
@@ -456,11 +470,11 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
             int64_t nnb = 0;
             bool live = false;
             if (is_main) {
-                bc = a.slot_bc[sidx];
+                bc = m_bc;
                 if (bc == BC_INTERIOR) {
-                    const int src = (a.slot_mask[sidx] >> fp) & 1;
-                    const int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
-                    const int nbc = a.slot_nb[sidx * 2 + src];
+                    const int src = (m_mask >> fp) & 1;
+                    const int row = (src ? m_blk1 : m_blk0) * NFP + fp;
+                    const int nbc = src ? m_nb1 : m_nb0;
                     live = nbc >= 0 && row >= 0 && row < a.n_rows;
                     if (live) {
                         nrow = row;
@@ -474,15 +488,17 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
             const unsigned long long wm = __ballot(wide);
             const int wslot = wm ? (int)((__ffsll((long long)wm) - 1) / NFP) : -1;
             const bool use_help = is_main && live && slot == wslot;
-            if (is_help && wslot >= 0) {
-                const int64_t ws = cell * 5 + wslot;
-                const int src = (a.slot_mask[ws] >> fp) & 1;
-                const int row = a.slot_blk[ws * 2 + src] * NFP + fp;
-                const int nbc = a.slot_nb[ws * 2 + src];
-                live = src == 0 && nbc >= 0 && row >= 0 && row < a.n_rows;
-                if (live) {
-                    nrow = row;
-                    nnb = nbc;
+            // helpers take the helped slot's row and source cell from its main lane of the same
+            // flux point (cross-lane, no memory round trip)
+            {
+                const int from = (wslot >= 0 ? wslot : 0) * NFP + fp;
+                const int h_live = __shfl(live ? 1 : 0, from);
+                const int h_row = __shfl(nrow, from);
+                const int h_nb = __shfl((int)nnb, from);
+                if (is_help && wslot >= 0 && h_live) {
+                    live = true;
+                    nrow = h_row;
+                    nnb = h_nb;
                 }
             }
             const bool has_nb = is_main ? (bc == BC_INTERIOR && nrow < a.n_rows) : live;
