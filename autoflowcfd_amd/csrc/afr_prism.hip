@@ -342,6 +342,17 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             const int nb = __builtin_amdgcn_readlane(nbv, 2 * j);
             base[1 + j] = nb >= 0 ? (int64_t)nb * CELL_DW : cell * CELL_DW;
         }
+        // the interface lanes' slot tables, loaded with the own cell (one memory round trip)
+        int m_bc = BC_INTERIOR, m_mask = 0, m_blk0 = -1, m_blk1 = -1, m_nb0 = -1, m_nb1 = -1;
+        if (t < 5 * NFP) {
+            const int64_t sidx = cell * 5 + t / NFP;
+            m_bc = a.slot_bc[sidx];
+            m_mask = a.slot_mask[sidx];
+            m_blk0 = a.slot_blk[sidx * 2];
+            m_blk1 = a.slot_blk[sidx * 2 + 1];
+            m_nb0 = a.slot_nb[sidx * 2];
+            m_nb1 = a.slot_nb[sidx * 2 + 1];
+        }
         dma_cells<L::QN_DW / 64, CELL_DW>(dQw, base, sQNw, t);
         vm_wait_all();
         wave_sync();
@@ -462,7 +473,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
         if (t < 5 * NFP) {
             const int slot = t / NFP, fp = t % NFP;
             const int64_t sidx = cell * 5 + slot;
-            const int bc = a.slot_bc[sidx];
+            const int bc = m_bc;
             const int code = slot < 3 ? slot : (slot == 3 ? 4 : 5);
             const double side = (code & 1) ? 1.0 : -1.0;
             real dqo[5] = {0, 0, 0, 0, 0};
@@ -488,9 +499,9 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
                             (double)a.slot_tn[sidx * 3 + 2]};
             if (bc == BC_INTERIOR) {
-                const int src = (a.slot_mask[sidx] >> fp) & 1;
-                int row = a.slot_blk[sidx * 2 + src] * NFP + fp;
-                const int nbc = a.slot_nb[sidx * 2 + src];
+                const int src = (m_mask >> fp) & 1;
+                int row = (src ? m_blk1 : m_blk0) * NFP + fp;
+                const int nbc = src ? m_nb1 : m_nb0;
                 if (row < 0 || row >= a.n_rows || nbc < 0) row = a.n_rows;  // all-zero row
                 int sp[KMAX];
                 real rw[KMAX];

@@ -231,8 +231,12 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
             for (int s = 0; s < NS; ++s) acc += a.det[cell * NS + s];
             a.h_ip[cell] = (real)cbrt((double)(acc / real(NS)));
         }
-        // 2. gradients: lane (s, field), sparse derivative rows
-        for (int o = t; o < 4 * NS; o += 64) {
+        // 2. gradients: lane (s, field), sparse derivative rows (fixed trip count, unrolled: the
+        //    metric loads of every pass are issued together)
+#pragma unroll
+        for (int jo = 0; jo < (4 * NS + 63) / 64; ++jo) {
+            const int o = t + 64 * jo;
+            if (o >= 4 * NS) break;
             const int s = o >> 2, f = o & 3;
             real d[3];
 #pragma unroll
@@ -287,7 +291,10 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
         wave_sync();
         t = lane_fresh(t);
         // 4. volume term R_vol = sum_m D_m G~_m / det
-        for (int o = t; o < NSV; o += 64) {
+#pragma unroll
+        for (int jo = 0; jo < (NSV + 63) / 64; ++jo) {
+            const int o = t + 64 * jo;
+            if (o >= NSV) break;
             const int s = o / 5, v = o % 5;
             real acc = 0;
             if (v > 0) {
@@ -611,27 +618,35 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
         }
         wave_sync();
         t = lane_fresh(t);
-        // 2. distribution, / det, + volume term
-        for (int o = t; o < NSV; o += 64) {
-            const int s = o / 5, v = o % 5;
-            real corr = 0;
+        // 2. distribution, / det, + volume term; the accumulated R is read here too, so that its
+        //    latency overlaps the outlier phase instead of ending the cell
+        constexpr int NO = (NSV + 63) / 64;
+        real r_old[NO];
 #pragma unroll
-            for (int slot = 0; slot < NSLOT; ++slot) {
-                const int code = slot_code(slot), axis = code >> 1;
-                const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
-                corr += g * sJ[(slot * NFP + s_fp[axis * NS + s]) * 5 + v];
+        for (int j = 0; j < NO; ++j) {
+            const int o = t + 64 * j;
+            r_old[j] = real(0);
+            if (o < NSV) {
+                const int s = o / 5, v = o % 5;
+                if (a.accumulate) r_old[j] = a.R[cell * NSV + o];
+                real corr = 0;
+#pragma unroll
+                for (int slot = 0; slot < NSLOT; ++slot) {
+                    const int code = slot_code(slot), axis = code >> 1;
+                    const real g = s_g[((code & 1) ? N1 : 0) + s_co[axis * NS + s]];
+                    corr += g * sJ[(slot * NFP + s_fp[axis * NS + s]) * 5 + v];
+                }
+                sR[o] = a.rvol[cell * NSV + o] + corr * recip(a.det[cell * NS + s]);
             }
-            sR[o] = a.rvol[cell * NSV + o] + corr * recip(a.det[cell * NS + s]);
         }
         wave_sync();
         t = lane_fresh(t);
         if (a.suppress) outlier_zeroing<NS, real>(sR, a.umean + cell * 5, t);
         wave_sync();
-        for (int o = t; o < NSV; o += 64) {
-            if (a.accumulate)
-                a.R[cell * NSV + o] += sR[o];
-            else
-                a.R[cell * NSV + o] = sR[o];
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            const int o = t + 64 * j;
+            if (o < NSV) a.R[cell * NSV + o] = a.accumulate ? r_old[j] + sR[o] : sR[o];
         }
         wave_sync();
     }
