@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             base[1 + j] = nb >= 0 ? (int64_t)nb * CELL_DW : cell * CELL_DW;
         }
         // the interface lanes' slot tables, loaded with the own cell (one memory round trip)
-        int m_bc = BC_INTERIOR, m_mask = 0, m_blk0 = -1, m_blk1 = -1, m_nb0 = -1, m_nb1 = -1;
+        int m_bc = BC_INTERIOR, m_mask = 0, m_blk0 = -1, m_blk1 = -1, m_nb0 = -1, m_nb1 = -1, m_fb = 0;
+        real m_tn[3] = {0, 0, 0}, m_ar[3] = {0, 0, 0};
         if (t < 5 * NFP) {
             const int64_t sidx = cell * 5 + t / NFP;
             m_bc = a.slot_bc[sidx];
@@ -352,6 +353,13 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             m_blk1 = a.slot_blk[sidx * 2 + 1];
             m_nb0 = a.slot_nb[sidx * 2];
             m_nb1 = a.slot_nb[sidx * 2 + 1];
+            m_fb = a.slot_fb[sidx];
+            const real* ar = a.slot_adjrow + (sidx * NFP + t % NFP) * 3;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                m_tn[q] = a.slot_tn[sidx * 3 + q];
+                m_ar[q] = ar[q];
+            }
         }
         dma_cells<L::QN_DW / 64, CELL_DW>(dQw, base, sQNw, t);
         vm_wait_all();
@@ -496,8 +504,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             double qo[5], qn[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) qo[v] = s_ref[v] + (double)dqo[v];
-            double tn[3] = {(double)a.slot_tn[sidx * 3 + 0], (double)a.slot_tn[sidx * 3 + 1],
-                            (double)a.slot_tn[sidx * 3 + 2]};
+            double tn[3] = {(double)m_tn[0], (double)m_tn[1], (double)m_tn[2]};
             if (bc == BC_INTERIOR) {
                 const int src = (m_mask >> fp) & 1;
                 int row = (src ? m_blk1 : m_blk0) * NFP + fp;
@@ -529,12 +536,11 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
             } else {
                 ghost_state(bc, qo, tn, a.bcp + (a.cell_case ? 40 * a.cell_case[cell] : 0), qn);
             }
-            const real* ar = a.slot_adjrow + (sidx * NFP + fp) * 3;
-            const double A0 = ar[0], A1 = ar[1], A2 = ar[2];
+            const double A0 = m_ar[0], A1 = m_ar[1], A2 = m_ar[2];
             double mag = sqrt_r(A0 * A0 + A1 * A1 + A2 * A2);
             mag = mag > 1e-300 ? mag : 1e-300;
             double d[3];
-            if ((a.slot_fb[sidx] >> fp) & 1) {
+            if ((m_fb >> fp) & 1) {
                 d[0] = tn[0]; d[1] = tn[1]; d[2] = tn[2];
             } else {
                 const double im = side * recip(mag);
