@@ -183,8 +183,16 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
         const bool live = h == 0 || pos0 + 1 < a.n_cells;
         const int64_t cell = a.cell_begin + (h == 0 ? c0 : c1);
         const int64_t cb = cell * NS;                // first solution point of the cell
+        // the lane's wall distance and adjugate (used by the same lane in the point phase) are
+        // loaded here with the state
+        real d_s = real(0), adj_s[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) adj_s[q] = real(0);
         if (live && tl < NS) {
             const int64_t i = cb + tl;
+            d_s = a.d_wall[i];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) adj_s[q] = a.adj[i * 9 + q];
             real u5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) u5[v] = a.U[i * 5 + v];
@@ -231,7 +239,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             const real* r = sv + s * NF;
             const real* gs = sg + s * G_;
             const double rho = (double)r[0], k = (double)r[4], w = (double)r[5];
-            const double d = (double)a.d_wall[i];
+            const double d = (double)d_s;
             // strain rate of the momentum gradient
             double S2 = 0.0;
 #pragma unroll
@@ -307,7 +315,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
             nbp[3] = (real)Gw;
             // contravariant transport fluxes (raw gradients in the diffusion), convective +
             // diffusive summed per field: sF[s][f][m]
-            const real* ad = a.adj + i * 9;
+            const real* ad = adj_s;
             const real rr = r[0];
             const real ru[3] = {rr * r[1], rr * r[2], rr * r[3]};
             const real gkr = (real)Gk, gwr = (real)Gw, kr = r[4], wr = r[5];

@@ -194,10 +194,16 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
     for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
         const int64_t cell = a.cell_begin + walk_cell(a.schedule, cw.it);
         t = lane_fresh(t);
-        // 1. primitives, perturbations, |U|
+        // 1. primitives, perturbations, |U|; the lane's adjugate (used in stage 3 by the same
+        //    lane) is loaded here with the state
         real mut_s = real(0);
+        real adj_s[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) adj_s[q] = real(0);
         if (t < NS) {
             const int s = t;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) adj_s[q] = a.adj[(cell * NS + s) * 9 + q];
             double u[5], q[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) u[v] = (double)a.U[(cell * NS + s) * 5 + v];
@@ -281,7 +287,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
                 gT[i] = sPk[s * kVP + 14 + i];
             }
             viscous_flux(q, gv, gT, (real)a.mu, (real)a.Pr, mut_s, (real)a.Prt, G);
-            const real* ad = a.adj + (cell * NS + s) * 9;
+            const real* ad = adj_s;
 #pragma unroll
             for (int m = 0; m < 3; ++m)
 #pragma unroll
