@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
                 }
             }
             // the first main lane with a wide first-source neighbour row names the helped slot
-            const bool wide = live && a.rows_w[(int64_t)nrow * KMAX + N1] != real(0);
+            const bool wide = live && ((m_mask >> 30) & 1);   // host-set: first-source rows wider than N1
             const unsigned long long wm = __ballot(wide);
             const int wslot = wm ? (int)((__ffsll((long long)wm) - 1) / NFP) : -1;
             const bool use_help = is_main && live && slot == wslot;

@@ -137,11 +137,22 @@ class DeviceMesh:
             return torch.as_tensor(np.ascontiguousarray(x, dtype=np.int32)).to(self.device)
 
         self.slot_nb, self.slot_blk = devi(sl.nb), devi(sl.blk)
-        self.slot_mask, self.slot_bc = devi(sl.mask), devi(sl.bc)
+        self.slot_bc = devi(sl.bc)
         self.slot_tn = dev(sl.tn)
         self.rows = dev(sl.rows)
         # sparse rows plus one trailing all-zero row (index n_rows) for lanes without a source
         nnz, idx, w = sparse_rows(np.concatenate([sl.rows, np.zeros((1, sl.rows.shape[1]))]))
+        # slot mask: bit fp = flux point fp takes source 1; bit 30 = the slot's first-source rows
+        # are wider than order + 1 (the viscous interface kernel's helper lanes need no row load
+        # to find them)
+        mask = np.asarray(sl.mask, dtype=np.int64).copy()
+        if T.nfp > 30:
+            raise ValueError("slot masks hold at most 30 flux points")
+        blk0 = np.asarray(sl.blk)[..., 0]
+        n_blk = max(int(sl.n_blocks), 1)
+        blk_wide = nnz[:n_blk * T.nfp].reshape(-1, T.nfp).max(axis=1) > order + 1 if sl.n_blocks else np.zeros(1, bool)
+        wide = (blk0 >= 0) & blk_wide[np.clip(blk0, 0, n_blk - 1)]
+        self.slot_mask = devi(mask | (wide.astype(np.int64) << 30))
         # fixed row width: (P+1)^2 when every row fits (prism sources), else (P+1)^3, zero-padded,
         # so the kernels read a compile-time number of entries per row without a bound check
         width = (order + 1) ** 2 if idx.shape[1] <= (order + 1) ** 2 else T.ns
