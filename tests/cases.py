@@ -41,6 +41,20 @@ def odd_prism_mesh(seed=5):
     return m
 
 
+def rotated_prism_box(seed=0):
+    """A 4 x 3 x 3 hex wall box split into prisms whose vertex order is rotated at random within
+    each prism: which quad is a prism's collapsed-direction (b) face then varies from cell to cell,
+    so cells meet up to four neighbours through their b faces (up to four slots with 9-wide
+    neighbour rows; the unrotated box has at most one).  All boundaries FARFIELD."""
+    from autoflowcfd_amd.mesh import BC_FARFIELD, mesh_from_cells
+    src = wall_box(n=(4, 3, 3), seed=3)
+    rng = np.random.default_rng(seed)
+    pc = src.prism_conn.copy()
+    for i, r in enumerate(rng.integers(0, 3, len(pc))):
+        pc[i] = np.concatenate([np.roll(pc[i, :3], r), np.roll(pc[i, 3:], r)])
+    return mesh_from_cells(src.nodes, pc, default_bc=BC_FARFIELD)
+
+
 def tet_box(n=3, seed=0, displacement=0.1, periodic=False):
     return hex_box(n, n, n, split="tet", displacement=displacement, seed=seed, periodic=periodic)
 
