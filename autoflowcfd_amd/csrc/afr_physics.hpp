@@ -13,13 +13,14 @@
 // register allocation to the compiler; n > 0 asks for at least n waves per SIMD (the compiler
 // then budgets registers for it, spilling if it must).  Measured on the 82^3 P2 SST residual:
 // the interface kernel 3.52 (7 waves, 28 B/lane spilled), 3.22 (6 waves, no spill), 3.55 ms
-// (compiler's choice, 5 waves); the SST update 8 waves best (SST 6.93 -> 6.34 ms); the SST cell,
-// inviscid and viscous volume kernels are bounded by LDS or already at 8.
+// (compiler's choice, 5 waves); the SST update 3.90 (8 waves, 56 B/lane spilled), 3.83 (7 waves,
+// 20 B), 3.88 ms (6 waves, no spill) for the whole SST phase; the SST cell, inviscid and viscous
+// volume kernels are bounded by LDS or already at their register limit.
 #ifndef AFR_WPE_SST_CELL
 #define AFR_WPE_SST_CELL 0
 #endif
 #ifndef AFR_WPE_SST_UPDATE
-#define AFR_WPE_SST_UPDATE 8
+#define AFR_WPE_SST_UPDATE 7
 #endif
 #ifndef AFR_WPE_VISC_IFACE
 #define AFR_WPE_VISC_IFACE 6
