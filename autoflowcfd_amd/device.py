@@ -34,6 +34,21 @@ def sparse_rows(rows, drop=ROW_DROP):
     return nnz, idx, w
 
 
+def slot_mask_with_wide_flag(mask, blk0, rows_nnz, n_blocks, nfp, order):
+    """Slot masks (bit fp: flux point fp takes source 1) with bit 30 set on the slots whose
+    first-source row block (blk0, -1 for none) has a row with more than order + 1 non-zeros
+    (rows_nnz: non-zeros of every row, blocks of nfp rows)."""
+    if nfp > 30:
+        raise ValueError("slot masks hold at most 30 flux points")
+    mask = np.asarray(mask, dtype=np.int64)
+    blk0 = np.asarray(blk0, dtype=np.int64)
+    if n_blocks <= 0:
+        return mask.copy()
+    blk_wide = np.asarray(rows_nnz)[:n_blocks * nfp].reshape(n_blocks, nfp).max(axis=1) > order + 1
+    wide = (blk0 >= 0) & blk_wide[np.clip(blk0, 0, n_blocks - 1)]
+    return mask | (wide.astype(np.int64) << 30)
+
+
 def bc_params_table(Q_free, Q_inlet=None, p_outlet=None):
     """(8,5) per-BC-code reference states (src/autoflowcfd/core/fr_solver/boundary.py:52-145)."""
     P = np.zeros((8, 5))
@@ -145,14 +160,8 @@ class DeviceMesh:
         # slot mask: bit fp = flux point fp takes source 1; bit 30 = the slot's first-source rows
         # are wider than order + 1 (the viscous interface kernel's helper lanes need no row load
         # to find them)
-        mask = np.asarray(sl.mask, dtype=np.int64).copy()
-        if T.nfp > 30:
-            raise ValueError("slot masks hold at most 30 flux points")
-        blk0 = np.asarray(sl.blk)[..., 0]
-        n_blk = max(int(sl.n_blocks), 1)
-        blk_wide = nnz[:n_blk * T.nfp].reshape(-1, T.nfp).max(axis=1) > order + 1 if sl.n_blocks else np.zeros(1, bool)
-        wide = (blk0 >= 0) & blk_wide[np.clip(blk0, 0, n_blk - 1)]
-        self.slot_mask = devi(mask | (wide.astype(np.int64) << 30))
+        self.slot_mask = devi(slot_mask_with_wide_flag(sl.mask, np.asarray(sl.blk)[..., 0], nnz, int(sl.n_blocks),
+                                                       T.nfp, order))
         # fixed row width: (P+1)^2 when every row fits (prism sources), else (P+1)^3, zero-padded,
         # so the kernels read a compile-time number of entries per row without a bound check
         width = (order + 1) ** 2 if idx.shape[1] <= (order + 1) ** 2 else T.ns

@@ -276,3 +276,24 @@ def test_graph_entry_points_reject_bad_arguments_without_a_gpu():
     from autoflowcfd_amd.device import LibGraph
     with pytest.raises(RuntimeError, match="nothing captured"):
         LibGraph(lib).launch(None)
+
+
+def test_slot_mask_wide_flag():
+    """Bit 30 of the slot mask marks slots whose first-source row block has a row wider than
+    order + 1; the per-flux-point source bits are kept; slots without a source stay unflagged."""
+    import numpy as np
+    from autoflowcfd_amd.device import slot_mask_with_wide_flag
+    nfp, order = 9, 2
+    nnz = np.array([3] * 9 + [3] * 8 + [9] + [0])          # block 0 narrow, block 1 one wide row, + zero row
+    mask = np.array([[0, 5, 0], [1, 0, 0]])
+    blk0 = np.array([[0, 1, -1], [1, 0, -1]])
+    out = slot_mask_with_wide_flag(mask, blk0, nnz, 2, nfp, order)
+    assert out.tolist() == [[0, 5 | (1 << 30), 0], [1 | (1 << 30), 0, 0]]
+    assert ((out & ((1 << 30) - 1)) == mask).all()
+    m = wall_box(seed=1)
+    from autoflowcfd_amd import face_slots
+    from autoflowcfd_amd.device import sparse_rows
+    sl = face_slots.slots_for(m, 2)
+    nz, _, _ = sparse_rows(np.concatenate([sl.rows, np.zeros((1, sl.rows.shape[1]))]))
+    flags = (slot_mask_with_wide_flag(sl.mask, np.asarray(sl.blk)[..., 0], nz, sl.n_blocks, 9, 2) >> 30) & 1
+    assert flags.sum(axis=1).max() <= 1 and flags.any()     # a graded prism box: at most one wide slot per cell
