@@ -429,11 +429,24 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
     __shared__ int s_fp[3 * NS], s_co[3 * NS];
     __shared__ __attribute__((aligned(16))) real s_w[kWaves][NS * kVP + 5 * NFP * 5 + NSV];
     __shared__ double s_ref[5];
+    // balanced lanes: the neighbour interpolation rows in LDS when the mesh has few row blocks
+    // (deduplicated, e.g. 8 blocks for a graded prism box; a batch of cases has more and reads
+    // them from global memory)
+    constexpr int ROWS_CAP = B::ON ? 8 * NFP + 1 : 1;
+    __shared__ real s_rw[ROWS_CAP * KMAX];
+    __shared__ int s_ri[ROWS_CAP * KMAX];
+    const bool rows_lds = B::ON && a.n_rows + 1 <= ROWS_CAP;
     const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int t = tid & 63;
     for (int i = tid; i < 5 * NFP * KE; i += 64 * kWaves) {
         s_Ew[i] = a.Esp_w[i];
         s_Ei[i] = a.Esp_i[i];
+    }
+    if (rows_lds) {
+        for (int i = tid; i < (a.n_rows + 1) * KMAX; i += 64 * kWaves) {
+            s_rw[i] = a.rows_w[i];
+            s_ri[i] = a.rows_idx[i];
+        }
     }
     for (int i = tid; i < 3 * NS; i += 64 * kWaves) {
         s_fp[i] = a.fp_of_sp[i];
@@ -541,8 +554,23 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
                     const int* const ri = a.rows_idx + (int64_t)nrow * KMAX;
                     const real* const rw = a.rows_w + (int64_t)nrow * KMAX;
                     const int k0 = part * N1;
+                    int ik[N1];
+                    real wk[N1];
+                    if (rows_lds) {
 #pragma unroll
-                    for (int j = 0; j < N1; ++j) add_pack_point(pn, rw[k0 + j], a.vpack + (nnb * NS + ri[k0 + j]) * kVP);
+                        for (int j = 0; j < N1; ++j) {
+                            ik[j] = s_ri[nrow * KMAX + k0 + j];
+                            wk[j] = s_rw[nrow * KMAX + k0 + j];
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < N1; ++j) {
+                            ik[j] = ri[k0 + j];
+                            wk[j] = rw[k0 + j];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) add_pack_point(pn, wk[j], a.vpack + (nnb * NS + ik[j]) * kVP);
                     if (is_main && !use_help) {
 #pragma unroll 1
                         for (int k = N1; k < KMAX; ++k) {
