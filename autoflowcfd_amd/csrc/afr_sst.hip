@@ -375,6 +375,18 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
 template <int P, int KMAX, typename real>
 __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
     constexpr int N1 = P + 1, NS = N1 * N1 * N1, NFP = N1 * N1;
+    // the deduplicated neighbour row blocks in LDS when they fit (8 blocks for a graded prism box)
+    constexpr int ROWS_CAP = 8 * NFP + 1;
+    __shared__ real s_rw[ROWS_CAP * KMAX];
+    __shared__ int s_ri[ROWS_CAP * KMAX];
+    const bool rows_lds = a.n_rows + 1 <= ROWS_CAP;
+    if (rows_lds) {
+        for (int i = threadIdx.x; i < (a.n_rows + 1) * KMAX; i += 256) {
+            s_rw[i] = a.rows_w[i];
+            s_ri[i] = a.rows_idx[i];
+        }
+    }
+    __syncthreads();
     const int64_t n = a.n_records * NFP;
     for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
         const int64_t r = g / NFP;
@@ -398,10 +410,18 @@ __global__ void __launch_bounds__(256) k_sst_face(SstArgs<real> a) {
             for (int q = 0; q < 4; ++q) pn[q] = 0.0;
             int spk[KMAX];
             real wk[KMAX];
+            if (rows_lds) {
 #pragma unroll
-            for (int kk = 0; kk < KMAX; ++kk) {  // rows are KMAX wide (zero-padded)
-                spk[kk] = a.rows_idx[(int64_t)row * KMAX + kk];
-                wk[kk] = a.rows_w[(int64_t)row * KMAX + kk];
+                for (int kk = 0; kk < KMAX; ++kk) {  // rows are KMAX wide (zero-padded)
+                    spk[kk] = s_ri[row * KMAX + kk];
+                    wk[kk] = s_rw[row * KMAX + kk];
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < KMAX; ++kk) {
+                    spk[kk] = a.rows_idx[(int64_t)row * KMAX + kk];
+                    wk[kk] = a.rows_w[(int64_t)row * KMAX + kk];
+                }
             }
 #pragma unroll
             for (int kk = 0; kk < KMAX; ++kk) {
