@@ -7,7 +7,13 @@ the checker (or the timed CPU baseline), never as the thing measured.
 
 Every function restates the reference algorithm of AutoFlowCFD's Numba/NumPy
 CPU path (the reference's own FP64 semantics) and cites the reference
-``file:line`` it follows.  Pins (closed forms, invariants, brute force) live
-in ``tests/test_oracle_*.py``; functions no check pins are marked
-"parity unpinned" in their docstring.
+``file:line`` it follows.  Pins (closed forms, invariants, brute force, figures
+the reference states in its source) live in ``tests/test_oracle_*.py``.
+
+Parity unpinned (no reference fixture and no closed form pins them beyond special
+states; each says so in its docstring): the values of ``residual.inviscid_residual``
+and ``residual.viscous_residual`` on general states, ``residual.suppress_outliers``
+decisions on general fields, ``sst.source_terms``, ``sst.transport`` and
+``sst.turbulence_step`` values, ``time_step.local_time_step`` values and the
+``time_step.ns_step`` trajectory.
 """

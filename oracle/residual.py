@@ -160,7 +160,10 @@ def ghost_state(bc, Q_int, normal, bc_params):
 
 
 def suppress_outliers(R, Uref, factor=OUTLIER_FACTOR, floor=OUTLIER_FIELD_FLOOR):
-    """troubled_cell.py:347-408."""
+    """troubled_cell.py:347-408.
+
+    Pinned by the reference's test_troubled_cell cases; parity unpinned for decisions on general
+    residual fields (the rule is discontinuous at its threshold)."""
     sib = np.median(np.abs(R), axis=1, keepdims=True)
     fld = floor * np.mean(np.abs(Uref), axis=1, keepdims=True)
     ref = np.maximum(np.maximum(sib, fld), 1e-300)
@@ -215,7 +218,10 @@ def ghost_states_at_owner(ops, geom, fl, Q, bc_params):
 
 
 def inviscid_residual(U, geom, ops, fl, bc_params, suppress=True):
-    """dU/dt inviscid part (C, n_sps, 5); inviscid.py:139-345."""
+    """dU/dt inviscid part (C, n_sps, 5); inviscid.py:139-345.
+
+    Pinned at special states (free stream P2 3e-5 / P3 5e-3, linear shear, uniform mass flux, the
+    reference's own test cases); parity unpinned for the values on general states."""
     C = geom.n_cells
     Q = conserved_to_primitive(U[..., :5])
 
@@ -292,7 +298,10 @@ def viscous_flux(Q, gv, gT, mu, Pr, mu_t, Pr_t):
 
 def viscous_residual(U, geom, ops, fl, bc_params, mu, mu_t=None, Pr=0.72, Pr_t=0.9, suppress=True,
                      return_grads=False):
-    """dU/dt viscous part (C, n_sps, 5); viscous_flux.py:113-302."""
+    """dU/dt viscous part (C, n_sps, 5); viscous_flux.py:113-302.
+
+    Pinned by the zero residual of uniform flow and closed-form stresses / heat flux; parity
+    unpinned for the values on general states."""
     C, nsp = geom.n_cells, ops.n_sps
     mu_t = np.zeros((C, nsp)) if mu_t is None else mu_t
     Q = conserved_to_primitive(U[..., :5])

@@ -73,7 +73,8 @@ def eddy_viscosity(k, w, rho, S, F2, mu):
 
 
 def source_terms(Q, gv, d, mu, k_field, w_field, gk, gw):
-    """(Sk, Somega, nu_t, beta_blend); sst.py:203-330."""
+    """(Sk, Somega, nu_t, beta_blend); sst.py:203-330.  Parity unpinned (no reference test pins
+    the SST source values)."""
     rho = Q[..., 0]
     nu = mu / np.maximum(rho, 1e-10)
     k = np.minimum(k_field, 1e40)
@@ -151,7 +152,8 @@ def _volume_div(Ft, geom, ops):
 
 
 def transport(k, w, nut, Q, geom, ops, fl, d, mu, wall_records):
-    """(dk/dt, domega/dt) transport contributions; transport.py:122-520."""
+    """(dk/dt, domega/dt) transport contributions; transport.py:122-520.  Pinned by uniform
+    transport (zero); parity unpinned for general fields."""
     rho, vel = Q[..., 0], Q[..., 1:4]
     gk = clipped_gradient(k, geom, ops)
     gw = clipped_gradient(w, geom, ops)
@@ -206,7 +208,8 @@ def update_fields(k, w, dt, Sk_rho, Sw_rho, beta, tk, tw):
 
 
 def turbulence_step(U, k, w, geom, ops, fl, d, dt, mu_molecular, wall_records):
-    """compute_turbulence_source (turbulence.py:178-330): returns (k_new, w_new, nu_t, Sk, Somega)."""
+    """compute_turbulence_source (turbulence.py:178-330): returns (k_new, w_new, nu_t, Sk, Somega).
+    Parity unpinned (no reference test pins the SST update values)."""
     Q = update_primitives(U)
     gU = physical_gradient(U[..., :5], geom, ops)
     gv = gU[:, :, 1:4, :]

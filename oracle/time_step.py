@@ -38,6 +38,8 @@ def residual_norm(dUdt, n_vars=None):
 
 
 def local_time_step(Q, geom, order, mu, mu_t=None):
+    """cfl.py:12-147 (the three bounds of the local time step).  Parity unpinned for its values
+    (no reference test pins them)."""
     rho, u, v, w, p = (Q[..., i] for i in range(5))
     a = np.sqrt(np.maximum(1.4 * p / np.maximum(rho, 1e-10), 1e-10))
     ws = np.maximum(np.sqrt(u * u + v * v + w * w) + a, 1e-10)
@@ -98,7 +100,9 @@ def ssp_rk3(U0, rhs, dt, ops, n_prism, R0=None):
 
 
 def ns_step(U, geom, ops, fl, bc_params, order, mu=1.8e-5, mu_t=None, Pr=0.72, Pr_t=0.9, viscous=True):
-    """One pseudo-step: returns (U_new, RMS norm of dU/dt at U, dt_local)."""
+    """One pseudo-step: returns (U_new, RMS norm of dU/dt at U, dt_local).  The SSP-RK3 stages
+    are pinned by the linear-ODE order test; the step trajectory on general states is parity
+    unpinned."""
     Q = update_primitives(U)
     dt = local_time_step(Q, geom, order, mu, mu_t)
 
