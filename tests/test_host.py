@@ -297,3 +297,32 @@ def test_slot_mask_wide_flag():
     nz, _, _ = sparse_rows(np.concatenate([sl.rows, np.zeros((1, sl.rows.shape[1]))]))
     flags = (slot_mask_with_wide_flag(sl.mask, np.asarray(sl.blk)[..., 0], nz, sl.n_blocks, 9, 2) >> 30) & 1
     assert flags.sum(axis=1).max() <= 1 and flags.any()     # a graded prism box: at most one wide slot per cell
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N with N > 1 exits (status 2) with a message, before
+    touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_bench_algorithmic_byte_models():
+    """The bench's algorithmic-byte models at the C2 sizes (the figures quoted for the kernel
+    rooflines): inviscid 4.196 GB, viscous volume 5.27 GB, viscous interface 4.87 GB, and the
+    full-residual B_res 5.863 GB."""
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_models", os.path.join(root, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    C, ns, nfine, nfp, nfaces = 1102736, 27, 64, 9, 2783736
+    n_sp, n_fine = C * ns, C * nfine
+    assert abs(b.algorithmic_bytes_inviscid(n_sp, n_fine, nfaces, nfp) / 1e9 - 4.196) < 0.01
+    assert abs(b.algorithmic_bytes_visc_volume(n_sp, C) / 1e9 - 5.27) < 0.01
+    assert abs(b.algorithmic_bytes_visc_iface(n_sp, C, nfp) / 1e9 - 4.87) < 0.01
+    assert abs(b.algorithmic_bytes_residual(n_sp, n_fine, nfaces, nfp) / 1e9 - 5.863) < 0.01
