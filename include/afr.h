@@ -47,7 +47,8 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   slot_mask  (C, S)    int32         bit fp set: flux point fp uses source 1; bit 30 set: the
  *                                       slot's source-0 rows have more than order+1 non-zeros
  *                                       (a scheduling hint of the viscous interface kernel:
- *                                       results do not depend on it)
+ *                                       without it the wide rows are summed by one lane, with
+ *                                       it in three parts; the two differ by round-off only)
  *   slot_bc    (C, S)    int32         BC code, -1 interior, -2 unused slot
  *   slot_fallback (C, S) int32         bit fp set: direction falls back to tn
  *   slot_tn    (C, S, 3)               role-signed true normal
