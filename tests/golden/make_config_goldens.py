@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 GAMMA = 1.4
+SAMPLE_SEED = 0  # seed of the 2000 stored sample points (PCG64)
 
 
 def c1_setup():
