@@ -92,6 +92,8 @@ SIGNATURES = {
     "afr_graph_destroy": (ctypes.c_int, [c_vp]),
     "afr_residual_norm_cases": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
     "afr_forces_cases_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
+    "afr_suppress_residual_outliers": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                      ctypes.c_int32, ctypes.c_int32, c_vp]),
 }
 
 _lib = None

@@ -18,6 +18,7 @@
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
 #include "afr_prism.hpp"
+#include "afr_outlier.hpp"
 
 using namespace afr;
 
@@ -151,110 +152,6 @@ struct DmaCells {
 template <int NCH, int CELL_DW>
 __device__ __forceinline__ void dma_cells(const uint32_t* src, const int64_t (&base)[6], uint32_t* dst, int t) {
     DmaCells<0, NCH, CELL_DW>::run(src, base, dst, t);
-}
-
-// Exact median-based outlier zeroing of the (NS x 5) residual block held in
-// sR ([s][v]), as in troubled_cell.py:339-408: R[s][v] = 0 where
-// |R| > 1e4 max(median_s |R[:,v]|, 1e-9 mean_s |U[:,v]|, tiny).  A cheap
-// certificate runs first: with M = max_s |R[:,v]| and t = 1.0001e-4 M, if at
-// most k1 = (NS-1)/2 values lie below t then median >= t, so no value can
-// exceed the threshold and the variable is left as is.  Only when the
-// certificate fails for some variable is the median selected exactly.
-template <int NS, typename real>
-__device__ __forceinline__ void outlier_zeroing(real* sR, const real* umean, int t) {
-    constexpr int NSV = NS * 5, NR = (NSV + 63) / 64;
-    constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
-    constexpr int G = 8, J = (NS + G - 1) / G;  // 8 lanes per variable, J values each
-    const int v = t >> 3, g = t & 7;
-    real mx = 0;
-    if (t < 5 * G) {
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int s = g + G * j;
-            if (s < NS) {
-                const real x = fabs(sR[s * 5 + v]);
-                mx = x > mx ? x : mx;
-            }
-        }
-    }
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) {
-        const real o = __shfl_xor(mx, m);
-        mx = o > mx ? o : mx;
-    }
-    const real tt = mx * real(1.0001e-4);
-    int cnt = 0;
-    if (t < 5 * G) {
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int s = g + G * j;
-            if (s < NS) cnt += fabs(sR[s * 5 + v]) < tt;
-        }
-    }
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) cnt += __shfl_xor(cnt, m);
-    const bool need = (t < 5 * G) && cnt > k1;
-    if (!__any(need)) return;
-    // exact selection: rank counting, then zeroing
-    real x[NR], m1[NR], m2[NR];
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const int o = t + 64 * k;
-        x[k] = real(0);
-        m1[k] = m2[k] = real(0);
-        if (o < NSV) {
-            const int vv = o % 5;
-            x[k] = fabs(sR[o]);
-            int lt = 0, le = 0;
-#pragma unroll 1
-            for (int s2 = 0; s2 < NS; ++s2) {  // rare path: rolled, off the register budget
-                const real y = fabs(sR[s2 * 5 + vv]);
-                lt += y < x[k];
-                le += y <= x[k];
-            }
-            m1[k] = (lt <= k1 && k1 < le) ? real(1) : real(0);
-            m2[k] = (lt <= k2 && k2 < le) ? real(1) : real(0);
-        }
-    }
-    // the median value of variable vv is held by the lanes flagged above: gather it by
-    // a max over flagged values (all flagged values of one rank are equal)
-    real med1[5], med2[5];
-#pragma unroll
-    for (int vv = 0; vv < 5; ++vv) {
-        real a1 = real(-1), a2 = real(-1);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const int o = t + 64 * k;
-            if (o < NSV && o % 5 == vv) {
-                if (m1[k] > real(0)) a1 = x[k] > a1 ? x[k] : a1;
-                if (m2[k] > real(0)) a2 = x[k] > a2 ? x[k] : a2;
-            }
-        }
-#pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-            const real o1 = __shfl_xor(a1, m), o2 = __shfl_xor(a2, m);
-            a1 = o1 > a1 ? o1 : a1;
-            a2 = o2 > a2 ? o2 : a2;
-        }
-        med1[vv] = a1;
-        med2[vv] = a2;
-    }
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        const int o = t + 64 * k;
-        if (o < NSV) {
-            const int vv = o % 5;
-            real m1v = med1[0], m2v = med2[0];
-#pragma unroll
-            for (int q = 1; q < 5; ++q)
-                if (vv == q) { m1v = med1[q]; m2v = med2[q]; }
-            const real med = (NS % 2) ? m1v : real(0.5) * (m1v + m2v);
-            const real um = real(1e-9) * umean[vv];
-            real ref = med > um ? med : um;
-            ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
-            if (x[k] > real(1e4) * ref) sR[o] = real(0);
-        }
-    }
 }
 
 template <int P, int KMAX, typename real>
@@ -603,7 +500,10 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
         wave_sync();
         // ---- 7. outlier suppression, coalesced write
         t = lane_fresh(t);
-        if (a.suppress) outlier_zeroing<NS, real>(sR, a.umean + cell * 5, t);
+        if (a.suppress) {
+            const real* um = a.umean + cell * 5;
+            outlier_zeroing<NS, 5, real>(sR, [um](int v) { return um[v]; }, t);
+        }
         wave_sync();
         for (int o = t; o < NSV; o += 64) a.R[cell * NSV + o] = sR[o];
         wave_sync();

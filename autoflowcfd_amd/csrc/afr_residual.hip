@@ -31,6 +31,7 @@
 #include "afr_step.hpp"
 #include "afr_sst.hpp"
 #include "afr_forces.hpp"
+#include "afr_outlier.hpp"
 
 using namespace afr;
 
@@ -127,7 +128,8 @@ __global__ void __launch_bounds__(kThreads) k_primitives(const real* __restrict_
     }
 }
 
-// median of |x[0..n)| as in numpy.median (mean of the two middle values for even n)
+// median of |x[0..n)| as troubled_cell.py:347-379 takes it: the sorted values (NaN last), the
+// middle one or the mean of the two middle ones for even n; NaN when a median position holds NaN
 template <typename real, int N>
 __device__ real median_abs(const real* x, int stride) {
     real a[N];
@@ -137,7 +139,8 @@ __device__ real median_abs(const real* x, int stride) {
         a[i] = t < real(0) ? -t : t;
     }
     const int k1 = (N - 1) / 2, k2 = N / 2;
-    real m1 = real(0), m2 = real(0);
+    const real qnan = real(__builtin_nan(""));
+    real m1 = qnan, m2 = qnan;
     for (int i = 0; i < N; ++i) {
         int lt = 0, le = 0;
 #pragma unroll
@@ -327,7 +330,7 @@ __global__ void __launch_bounds__(kThreads) k_inviscid(Args<real> a) {
             if (c < ncell) {
                 real med = median_abs<real, NS>(s_R + c * NS * 5 + v, 5);
                 const real mu = a.umean[(first + c) * 5 + v];
-                real ref = rmax(rmax(med, real(1e-9) * mu), Tiny<real>::v);
+                real ref = nan_max(nan_max(med, real(1e-9) * mu), Tiny<real>::v);
                 real thr = real(1e4) * ref;
                 for (int s = 0; s < NS; ++s) {
                     real x = s_R[(c * NS + s) * 5 + v];

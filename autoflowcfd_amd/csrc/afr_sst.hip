@@ -23,6 +23,7 @@
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
 #include "afr_sst.hpp"
+#include "afr_outlier.hpp"
 
 using namespace afr;
 
@@ -504,54 +505,16 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
             sf[t] = fabs((double)(f == 0 ? a.k[i] : a.w[i]));
         }
         wave_sync();
-        // outlier zeroing per field (troubled_cell.py:339-408, n_vars = 1).  Certificate first:
-        // when at most (NS-1)/2 values of a field lie below 1.0001e-4 of its largest |value|, the
-        // median is at least that, so no value exceeds 1e4 x max(median, floor) and nothing is
-        // zeroed; only otherwise is the median selected exactly.
-        double tv = 0.0;
-        bool need = false;
-        if (t < 2 * NS) {
-            const int f = t & 1;
-            double mx = 0.0;
-            for (int s2 = 0; s2 < NS; ++s2) {
-                const double y = fabs(st[s2 * 2 + f]);
-                mx = y > mx ? y : mx;
-            }
-            const double thr = mx * 1.0001e-4;
-            int cnt = 0;
-            for (int s2 = 0; s2 < NS; ++s2) cnt += fabs(st[s2 * 2 + f]) < thr;
-            need = cnt > (NS - 1) / 2;
-            tv = isfinite(st[t]) ? st[t] : 0.0;
-        }
-        if (__any(need) && t < 2 * NS) {
-            const int f = t & 1;
-            const double x = fabs(st[t]);
-            double m1 = 0.0, m2 = 0.0, um = 0.0;
-            constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
-            // rare path (the certificate failed): kept rolled so that it does not set the
-            // register budget of the whole kernel
-#pragma unroll 1
-            for (int s2 = 0; s2 < NS; ++s2) {
-                const double y = fabs(st[s2 * 2 + f]);
-                int lt = 0, le = 0;
-#pragma unroll 1
-                for (int s3 = 0; s3 < NS; ++s3) {
-                    const double z = fabs(st[s3 * 2 + f]);
-                    lt += z < y;
-                    le += z <= y;
-                }
-                if (lt <= k1 && k1 < le) m1 = y;
-                if (lt <= k2 && k2 < le) m2 = y;
-                um += sf[s2 * 2 + f];
-            }
-            const double med = (NS % 2) ? m1 : 0.5 * (m1 + m2);
-            double ref = dmax(med, 1e-9 * um / NS);
-            ref = dmax(ref, 1e-300);
-            tv = x > 1e4 * ref ? 0.0 : st[t];
-            tv = isfinite(tv) ? tv : 0.0;
-        }
+        // outlier zeroing per field (troubled_cell.py:339-408 with n_vars = 1, once for k and once
+        // for omega, each field's own mean |value| as the reference magnitude), then non-finite
+        // transport values -> 0 (transport.py:509-515)
+        outlier_zeroing<NS, 2, double>(st, [sf](int f) {
+            double m = 0.0;
+            for (int s2 = 0; s2 < NS; ++s2) m += sf[s2 * 2 + f];
+            return m / NS;
+        }, t);
         wave_sync();
-        if (t < 2 * NS) st[t] = tv;
+        if (t < 2 * NS && !isfinite(st[t])) st[t] = 0.0;
         wave_sync();
         if (t < NS) {
             const int s = t;

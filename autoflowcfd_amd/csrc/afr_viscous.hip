@@ -18,6 +18,7 @@
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
 #include "afr_viscous.hpp"
+#include "afr_outlier.hpp"
 
 using namespace afr;
 
@@ -51,89 +52,6 @@ template <int P, int CT> struct VO {
 };
 
 __device__ __forceinline__ int slot_code(int slot) { return slot < 3 ? slot : (slot == 3 ? 4 : 5); }
-
-// median-certificate outlier zeroing over sR[NS][5] (same rule as the inviscid kernels)
-template <int NS, typename real>
-__device__ void outlier_zeroing(real* sR, const real* umean, int t) {
-    constexpr int NSV = NS * 5, NR = (NSV + 63) / 64;
-    constexpr int k1 = (NS - 1) / 2, k2 = NS / 2;
-    constexpr int G = 8, J = (NS + G - 1) / G;
-    const int v = t >> 3, g = t & 7;
-    real mx = 0;
-    if (t < 5 * G) {
-        for (int j = 0; j < J; ++j) {
-            const int s = g + G * j;
-            if (s < NS) {
-                const real x = fabs(sR[s * 5 + v]);
-                mx = x > mx ? x : mx;
-            }
-        }
-    }
-    for (int m = 1; m < G; m <<= 1) {
-        const real o = __shfl_xor(mx, m);
-        mx = o > mx ? o : mx;
-    }
-    const real tt = mx * real(1.0001e-4);
-    int cnt = 0;
-    if (t < 5 * G) {
-        for (int j = 0; j < J; ++j) {
-            const int s = g + G * j;
-            if (s < NS) cnt += fabs(sR[s * 5 + v]) < tt;
-        }
-    }
-    for (int m = 1; m < G; m <<= 1) cnt += __shfl_xor(cnt, m);
-    if (!__any((t < 5 * G) && cnt > k1)) return;
-    real x[NR], f1[NR], f2[NR];
-    for (int k = 0; k < NR; ++k) {
-        const int o = t + 64 * k;
-        x[k] = f1[k] = f2[k] = real(0);
-        if (o < NSV) {
-            const int vv = o % 5;
-            x[k] = fabs(sR[o]);
-            int lt = 0, le = 0;
-#pragma unroll 1
-            for (int s2 = 0; s2 < NS; ++s2) {  // rare path: rolled, off the register budget
-                const real y = fabs(sR[s2 * 5 + vv]);
-                lt += y < x[k];
-                le += y <= x[k];
-            }
-            f1[k] = (lt <= k1 && k1 < le) ? real(1) : real(0);
-            f2[k] = (lt <= k2 && k2 < le) ? real(1) : real(0);
-        }
-    }
-    real med1[5], med2[5];
-    for (int vv = 0; vv < 5; ++vv) {
-        real a1 = real(-1), a2 = real(-1);
-        for (int k = 0; k < NR; ++k) {
-            const int o = t + 64 * k;
-            if (o < NSV && o % 5 == vv) {
-                if (f1[k] > real(0)) a1 = x[k] > a1 ? x[k] : a1;
-                if (f2[k] > real(0)) a2 = x[k] > a2 ? x[k] : a2;
-            }
-        }
-        for (int m = 1; m < 64; m <<= 1) {
-            const real o1 = __shfl_xor(a1, m), o2 = __shfl_xor(a2, m);
-            a1 = o1 > a1 ? o1 : a1;
-            a2 = o2 > a2 ? o2 : a2;
-        }
-        med1[vv] = a1;
-        med2[vv] = a2;
-    }
-    for (int k = 0; k < NR; ++k) {
-        const int o = t + 64 * k;
-        if (o < NSV) {
-            const int vv = o % 5;
-            real m1v = med1[0], m2v = med2[0];
-            for (int q = 1; q < 5; ++q)
-                if (vv == q) { m1v = med1[q]; m2v = med2[q]; }
-            const real med = (NS % 2) ? m1v : real(0.5) * (m1v + m2v);
-            const real um = real(1e-9) * umean[vv];
-            real ref = med > um ? med : um;
-            ref = ref > Tiny<real>::v ? ref : Tiny<real>::v;
-            if (x[k] > real(1e4) * ref) sR[o] = real(0);
-        }
-    }
-}
 
 // ------------------------------------------------------------------ volume kernel
 // Row widths of the sparse collapsed-coordinate derivative matrices: d/da couples the
@@ -675,7 +593,10 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
         }
         wave_sync();
         t = lane_fresh(t);
-        if (a.suppress) outlier_zeroing<NS, real>(sR, a.umean + cell * 5, t);
+        if (a.suppress) {
+            const real* um = a.umean + cell * 5;
+            outlier_zeroing<NS, 5, real>(sR, [um](int v) { return um[v]; }, t);
+        }
         wave_sync();
 #pragma unroll
         for (int j = 0; j < NO; ++j) {

@@ -83,6 +83,25 @@ def sparse_fixed(rows, widths, drop=ROW_DROP):
     return w, idx
 
 
+def suppress_residual_outliers(R, U):
+    """Zero the outliers of the residual field R (C, ns, nv) in place on the device, with U (C, ns,
+    >= nv) the reference field, as suppress_residual_outliers
+    (src/autoflowcfd/core/fr_operators/troubled_cell.py:382-408); returns R."""
+    if R.device.type != "cuda" or U.device != R.device or R.dtype != U.dtype:
+        raise ValueError("R and U must be device tensors of one precision")
+    if R.dim() != 3 or U.dim() != 3 or tuple(U.shape[:2]) != tuple(R.shape[:2]) or U.shape[2] < R.shape[2]:
+        raise ValueError(f"shapes R {tuple(R.shape)} and U {tuple(U.shape)} do not match")
+    if not (R.is_contiguous() and U.is_contiguous()) or R.dtype not in (torch.float32, torch.float64):
+        raise ValueError("R and U must be contiguous FP32 or FP64 tensors")
+    lib = _lib.load()
+    st = ctypes.c_void_p(torch.cuda.current_stream(R.device).cuda_stream)
+    _lib.check(lib.afr_suppress_residual_outliers(ctypes.c_void_p(R.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                                                  int(R.shape[0]), int(R.shape[1]), int(R.shape[2]),
+                                                  int(U.shape[2]), 0 if R.dtype == torch.float32 else 1, st),
+               "afr_suppress_residual_outliers")
+    return R
+
+
 class LibGraph:
     """A HIP graph of afr_* calls captured and replayed through the C ABI (afr_graph_begin /
     afr_graph_end / afr_graph_launch), for callers that do not use torch's graphs:

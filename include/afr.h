@@ -323,6 +323,17 @@ int afr_sst_step(const AfrMesh *mesh, const void *U, void *k, void *omega, const
 int afr_sst_rk3_step(const AfrMesh *mesh, void *U, void *k, void *omega, const void *d_wall, void *nu_t, void *mu_t,
                      double mu, double Pr, double Pr_t, int32_t n_vars_total, double *norm, void *stream);
 
+/* Residual outlier zeroing of a field R (n_cells, ns, nv) in place, as
+ * suppress_residual_outliers (src/autoflowcfd/core/fr_operators/troubled_cell.py:382-408) with its
+ * default factor 1e4 and field floor 1e-9: R[c][s][v] = 0 where |R| > 1e4 max(median_s |R[c][:][v]|,
+ * 1e-9 mean_s |U[c][:][v]|, 1e-300); the median as troubled_cell.py:347-379 takes it (sorted, NaN
+ * last, mean of the two middle values for even ns), a NaN median zeroes nothing.  U (n_cells, ns,
+ * u_stride) is the reference field, its first nv variables used.  ns 1, 8, 27 or 64; nv 1, 2, 5
+ * or 7; dtype AFR_F32 / AFR_F64 for both arrays.  The residual entry points apply the same rule
+ * inside their kernels when `suppress` is set. */
+int afr_suppress_residual_outliers(void *R, const void *U, int64_t n_cells, int32_t ns, int32_t nv,
+                                   int32_t u_stride, int32_t dtype, void *stream);
+
 /* Persistent grid size of one kernel on this mesh: grid_cap workgroups (0 = the built-in
  * default; otherwise 8 .. 65536, rounded down to a multiple of the 8 XCDs).  Results do not
  * depend on it; the autotuner (autoflowcfd_amd/autotune.py) picks it by timing.  No

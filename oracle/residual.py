@@ -159,12 +159,24 @@ def ghost_state(bc, Q_int, normal, bc_params):
     return out
 
 
+def median_abs_over_sps(R):
+    """troubled_cell.py:347-379: per (cell, variable) the |R| over the solution points sorted (NaN
+    last, as numpy and numba sort), then the middle value, or the mean of the two middle values for
+    an even count.  Differs from np.median only with NaN present: a few NaN leave it finite."""
+    srt = np.sort(np.abs(R), axis=1)
+    n = R.shape[1]
+    half = n // 2
+    if n % 2:
+        return srt[:, half:half + 1]
+    return 0.5 * (srt[:, half - 1:half] + srt[:, half:half + 1])
+
+
 def suppress_outliers(R, Uref, factor=OUTLIER_FACTOR, floor=OUTLIER_FIELD_FLOOR):
     """troubled_cell.py:347-408.
 
     Pinned by the reference's test_troubled_cell cases; parity unpinned for decisions on general
     residual fields (the rule is discontinuous at its threshold)."""
-    sib = np.median(np.abs(R), axis=1, keepdims=True)
+    sib = median_abs_over_sps(R)
     fld = floor * np.mean(np.abs(Uref), axis=1, keepdims=True)
     ref = np.maximum(np.maximum(sib, fld), 1e-300)
     out = np.abs(R) > factor * ref

@@ -1,0 +1,180 @@
+"""GPU parity of the residual outlier zeroing (src/autoflowcfd/core/fr_operators/troubled_cell.py:339-408).
+
+* The entry point afr_suppress_residual_outliers, on seeded fields with planted spikes (1e6 x the
+  cell's other values), ties, a few NaN (the reference's sorted median stays finite), a majority of
+  NaN (median NaN: nothing zeroed, even beside a spike), infinities and floor-dominated columns, for
+  P1 (8 points: the even-count median), P2 (27) and P3 (64) and 1, 2, 5, 7 variables: the zeroed
+  entries must be exactly the oracle's.
+* The fused zeroing of the inviscid kernels (prism and tet) and of the SST transport, on the real
+  path: a single solution point of a few cells is perturbed in an otherwise uniform state, so the
+  neighbours that share a collapsed-coordinate face with such a cell see a jump at one flux point
+  only, i.e. a residual at the few solution points on that line and round-off elsewhere: those are
+  zeroed.  The device residual (zeroing on) must equal the oracle's, zeroed entries included.
+  The viscous interface kernel applies the same device routine (afr_outlier.hpp); its BR1 jumps
+  carry the neighbour's gradients, which are dense in the cell, so no state localises them.
+"""
+
+import numpy as np
+import pytest
+
+from cases import freestream, prim_to_cons, tet_box, wall_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic(C, ns, nv, seed):
+    rng = np.random.default_rng(seed)
+    R = rng.standard_normal((C, ns, nv)) * 10.0 ** rng.uniform(-3, 3, (C, 1, nv))
+    U = rng.uniform(0.5, 2.0, (C, ns, nv + 2)) * 10.0 ** rng.uniform(-2, 5, (C, 1, nv + 2))
+    half = ns // 2
+    for c in range(C):
+        kind = c % 8
+        v = rng.integers(nv)
+        pts = rng.choice(ns, size=3, replace=False)
+        if kind == 1:                      # planted spikes
+            R[c, pts[: 1 + c % 3], v] *= 1e6
+        elif kind == 2:                    # ties at zero below the median, one spike
+            R[c, : half - 1, v] = 0.0
+            R[c, pts[0], v] = 1e8
+        elif kind == 3:                    # a few NaN: the sorted median stays finite
+            R[c, pts[:2], v] = np.nan
+            R[c, pts[2], v] *= 1e7
+        elif kind == 4:                    # a majority of NaN: median NaN, nothing zeroed
+            R[c, : half + 1, v] = np.nan
+            R[c, -1, v] = 1e12
+        elif kind == 5:                    # an infinite residual
+            R[c, pts[0], v] = np.inf
+        elif kind == 6:                    # tiny residuals: the field floor decides
+            R[c, :, v] = 1e-14 * rng.standard_normal(ns)
+            R[c, pts[0], v] = 3e-5 * U[c, :, v].mean() if c % 16 == 6 else 3e-6 * U[c, :, v].mean()
+        elif kind == 7:                    # NaN in the reference field
+            U[c, pts[0], v] = np.nan
+            R[c, pts[1], v] *= 1e6
+    return R, U
+
+
+@pytest.mark.parametrize("ns", [8, 27, 64])
+@pytest.mark.parametrize("nv", [1, 2, 5, 7])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_suppress_entry_matches_oracle(ns, nv, prec):
+    import torch
+    from autoflowcfd_amd.device import suppress_residual_outliers
+    from oracle.residual import suppress_outliers
+    print(f"[gpu] suppress entry ns={ns} nv={nv} {prec}", flush=True)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    R, U = _synthetic(160, ns, nv, seed=ns * 10 + nv)
+    if dt == torch.float32:
+        R, U = R.astype(np.float32).astype(np.float64), U.astype(np.float32).astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        ref = suppress_outliers(R, U[..., :nv])
+    Rd = torch.as_tensor(R, dtype=dt, device="cuda").contiguous()
+    Ud = torch.as_tensor(U, dtype=dt, device="cuda").contiguous()
+    out = suppress_residual_outliers(Rd, Ud).double().cpu().numpy()
+    zeroed_ref = (ref == 0) & (R != 0)
+    zeroed_dev = (out == 0) & (R != 0)
+    assert zeroed_ref.sum() > 0
+    assert np.array_equal(zeroed_dev, zeroed_ref), f"{int((zeroed_dev != zeroed_ref).sum())} decisions differ"
+    assert np.array_equal(out, ref, equal_nan=True)
+    # a majority of NaN makes the median NaN: the 1e12 spike beside them stays (np.nanmedian would
+    # have zeroed it)
+    for c in range(4, len(R), 8):
+        assert not zeroed_dev[c, -1].any()
+
+
+def _spiked_state(mesh, ns, seed, n_spikes):
+    Q0 = freestream(0.2, 3.0)
+    U = np.broadcast_to(prim_to_cons(Q0), (mesh.n_cells, ns, 5)).copy()
+    rng = np.random.default_rng(seed)
+    for c in rng.choice(mesh.n_cells, n_spikes, replace=False):
+        U[c, rng.integers(ns)] *= 1.01
+    return Q0, U
+
+
+@pytest.mark.parametrize("kind", ["prism", "tet"])
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_inviscid_kernel_zeroing_matches_oracle(kind, order, prec):
+    import torch
+    from autoflowcfd_amd.device import DeviceMesh, bc_params_table
+    from oracle import fr_ops
+    from oracle.faces import build_flat_faces
+    from oracle.geometry import Geometry
+    from oracle.residual import inviscid_residual
+    print(f"[gpu] inviscid outliers {kind} P{order} {prec}", flush=True)
+    mesh = wall_box(seed=1) if kind == "prism" else tet_box(3, seed=1)
+    ops = fr_ops.generate_operators(order)
+    g = Geometry(mesh, ops)
+    fl = build_flat_faces(mesh, ops)
+    Q0, U = _spiked_state(mesh, ops.n_sps, seed=0, n_spikes=4 if kind == "prism" else 8)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    if dt == torch.float32:
+        U = U.astype(np.float32).astype(np.float64)
+    bcp = bc_params_table(Q0)
+    bcd = {c: bcp[c] for c in range(8)}
+    raw = inviscid_residual(U, g, ops, fl, bcd, suppress=False)
+    ref = inviscid_residual(U, g, ops, fl, bcd, suppress=True)
+    zeroed_ref = (ref == 0) & (raw != 0)
+    assert zeroed_ref.sum() > 0
+    dm = DeviceMesh(mesh, order, dtype=dt, bc_params=bcp)
+    Ud = torch.as_tensor(U, dtype=dt, device="cuda").contiguous()
+    out = dm.inviscid_residual(Ud, suppress=True).double().cpu().numpy()
+    rawd = dm.inviscid_residual(Ud, suppress=False).double().cpu().numpy()
+    zeroed_dev = (out == 0) & (rawd != 0)
+    assert np.array_equal(zeroed_dev, zeroed_ref), f"{int((zeroed_dev != zeroed_ref).sum())} decisions differ"
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err < (1e-10 if prec == "f64" else 1e-5), f"rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sst_transport_zeroing_matches_oracle(order, prec):
+    import torch
+    from autoflowcfd_amd.device import DeviceMesh, bc_params_table
+    from autoflowcfd_amd.mesh import BC_WALL
+    from oracle import fr_ops, sst
+    from oracle.faces import build_flat_faces
+    from oracle.geometry import Geometry
+    from oracle.time_step import local_time_step, update_primitives
+    print(f"[gpu] sst outliers P{order} {prec}", flush=True)
+    mesh = wall_box(seed=2)
+    ops = fr_ops.generate_operators(order)
+    g = Geometry(mesh, ops)
+    fl = build_flat_faces(mesh, ops)
+    Q0 = freestream(0.15)
+    U = np.broadcast_to(prim_to_cons(Q0), (mesh.n_cells, ops.n_sps, 5)).copy()
+    k = np.full((mesh.n_cells, ops.n_sps), 0.3908)
+    w = np.full_like(k, 2660.0)
+    rng = np.random.default_rng(0)
+    for c in rng.choice(mesh.n_cells, 4, replace=False):
+        k[c, rng.integers(ops.n_sps)] *= 1.5
+        w[rng.integers(mesh.n_cells), rng.integers(ops.n_sps)] *= 1.5
+    dt = torch.float64 if prec == "f64" else torch.float32
+    if dt == torch.float32:
+        U, k, w = (x.astype(np.float32).astype(np.float64) for x in (U, k, w))
+    mu = 1.8e-5
+    counts = []
+    orig = sst.suppress_outliers
+
+    def counting(Rr, Uref, *a, **kw):
+        out = orig(Rr, Uref, *a, **kw)
+        counts.append(int(((out == 0) & (Rr != 0)).sum()))
+        return out
+    sst.suppress_outliers = counting
+    try:
+        d = sst.wall_distance(g.sps_xyz, mesh.nodes[mesh.wall_nodes])
+        dtl = local_time_step(update_primitives(U), g, order, mu, None)
+        kn, wn, nut, _, _ = sst.turbulence_step(U, k, w, g, ops, fl, d, dtl, mu, fl.bc == BC_WALL)
+    finally:
+        sst.suppress_outliers = orig
+    assert counts[0] > 0 and counts[1] > 0, counts
+    dm = DeviceMesh(mesh, order, dtype=dt, bc_params=bc_params_table(Q0), turbulence="SST")
+    Ud = torch.as_tensor(U, dtype=dt, device="cuda").contiguous()
+    kd = torch.as_tensor(k, dtype=dt, device="cuda").contiguous()
+    wd = torch.as_tensor(w, dtype=dt, device="cuda").contiguous()
+    dtd = dm.local_time_step(Ud, mu=mu)
+    nutd, mutd = torch.zeros_like(kd), torch.zeros_like(kd)
+    dm.sst_step(Ud, kd, wd, dtd, nutd, mutd, mu=mu)
+    e = lambda a, b, x: np.abs(a - b).max() / np.abs(b - x).max()  # noqa: E731
+    kg, wg = kd.double().cpu().numpy(), wd.double().cpu().numpy()
+    tol = 1e-9 if prec == "f64" else 1e-3
+    assert e(kg, kn, k) < tol and e(wg, wn, w) < tol, f"k {e(kg, kn, k):.3e} omega {e(wg, wn, w):.3e}"

@@ -132,3 +132,28 @@ def test_outlier_zeroing_matches_median_rule():
     assert R.suppress_outliers(res, np.ones((1, 4, 5)))[0, 3, 0] == 2.5e4
     res[0, 3, 0] = 2.5001e4
     assert R.suppress_outliers(res, np.ones((1, 4, 5)))[0, 3, 0] == 0.0
+
+
+def test_outlier_median_is_the_sorted_middle_with_nan_last():
+    """troubled_cell.py:347-379 sorts each (cell, variable) column and takes the middle value(s):
+    equal to np.median without NaN (tests/unit/test_troubled_cell.py:28-47), and with NaN the
+    middle of the sorted order with NaN placed last (Python's sorted with a NaN-last key)."""
+    rng = np.random.default_rng(5)
+    for ns in (1, 8, 27, 64):
+        res = rng.standard_normal((30, ns, 3))
+        assert np.array_equal(R.median_abs_over_sps(res)[:, 0], np.median(np.abs(res), axis=1))
+        for c in range(30):
+            res[c, rng.choice(ns, size=rng.integers(0, ns + 1), replace=False), c % 3] = np.nan
+        med = R.median_abs_over_sps(res)[:, 0]
+        for c in range(30):
+            for v in range(3):
+                col = sorted(np.abs(res[c, :, v]), key=lambda x: (np.isnan(x), x))
+                want = col[ns // 2] if ns % 2 else 0.5 * (col[ns // 2 - 1] + col[ns // 2])
+                assert (np.isnan(want) and np.isnan(med[c, v])) or want == med[c, v]
+    # a NaN median zeroes nothing (np.maximum propagates it); a finite one still zeroes the spike
+    res = np.ones((2, 8, 1))
+    res[:, 7, 0] = 1e9
+    res[0, :5, 0] = np.nan
+    res[1, :2, 0] = np.nan
+    out = R.suppress_outliers(res, np.ones((2, 8, 1)))
+    assert out[0, 7, 0] == 1e9 and out[1, 7, 0] == 0.0
