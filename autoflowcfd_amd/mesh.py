@@ -140,17 +140,22 @@ def graded_coords(n: int, length: float, ratio: Optional[float]) -> np.ndarray:
 
 def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
             displacement=0.0, seed=0, permute=True, bc_by_plane=None,
-            periodic=False) -> Mesh:
-    """Structured hex box split into tets or y-extruded prisms.
+            periodic=False, extrude="y") -> Mesh:
+    """Structured hex box split into tets or extruded prisms.
 
     Args:
-        split: "tet" (6 per hex) or "prism" (2 per hex, extruded along y).
+        split: "tet" (6 per hex) or "prism" (2 per hex, extruded along ``extrude``).
         grading_y: geometric growth ratio of y spacing away from y=0 (wall-normal grading).
         displacement: relative amplitude of the seeded sinusoidal displacement of interior nodes
             (0.1 -> +-10% of the local spacing per axis).
         permute: apply a seeded random permutation to node ids and to cell order.
         bc_by_plane: {"x0","x1","y0","y1","z0","z1"} -> BC type code; default FARFIELD.
-        periodic: pair x0/x1, y0/y1, z0/z1 planes into interior records (triply periodic box).
+        periodic: True pairs the x0/x1, y0/y1 and z0/z1 planes into interior records (triply
+            periodic box); a string of axis letters ("x", "xz") pairs only those planes, the others
+            keep their ``bc_by_plane`` codes.
+        extrude: prism extrusion axis, "y" (wall-normal, tests/validation/_channel_mesh.py:159-229)
+            or "x" (along the periodic direction, tests/validation/_periodic_mesh.py:22-60: the
+            x planes are then single triangle caps).
     """
     rng = np.random.default_rng(seed)
     xs = np.linspace(0.0, Lx, nx + 1)
@@ -194,10 +199,16 @@ def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
     elif split == "prism":
         tri_a = [(0, 0), (1, 0), (1, 1)]
         tri_b = [(0, 0), (1, 1), (0, 1)]
+        if extrude == "y":      # triangles in (x, z), layers in y
+            corner = lambda dp, dq, de: gid(I + dp, J + de, K + dq)  # noqa: E731
+        elif extrude == "x":    # triangles in (y, z), layers in x
+            corner = lambda dp, dq, de: gid(I + de, J + dp, K + dq)  # noqa: E731
+        else:
+            raise ValueError(extrude)
         out = []
         for tri in (tri_a, tri_b):
-            vv = [gid(I + di, J, K + dk) for di, dk in tri]
-            ww = [gid(I + di, J + 1, K + dk) for di, dk in tri]
+            vv = [corner(dp, dq, 0) for dp, dq in tri]
+            ww = [corner(dp, dq, 1) for dp, dq in tri]
             out.append(np.stack(vv + ww, axis=1))
         prisms = np.stack(out, axis=1).reshape(-1, 6)
     else:
@@ -217,7 +228,7 @@ def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
     tets = fix_tet_orientation(tets, nodes) if len(tets) else tets
     mesh = Mesh(nodes=nodes, prism_conn=prisms.astype(np.int64), tet_conn=tets.astype(np.int64))
     mesh.meta.update(dict(nx=nx, ny=ny, nz=nz, L=(Lx, Ly, Lz), split=split, seed=seed,
-                          grading_y=grading_y, displacement=displacement))
+                          grading_y=grading_y, displacement=displacement, extrude=extrude))
     build_faces(mesh, bc_by_plane or {}, (Lx, Ly, Lz), periodic=periodic)
     return mesh
 
@@ -354,7 +365,8 @@ def build_topology(mesh: Mesh) -> FaceRecords:
 
 
 def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_rel=1e-7):
-    """Topology plus face-exact box-plane boundary classification (or triple periodicity)."""
+    """Topology plus face-exact box-plane boundary classification and periodic pairing
+    (``periodic``: True for all three axes, or a string of axis letters)."""
     fr = build_topology(mesh)
     bnd = np.flatnonzero(fr.is_boundary)
     Lx, Ly, Lz = L
@@ -369,31 +381,34 @@ def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_
         raise RuntimeError("boundary face off every box plane")
 
     keep = np.ones(fr.n_faces, dtype=bool)
-    if periodic:
-        for ax, (lo, hi), Lax in ((0, ("x0", "x1"), Lx), (1, ("y0", "y1"), Ly), (2, ("z0", "z1"), Lz)):
-            ia = bnd[plane == lo]
-            ib = bnd[plane == hi]
-            shift = np.zeros(3)
-            shift[ax] = Lax
-            if len(ia) != len(ib):
-                raise RuntimeError("periodic planes are not congruent")
-            ka = np.round((fr.center[ia] + shift) / tol).astype(np.int64)
-            kb = np.round(fr.center[ib] / tol).astype(np.int64)
-            da = {tuple(k): i for k, i in zip(ka, ia)}
-            for k, fb in zip(kb, ib):
-                fa = da.get(tuple(k))
-                if fa is None:
-                    raise RuntimeError("periodic pairing failed (face_connectivity.py:279-425)")
-                fr.neighbor[fa] = fr.owner[fb]
-                fr.neighbor_code[fa] = fr.owner_code[fb]
-                fr.translation[fa] = shift
-                keep[fb] = False
-        wall_nodes = np.zeros(0, np.int64)
-    else:
-        fr.bc[bnd] = [bc_by_plane.get(p, BC_FARFIELD) for p in plane]
-        wall_planes = [p for p, t in bc_by_plane.items() if t == BC_WALL]
-        wall_nodes = (np.unique(fr.node_ids[bnd][np.isin(plane, wall_planes)])
-                      if wall_planes else np.zeros(0, np.int64))
+    axes = "xyz" if periodic is True else (periodic or "")
+    paired = np.zeros(len(bnd), dtype=bool)
+    for ax, (lo, hi), Lax in ((0, ("x0", "x1"), Lx), (1, ("y0", "y1"), Ly), (2, ("z0", "z1"), Lz)):
+        if "xyz"[ax] not in axes:
+            continue
+        ia = bnd[plane == lo]
+        ib = bnd[plane == hi]
+        paired |= (plane == lo) | (plane == hi)
+        shift = np.zeros(3)
+        shift[ax] = Lax
+        if len(ia) != len(ib):
+            raise RuntimeError("periodic planes are not congruent")
+        ka = np.round((fr.center[ia] + shift) / tol).astype(np.int64)
+        kb = np.round(fr.center[ib] / tol).astype(np.int64)
+        da = {tuple(k): i for k, i in zip(ka, ia)}
+        for k, fb in zip(kb, ib):
+            fa = da.get(tuple(k))
+            if fa is None:
+                raise RuntimeError("periodic pairing failed (face_connectivity.py:279-425)")
+            fr.neighbor[fa] = fr.owner[fb]
+            fr.neighbor_code[fa] = fr.owner_code[fb]
+            fr.translation[fa] = shift
+            keep[fb] = False
+    rest = bnd[~paired]
+    fr.bc[rest] = [bc_by_plane.get(p, BC_FARFIELD) for p in plane[~paired]]
+    wall_planes = [p for p, t in bc_by_plane.items() if t == BC_WALL and p[0] not in axes]
+    wall_nodes = (np.unique(fr.node_ids[rest][np.isin(plane[~paired], wall_planes)])
+                  if wall_planes else np.zeros(0, np.int64))
     mesh.wall_nodes = wall_nodes
     mesh.faces = FaceRecords(
         owner=fr.owner[keep], neighbor=fr.neighbor[keep], owner_code=fr.owner_code[keep],

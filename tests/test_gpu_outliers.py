@@ -99,7 +99,7 @@ def test_inviscid_kernel_zeroing_matches_oracle(kind, order, prec):
     from oracle import fr_ops
     from oracle.faces import build_flat_faces
     from oracle.geometry import Geometry
-    from oracle.residual import inviscid_residual
+    from oracle.residual import inviscid_residual, suppress_outliers
     print(f"[gpu] inviscid outliers {kind} P{order} {prec}", flush=True)
     mesh = wall_box(seed=1) if kind == "prism" else tet_box(3, seed=1)
     ops = fr_ops.generate_operators(order)
@@ -120,7 +120,17 @@ def test_inviscid_kernel_zeroing_matches_oracle(kind, order, prec):
     out = dm.inviscid_residual(Ud, suppress=True).double().cpu().numpy()
     rawd = dm.inviscid_residual(Ud, suppress=False).double().cpu().numpy()
     zeroed_dev = (out == 0) & (rawd != 0)
-    assert np.array_equal(zeroed_dev, zeroed_ref), f"{int((zeroed_dev != zeroed_ref).sum())} decisions differ"
+    # The decisions are taken on the kernel's own residual: the reference rule applied by the
+    # oracle to the device's unsuppressed residual must zero exactly the same entries.  (Against
+    # the oracle's residual, decisions may differ where the median of a cell's column is round-off:
+    # the collapsed P2 tet operators reach 4e5, so that round-off differs between summation
+    # orders by O(1) and moves the threshold of the round-off-sized entries.)
+    with np.errstate(invalid="ignore"):
+        rule = suppress_outliers(rawd, U)
+    zeroed_rule = (rule == 0) & (rawd != 0)
+    assert zeroed_rule.sum() > 0
+    assert np.array_equal(zeroed_dev, zeroed_rule), f"{int((zeroed_dev != zeroed_rule).sum())} decisions differ"
+    assert np.array_equal(out, rule)
     err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
     assert err < (1e-10 if prec == "f64" else 1e-5), f"rel L2 {err:.3e}"
 
@@ -176,5 +186,8 @@ def test_sst_transport_zeroing_matches_oracle(order, prec):
     dm.sst_step(Ud, kd, wd, dtd, nutd, mutd, mu=mu)
     e = lambda a, b, x: np.abs(a - b).max() / np.abs(b - x).max()  # noqa: E731
     kg, wg = kd.double().cpu().numpy(), wd.double().cpu().numpy()
-    tol = 1e-9 if prec == "f64" else 1e-3
+    # FP32: the transport contractions run in the storage precision, so the increments carry
+    # ~1e-3 of their maximum (measured 1.05e-3 at P2); a zeroing decision taken differently would
+    # leave an error of the size of the increment itself
+    tol = 1e-9 if prec == "f64" else 3e-3
     assert e(kg, kn, k) < tol and e(wg, wn, w) < tol, f"k {e(kg, kn, k):.3e} omega {e(wg, wn, w):.3e}"
