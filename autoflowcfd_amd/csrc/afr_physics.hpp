@@ -51,7 +51,7 @@ constexpr double kGamma = 1.4;
 constexpr double kRair = 287.0;
 
 enum BcCode { BC_INTERIOR = -1, BC_WALL = 0, BC_FARFIELD = 1, BC_INLET = 2, BC_OUTLET = 3,
-              BC_SYMMETRY = 4, BC_SLIP_WALL = 5, BC_EXTRAP = 6 };
+              BC_SYMMETRY = 4, BC_SLIP_WALL = 5, BC_EXTRAP = 6, BC_MOVING_WALL = 7 };
 
 template <typename real>
 __host__ __device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : b; }
@@ -208,15 +208,25 @@ __host__ __device__ __forceinline__ void ausm_up(const real qL[5], const real qR
     out[4] = mass * (up ? hL : hR);
 }
 
-// primitive ghost state of boundary code bc; bcp = (8,5) per-code reference state
+// primitive ghost state of a boundary slot (src/autoflowcfd/boundary/fr_ghost_state.py:26-189).
+// bc = type | (row << 4): the boundary type (BcCode) and the row of the (8, 5) table bcp that holds
+// its reference state, so that boundary groups of one type may carry states of their own
+// (FARFIELD / INLET: the state; OUTLET: the pressure in column 4; MOVING_WALL: the wall velocity in
+// columns 1-3, ghost velocity 2 v_wall - v as fr_ghost_state.py:26-53 forms it).
 template <typename real>
-__host__ __device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real n[3], const real* bcp, real qg[5]) {
+__host__ __device__ __forceinline__ void ghost_state(int bc, const real qi[5], const real n[3], const double* bcp,
+                                                     real qg[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) qg[v] = qi[v];
+    const double* p = bcp + 5 * (bc >> 4);
     real vn = qi[1] * n[0] + qi[2] * n[1] + qi[3] * n[2];
-    switch (bc) {
+    switch (bc & 15) {
         case BC_WALL:
             qg[1] = -qi[1]; qg[2] = -qi[2]; qg[3] = -qi[3];
+            break;
+        case BC_MOVING_WALL:
+#pragma unroll
+            for (int v = 1; v < 4; ++v) qg[v] = real(2) * (real)p[v] - qi[v];
             break;
         case BC_SLIP_WALL:
         case BC_SYMMETRY:
@@ -226,16 +236,16 @@ __host__ __device__ __forceinline__ void ghost_state(int bc, const real qi[5], c
             break;
         case BC_FARFIELD:
 #pragma unroll
-            for (int v = 0; v < 5; ++v) qg[v] = bcp[BC_FARFIELD * 5 + v];
+            for (int v = 0; v < 5; ++v) qg[v] = (real)p[v];
             break;
         case BC_INLET:
             if (vn < real(0)) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) qg[v] = bcp[BC_INLET * 5 + v];
+                for (int v = 0; v < 5; ++v) qg[v] = (real)p[v];
             }
             break;
         case BC_OUTLET:
-            if (vn > real(0)) qg[4] = bcp[BC_OUTLET * 5 + 4];
+            if (vn > real(0)) qg[4] = (real)p[4];
             break;
         default:  // BC_EXTRAP and anything else: zero-gradient copy
             break;

@@ -171,7 +171,9 @@ class DeviceMesh:
             return torch.as_tensor(np.ascontiguousarray(x, dtype=np.int32)).to(self.device)
 
         self.slot_nb, self.slot_blk = devi(sl.nb), devi(sl.blk)
-        self.slot_bc = devi(sl.bc)
+        # boundary slots: type | (row << 4), the row of the boundary-state table holding the slot's
+        # reference state (the type code itself unless boundary groups set their own rows)
+        self.slot_bc = devi(np.where(sl.bc >= 0, sl.bc | (np.maximum(sl.rows_of_bc(), 0) << 4), sl.bc))
         self.slot_tn = dev(sl.tn)
         self.rows = dev(sl.rows)
         # sparse rows plus one trailing all-zero row (index n_rows) for lanes without a source
@@ -362,9 +364,9 @@ class DeviceMesh:
 
     def _build_wall_records(self, dev, devi):
         """Owner-primary WALL records of the force integral (fr_coefficients.py:108-112)."""
-        from .mesh import BC_WALL
+        from .mesh import is_wall
         fr = self.mesh.faces
-        sel = np.flatnonzero((fr.bc == BC_WALL) & self.owner_primary(self.mesh))
+        sel = np.flatnonzero(is_wall(fr.bc) & self.owner_primary(self.mesh))
         self.n_wall_rec = len(sel)
         if not len(sel):
             return
@@ -419,13 +421,13 @@ class DeviceMesh:
         (src/autoflowcfd/fr/face_flux_points_merge.py:101-110); every other record takes a
         Neumann ghost in the transport (src/autoflowcfd/core/turbulence/transport_kernel.py:103-170).
         """
-        from .mesh import BC_WALL
+        from .mesh import is_wall
         mesh, T, C = self.mesh, self.T, self.n_cells
         fr = mesh.faces
         F = fr.n_faces
         bnd = fr.neighbor < 0
         primary = self.owner_primary(mesh)
-        flags = ((~bnd & primary).astype(np.int32)) | ((fr.bc == BC_WALL).astype(np.int32) << 1)
+        flags = ((~bnd & primary).astype(np.int32)) | (is_wall(fr.bc).astype(np.int32) << 1)
         code_to_slot = np.full(6, -1, dtype=np.int64)
         for si, code in enumerate(SLOT_CODES[0]):
             code_to_slot[code] = si
@@ -591,8 +593,11 @@ class DeviceMesh:
             raise ValueError("interior slot without a source cell")
         if np.any((sl.blk + 1) * T.nfp > n_rows):
             raise ValueError("slot row block out of range")
-        if np.any((sl.bc < -2) | (sl.bc > 6)):
+        if np.any((sl.bc < -2) | (sl.bc > 7)):
             raise ValueError("unknown boundary code")
+        rows = sl.rows_of_bc()
+        if np.any((sl.bc >= 0) & ((rows < 0) | (rows >= 8))):
+            raise ValueError("boundary-state row out of range (8 rows per case)")
         if np.any(sl.bc[:n_prism] == -2):
             raise ValueError("prism with an unused face slot")
 

@@ -21,6 +21,8 @@ slots index it.  A prism quad face split between two different cells
 
 from dataclasses import dataclass
 
+from typing import Optional
+
 import numpy as np
 
 from .fr_tables import face_pts, gl
@@ -47,6 +49,11 @@ class Slots:
     n_blocks: int
     keyed_slots: int
     split_slots: int
+    bc_row: Optional[np.ndarray] = None  # (C,5) int32 boundary-state table row (None: the BC code)
+
+    def rows_of_bc(self) -> np.ndarray:
+        """Boundary-state table row of every slot (its BC code unless set; < 0 off the boundary)."""
+        return self.bc if self.bc_row is None else self.bc_row
 
 
 def _cells_nodes(mesh, cells):
@@ -231,6 +238,9 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slo
     is_own = role == 0
     bc = np.where(valid, np.where(is_own, fr.bc[r], -1), BC_NONE).astype(np.int32)
     bc[valid & bnd[r]] = fr.bc[r][valid & bnd[r]]
+    bc_row = None
+    if fr.bc_row is not None:
+        bc_row = np.where(valid & bnd[r], fr.bc_row[r], -1).astype(np.int32)
     sgn = np.where(is_own, 1.0, -1.0)
     tn = fr.normal[r] * sgn[..., None]
     trans = fr.translation[r] * sgn[..., None]      # search point = own FP + trans
@@ -372,7 +382,7 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slo
                 fb = (al < 0.5)
                 fallback[g, si] = np.sum(fb.astype(np.int64) << np.arange(nfp), axis=1).astype(np.int32)
     return Slots(nb=nb, blk=blk, mask=mask, bc=bc, tn=tn, adjrow=adjrow, fallback=fallback, rows=rows,
-                 n_blocks=n_blocks, keyed_slots=keyed_slots, split_slots=len(sc))
+                 n_blocks=n_blocks, keyed_slots=keyed_slots, split_slots=len(sc), bc_row=bc_row)
 
 
 def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
@@ -395,6 +405,8 @@ def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
 
     out = {k: alloc(getattr(s0, k), f) for k, f in (("nb", -1), ("blk", 0), ("mask", 0), ("bc", -2),
                                                      ("tn", 0.0), ("fallback", 0), ("adjrow", 0.0))}
+    any_rows = any(sl.bc_row is not None for sl in slot_list)
+    out["bc_row"] = np.full(out["bc"].shape, -1, dtype=np.int32) if any_rows else None
     rows, blk_off = [], 0
     for i, sl in enumerate(slot_list):
         Pi = int(P[i])
@@ -410,11 +422,14 @@ def concat_slots(slot_list, n_prism_list, n_tet_list, nfp):
         for k in ("mask", "bc", "tn", "fallback", "adjrow"):
             if out[k] is not None:
                 out[k][g] = getattr(sl, k)
+        if any_rows:
+            out["bc_row"][g] = np.where(sl.bc >= 0, sl.rows_of_bc(), -1)
         rows.append(sl.rows)
         blk_off += sl.rows.shape[0] // nfp
     return Slots(nb=out["nb"], blk=out["blk"], mask=out["mask"], bc=out["bc"], tn=out["tn"], adjrow=out["adjrow"],
                  fallback=out["fallback"], rows=np.concatenate(rows), n_blocks=blk_off,
-                 keyed_slots=sum(s.keyed_slots for s in slot_list), split_slots=sum(s.split_slots for s in slot_list))
+                 keyed_slots=sum(s.keyed_slots for s in slot_list), split_slots=sum(s.split_slots for s in slot_list),
+                 bc_row=out["bc_row"])
 
 
 def slots_for(mesh, order, storage="locality"):

@@ -43,9 +43,17 @@ BC_OUTLET = 3
 BC_SYMMETRY = 4
 BC_SLIP_WALL = 5
 BC_EXTRAP = 6   # ghost = owner trace (DefaultGhostProvider, src/autoflowcfd/core/fr_residual/inviscid.py:92-104)
+BC_MOVING_WALL = 7  # no-slip wall with a wall velocity (fr_ghost_state.py:26-53, wall_velocity)
 BC_NAMES = {BC_WALL: "WALL", BC_FARFIELD: "FARFIELD", BC_INLET: "INLET",
             BC_OUTLET: "OUTLET", BC_SYMMETRY: "SYMMETRY", BC_SLIP_WALL: "SLIP_WALL",
-            BC_EXTRAP: "EXTRAPOLATE"}
+            BC_EXTRAP: "EXTRAPOLATE", BC_MOVING_WALL: "MOVING_WALL"}
+N_BC_ROWS = 8   # rows of the per-case boundary-state table
+
+
+def is_wall(bc):
+    """No-slip wall records (resting or moving): force integral, k Dirichlet, wall distance."""
+    bc = np.asarray(bc)
+    return (bc == BC_WALL) | (bc == BC_MOVING_WALL)
 
 
 @dataclass
@@ -61,6 +69,11 @@ class FaceRecords:
     node_ids: np.ndarray       # (F,3) sorted node triple
     translation: np.ndarray    # (F,3) periodic shift owner->neighbour image (0 if none)
     bc: np.ndarray             # (F,) int64 BC type code, BC_INTERIOR for interior records
+    bc_row: Optional[np.ndarray] = None  # (F,) row of the boundary-state table (None: the type code)
+
+    def rows(self) -> np.ndarray:
+        """Row of the boundary-state table of every record (its type code unless set)."""
+        return self.bc if self.bc_row is None else self.bc_row
 
     @property
     def n_faces(self) -> int:
@@ -406,14 +419,15 @@ def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_
             keep[fb] = False
     rest = bnd[~paired]
     fr.bc[rest] = [bc_by_plane.get(p, BC_FARFIELD) for p in plane[~paired]]
-    wall_planes = [p for p, t in bc_by_plane.items() if t == BC_WALL and p[0] not in axes]
+    wall_planes = [p for p, t in bc_by_plane.items() if is_wall(t) and p[0] not in axes]
     wall_nodes = (np.unique(fr.node_ids[rest][np.isin(plane[~paired], wall_planes)])
                   if wall_planes else np.zeros(0, np.int64))
     mesh.wall_nodes = wall_nodes
     mesh.faces = FaceRecords(
         owner=fr.owner[keep], neighbor=fr.neighbor[keep], owner_code=fr.owner_code[keep],
         neighbor_code=fr.neighbor_code[keep], normal=fr.normal[keep], area=fr.area[keep],
-        center=fr.center[keep], node_ids=fr.node_ids[keep], translation=fr.translation[keep], bc=fr.bc[keep])
+        center=fr.center[keep], node_ids=fr.node_ids[keep], translation=fr.translation[keep], bc=fr.bc[keep],
+        bc_row=None if fr.bc_row is None else fr.bc_row[keep])
     return mesh.faces
 
 
@@ -436,7 +450,7 @@ def concat_meshes(meshes):
     P_tot = int(p_off[-1])
     nodes, prisms, tets = [], [], []
     fields = {k: [] for k in ("owner", "neighbor", "owner_code", "neighbor_code", "normal", "area", "center",
-                              "node_ids", "translation", "bc")}
+                              "node_ids", "translation", "bc", "bc_row")}
     wall = []
     wall_off = [0]
     rec_off = [0]
@@ -461,6 +475,7 @@ def concat_meshes(meshes):
         fields["node_ids"].append(fr.node_ids + n_off[i])
         fields["translation"].append(fr.translation)
         fields["bc"].append(fr.bc)
+        fields["bc_row"].append(fr.rows())
         rec_off.append(rec_off[-1] + fr.n_faces)
         if m.wall_nodes is not None and len(m.wall_nodes):
             wall.append(m.wall_nodes + n_off[i])
@@ -497,7 +512,8 @@ def renumber_cells(mesh, prism_order=None, tet_order=None):
     nb = np.asarray(fr.neighbor)
     out.faces = FaceRecords(owner=new_of_old[np.asarray(fr.owner)], neighbor=np.where(nb >= 0, new_of_old[np.maximum(nb, 0)], -1),
                             owner_code=fr.owner_code, neighbor_code=fr.neighbor_code, normal=fr.normal, area=fr.area,
-                            center=fr.center, node_ids=fr.node_ids, translation=fr.translation, bc=fr.bc)
+                            center=fr.center, node_ids=fr.node_ids, translation=fr.translation, bc=fr.bc,
+                            bc_row=fr.bc_row)
     out.wall_nodes = mesh.wall_nodes
     out.meta = dict(mesh.meta)
     return out, new_of_old

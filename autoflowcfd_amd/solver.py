@@ -20,6 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from .boundary import as_mesh, boundary_table
 from .device import DeviceMesh, bc_params_table, locality_renumbered
 from .mesh import BC_FARFIELD, BC_INLET, BC_OUTLET
 
@@ -81,14 +82,22 @@ class GPUFRSolver:
         self.cfl = cfl
         self.freestream = {"rho_inf": rho_inf, "vel_inf": vel_inf, "p_inf": p_inf}
         Q_free = np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf])
-        overrides = dict(boundary_ghost_provider or {})
+        mesh = as_mesh(mesh)
+        self.mesh = mesh
+        provider = boundary_ghost_provider if hasattr(boundary_ghost_provider, "group_code") else None
+        overrides = {} if provider is not None else dict(boundary_ghost_provider or {})
         overrides.update(bc_overrides or {})
-        Q_inlet = overrides.get(BC_INLET)
-        p_out = overrides.get(BC_OUTLET)
-        if BC_FARFIELD in overrides:
-            Q_free = np.asarray(overrides[BC_FARFIELD], dtype=np.float64)
-        p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
-        bcp = bc_params_table(Q_free, Q_inlet, p_out)
+        # boundary groups (reference meshes) or a reference provider: one table row per distinct state
+        bcp = boundary_table(mesh, rho_inf, vel_inf, p_inf, bc_overrides=overrides, provider=provider)
+        if bcp is not None and ref_state is None:
+            ref_state = Q_free   # the table's rows follow the groups, not the type codes
+        if bcp is None:
+            Q_inlet = overrides.get(BC_INLET)
+            p_out = overrides.get(BC_OUTLET)
+            if BC_FARFIELD in overrides:
+                Q_free = np.asarray(overrides[BC_FARFIELD], dtype=np.float64)
+            p_out = None if p_out is None else float(np.asarray(p_out).ravel()[-1])
+            bcp = bc_params_table(Q_free, Q_inlet, p_out)
         if storage == "locality":
             dev_mesh, new_of_old = locality_renumbered(mesh)
         elif storage == "input":

@@ -219,6 +219,7 @@ class FlatFaces:
     ow_src1_cell: np.ndarray
     ow_src1_mat: np.ndarray
     bc: np.ndarray
+    bc_row: np.ndarray  # (F,) row of the boundary-state table (the BC code unless groups set it)
     area: np.ndarray
 
 
@@ -350,7 +351,7 @@ def build_flat_faces(mesh, ops) -> FlatFaces:
         true_normal=np.repeat(fr.normal[:, None, :], nfp, axis=1),
         nb_src0_cell=nb0c, nb_src0_mat=nb0m, nb_src1_idx=nb1i, nb_src1_cell=nb1c, nb_src1_mat=nb1m,
         ow_src0_cell=ow0c, ow_src0_mat=ow0m, ow_src1_idx=ow1i, ow_src1_cell=ow1c, ow_src1_mat=ow1m,
-        bc=fr.bc.copy(), area=fr.area.copy(),
+        bc=fr.bc.copy(), bc_row=np.asarray(fr.rows()).copy(), area=fr.area.copy(),
     )
 
 

@@ -12,7 +12,7 @@ Only tests/, smoke() and bench.py's CPU baseline may use this module.
 
 import numpy as np
 
-from .residual import BC_WALL, physical_gradient
+from .residual import BC_MOVING_WALL, BC_WALL, physical_gradient
 from .time_step import update_primitives
 
 
@@ -24,7 +24,7 @@ def wall_forces(U, geom, ops, fl, mu, mu_t=None, include_viscous=True):
     rel = rel / rel.sum()
     gQ = physical_gradient(Q, geom, ops) if include_viscous else None
     fp_, fv_ = np.zeros(3), np.zeros(3)
-    sel = np.flatnonzero((fl.bc == BC_WALL) & fl.owner_primary)
+    sel = np.flatnonzero(((fl.bc == BC_WALL) | (fl.bc == BC_MOVING_WALL)) & fl.owner_primary)
     for f in sel:
         c = fl.owner[f]
         ti = 0 if c < fl.n_prism else 1

@@ -22,8 +22,8 @@ inviscid_kernel.py:40-52); results agree to FP64 rounding.
 
 import numpy as np
 
-from autoflowcfd_amd.mesh import (BC_EXTRAP, BC_FARFIELD, BC_INLET, BC_OUTLET, BC_SLIP_WALL,
-                                  BC_SYMMETRY, BC_WALL)
+from autoflowcfd_amd.mesh import (BC_EXTRAP, BC_FARFIELD, BC_INLET, BC_MOVING_WALL, BC_OUTLET,
+                                  BC_SLIP_WALL, BC_SYMMETRY, BC_WALL)
 
 from .faces import gather_sources
 
@@ -127,32 +127,37 @@ def ausm_up(qL, qR, n):
     return out
 
 
-def ghost_state(bc, Q_int, normal, bc_params):
-    """Primitive ghost states; fr_ghost_state.py:26-189 (constant wall velocity 0).
+def ghost_state(bc, Q_int, normal, bc_params, rows=None):
+    """Primitive ghost states; fr_ghost_state.py:26-189.
 
-    bc: (K,) BC type codes, Q_int/normal: (K, n_fp, 5|3),
-    bc_params: dict code -> (5,) state (FARFIELD / INLET: the state; OUTLET: p in slot 4).
+    bc: (K,) BC type codes, Q_int/normal: (K, n_fp, 5|3), rows: (K,) row of each record's reference
+    state in bc_params (default: its type code, i.e. one state per type), bc_params: dict row ->
+    (5,) state (FARFIELD / INLET: the state; OUTLET: p in slot 4; MOVING_WALL: the wall velocity in
+    slots 1-3, ghost velocity 2 v_wall - v as fr_ghost_state.py:26-53 forms it).
     """
+    rows = bc if rows is None else rows
     out = Q_int.copy()
     vel = Q_int[..., 1:4]
     vn = np.sum(vel * normal, axis=-1, keepdims=True)
-    for code in np.unique(bc):
-        sel = bc == code
+    for code, row in sorted(set(zip(np.asarray(bc).tolist(), np.asarray(rows).tolist()))):
+        sel = (bc == code) & (rows == row)
         if code == BC_EXTRAP:
             continue
         if code == BC_WALL:
             out[sel, :, 1:4] = -vel[sel]
+        elif code == BC_MOVING_WALL:
+            out[sel, :, 1:4] = 2.0 * np.asarray(bc_params[row])[1:4] - vel[sel]
         elif code in (BC_SLIP_WALL, BC_SYMMETRY):
             out[sel, :, 1:4] = vel[sel] - 2.0 * vn[sel] * normal[sel]
         elif code == BC_FARFIELD:
-            out[sel] = np.asarray(bc_params[BC_FARFIELD])
+            out[sel] = np.asarray(bc_params[row])
         elif code == BC_INLET:
             inflow = vn[sel][..., 0] < 0.0
-            out[sel] = np.where(inflow[..., None], np.asarray(bc_params[BC_INLET]), Q_int[sel])
+            out[sel] = np.where(inflow[..., None], np.asarray(bc_params[row]), Q_int[sel])
         elif code == BC_OUTLET:
             outflow = vn[sel][..., 0] > 0.0
             q = Q_int[sel].copy()
-            q[..., 4] = np.where(outflow, bc_params[BC_OUTLET][4], q[..., 4])
+            q[..., 4] = np.where(outflow, bc_params[row][4], q[..., 4])
             out[sel] = q
         else:
             raise ValueError(f"unknown BC code {code}")
@@ -225,7 +230,7 @@ def ghost_states_at_owner(ops, geom, fl, Q, bc_params):
     sel = np.flatnonzero(fl.is_boundary & fl.owner_primary)
     if len(sel):
         (Qo,) = _side_extrap(ops, geom, fl, fl.owner[sel], fl.owner_axis[sel], fl.owner_side[sel], [Q])
-        Qg[sel] = ghost_state(fl.bc[sel], Qo, fl.true_normal[sel], bc_params)
+        Qg[sel] = ghost_state(fl.bc[sel], Qo, fl.true_normal[sel], bc_params, fl.bc_row[sel])
     return Qg
 
 
