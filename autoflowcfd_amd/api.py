@@ -1,8 +1,10 @@
 """run_steady: the AutoFlowCFDAPI.run_steady surface (src/autoflowcfd/api.py:123-198) on the GPU backend.
 
-The mesh argument is an autoflowcfd_amd.mesh.Mesh (or a path readable by
-autoflowcfd_amd.meshio.load_mesh).  Returns a SolverResult; the solver is kept on the
-returned object as ``result.solver`` for force / state queries.
+The mesh argument is the reference's ``VolumeMeshData`` (read by its attributes, with its boundary
+groups: autoflowcfd_amd.boundary), an autoflowcfd_amd.mesh.Mesh, or a path readable by
+autoflowcfd_amd.meshio.load_mesh.  With SST the wall distance is taken to the wall nodes of the
+WALL groups (src/autoflowcfd/cli/solve_wall_distance.py:30-60).  Returns a SolverResult; the solver
+is kept on the returned object as ``result.solver`` for force / state queries.
 """
 
 from .solver import GPUFRSolver, SolverResult
@@ -17,6 +19,8 @@ def run_steady(volume_mesh, backend: str = "gpu", order: int = 2, turbulence_mod
     if isinstance(volume_mesh, str):
         from .meshio import load_mesh
         volume_mesh = load_mesh(volume_mesh)
+    from .boundary import as_mesh
+    volume_mesh = as_mesh(volume_mesh)
     tm = {"none": "NONE", "sst": "SST"}.get(str(turbulence_model).lower())
     if tm is None:
         raise NotImplementedError(f"turbulence model {turbulence_model!r}: this backend implements none and sst")
