@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -823,6 +824,60 @@ int afr_residual_norm_f64(const AfrMesh* mesh, const void* R, int32_t n_vars_tot
                                                       out, st)
                        : afr_step::launch_norm<double>((const double*)R, n, n_total, d.work_part,
                                                        afr_step::kNormParts, out, st);
+    return rc ? fail("residual norm launch failed") : 0;
+}
+
+}  // extern "C"
+
+// Partial sums of afr_residual_norm_f64_noalloc: a file-scope device array, one copy per GPU, held
+// until the process ends.  Host code never passes `g_norm_scratch` itself (in host code the name
+// is a host-side stand-in, not a device address): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) double g_norm_scratch[afr_step::kNormParts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this GPU's g_norm_scratch (nullptr on failure, with the error set)
+double* norm_scratch_of_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        fail("afr_residual_norm_f64_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_norm_scratch)) != hipSuccess || !ptr) {
+            fail("afr_residual_norm_f64_noalloc: hipGetSymbolAddress(g_norm_scratch) failed");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int afr_residual_norm_f64_noalloc(const AfrMesh* mesh, const void* R, int32_t n_vars_total, double* out,
+                                  void* stream) {
+    if (!mesh || !R || !out) return fail("afr_residual_norm_f64_noalloc: null argument");
+    const AfrMeshDesc& d = mesh->d;
+    if (n_vars_total < 5) return fail("afr_residual_norm_f64_noalloc: n_vars_total < 5");
+    double* part = norm_scratch_of_current_device();
+    if (!part) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = d.n_cells * ns_of(d) * 5;
+    const double n_total = (double)d.n_cells * ns_of(d) * n_vars_total;
+    // the partition is the one of afr_residual_norm_f64: at most kNormParts partial sums, whatever n
+    const int rc = d.dtype == AFR_F32
+                       ? afr_step::launch_norm<float>((const float*)R, n, n_total, part, afr_step::kNormParts, out,
+                                                      st)
+                       : afr_step::launch_norm<double>((const double*)R, n, n_total, part, afr_step::kNormParts,
+                                                       out, st);
     return rc ? fail("residual norm launch failed") : 0;
 }
 

@@ -283,6 +283,19 @@ int afr_rk_stage(const AfrMesh *mesh, const void *U0, const void *Ua, const void
  * residual entries are zero. */
 int afr_residual_norm_f64(const AfrMesh *mesh, const void *R, int32_t n_vars_total, double *out, void *stream);
 
+/* afr_residual_norm_f64 without the mesh's work_part buffer: the FP64 partial sums go to a
+ * file-scope device array of the library (4096 doubles = 32 KiB per GPU, held until the process
+ * ends), so the call allocates nothing and needs no workspace.  No counterpart in the reference;
+ * it is for callers that cannot allocate or bring a workspace (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same partition and summation order as
+ * afr_residual_norm_f64, hence the same bits.  Every length of R fits: the partial count is
+ * capped at the array's 4096 entries whatever the mesh size, and each call writes every entry
+ * it reads, so nothing needs clearing.  Calls that overlap on one GPU (two streams, two threads)
+ * race on that array: serialise them.  Errors: null arguments, n_vars_total < 5, no current HIP
+ * device, the device address of the array not available. */
+int afr_residual_norm_f64_noalloc(const AfrMesh *mesh, const void *R, int32_t n_vars_total, double *out,
+                                  void *stream);
+
 /* One steady SSP-RK3 pseudo-step of the Navier-Stokes state U (C, ns, 5), in place:
  * dt_local, R0 = residual(U), three stages with positivity and filter, and the RMS
  * norm of R0 into norm[0] (device pointer; on a batched mesh norm[i] per case, as
