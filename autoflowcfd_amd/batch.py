@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from .device import DeviceMesh, bc_params_table
-from .face_slots import build_slots, concat_slots
+from .face_slots import concat_slots
 from .fr_tables import tables
 from .mesh import concat_meshes, locality_renumbered
 
@@ -75,11 +75,25 @@ def free_stream_coefficients(F, Q_free, reference_area, include_viscous=True):
             "F_pressure": F[:3].tolist(), "F_viscous": F[3:].tolist()}
 
 
+def case_slots(meshes, orders, host_workers=0):
+    """Slot tables of every case in the locality order GPUFRSolver stores it (face_slots.slots_for),
+    serially or in ``host_workers`` spawned processes (tables are deterministic: the same either way)."""
+    from .face_slots import slots_for
+    if host_workers and len(meshes) > 1:
+        import multiprocessing
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=host_workers, mp_context=multiprocessing.get_context("spawn")) as pool:
+            return list(pool.map(slots_for, meshes, orders))
+    return [slots_for(m, p) for m, p in zip(meshes, orders)]
+
+
 class CaseBatch:
     """Many independent steady cases of one order on one GPU, one launch per kernel per stage."""
 
     def __init__(self, cases, order=2, dtype=torch.float32, turb_model="SST", mu_molecular=1.8e-5, Pr=0.72,
-                 Pr_t=0.9, device_id=0, ref_state=None):
+                 Pr_t=0.9, device_id=0, ref_state=None, slots_list=None, host_workers=0):
+        """slots_list: each case's slot tables at this order (face_slots.slots_for), else built here,
+        in ``host_workers`` spawned processes when > 0 (NumPy only, never the GPU)."""
         if not torch.cuda.is_available():
             raise RuntimeError("CaseBatch needs a visible GPU (HIP device)")
         if not cases:
@@ -106,8 +120,9 @@ class CaseBatch:
         bcp = np.stack([bc_params_table(np.asarray(c.Q_free, dtype=np.float64)) for c in self.cases])
         # slot tables case by case: each cell gets exactly the rows its case alone has
         T = tables(order)
-        slots = concat_slots([build_slots(p, T, with_geometry=False) for p in parts], [p.n_prism for p in parts],
-                             [p.n_tet for p in parts], T.nfp)
+        if slots_list is None:
+            slots_list = case_slots([c.mesh for c in self.cases], [order] * self.n_cases, host_workers)
+        slots = concat_slots(list(slots_list), [p.n_prism for p in parts], [p.n_tet for p in parts], T.nfp)
         self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=self.ref_state,
                              slots=slots, turbulence="SST" if self.turb == "SST" else None, cell_case=cell_case)
         ns = self.dm.ns
@@ -204,3 +219,107 @@ class CaseBatch:
         F = self.wall_forces()
         return [free_stream_coefficients(F[i], c.Q_free, c.reference_area, include_viscous)
                 for i, c in enumerate(self.cases)]
+
+
+class MixedBatch:
+    """Cases of different orders (P1 and P2) stepped together.
+
+    Cases are grouped by order into one ``CaseBatch`` each (its kernels are specialised for one
+    order: per-order register and LDS budgets, so P1 work is not run at P2 occupancy), all with one
+    common reference state, and ONE HIP graph per pseudo-step holds every group's step on its own
+    branch: the branches fork from the capturing stream onto default-priority side streams and
+    join back, so the order groups run concurrently on the device and a whole mixed pseudo-step is
+    one graph launch.  ``order_of[i]`` is case i's order; results come back in case order.  A case
+    stepped alone with ``ref_state`` follows its trajectory here bit for bit."""
+
+    def __init__(self, cases, orders, dtype=torch.float32, turb_model="SST", mu_molecular=1.8e-5, Pr=0.72,
+                 Pr_t=0.9, device_id=0, ref_state=None, host_workers=0):
+        if len(cases) != len(orders) or not cases:
+            raise ValueError("one order per case, at least one case")
+        self.cases, self.order_of = list(cases), [int(p) for p in orders]
+        self.n_cases = len(self.cases)
+        self.device = torch.device(f"cuda:{device_id}")
+        self.ref_state = static_ref_state(self.cases) if ref_state is None else np.asarray(ref_state, np.float64)
+        self.members = {}
+        self.groups = {}
+        slots = case_slots([c.mesh for c in self.cases], self.order_of, host_workers)
+        for p in sorted(set(self.order_of)):
+            idx = [i for i, q in enumerate(self.order_of) if q == p]
+            self.members[p] = idx
+            self.groups[p] = CaseBatch([self.cases[i] for i in idx], order=p, dtype=dtype, turb_model=turb_model,
+                                       mu_molecular=mu_molecular, Pr=Pr, Pr_t=Pr_t, device_id=device_id,
+                                       ref_state=self.ref_state, slots_list=[slots[i] for i in idx])
+        self.history = []
+        self.iteration = 0
+        self._graph = None
+        self._side = {}
+
+    def _launch_step(self):
+        """Enqueue one pseudo-step of every group: the first on the current stream, the others on
+        side streams forked from it and joined back."""
+        cur = torch.cuda.current_stream(self.device)
+        orders = sorted(self.groups)
+        for p in orders[1:]:
+            s = self._side.setdefault(p, torch.cuda.Stream(self.device))
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                self.groups[p]._launch_step()
+        self.groups[orders[0]]._launch_step()
+        for p in orders[1:]:
+            cur.wait_stream(self._side[p])
+
+    def capture_graph(self):
+        """One eager warm-up step (counted), then record one mixed pseudo-step into a HIP graph."""
+        torch.cuda.synchronize(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self._launch_step()
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        self.history.append(self.norms())
+        self.iteration += 1
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._launch_step()
+        self._graph = g
+        return g
+
+    def norms(self):
+        """Per-case FP64 residual norms of the last step (host), in case order."""
+        out = np.zeros(self.n_cases)
+        for p, b in self.groups.items():
+            out[self.members[p]] = b.norms.cpu().numpy()
+        return out
+
+    def step(self, sync=True):
+        if self._graph is not None:
+            self._graph.replay()
+        else:
+            self._launch_step()
+        self.iteration += 1
+        if not sync:
+            return None
+        n = self.norms()
+        self.history.append(n)
+        return n
+
+    def run(self, n_steps, sync_every=1):
+        for i in range(n_steps):
+            self.step(sync=((i + 1) % sync_every == 0) or i == n_steps - 1)
+        return np.array(self.history)
+
+    def _where(self, i):
+        p = self.order_of[i]
+        return self.groups[p], self.members[p].index(i)
+
+    def case_state(self, i):
+        b, j = self._where(i)
+        return b.case_state(j)
+
+    def coefficients(self, include_viscous=True):
+        out = [None] * self.n_cases
+        for p, b in self.groups.items():
+            for j, c in enumerate(b.coefficients(include_viscous)):
+                out[self.members[p][j]] = c
+        return out

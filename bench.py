@@ -92,6 +92,20 @@ def algorithmic_bytes_visc_volume(n_sp, n_cells, w=4):
     return n_sp * (5 + 1 + 1 + 9 + 5 + 18 + 5) * w + n_cells * 6 * w
 
 
+def algorithmic_bytes_dt_local(n_sp, n_cells, w=4):
+    """Compulsory traffic of the local-time-step launch: read U (5), nu_t, det and the metric
+    flux scale per solution point, the volume per cell; write dt per point."""
+    return n_sp * (5 + 1 + 1 + 1 + 1) * w + n_cells * w
+
+
+def algorithmic_bytes_sst(n_sp, n_cells, n_records, w=4):
+    """Compulsory traffic of the SST phase (sources, transport, update rates: k_sst_cell,
+    k_sst_face, k_sst_update): read U (5), k, omega, wall distance, dt, det and adj (9) per
+    solution point, write dk/dt, domega/dt, nu_t and mu_t per point; per face record its unit
+    normal (3) and owner, owner slot and flags; per cell its eight record incidences."""
+    return n_sp * (5 + 1 + 1 + 1 + 1 + 1 + 9 + 4) * w + n_records * (3 * w + 3 * 4) + n_cells * 8 * 4
+
+
 def algorithmic_bytes_visc_iface(n_sp, n_cells, nfp, w=4):
     """Compulsory traffic of the viscous interface launch: read the 18 pack values, the volume
     residual (5) and det per point, read-modify-write R (5); per cell the five face slots'
@@ -99,10 +113,9 @@ def algorithmic_bytes_visc_iface(n_sp, n_cells, nfp, w=4):
     return n_sp * (18 + 5 + 1 + 10) * w + n_cells * (5 * (nfp * 3 + 3) * w + 5 * 6 * 4 + 6 * w)
 
 
-def cpu_baseline(order, n_sample=6, budget_s=20.0):
-    """FP64 oracle (NumPy restatement of the reference CPU path), single thread, on a
-    bounded sample: an n_sample^3 hex version of the same graded wall box, one full
-    SST RANS residual evaluation (turbulence sources + transport + inviscid + viscous)."""
+def _cpu_sample(order, n_sample, budget_s, seed):
+    """One single-threaded worker of the CPU baseline: (solution points, median seconds, evaluations)
+    of the FP64 oracle's full P2 SST RANS residual on an n_sample^3-hex graded wall box."""
     from threadpoolctl import threadpool_limits
     from oracle import fr_ops, sst
     from oracle.faces import build_flat_faces
@@ -110,7 +123,7 @@ def cpu_baseline(order, n_sample=6, budget_s=20.0):
     from oracle.residual import BC_WALL, inviscid_residual, viscous_residual
     from oracle.time_step import local_time_step, update_primitives
     from autoflowcfd_amd.device import bc_params_table
-    mesh = wall_box_mesh(n_sample, seed=1)
+    mesh = wall_box_mesh(n_sample, seed=seed)
     ops = fr_ops.generate_operators(order)
     g = Geometry(mesh, ops)
     fl = build_flat_faces(mesh, ops)
@@ -139,12 +152,28 @@ def cpu_baseline(order, n_sample=6, budget_s=20.0):
             t0 = time.perf_counter()
             residual()
             times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    n_sp = mesh.n_cells * ops.n_sps
-    return {"value": n_sp * 7 / t / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy oracle, one P{order} SST RANS residual (turbulence sources + transport + "
-                      f"inviscid + viscous), {n_sample}^3-hex graded wall box ({mesh.n_cells} prisms), "
-                      f"median of {len(times)} evaluations, {t:.3f} s each, 1 thread"}
+    return mesh.n_cells * ops.n_sps, float(np.median(times)), len(times)
+
+
+def cpu_baseline(order, n_sample=6, budget_s=20.0, workers=None):
+    """FP64 oracle (NumPy restatement of the reference CPU path) on the host cores: one
+    single-threaded process per core, each timing full SST RANS residual evaluations of its own
+    seeded n_sample^3-hex graded wall box for budget_s; the throughput is their sum (the cases
+    are independent, as the reference's one-case-per-process runs are)."""
+    import multiprocessing as mp
+    n_host = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = workers or max(1, min(16, n_host))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.starmap(_cpu_sample, [(order, n_sample, budget_s, 1 + i) for i in range(workers)])
+    gdof = sum(n_sp * 7 / t for n_sp, t, _ in res) / 1e9
+    one = res[0][0] * 7 / res[0][1] / 1e9
+    return {"value": gdof, "unit": "GDOF/s", "cores": workers, "kind": "port", "single_core_value": one,
+            "host_cpus": n_host,
+            "sample": f"FP64 NumPy oracle, P{order} SST RANS residual (turbulence sources + transport + inviscid + "
+                      f"viscous) on {workers} single-threaded processes at once, each its own seeded {n_sample}^3-hex "
+                      f"graded wall box ({res[0][0] // ((order + 1) ** 3)} prisms), median of "
+                      f"{min(r[2] for r in res)}-{max(r[2] for r in res)} evaluations of {budget_s:.0f} s per process "
+                      f"({res[0][1]:.3f} s each on one core)"}
 
 
 def traffic_from_profiles(kernel, workload="wallbox82_p2_sst_fp32"):
@@ -197,40 +226,48 @@ def served_leg(args):
 
 
 def batched_leg(args):
-    """Batched multi-case form vs the plain one-at-a-time path on the same seeded cases (fixed
-    step count, set-up included in both): solves/h and the largest Cd difference."""
+    """Batched multi-case form vs the plain one-at-a-time path on the same seeded SST cases (fixed
+    step count, set-up included in both): P1 and P2 cases in one MixedBatch (one HIP graph per
+    pseudo-step holding both order groups), solves/h and the largest Cd difference to the plain
+    path."""
     import torch
-    from autoflowcfd_amd.batch import CaseBatch, static_ref_state
+    from autoflowcfd_amd.batch import MixedBatch, static_ref_state
     from autoflowcfd_amd.serve import solve_alone, synthetic_requests
-    reqs = synthetic_requests(args.batch_cases, seed=args.serve_seed + 7, hex_range=(2e3, 2e4), rate=1e9,
+    lo, hi = (float(x) for x in args.batch_hex.split(","))
+    reqs = synthetic_requests(args.batch_cases, seed=args.serve_seed + 7, hex_range=(lo, hi), rate=1e9,
                               max_iter=args.batch_steps, rel_tol=0.0)
     ref = static_ref_state([r.case for r in reqs])
     n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
-    log(f"[bench] batched leg: {len(reqs)} cases, {n_sp} solution points, {args.batch_steps} pseudo-steps each")
-    solve_alone(reqs[:1], dtype=torch.float32, ref_state=ref)          # warm the code paths
+    log(f"[bench] batched leg: {len(reqs)} SST cases, {n_sp} solution points, {args.batch_steps} pseudo-steps each")
+    solve_alone(reqs[:1], dtype=torch.float32, turb_model="SST", ref_state=ref, use_graph=True)   # warm the paths
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain = solve_alone(reqs, dtype=torch.float32, ref_state=ref)
+    plain = solve_alone(reqs, dtype=torch.float32, turb_model="SST", ref_state=ref, use_graph=True)
     t_plain = time.perf_counter() - t0
     t0 = time.perf_counter()
-    cd = {}
-    for order in sorted({r.order for r in reqs}):
-        grp = [r for r in reqs if r.order == order]
-        b = CaseBatch([r.case for r in grp], order=order, dtype=torch.float32, turb_model="NONE", ref_state=ref)
-        b.capture_graph()                                   # one eager warm-up step, counted
-        for _ in range(args.batch_steps - 1):
-            b.step(sync=False)
-        torch.cuda.synchronize()
-        for r, c in zip(grp, b.coefficients()):
-            cd[r.rid] = c["Cd"]
-        del b
+    b = MixedBatch([r.case for r in reqs], [r.order for r in reqs], dtype=torch.float32, turb_model="SST",
+                   ref_state=ref, host_workers=args.batch_host_workers)
+    t_setup = time.perf_counter() - t0
+    b.capture_graph()                                       # one eager warm-up step, counted
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps - 1):
+        b.step(sync=False)
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t1
+    coef = b.coefficients()
     t_batch = time.perf_counter() - t0
-    log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s")
-    dcd = max(abs(cd[r.rid] - r.coefficients["Cd"]) for r in plain)
-    return {"workload": f"{len(reqs)} laminar NS cases, graded wall boxes of 2e3-2e4 hex split into prisms, P1/P2 "
-                        f"(one batch per order), {args.batch_steps} pseudo-steps each, set-up included",
+    log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s (set-up {t_setup:.2f}s, "
+        f"{t_steps / max(args.batch_steps - 1, 1) * 1e3:.2f} ms per mixed pseudo-step)")
+    dcd = max(abs(coef[r.rid]["Cd"] - r.coefficients["Cd"]) for r in plain)
+    del b
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(reqs)} SST cases, graded wall boxes of {lo:g}-{hi:g} hex (log-uniform, aspect 2-6) "
+                        f"split into prisms, P1/P2 drawn 50/50 in one mixed batch (one HIP graph per pseudo-step), "
+                        f"Mach 0.05-0.3, yaw 0-10 deg, {args.batch_steps} pseudo-steps each, set-up included",
             "solves_per_hour": len(reqs) / t_batch * 3600.0, "plain_solves_per_hour": len(reqs) / t_plain * 3600.0,
-            "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd}
+            "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd,
+            "ms_per_mixed_pseudo_step": t_steps / max(args.batch_steps - 1, 1) * 1e3,
+            "solution_points": n_sp}
 
 
 def tgv_noise(xyz, v0, seed=7, modes=4):
@@ -348,8 +385,10 @@ def main():
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--serve-host-workers", type=int, default=2, help="processes building host tables")
-    ap.add_argument("--batch-cases", type=int, default=16)
-    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-hex", default="2e3,2e4", help="hex-count range of the batched cases (lo,hi)")
+    ap.add_argument("--batch-host-workers", type=int, default=8, help="processes building the batch's host tables")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
     ap.add_argument("--tgv-p3", type=int, default=0,
                     help="run only the P3 Taylor-Green vortex leg on N^3 hex x 6 tets (prints its JSON)")
@@ -461,22 +500,30 @@ def main():
     is_quad = np.isin(fr.owner_code, (0, 1, 2)) & (fr.owner < mesh.n_prism)
     n_phys_faces = int(fr.n_faces - is_quad.sum() // 2)
     b_inv = algorithmic_bytes_inviscid(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
-    # per-kernel rooflines of the three single-kernel phases (prism-only mesh: one launch each);
-    # the dominant one (longest launch) is the headline roofline
+    # per-phase rooflines (prism-only mesh: one launch per phase, three for SST), timed with HIP
+    # events on the launch stream; PMC traffic from the committed rocprofv3 summary
     P, knz = args.order, dm.rows_knz
+    sst_kernels = [f"afr_sst::k_sst_cell<{P}, 0, float>", f"afr_sst::k_sst_face<{P}, {knz}, float>",
+                   f"afr_sst::k_sst_update<{P}, 0, float>"]
     kr = {}
-    for kname, phase, b in ((f"afr_prism::k_prism_inviscid<{P}, {knz}, float>", "inviscid", b_inv),
-                            (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
-                             algorithmic_bytes_visc_volume(n_sp, mesh.n_cells)),
-                            (f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>", "visc_interface",
-                             algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp))):
+    for kname, phase, b, parts in (
+            ("afr_step::k_dt_local<float>", "dt_local", algorithmic_bytes_dt_local(n_sp, mesh.n_cells), None),
+            (" + ".join(sst_kernels), "sst", algorithmic_bytes_sst(n_sp, mesh.n_cells, fr.n_faces), sst_kernels),
+            (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
+             algorithmic_bytes_visc_volume(n_sp, mesh.n_cells), None),
+            (f"afr_prism::k_prism_inviscid<{P}, {knz}, float>", "inviscid", b_inv, None),
+            (f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>", "visc_interface",
+             algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp), None)):
         t = float(ph[names.index(phase)]) * 1e-3
-        kr[kname] = {"bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b / t / HBM_PEAK, "traffic": traffic_from_profiles(kname), "kernel_ms": t * 1e3,
+        tr = [traffic_from_profiles(k) for k in (parts or [kname])]
+        kr[kname] = {"phase": phase, "bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": b / t / HBM_PEAK,
+                     "traffic": None if any(x is None for x in tr) else float(sum(tr)), "kernel_ms": t * 1e3,
                      "algorithmic_bytes": b}
     dominant = max(kr, key=lambda k: kr[k]["kernel_ms"])
     b_res = algorithmic_bytes_residual(n_sp, mesh.n_cells * dm.nf, n_phys_faces, dm.nfp)
     t_res = ms_per_step * 1e-3
+    res_traffic = [v["traffic"] for v in kr.values()]
     value = n_sp * 7 / t_res / 1e9
     line = {
         "metric": "P2 residual GDOF/s + %HBM roofline; served solves/h & p99 latency vs plain",
@@ -495,10 +542,14 @@ def main():
                                f"transport), {args.n}^3-hex graded wall box split into {mesh.n_cells} prisms, "
                                f"{n_sp} solution points x 7 variables",
                    "cells": mesh.n_cells, "order": args.order, "n_vars": 7},
-        "roofline": dict(kr[dominant], kernel=dominant),
+        # headline: one full residual evaluation (all seven kernels) against the HBM roofline
+        "roofline": {"bound": "hbm", "achieved": b_res / t_res / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b_res / t_res / HBM_PEAK,
+                     "traffic": None if any(x is None for x in res_traffic) else float(sum(res_traffic)),
+                     "algorithmic_bytes": b_res, "ms": t_res * 1e3,
+                     "kernel": "full residual evaluation (" + ", ".join(kr) + ")",
+                     "dominant_kernel": dominant},
         "kernel_rooflines": kr,
-        "residual_roofline": {"achieved_GBs": b_res / t_res / 1e9, "frac": b_res / t_res / HBM_PEAK,
-                              "algorithmic_bytes": b_res},
         "phase_ms": {n: float(t) for n, t in zip(names, ph)},
         "ms_per_rk3_pseudo_step": ms_rk,
         "autotuned_grids": None if tuned is None else {n: v["grid"] for n, v in tuned.items()},

@@ -10,10 +10,18 @@ CPU path (the reference's own FP64 semantics) and cites the reference
 ``file:line`` it follows.  Pins (closed forms, invariants, brute force, figures
 the reference states in its source) live in ``tests/test_oracle_*.py``.
 
-Parity unpinned (no reference fixture and no closed form pins them beyond special
-states; each says so in its docstring): the values of ``residual.inviscid_residual``
-and ``residual.viscous_residual`` on general states, ``residual.suppress_outliers``
-decisions on general fields, ``sst.source_terms``, ``sst.transport`` and
-``sst.turbulence_step`` values, ``time_step.local_time_step`` values and the
-``time_step.ns_step`` trajectory.
+Pinned beyond the reference's own special states (tests/test_oracle_pins.py,
+tests/test_oracle_convergence.py, tests/test_oracle_conditioning.py):
+``residual.inviscid_residual`` and ``residual.viscous_residual`` on discontinuous states by
+per-cell discrete conservation (prisms and tets, interior and boundary faces) and pointwise by the
+one-dimensional FR reduction on layered prisms, and on smooth states by convergence to exact
+solutions (isentropic vortex: rate P; manufactured viscous field: rate P - 1, the reference's
+unlifted BR1 gradient) on prisms; each check rejects planted mistakes.
+
+Parity unpinned (no reference fixture and no closed form pins them; each says so in its
+docstring): the pointwise distribution of the tet interface corrections (the collapsed tet scheme
+does not converge under refinement, so only its per-cell sums are pinned),
+``residual.suppress_outliers`` decisions on general fields beyond the reference's own test cases,
+``sst.source_terms``, ``sst.transport`` and ``sst.turbulence_step`` values,
+``time_step.local_time_step`` values and the ``time_step.ns_step`` trajectory.
 """

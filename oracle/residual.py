@@ -237,8 +237,10 @@ def ghost_states_at_owner(ops, geom, fl, Q, bc_params):
 def inviscid_residual(U, geom, ops, fl, bc_params, suppress=True):
     """dU/dt inviscid part (C, n_sps, 5); inviscid.py:139-345.
 
-    Pinned at special states (free stream P2 3e-5 / P3 5e-3, linear shear, uniform mass flux, the
-    reference's own test cases); parity unpinned for the values on general states."""
+    Pinned at the reference's special states (free stream P2 3e-5 / P3 5e-3, linear shear,
+    uniform mass flux), on discontinuous states by per-cell conservation and the 1-D reduction, and
+    on smooth states by convergence at rate P to the exact isentropic vortex (prisms); the
+    pointwise tet distribution is parity unpinned (only its per-cell sums are pinned)."""
     C = geom.n_cells
     Q = conserved_to_primitive(U[..., :5])
 
@@ -317,8 +319,10 @@ def viscous_residual(U, geom, ops, fl, bc_params, mu, mu_t=None, Pr=0.72, Pr_t=0
                      return_grads=False):
     """dU/dt viscous part (C, n_sps, 5); viscous_flux.py:113-302.
 
-    Pinned by the zero residual of uniform flow and closed-form stresses / heat flux; parity
-    unpinned for the values on general states."""
+    Pinned by the zero residual of uniform flow, closed-form stresses / heat flux, per-cell
+    conservation on cell-wise linear fields, the 1-D reduction, and convergence at rate P - 1 to a
+    manufactured field (prisms; the reference's heat flux enters with +div(-k grad T), kept); the
+    pointwise tet distribution is parity unpinned."""
     C, nsp = geom.n_cells, ops.n_sps
     mu_t = np.zeros((C, nsp)) if mu_t is None else mu_t
     Q = conserved_to_primitive(U[..., :5])
