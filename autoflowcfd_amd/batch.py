@@ -209,6 +209,11 @@ class CaseBatch:
             out["omega"] = self.omega.index_select(0, idx).double().cpu().numpy()
         return out
 
+    def set_case_state(self, i, U):
+        """Set the conserved state (C_i, ns, 5) of case i (the case mesh's own cell order)."""
+        idx = torch.as_tensor(self.case_cells(i), device=self.device)
+        self.U.index_copy_(0, idx, torch.as_tensor(np.ascontiguousarray(U), dtype=self.dtype).to(self.device))
+
     def wall_forces(self):
         """(n_cases, 6) FP64 host array: pressure force, viscous force of every case."""
         mu_t = self.mu_t if self.turb == "SST" else None
@@ -317,9 +322,86 @@ class MixedBatch:
         b, j = self._where(i)
         return b.case_state(j)
 
+    def set_case_state(self, i, U):
+        b, j = self._where(i)
+        b.set_case_state(j, U)
+
     def coefficients(self, include_viscous=True):
         out = [None] * self.n_cases
         for p, b in self.groups.items():
             for j, c in enumerate(b.coefficients(include_viscous)):
                 out[self.members[p][j]] = c
         return out
+
+
+class ResidentBatch:
+    """Resident solves of one order (GPUFRSolver objects, each holding its own device mesh and
+    state) stepped as ONE batched mesh assembled on the device (DeviceMesh.concat): nothing is
+    rebuilt on the host, so membership can change as solves arrive and retire.  The members' states
+    are gathered into the batch's buffers at construction and written back to each member by
+    ``write_back`` (before the batch is replaced, or when a member retires).  Every member must have
+    been built with the same ``ref_state``; each then follows its trajectory alone bit for bit."""
+
+    def __init__(self, solvers, ref_state):
+        from .device import DeviceMesh
+        if not solvers:
+            raise ValueError("empty batch")
+        self.members = list(solvers)
+        s0 = self.members[0]
+        self.turb = s0.turb_model_name
+        self.order, self.device = s0.order, s0.device
+        self.mu, self.Pr, self.Pr_t, self.n_vars = s0.mu_molecular, s0.Pr, s0.Pr_t, s0.n_vars
+        for s in self.members:
+            if (s.turb_model_name, s.order, s.mu_molecular, s.Pr, s.Pr_t) != (self.turb, self.order, self.mu,
+                                                                                self.Pr, self.Pr_t):
+                raise ValueError("members differ in turbulence model, order, viscosity or Prandtl numbers")
+        bcp = np.stack([s.dm.bc_params.cpu().numpy().reshape(8, 5) for s in self.members])
+        self.dm = DeviceMesh.concat([s.dm for s in self.members], bcp, ref_state)
+        self.idx = [self.dm.part_cells(i) for i in range(len(self.members))]
+        self.U = self._gather(lambda s: s.U)
+        if self.turb == "SST":
+            self.k, self.omega = self._gather(lambda s: s.k), self._gather(lambda s: s.omega)
+            self.nu_t, self.mu_t = self._gather(lambda s: s.nu_t), self._gather(lambda s: s.mu_t)
+        self.norms = torch.zeros((len(self.members),), dtype=torch.float64, device=self.device)
+        self._graph = None
+
+    def _gather(self, get):
+        x0 = get(self.members[0])
+        out = torch.empty((self.dm.n_cells,) + tuple(x0.shape[1:]), dtype=x0.dtype, device=self.device)
+        for i, s in enumerate(self.members):
+            out.index_copy_(0, self.idx[i], get(s))
+        return out
+
+    def write_back(self, which=None):
+        """Copy the batch's state of the members ``which`` (default all) into their own buffers."""
+        for i in (range(len(self.members)) if which is None else which):
+            s, ix = self.members[i], self.idx[i]
+            s.U.copy_(self.U.index_select(0, ix))
+            if self.turb == "SST":
+                for name in ("k", "omega", "nu_t", "mu_t"):
+                    getattr(s, name).copy_(getattr(self, name).index_select(0, ix))
+
+    def _launch_step(self):
+        if self.turb == "SST":
+            return self.dm.sst_rk3_step(self.U, self.k, self.omega, self.nu_t, self.mu_t, mu=self.mu, Pr=self.Pr,
+                                        Pr_t=self.Pr_t, n_vars_total=self.n_vars, norm=self.norms)
+        return self.dm.ssp_rk3_step(self.U, None, mu=self.mu, Pr=self.Pr, Pr_t=self.Pr_t, n_vars_total=self.n_vars,
+                                    norm=self.norms)
+
+    def capture_graph(self, capture_error_mode="thread_local"):
+        """Record one batched pseudo-step into a HIP graph (nothing is run: every step counts)."""
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, capture_error_mode=capture_error_mode):
+            self._launch_step()
+        self._graph = g
+        return g
+
+    def launch(self):
+        """Enqueue one pseudo-step of every member on the current stream (per-member norms in
+        self.norms)."""
+        if self._graph is not None:
+            self._graph.replay()
+        else:
+            self._launch_step()
+        for s in self.members:
+            s.iteration += 1

@@ -285,13 +285,22 @@ class DeviceMesh:
             if turbulence != "SST":
                 raise ValueError(f"unsupported turbulence model {turbulence!r}")
             self._build_sst(dev, devi)
+        self._create(int(sl.rows.shape[0]), mesh.n_prism,
+                     mesh.meta["prism_off"] if self.cell_case is not None else None,
+                     mesh.meta["tet_off"] if self.cell_case is not None else None)
+
+    def _create(self, n_rows, n_prism, prism_off=None, tet_off=None):
+        """Fill the AfrMeshDesc from this object's device arrays and create the library's mesh."""
+        order, dtype, C = self.order, self.dtype, self.n_cells
+        turbulence = self.turbulence
         d = _lib.AfrMeshDesc()
         d.abi_version = _lib.ABI_VERSION
         d.order = order
         d.dtype = 0 if dtype == torch.float32 else 1
-        d.n_rows = int(sl.rows.shape[0])
+        d.n_rows = int(n_rows)
         d.n_cells = C
-        d.n_prism = mesh.n_prism
+        d.n_prism = int(n_prism)
+        self.n_prism, self.n_rows = int(n_prism), int(n_rows)
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
         d.det, d.adj, d.adj_fine = p(self.det), p(self.adj), p(self.adj_fine)
         d.slot_nb, d.slot_blk, d.slot_mask = p(self.slot_nb), p(self.slot_blk), p(self.slot_mask)
@@ -300,7 +309,7 @@ class DeviceMesh:
         d.rows_nnz, d.rows_idx, d.rows_w = p(self.rows_nnz), p(self.rows_idx), p(self.rows_w)
         d.rows_knz = self.rows_knz
         d.prism_tab = p(self.prism_tab)
-        d.prism_schedule = p(self.prism_schedule) if mesh.n_prism else None
+        d.prism_schedule = p(self.prism_schedule) if n_prism else None
         for ct in (0, 1):
             d.c2f[ct], d.op[ct], d.E[ct] = p(self.c2f[ct]), p(self.op[ct]), p(self.E[ct])
             d.D[ct], d.filt[ct] = p(self.D[ct]), p(self.filt[ct])
@@ -309,8 +318,8 @@ class DeviceMesh:
         d.gcl, d.bc_params = p(self.gcl), p(self.bc_params)
         if self.cell_case is not None:
             # host copies of the case offsets: prisms, tets (from the mesh) and wall records
-            po = np.ascontiguousarray(mesh.meta["prism_off"], dtype=np.int64)
-            to = np.ascontiguousarray(mesh.meta["tet_off"], dtype=np.int64)
+            po = np.ascontiguousarray(prism_off, dtype=np.int64)
+            to = np.ascontiguousarray(tet_off, dtype=np.int64)
             self.case_prism_off, self.case_tet_off = po, to
             d.cell_case, d.n_cases = p(self.cell_case), self.n_cases
             d.case_prism_off = po.ctypes.data_as(ctypes.c_void_p)
@@ -343,6 +352,150 @@ class DeviceMesh:
         handle = ctypes.c_void_p()
         _lib.check(self.lib.afr_mesh_create(ctypes.byref(d), ctypes.byref(handle)), "afr_mesh_create")
         self.handle = handle
+
+
+    @classmethod
+    def concat(cls, parts, bc_params, ref_state):
+        """The batched mesh of independent cases from their resident DeviceMeshes, built on the
+        device: every per-cell array is concatenated (all prisms case by case, then all tets case
+        by case, as mesh.concat_meshes numbers them), cell, row-block and face-record indices are
+        offset, nothing is recomputed.  Every cell keeps exactly its case's data, so a case in the
+        batch follows its trajectory alone bit for bit (same ``ref_state`` in every part).
+        bc_params: (n_cases, 8, 5) boundary states; the parts must share order, precision and
+        turbulence model."""
+        if not parts:
+            raise ValueError("no parts to concatenate")
+        a = parts[0]
+        for q in parts[1:]:
+            if (q.order, q.dtype, q.turbulence, q.device) != (a.order, a.dtype, a.turbulence, a.device):
+                raise ValueError("parts differ in order, precision, turbulence model or device")
+        ref = np.asarray(ref_state, dtype=np.float64)
+        for q in parts:
+            if not np.array_equal(np.asarray(q.ref_state, dtype=np.float64), ref):
+                raise ValueError("every part must have been built with the batch's ref_state")
+        bcp = np.asarray(bc_params, dtype=np.float64)
+        if bcp.shape != (len(parts), 8, 5):
+            raise ValueError("bc_params must be (n_cases, 8, 5)")
+        self = cls.__new__(cls)
+        dv = a.device
+        self.lib, self.turbulence, self.mesh, self.slots = a.lib, a.turbulence, None, None
+        self.order, self.dtype, self.device, self.T = a.order, a.dtype, dv, a.T
+        self.ns, self.nfp, self.nf = a.ns, a.nfp, a.nf
+        n = len(parts)
+        P = np.array([q.n_prism for q in parts], dtype=np.int64)
+        Cs = np.array([q.n_cells for q in parts], dtype=np.int64)
+        p_off = np.concatenate([[0], np.cumsum(P)])
+        t_off = np.concatenate([[0], np.cumsum(Cs - P)])
+        P_tot, C = int(p_off[-1]), int(Cs.sum())
+        self.n_cells, self.n_cases = C, n
+
+        def cells(get):
+            xs = [get(q) for q in parts]
+            return torch.cat([x[:q.n_prism] for x, q in zip(xs, parts)] +
+                             [x[q.n_prism:] for x, q in zip(xs, parts)]).contiguous()
+
+        def remap(x, i):
+            """Cell indices of part i (-1 kept) in the batch numbering."""
+            q = parts[i]
+            x = x.long()
+            out = torch.where(x < q.n_prism, x + int(p_off[i]), x - q.n_prism + P_tot + int(t_off[i]))
+            return torch.where(x < 0, x, out).to(torch.int32)
+
+        def cells_i(get):
+            """Per-cell arrays holding cell indices, remapped part by part."""
+            xs = [remap(get(q), i) for i, q in enumerate(parts)]
+            return torch.cat([x[:q.n_prism] for x, q in zip(xs, parts)] +
+                             [x[q.n_prism:] for x, q in zip(xs, parts)]).contiguous()
+
+        self.slot_nb = cells_i(lambda q: q.slot_nb)
+        blk_off = np.concatenate([[0], np.cumsum([q.n_rows // q.nfp for q in parts])])
+        blks = [q.slot_blk + int(blk_off[i]) for i, q in enumerate(parts)]
+        self.slot_blk = torch.cat([b[:q.n_prism] for b, q in zip(blks, parts)] +
+                                  [b[q.n_prism:] for b, q in zip(blks, parts)]).contiguous()
+        for name in ("slot_mask", "slot_bc", "slot_tn", "slot_fb", "slot_adjrow", "det", "adj", "adj_fine", "gcl",
+                     "metric_flux_scale", "volume_dev"):
+            setattr(self, name, cells(lambda q, name=name: getattr(q, name)))
+        self.volume = np.concatenate([q.volume[:q.n_prism] for q in parts] + [q.volume[q.n_prism:] for q in parts])
+        self.det_mean = np.concatenate([q.det_mean[:q.n_prism] for q in parts] +
+                                       [q.det_mean[q.n_prism:] for q in parts])
+        # interpolation rows: each part's rows without its trailing zero row, then one zero row
+        knz = max(q.rows_knz for q in parts)
+
+        def padw(x):
+            return torch.nn.functional.pad(x, (0, knz - x.shape[1]))
+        self.rows = torch.cat([q.rows for q in parts]).contiguous()
+        self.rows_knz = knz
+        n_rows = int(self.rows.shape[0])
+        self.rows_nnz = torch.cat([q.rows_nnz[:q.n_rows] for q in parts] + [a.rows_nnz[-1:]]).contiguous()
+        self.rows_idx = torch.cat([padw(q.rows_idx[:q.n_rows]) for q in parts] + [padw(a.rows_idx[-1:])]).contiguous()
+        self.rows_w = torch.cat([padw(q.rows_w[:q.n_rows]) for q in parts] + [padw(a.rows_w[-1:])]).contiguous()
+        self.prism_schedule = torch.cat([q.prism_schedule + int(p_off[i]) for i, q in enumerate(parts)]).to(
+            torch.int32).contiguous()
+        for name in ("prism_tab", "E", "E64", "dsp_w", "dsp_i", "esp_w", "esp_i", "c2f", "op", "D", "filt", "gL", "gR",
+                     "fp_of_sp", "coord_of_sp"):
+            setattr(self, name, getattr(a, name))
+        cc = np.concatenate([np.repeat(np.arange(n), P), np.repeat(np.arange(n), Cs - P)]).astype(np.int32)
+        self.cell_case = torch.as_tensor(cc).to(dv)
+        self.bc_params = torch.as_tensor(bcp).to(dv)
+        self.ref_state = ref
+        dt = self.dtype
+        ns = self.ns
+        self.work_q = torch.empty((C, ns, 5), dtype=dt, device=dv)
+        self.work_grad = torch.empty((C, ns, 12), dtype=dt, device=dv)
+        self.work_umean = torch.empty((C, 5), dtype=dt, device=dv)
+        self.work_rvol = torch.empty((C, ns, 5), dtype=dt, device=dv)
+        self.work_vpack = torch.empty((C, ns, 20), dtype=dt, device=dv)
+        self.work_h = torch.empty((C,), dtype=dt, device=dv)
+        self.work_r = torch.empty((C, ns, 5), dtype=dt, device=dv)
+        self.work_s1 = torch.empty((C, ns, 5), dtype=dt, device=dv)
+        self.work_s2 = torch.empty((C, ns, 5), dtype=dt, device=dv)
+        self.work_dt = torch.empty((C, ns), dtype=dt, device=dv)
+        self.work_part = torch.zeros((4096,), dtype=torch.float64, device=dv)
+        self.norm_dev = torch.zeros((n,), dtype=torch.float64, device=dv)
+        # wall records, grouped case by case (each part's records in its own order)
+        self.n_wall_rec = int(sum(q.n_wall_rec for q in parts))
+        if self.n_wall_rec:
+            have = [i for i, q in enumerate(parts) if q.n_wall_rec]
+            self.wrec_owner = torch.cat([remap(parts[i].wrec_owner, i) for i in have]).contiguous()
+            self.wrec_oslot = torch.cat([parts[i].wrec_oslot for i in have]).contiguous()
+            self.wrec_geom = torch.cat([parts[i].wrec_geom for i in have]).contiguous()
+            self.w1d = a.w1d if a.n_wall_rec else parts[have[0]].w1d
+            self.case_wall_off = np.concatenate([[0], np.cumsum([q.n_wall_rec for q in parts])]).astype(np.int64)
+            self.work_fpart = torch.zeros((self.n_wall_rec, 6), dtype=torch.float64, device=dv)
+            self.forces_dev = torch.zeros((6,), dtype=torch.float64, device=dv)
+        if self.turbulence is not None:
+            rec_off = np.concatenate([[0], np.cumsum([q.n_records for q in parts])])
+            self.n_records = int(rec_off[-1])
+            self.rec_owner = torch.cat([remap(q.rec_owner, i) for i, q in enumerate(parts)]).contiguous()
+            self.rec_oslot = torch.cat([q.rec_oslot for q in parts]).contiguous()
+            self.rec_flags = torch.cat([q.rec_flags for q in parts]).contiguous()
+            self.rec_tn = torch.cat([q.rec_tn for q in parts]).contiguous()
+            incs = [torch.where(q.cell_inc >= 0, q.cell_inc + 16 * int(rec_off[i]), q.cell_inc)
+                    for i, q in enumerate(parts)]
+            self.cell_inc = torch.cat([x[:q.n_prism] for x, q in zip(incs, parts)] +
+                                      [x[q.n_prism:] for x, q in zip(incs, parts)]).contiguous()
+            if self.n_records * 16 >= 2 ** 31:
+                raise ValueError("too many face records for the int32 incidence encoding")
+            self.d_wall = cells(lambda q: q.d_wall)
+            self.sps_xyz = cells(lambda q: q.sps_xyz) if all(q.sps_xyz is not None for q in parts) else None
+            nfp = self.nfp
+            self.work_pack = torch.empty((C, 5, nfp, 8), dtype=dt, device=dv)
+            self.work_nbp = torch.empty((C, ns, 4), dtype=dt, device=dv)
+            self.work_tvol = torch.empty((C, ns, 2), dtype=dt, device=dv)
+            self.work_tsrc = torch.empty((C, ns, 3), dtype=dt, device=dv)
+            self.work_crec = torch.empty((self.n_records, nfp, 2), dtype=dt, device=dv)
+        else:
+            self.sps_xyz = None
+        self.part_prism_off, self.part_tet_off = p_off, t_off
+        self._create(n_rows, P_tot, p_off, t_off)
+        return self
+
+    def part_cells(self, i):
+        """Device cell indices of part i of a concatenated mesh, in that part's own cell order."""
+        p0, p1 = int(self.part_prism_off[i]), int(self.part_prism_off[i + 1])
+        t0, t1 = int(self.part_tet_off[i]), int(self.part_tet_off[i + 1])
+        return torch.cat([torch.arange(p0, p1, device=self.device),
+                          self.n_prism + torch.arange(t0, t1, device=self.device)])
 
     @staticmethod
     def owner_primary(mesh):

@@ -154,10 +154,22 @@ class SolveServer:
     """Continuous batching of steady solves on one GPU (see the module docstring)."""
 
     def __init__(self, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, max_active=8, hbm_budget=None,
-                 use_graph=True, clock=time.perf_counter, host_workers=2):
+                 use_graph=True, clock=time.perf_counter, host_workers=2, batched=False, ref_state=None,
+                 setup_threads=3):
+        """batched: step the resident solves of each order as ONE batched mesh assembled on the device
+        (batch.ResidentBatch), rebuilt whenever solves join or retire, instead of one graph per solve
+        on its own stream; every solve is then built with the common ``ref_state`` (default: the
+        static state rho 1.225, p 101325 at rest), and a plain run with the same ref_state gives
+        the same results bit for bit.  setup_threads: solves set up concurrently in batched mode (no
+        graph capture there, so set-ups may overlap each other)."""
         if not torch.cuda.is_available():
             raise RuntimeError("SolveServer needs a visible GPU (HIP device)")
         self.dtype, self.turb, self.mu = dtype, turb_model, mu
+        self.batched = bool(batched)
+        self.setup_threads = max(1, int(setup_threads))
+        self.ref_state = (np.array([1.225, 0.0, 0.0, 0.0, 101325.0]) if ref_state is None
+                          else np.asarray(ref_state, dtype=np.float64))
+        self.rebatches = 0
         self.max_active = int(max_active)
         free, _ = torch.cuda.mem_get_info()
         self.hbm_budget = int(0.8 * free) if hbm_budget is None else int(hbm_budget)
@@ -210,7 +222,28 @@ class SolveServer:
         used = max(torch.cuda.memory_allocated() - before, 1)
         return s, stream, first, used, self.clock() - t_start
 
+    def _prepare_batched(self, req, host_future):
+        """Worker thread (batched mode): build the solve with the common ref_state and run its first
+        pseudo-step on the thread's stream; no graph (the batch is stepped as a whole)."""
+        t_start = self.clock()
+        slots = host_future.result() if host_future is not None else None
+        stream = torch.cuda.Stream()
+        before = torch.cuda.memory_allocated()
+        with torch.cuda.stream(stream):
+            s = build_solver(req, self.dtype, self.turb, self.mu, ref_state=self.ref_state, slots=slots)
+            norm = s._launch_step()
+            host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
+            host.copy_(norm[:1], non_blocking=True)
+        stream.synchronize()
+        first = float(host[0])
+        s.residual_history.append(first)
+        s.iteration += 1
+        used = max(torch.cuda.memory_allocated() - before, 1)
+        return s, stream, first, used, self.clock() - t_start
+
     def run(self, requests) -> List[SolveResult]:
+        if self.batched:
+            return self._run_batched(requests)
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         t0 = self.clock()
         now = lambda: self.clock() - t0  # noqa: E731
@@ -282,6 +315,127 @@ class SolveServer:
             pool.shutdown(wait=True)
             self._graveyard.clear()
         return sorted(results, key=lambda r: r.rid)
+
+    def _run_batched(self, requests) -> List[SolveResult]:
+        """The scheduler of the batched mode: per order, the active solves are one ResidentBatch
+        stepped on that order's stream; after every step the per-solve norms come back by one
+        asynchronous copy, retiring solves are written back and retired by the plain path's rule,
+        and the batch is rebuilt (device concatenation) when solves have joined or left."""
+        from .batch import ResidentBatch
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        t0 = self.clock()
+        now = lambda: self.clock() - t0  # noqa: E731
+        waiting, results = [], []
+        host = {}
+        i_next = 0
+        used_bytes = 0
+        preps = []
+        groups = {}
+        pool = ThreadPoolExecutor(max_workers=self.setup_threads)
+
+        def group(p):
+            if p not in groups:
+                groups[p] = {"active": [], "batch": None, "stream": torch.cuda.Stream(), "dirty": False,
+                             "event": torch.cuda.Event(), "in_flight": False,
+                             "host": torch.zeros(max(self.max_active, 1), dtype=torch.float64, pin_memory=True)}
+            return groups[p]
+
+        def n_active():
+            return sum(len(g["active"]) for g in groups.values())
+
+        try:
+            while len(results) < len(reqs):
+                t = now()
+                while i_next < len(reqs) and reqs[i_next].arrival <= t:
+                    r_new = reqs[i_next]
+                    waiting.append(r_new)
+                    host[r_new.rid] = (self.host_pool.submit(slots_for, r_new.case.mesh, r_new.order)
+                                       if self.host_pool is not None else None)
+                    i_next += 1
+                while (waiting and len(preps) < self.setup_threads
+                       and n_active() + len(preps) < self.max_active):
+                    need = self.estimate_bytes(waiting[0])
+                    if not (used_bytes + need <= self.hbm_budget or not (n_active() or preps)):
+                        break
+                    req = waiting.pop(0)
+                    used_bytes += need
+                    preps.append((pool.submit(self._prepare_batched, req, host.pop(req.rid)), req, t, need))
+                for prep in [q for q in preps if q[0].done()]:
+                    preps.remove(prep)
+                    fut, req, t_adm, need = prep
+                    s, _, first, used, setup = fut.result()
+                    ns = (req.order + 1) ** 3
+                    self.bytes_per_point[req.order] = max(self.bytes_per_point.get(req.order, 0.0),
+                                                          used / (s.dm.n_cells * ns))
+                    used_bytes += used - need
+                    a = _Active(req, s, None, None, None, t_adm, used, hist=[first], setup_s=setup)
+                    stop, conv = _stop(a.hist, 1, req)
+                    if stop:
+                        results.append(self._retire_batched(a, conv, now, torch.cuda.current_stream()))
+                        used_bytes -= a.nbytes
+                    else:
+                        g = group(req.order)
+                        g["active"].append(a)
+                        g["dirty"] = True
+                    self.peak_active = max(self.peak_active, n_active())
+                    self.peak_bytes = max(self.peak_bytes, used_bytes)
+                progressed = False
+                for p, g in groups.items():
+                    if g["in_flight"]:
+                        if not g["event"].query():
+                            continue
+                        g["in_flight"] = False
+                        b = g["batch"]
+                        for i, a in enumerate(b.members_active):
+                            a.hist.append(float(g["host"][i]))
+                            stop, conv = _stop(a.hist, len(a.hist), a.req)
+                            if stop:
+                                with torch.cuda.stream(g["stream"]):
+                                    b.write_back([i])
+                                results.append(self._retire_batched(a, conv, now, g["stream"]))
+                                used_bytes -= a.nbytes
+                                g["active"].remove(a)
+                                g["dirty"] = True
+                        progressed = True
+                    if g["dirty"]:
+                        with torch.cuda.stream(g["stream"]):
+                            if g["batch"] is not None:
+                                old = g["batch"]
+                                stay = [i for i, a in enumerate(old.members_active) if a in g["active"]]
+                                old.write_back(stay)
+                            g["batch"] = None
+                            if g["active"]:
+                                b = ResidentBatch([a.solver for a in g["active"]], self.ref_state)
+                                b.members_active = list(g["active"])
+                                g["batch"] = b
+                                self.rebatches += 1
+                        g["dirty"] = False
+                    if g["batch"] is not None and not g["in_flight"]:
+                        b = g["batch"]
+                        n = len(b.members_active)
+                        with torch.cuda.stream(g["stream"]):
+                            b.launch()
+                            g["host"][:n].copy_(b.norms, non_blocking=True)
+                            g["event"].record(g["stream"])
+                        g["in_flight"] = True
+                        self.steps_launched += n
+                        progressed = True
+                self.loop_rounds += 1
+                if not progressed:
+                    self.idle_rounds += 1
+                    time.sleep(0.0002)
+        finally:
+            pool.shutdown(wait=True)
+        return sorted(results, key=lambda r: r.rid)
+
+    def _retire_batched(self, a, converged, now, stream):
+        s = a.solver
+        with torch.cuda.stream(stream):
+            coef = _coefficients(s, a.req.case)
+        stream.synchronize()
+        s.residual_history = list(a.hist)
+        return SolveResult(a.req.rid, a.req.order, s.dm.n_cells, len(a.hist), converged, a.hist[-1], list(a.hist),
+                           coef, a.req.arrival, a.admitted, now(), a.setup_s)
 
     def _launch(self, a):
         """Enqueue one pseudo-step of an active solve on its stream, with the norm read-back."""

@@ -189,24 +189,36 @@ def traffic_from_profiles(kernel, workload="wallbox82_p2_sst_fp32"):
 
 def served_leg(args):
     """Served stream vs the plain one-at-a-time GPU path on the same seeded requests: solves/h,
-    p50 / p99 latency, and the largest Cd difference between the two (same-result check)."""
+    p50 / p99 latency, and the largest Cd difference between the two (same-result check).  The
+    server steps its resident solves of each order as one batched mesh assembled on the device
+    (continuous batching); both paths use the server's common reference state, so every solve's
+    residual history must be the same bit for bit."""
     import torch
     from autoflowcfd_amd.serve import SolveServer, latency_stats, solve_alone, synthetic_requests
-    reqs = synthetic_requests(args.serve_requests, seed=args.serve_seed, hex_range=(2e3, 2e4), rate=args.serve_rate,
+    lo, hi = (float(x) for x in args.serve_hex.split(","))
+    turb = args.serve_turb.upper()
+    reqs = synthetic_requests(args.serve_requests, seed=args.serve_seed, hex_range=(lo, hi), rate=args.serve_rate,
                               max_iter=args.serve_max_iter, rel_tol=1e-6)
     n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
-    log(f"[bench] served leg: {len(reqs)} requests, {n_sp} solution points in total")
-    # warm the code paths once (library load, first graph capture) outside both timings
-    SolveServer(dtype=torch.float32, max_active=2).run(reqs[:1])
-    solve_alone(reqs[:1], dtype=torch.float32)
+    log(f"[bench] served leg: {len(reqs)} {turb} requests, {n_sp} solution points in total")
+    batched = args.serve_mode == "batched"
+    mk = lambda n, hw: SolveServer(dtype=torch.float32, turb_model=turb, max_active=n, host_workers=hw,  # noqa: E731
+                                   batched=batched)
+    # warm the code paths once (library load, first batch / graph) outside both timings
+    warm = mk(2, 0)
+    warm.run(reqs[:1])
+    ref = warm.ref_state if batched else None
+    solve_alone(reqs[:1], dtype=torch.float32, turb_model=turb, ref_state=ref)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain = solve_alone(reqs, dtype=torch.float32)
+    plain = solve_alone(reqs, dtype=torch.float32, turb_model=turb, ref_state=ref)
     log(f"[bench] plain path done in {time.perf_counter() - t0:.1f}s")
-    srv = SolveServer(dtype=torch.float32, max_active=args.serve_active, host_workers=args.serve_host_workers)
+    srv = mk(args.serve_active, args.serve_host_workers)
     t0 = time.perf_counter()
     served = srv.run(reqs)
-    log(f"[bench] served path done in {time.perf_counter() - t0:.1f}s (peak {srv.peak_active} resident solves)")
+    log(f"[bench] served path ({args.serve_mode}) done in {time.perf_counter() - t0:.1f}s "
+        f"(peak {srv.peak_active} resident solves, {srv.rebatches} batch rebuilds)")
+    srv.close()
     sp, ss = latency_stats(plain), latency_stats(served)
     busy = [r.finished - r.admitted - r.setup_s for r in plain]
     log(f"[bench] plain: mean set-up {np.mean([r.setup_s for r in plain]):.3f}s, mean stepping {np.mean(busy):.3f}s "
@@ -214,14 +226,15 @@ def served_leg(args):
     dcd = max(abs(a.coefficients.get("Cd", 0.0) - b.coefficients.get("Cd", 0.0)) for a, b in zip(served, plain))
     same = all(a.residual_history == b.residual_history for a, b in zip(served, plain))
     return {
-        "workload": f"{len(reqs)} steady laminar NS solves, graded wall boxes of 2e3-2e4 hex (log-uniform) "
-                    f"split into prisms, P1/P2, Mach 0.05-0.3, yaw 0-10 deg, Poisson arrivals "
-                    f"{args.serve_rate}/s, stop at res/res0 < 1e-6 or {args.serve_max_iter} pseudo-steps",
+        "workload": f"{len(reqs)} steady {'laminar NS' if turb == 'NONE' else 'SST RANS'} solves, graded wall boxes of "
+                    f"{lo:g}-{hi:g} hex (log-uniform) split into prisms, P1/P2, Mach 0.05-0.3, yaw 0-10 deg, Poisson "
+                    f"arrivals {args.serve_rate}/s, stop at res/res0 < 1e-6 or {args.serve_max_iter} pseudo-steps; "
+                    f"server mode: {args.serve_mode}",
         "solves_per_hour": ss["solves_per_hour"], "p50_latency_s": ss["p50_s"], "p99_latency_s": ss["p99_s"],
         "plain_solves_per_hour": sp["solves_per_hour"], "plain_p50_latency_s": sp["p50_s"],
         "plain_p99_latency_s": sp["p99_s"], "speedup_vs_plain": ss["solves_per_hour"] / sp["solves_per_hour"],
         "max_abs_dCd_vs_plain": dcd, "histories_identical": same, "peak_resident_solves": srv.peak_active,
-        "iterations": [r.iterations for r in served],
+        "batch_rebuilds": srv.rebatches, "iterations": [r.iterations for r in served],
     }
 
 
@@ -380,6 +393,10 @@ def main():
     ap.add_argument("--grids", default="", help="explicit grid sizes instead of autotuning, e.g. visc_iface=16384,...")
     ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-hex", default="2e3,2e4", help="hex-count range of the served requests (lo,hi)")
+    ap.add_argument("--serve-turb", default="none", choices=("none", "sst"))
+    ap.add_argument("--serve-mode", default="batched", choices=("batched", "streams"),
+                    help="batched: one device-assembled batch per order; streams: one graph per solve")
     ap.add_argument("--serve-max-iter", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
@@ -388,7 +405,7 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-hex", default="2e3,2e4", help="hex-count range of the batched cases (lo,hi)")
-    ap.add_argument("--batch-host-workers", type=int, default=8, help="processes building the batch's host tables")
+    ap.add_argument("--batch-host-workers", type=int, default=0, help="processes building the batch's host tables")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
     ap.add_argument("--tgv-p3", type=int, default=0,
                     help="run only the P3 Taylor-Green vortex leg on N^3 hex x 6 tets (prints its JSON)")

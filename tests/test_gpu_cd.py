@@ -8,7 +8,10 @@ the flat wall, is a real signal; the FP32 GPUFRSolver takes 20 SSP-RK3 pseudo-st
 (src/autoflowcfd/core/fr_solver/step.py:15-221) and the FP64 oracle the same 20 from the same
 FP32-rounded start; the wall-force coefficients (src/autoflowcfd/postprocess/fr_coefficients.py:27-152,
 fixed axes) must agree to 0.1 % of Cd (and of |F|/q A for Cl and Cs, whose reference the
-pressure on the wall sets)."""
+pressure on the wall sets).  P2: at P1 the reference's per-stage modal filter removes every
+non-constant mode (sigma = exp(-alpha) = 2.2e-16 at eta = 1, src/autoflowcfd/fr/modal_filter.py:47-128),
+so the cells' own velocity gradients, and with them the skin friction, vanish after the first
+stage (tests/test_oracle_operators.py pins that filter)."""
 
 import numpy as np
 import pytest
@@ -21,7 +24,7 @@ MU = 1.8e-5
 N_STEPS = 20
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [2])
 def test_fp32_cd_after_fixed_steps_matches_fp64_oracle(order):
     import torch
     from autoflowcfd_amd.device import bc_params_table

@@ -86,3 +86,22 @@ def test_served_sst_equals_plain():
     plain = solve_alone(reqs, dtype=torch.float32, turb_model="SST")
     served = SolveServer(dtype=torch.float32, turb_model="SST", max_active=3).run(reqs)
     _same(served, plain)
+
+
+@pytest.mark.parametrize("turb,max_iter", [("NONE", 400), ("SST", 8)])
+def test_batched_server_equals_plain(turb, max_iter):
+    """Continuous batching proper: the resident solves of each order are stepped as one batched mesh
+    assembled on the device and rebuilt as solves join and retire; with the server's common
+    reference state every solve equals the plain path bit for bit.  The laminar solves all arrive
+    at once and run 388-400 steps, far longer than a set-up, so several are resident together."""
+    import torch
+    from autoflowcfd_amd.serve import SolveServer, solve_alone
+    print(f"[gpu] SolveServer batched {turb} vs plain", flush=True)
+    reqs = _requests(max_iter=max_iter, rel_tol=1e-12, stagger=False) if turb == "NONE" else _requests(max_iter)
+    srv = SolveServer(dtype=torch.float32, turb_model=turb, max_active=4, batched=True)
+    served = srv.run(reqs)
+    plain = solve_alone(reqs, dtype=torch.float32, turb_model=turb, ref_state=srv.ref_state)
+    _same(served, plain)
+    assert srv.rebatches >= 2
+    if turb == "NONE":
+        assert srv.peak_active >= 2

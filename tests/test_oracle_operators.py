@@ -224,3 +224,20 @@ def test_tet_filter_reproduces_reference_stated_norms():
     ratio = (F_total @ x).std() / (F_max @ x).std()
     assert abs(ratio - 17.5 / 0.63) < 0.5, ratio
     assert np.linalg.norm(modal_filter("tet", 3), 2) > 500.0
+
+
+def test_p1_filter_keeps_only_the_cell_mean():
+    """At P1 every non-constant mode has eta = max(i, j, k) / P = 1, so the filter of
+    src/autoflowcfd/fr/modal_filter.py:47-128 (sigma = exp(-alpha eta^8), alpha = -ln eps64) scales
+    them by eps64: applied after every stage, it leaves each cell's mean only (its P1 gradients
+    vanish).  At P2 the linear modes keep sigma = exp(-alpha / 256) = 0.87."""
+    from oracle import fr_ops
+    ops = fr_ops.generate_operators(1)
+    for ti in (0, 1):
+        F = ops.filt[ti]
+        assert np.allclose(F @ F, F, atol=1e-12)                  # a projection
+        assert np.linalg.matrix_rank(F, tol=1e-10) == 1           # onto one mode
+        assert np.allclose(F @ np.ones(ops.n_sps), 1.0)           # the constant
+    ops2 = fr_ops.generate_operators(2)
+    ev = np.sort(np.abs(np.linalg.eigvals(ops2.filt[0])))[::-1]
+    assert abs(ev[1] - np.exp(np.log(np.finfo(float).eps) / 256)) < 1e-9
