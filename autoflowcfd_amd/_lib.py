@@ -50,6 +50,17 @@ class AfrGeomDesc(ctypes.Structure):
     ]
 
 
+class AfrP0Desc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("dtype", ctypes.c_int32), ("n_cells", ctypes.c_int64),
+        ("n_records", ctypes.c_int64),
+        ("det", c_vp), ("adj", c_vp), ("mfs", c_vp), ("vol1", c_vp), ("cell_volume", c_vp),
+        ("rec_owner", c_vp), ("rec_nb", c_vp), ("rec_bc", c_vp), ("rec_flags", c_vp), ("rec_geom", c_vp),
+        ("cell_inc", c_vp), ("cell_ghost_rec", c_vp), ("bc_params", c_vp),
+        ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
+    ]
+
+
 # every function declared in include/afr.h: name -> (restype, argtypes)
 SIGNATURES = {
     "afr_version": (ctypes.c_int, []),
@@ -75,6 +86,11 @@ SIGNATURES = {
                                     ctypes.c_double, c_vp]),
     "afr_residual_norm_f64": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
     "afr_residual_norm_f64_noalloc": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int32, c_vp, c_vp]),
+    "afr_p0_residual": (ctypes.c_int, [ctypes.POINTER(AfrP0Desc), c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
+    "afr_p0_ssp_rk3_step": (ctypes.c_int, [ctypes.POINTER(AfrP0Desc), c_vp, c_vp, ctypes.c_double, ctypes.c_int32,
+                                           c_vp, c_vp]),
+    "afr_interpolate_order": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_vp, c_vp]),
     "afr_ssp_rk3_step_dt": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,

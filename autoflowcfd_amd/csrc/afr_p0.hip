@@ -1,0 +1,261 @@
+// The P0 phase of the order continuation and the interpolation between orders.
+//
+//   k_p0_residual    the P0 finite-volume inviscid residual of
+//                    src/autoflowcfd/core/fr_residual/inviscid_p0.py:17-143 (inviscid_p0_kernel.py:24-105):
+//                    every face record carries AUSM+up(Q_owner, Q_other, n) x area, minus on the owner,
+//                    plus on the neighbour, each over its cell volume (the target-order volume); a
+//                    boundary record's other state is the ghost state its owner cell keeps, that of
+//                    the cell's last boundary record (inviscid_p0.py:118-143).  Plus the viscous residual
+//                    at one solution point (viscous_p0_kernel.py:26-285): with the gradients of a constant
+//                    zero, only the interior-penalty term of the owner-primary boundary records remains,
+//                    distributed with g'(0) = -1/2 / +1/2 over det J at the point.  One thread per cell
+//                    gathers its records (no scatter, no atomics: a record's flux is evaluated by both of
+//                    its cells from the same inputs, hence the same bits).
+//   k_p0_stage       one Shu-Osher stage with the positivity projection, one thread per cell (the P0
+//                    filter is the identity).
+//   k_interp_order   X_new[c][a][v] = sum_b W[a][b] X_old[c][b][v]: the tensor-product linear
+//                    interpolation of src/autoflowcfd/core/utils/order_continuation.py:14-57.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/afr.h"
+#include "afr_physics.hpp"
+#include "afr_step.hpp"
+
+using namespace afr;
+
+void afr_set_error(const std::string& msg);
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kCip = 4.0;   // interior-penalty coefficient, viscous_flux.py:183-192
+
+template <typename real>
+__device__ __forceinline__ void prim_at(const real* U, int64_t c, double q[5]) {
+    double u[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = (double)U[c * 5 + v];
+    cons2prim(u, q);
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_p0_residual(AfrP0Desc d, const real* __restrict__ U,
+                                                          const real* __restrict__ mu_t, real* __restrict__ R,
+                                                          double mu) {
+    const real* det = (const real*)d.det;
+    const real* adj = (const real*)d.adj;
+    const double* vol = d.cell_volume;
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
+         c += (int64_t)gridDim.x * kThreads) {
+        double qc[5];
+        prim_at(U, c, qc);
+        double res[5] = {0, 0, 0, 0, 0};
+        // the ghost state the cell keeps: that of its last boundary record
+        double qg[5];
+        const int gr = d.cell_ghost_rec[c];
+        if (gr >= 0) {
+            const double* gg = d.rec_geom + (int64_t)gr * 4;
+            const double n[3] = {gg[0], gg[1], gg[2]};
+            ghost_state(d.rec_bc[gr], qc, n, d.bc_params, qg);
+        }
+        const double iv = 1.0 / vol[c];
+        for (int k = 0; k < 8; ++k) {
+            const int e = d.cell_inc[c * 8 + k];
+            if (e < 0) break;
+            const int r = e >> 4;
+            const int role = (e >> 3) & 1;
+            const double* gg = d.rec_geom + (int64_t)r * 4;
+            const double n[3] = {gg[0], gg[1], gg[2]};
+            const int nb = d.rec_nb[r];
+            double qo[5], qn[5], f[5];
+            if (role == 0) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qo[v] = qc[v];
+                if (nb >= 0) prim_at(U, nb, qn);
+                else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) qn[v] = qg[v];
+                }
+            } else {
+                prim_at(U, d.rec_owner[r], qo);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qn[v] = qc[v];
+            }
+            ausm_up(qo, qn, n, f);
+            const double s = (role == 0 ? -gg[3] : gg[3]) * iv;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) res[v] += f[v] * s;
+            // the boundary interior penalty of the owner-primary boundary records
+            const int fl = d.rec_flags[r];
+            if (role == 0 && nb < 0 && (fl & 1)) {
+                double qb[5];
+                ghost_state(d.rec_bc[r], qc, n, d.bc_params, qb);
+                const int axis = (fl >> 1) & 3;
+                const double side = (fl & 8) ? 1.0 : -1.0;
+                const real* ar = adj + c * 9 + axis * 3;
+                const double a0 = (double)ar[0], a1 = (double)ar[1], a2 = (double)ar[2];
+                const double mag = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+                const double dj = (double)det[c];
+                double h = cbrt(fabs(dj));
+                h = h > 1e-300 ? h : 1e-300;
+                const double eta = kCip * (mu + (mu_t ? (double)mu_t[c] : 0.0)) / h;
+                const double gp = side < 0 ? -0.5 : 0.5;
+#pragma unroll
+                for (int v = 1; v < 4; ++v) res[v] += gp * (-eta * mag * side * (qc[v] - qb[v])) / dj;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) R[c * 5 + v] = (real)res[v];
+    }
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_p0_stage(const real* U0, const real* Ua, const real* __restrict__ R,
+                                                       const real* __restrict__ dt, real* U_out, int64_t n,
+                                                       double c0, double c1, double c2, double p_floor) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double d = (double)dt[i];
+        double q[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
+            if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
+            q[v] = (double)(real)val;
+        }
+        // positivity (src/autoflowcfd/core/time_integration/positivity.py:13-45)
+        const double rho = q[0] > 1e-6 ? q[0] : 1e-6;
+        double vx = q[1] / rho, vy = q[2] / rho, vz = q[3] / rho;
+        const double vm = sqrt(vx * vx + vy * vy + vz * vz);
+        double m1 = q[1], m2 = q[2], m3 = q[3];
+        if (vm > 1e4) {
+            const double f = 1e4 / vm;
+            vx *= f; vy *= f; vz *= f;
+            m1 = rho * vx; m2 = rho * vy; m3 = rho * vz;
+        }
+        const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+        double E = q[4];
+        if (0.4 * (E - ke) < p_floor) E = p_floor / 0.4 + ke;
+        U_out[i * 5 + 0] = (real)rho;
+        U_out[i * 5 + 1] = (real)m1;
+        U_out[i * 5 + 2] = (real)m2;
+        U_out[i * 5 + 3] = (real)m3;
+        U_out[i * 5 + 4] = (real)E;
+    }
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_interp_order(const real* __restrict__ X, real* __restrict__ Y,
+                                                           const double* __restrict__ W, int64_t n_cells, int ns_old,
+                                                           int ns_new, int nv) {
+    const int64_t n = n_cells * ns_new * nv;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const int64_t c = i / ((int64_t)ns_new * nv);
+        const int a = (int)((i / nv) % ns_new), v = (int)(i % nv);
+        double acc = 0.0;
+        for (int b = 0; b < ns_old; ++b) acc += W[a * ns_old + b] * (double)X[(c * ns_old + b) * nv + v];
+        Y[i] = (real)acc;
+    }
+}
+
+int grid_of(int64_t n) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+bool desc_ok(const AfrP0Desc* d) {
+    return d && d->abi_version == AFR_ABI_VERSION && (d->dtype == AFR_F32 || d->dtype == AFR_F64) && d->n_cells >= 0 &&
+           d->det && d->adj && d->mfs && d->vol1 && d->cell_volume && d->rec_owner && d->rec_nb && d->rec_bc &&
+           d->rec_flags && d->rec_geom && d->cell_inc && d->cell_ghost_rec && d->bc_params;
+}
+
+template <typename real>
+int residual_t(const AfrP0Desc& d, const void* U, const void* mu_t, void* R, double mu, hipStream_t st) {
+    hipLaunchKernelGGL((k_p0_residual<real>), dim3(grid_of(d.n_cells)), dim3(kThreads), 0, st, d, (const real*)U,
+                       (const real*)mu_t, (real*)R, mu);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename real>
+int stage_t(const void* U0, const void* Ua, const void* R, const void* dt, void* out, int64_t n, double c0, double c1,
+            double c2, hipStream_t st) {
+    hipLaunchKernelGGL((k_p0_stage<real>), dim3(grid_of(n)), dim3(kThreads), 0, st, (const real*)U0,
+                       (const real*)(Ua ? Ua : U0), (const real*)R, (const real*)dt, (real*)out, n, c0, c1, c2, 1.0);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename real>
+int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm, hipStream_t st) {
+    const int64_t C = d.n_cells;
+    real* u = (real*)U;
+    real* r = (real*)d.work_r;
+    real* s1 = (real*)d.work_s1;
+    real* s2 = (real*)d.work_s2;
+    real* dt = (real*)d.work_dt;
+    if (afr_step::launch_dt_local<real>(u, (const real*)mu_t, (const real*)d.det, (const real*)d.mfs,
+                                        (const real*)d.vol1, dt, C, 1, mu, 0, 0, st))
+        return -1;
+    if (residual_t<real>(d, u, mu_t, r, mu, st)) return -1;
+    if (afr_step::launch_norm<real>(r, C * 5, (double)C * n_vars_total, d.work_part, afr_step::kNormParts, norm, st))
+        return -1;
+    if (stage_t<real>(u, nullptr, r, dt, s1, C, 1.0, 0.0, 1.0, st)) return -1;
+    if (residual_t<real>(d, s1, mu_t, r, mu, st)) return -1;
+    if (stage_t<real>(u, s1, r, dt, s2, C, 0.75, 0.25, 0.25, st)) return -1;
+    if (residual_t<real>(d, s2, mu_t, r, mu, st)) return -1;
+    return stage_t<real>(u, s2, r, dt, u, C, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int afr_p0_residual(const AfrP0Desc* d, const void* U, const void* mu_t, double mu, void* R, void* stream) {
+    if (!desc_ok(d) || !U || !R) {
+        afr_set_error("afr_p0_residual: bad descriptor or null argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = d->dtype == AFR_F64 ? residual_t<double>(*d, U, mu_t, R, mu, st)
+                                       : residual_t<float>(*d, U, mu_t, R, mu, st);
+    if (rc) afr_set_error("afr_p0_residual: launch failed");
+    return rc;
+}
+
+int afr_p0_ssp_rk3_step(const AfrP0Desc* d, void* U, const void* mu_t, double mu, int32_t n_vars_total, double* norm,
+                        void* stream) {
+    if (!desc_ok(d) || !U || !norm || !d->work_r || !d->work_s1 || !d->work_s2 || !d->work_dt || !d->work_part ||
+        n_vars_total < 5) {
+        afr_set_error("afr_p0_ssp_rk3_step: bad descriptor, work buffer or argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = d->dtype == AFR_F64 ? step_t<double>(*d, U, mu_t, mu, n_vars_total, norm, st)
+                                       : step_t<float>(*d, U, mu_t, mu, n_vars_total, norm, st);
+    if (rc) afr_set_error("afr_p0_ssp_rk3_step: launch failed");
+    return rc;
+}
+
+int afr_interpolate_order(const void* X_old, void* X_new, const double* W, int64_t n_cells, int32_t ns_old,
+                          int32_t ns_new, int32_t n_vars, int32_t dtype, void* stream) {
+    if (!X_old || !X_new || !W || n_cells < 0 || ns_old < 1 || ns_new < 1 || ns_old > 64 || ns_new > 64 ||
+        n_vars < 1 || (dtype != AFR_F32 && dtype != AFR_F64)) {
+        afr_set_error("afr_interpolate_order: bad argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = n_cells * ns_new * n_vars;
+    if (dtype == AFR_F64)
+        hipLaunchKernelGGL((k_interp_order<double>), dim3(grid_of(n)), dim3(kThreads), 0, st, (const double*)X_old,
+                           (double*)X_new, W, n_cells, ns_old, ns_new, n_vars);
+    else
+        hipLaunchKernelGGL((k_interp_order<float>), dim3(grid_of(n)), dim3(kThreads), 0, st, (const float*)X_old,
+                           (float*)X_new, W, n_cells, ns_old, ns_new, n_vars);
+    if (hipGetLastError() != hipSuccess) {
+        afr_set_error("afr_interpolate_order: launch failed");
+        return -1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -372,6 +372,48 @@ int afr_forces_cases_f64(const AfrMesh *mesh, const void *U, const void *mu_t, d
  * src/autoflowcfd/postprocess/fr_coefficients.py:27-152.  Cd = F_x/(q A), Cl = F_z/(q A). */
 int afr_forces_f64(const AfrMesh *mesh, const void *U, const void *mu_t, double mu, double *out, void *stream);
 
+/* The P0 phase of the order continuation (src/autoflowcfd/core/utils/order_continuation.py:207-425)
+ * on one mesh: plain arrays, no library object.  Per cell: det J, adj J (3x3) and the metric flux
+ * scale at the P0 point (the centre of the reference cube), vol1 = the one-point volume 8 det J
+ * (the local time step's h, cfl.py:96-101) in the storage precision, and cell_volume (FP64) = the
+ * target-order volume of the finite-volume residual.  Per face record: owner, neighbour (-1 on
+ * the boundary), bc = type | (row << 4) on the boundary (-1 inside), flags (bit 0 owner-primary,
+ * bits 1-2 the owner's face axis, bit 3 its side +1), geom (FP64: unit normal owner -> neighbour,
+ * area).  cell_inc (C, 8): record * 16 + role * 8 + face code (role 1: the cell is the
+ * neighbour), -1 padded; cell_ghost_rec (C): the cell's last boundary record (-1 none).
+ * bc_params (8, 5) FP64.  Work buffers (C, 5) x 3, dt (C), work_part 4096 doubles. */
+typedef struct AfrP0Desc {
+    int32_t abi_version;
+    int32_t dtype;
+    int64_t n_cells;
+    int64_t n_records;
+    const void *det, *adj, *mfs, *vol1;
+    const double *cell_volume;
+    const int32_t *rec_owner, *rec_nb, *rec_bc, *rec_flags;
+    const double *rec_geom;
+    const int32_t *cell_inc, *cell_ghost_rec;
+    const double *bc_params;
+    void *work_r, *work_s1, *work_s2, *work_dt;
+    double *work_part;
+} AfrP0Desc;
+
+/* P0 residual of the conserved state U (C, 1, 5) into R: the finite-volume inviscid residual of
+ * src/autoflowcfd/core/fr_residual/inviscid_p0.py:17-143 (each boundary record with the ghost
+ * state its owner cell keeps, that of the cell's last boundary record, inviscid_p0.py:118-143)
+ * plus the one-point viscous residual (viscous_p0_kernel.py:26-285: the boundary interior penalty,
+ * the only term a constant state leaves).  mu_t (C) may be NULL. */
+int afr_p0_residual(const AfrP0Desc *desc, const void *U, const void *mu_t, double mu, void *R, void *stream);
+
+/* One P0 SSP-RK3 pseudo-step of U (C, 1, 5) in place (dt_local at P0, three stages with
+ * positivity; the P0 filter is the identity) and the RMS norm of R0 into norm[0] (device). */
+int afr_p0_ssp_rk3_step(const AfrP0Desc *desc, void *U, const void *mu_t, double mu, int32_t n_vars_total,
+                        double *norm, void *stream);
+
+/* X_new (C, ns_new, n_vars) = W (ns_new, ns_old, FP64, device) applied to X_old (C, ns_old, n_vars)
+ * per cell: the interpolation between orders of order_continuation.py:14-57.  ns <= 64. */
+int afr_interpolate_order(const void *X_old, void *X_new, const double *W, int64_t n_cells, int32_t ns_old,
+                          int32_t ns_new, int32_t n_vars, int32_t dtype, void *stream);
+
 /* HIP graph of afr_* calls (no reference counterpart: the reference launches every kernel of a
  * pseudo-step from Python, src/autoflowcfd/core/gpu/gpu_solver.py:step).  afr_graph_begin starts
  * capturing the work this host thread issues on `stream` (thread-local capture: other threads

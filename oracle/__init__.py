@@ -17,11 +17,17 @@ per-cell discrete conservation (prisms and tets, interior and boundary faces) an
 one-dimensional FR reduction on layered prisms, and on smooth states by convergence to exact
 solutions (isentropic vortex: rate P; manufactured viscous field: rate P - 1, the reference's
 unlifted BR1 gradient) on prisms; each check rejects planted mistakes.
+``continuation`` (tests/test_oracle_continuation.py): the order interpolation reproduces every
+trilinear field exactly (inside and outside the old points) and its rows sum to one; the P0
+finite-volume residual preserves the free stream and is discretely conservative against the
+boundary flux with the per-cell ghost quirk; the P0 viscous residual equals the closed-form wall
+penalty g'(+-1/2)(-eta |a| side) 2u / det J and vanishes without walls.
 
 Parity unpinned (no reference fixture and no closed form pins them; each says so in its
 docstring): the pointwise distribution of the tet interface corrections (the collapsed tet scheme
 does not converge under refinement, so only its per-cell sums are pinned),
 ``residual.suppress_outliers`` decisions on general fields beyond the reference's own test cases,
 ``sst.source_terms``, ``sst.transport`` and ``sst.turbulence_step`` values,
-``time_step.local_time_step`` values and the ``time_step.ns_step`` trajectory.
+``time_step.local_time_step`` values and the ``time_step.ns_step`` trajectory, and so the
+trajectory of ``continuation.run`` (its P0 time step reuses the local time step formula).
 """

@@ -132,10 +132,12 @@ def test_abi_struct_layouts_match_ctypes(tmp_path):
     src = tmp_path / "layout.c"
     fields_m = [f[0] for f in _lib.AfrMeshDesc._fields_]
     fields_g = [f[0] for f in _lib.AfrGeomDesc._fields_]
+    fields_p = [f[0] for f in _lib.AfrP0Desc._fields_]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "afr.h"', 'int main(void) {',
-             'printf("%zu %zu\\n", sizeof(AfrMeshDesc), sizeof(AfrGeomDesc));']
+             'printf("%zu %zu %zu\\n", sizeof(AfrMeshDesc), sizeof(AfrGeomDesc), sizeof(AfrP0Desc));']
     lines += [f'printf("%zu\\n", offsetof(AfrMeshDesc, {f}));' for f in fields_m]
     lines += [f'printf("%zu\\n", offsetof(AfrGeomDesc, {f}));' for f in fields_g]
+    lines += [f'printf("%zu\\n", offsetof(AfrP0Desc, {f}));' for f in fields_p]
     lines += ['return 0; }']
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
@@ -143,10 +145,14 @@ def test_abi_struct_layouts_match_ctypes(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     vals = [int(x) for x in out]
     assert vals[0] == ctypes.sizeof(_lib.AfrMeshDesc) and vals[1] == ctypes.sizeof(_lib.AfrGeomDesc)
+    assert vals[2] == ctypes.sizeof(_lib.AfrP0Desc)
     off_m = [getattr(_lib.AfrMeshDesc, f).offset for f in fields_m]
     off_g = [getattr(_lib.AfrGeomDesc, f).offset for f in fields_g]
-    assert vals[2:2 + len(fields_m)] == off_m
-    assert vals[2 + len(fields_m):] == off_g
+    off_p = [getattr(_lib.AfrP0Desc, f).offset for f in fields_p]
+    n_m, n_g = len(fields_m), len(fields_g)
+    assert vals[3:3 + n_m] == off_m
+    assert vals[3 + n_m:3 + n_m + n_g] == off_g
+    assert vals[3 + n_m + n_g:] == off_p
 
 
 def test_error_paths_without_gpu():
