@@ -61,6 +61,16 @@ class AfrP0Desc(ctypes.Structure):
     ]
 
 
+class AfrTopoDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("pad_", ctypes.c_int32), ("n_nodes", ctypes.c_int64),
+        ("n_prism", ctypes.c_int64), ("n_tet", ctypes.c_int64),
+        ("nodes", c_vp), ("prism_conn", c_vp), ("tet_conn", c_vp), ("work", c_vp), ("work_bytes", ctypes.c_size_t),
+        ("owner", c_vp), ("neighbor", c_vp), ("owner_code", c_vp), ("neighbor_code", c_vp), ("node_ids", c_vp),
+        ("normal", c_vp), ("area", c_vp), ("center", c_vp), ("n_faces", c_vp), ("status", c_vp),
+    ]
+
+
 # every function declared in include/afr.h: name -> (restype, argtypes)
 SIGNATURES = {
     "afr_version": (ctypes.c_int, []),
@@ -89,6 +99,8 @@ SIGNATURES = {
     "afr_p0_residual": (ctypes.c_int, [ctypes.POINTER(AfrP0Desc), c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
     "afr_p0_ssp_rk3_step": (ctypes.c_int, [ctypes.POINTER(AfrP0Desc), c_vp, c_vp, ctypes.c_double, ctypes.c_int32,
                                            c_vp, c_vp]),
+    "afr_face_topology_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "afr_face_topology": (ctypes.c_int, [ctypes.POINTER(AfrTopoDesc), c_vp]),
     "afr_interpolate_order": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,

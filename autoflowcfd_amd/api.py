@@ -28,7 +28,7 @@ def run_steady(volume_mesh, backend: str = "gpu", order: int = 2, turbulence_mod
         from .meshio import load_mesh
         volume_mesh = load_mesh(volume_mesh)
     from .boundary import as_mesh
-    volume_mesh = as_mesh(volume_mesh)
+    volume_mesh = as_mesh(volume_mesh, topology="gpu")
     tm = {"none": "NONE", "sst": "SST"}.get(str(turbulence_model).lower())
     if tm is None:
         raise NotImplementedError(f"turbulence model {turbulence_model!r}: this backend implements none and sst")

@@ -240,11 +240,11 @@ def wall_nodes_of_groups(groups, bc_types, n_nodes, cell_conn) -> np.ndarray:
     return np.array(sorted(out), dtype=np.int64)
 
 
-def _mesh_from_parts(nodes, prism_conn, tet_conn, groups, bc_types, parameters):
+def _mesh_from_parts(nodes, prism_conn, tet_conn, groups, bc_types, parameters, topology="host"):
     nodes = np.asarray(nodes, dtype=np.float64)
     prisms = None if prism_conn is None or len(prism_conn) == 0 else np.asarray(prism_conn, dtype=np.int64)
     tets = None if tet_conn is None or len(tet_conn) == 0 else np.asarray(tet_conn, dtype=np.int64)
-    mesh = mesh_from_cells(nodes, prisms, tets, default_bc=BC_FARFIELD)
+    mesh = mesh_from_cells(nodes, prisms, tets, default_bc=BC_FARFIELD, topology=topology)
     groups = {k: np.asarray(v) for k, v in (groups or {}).items()}
     bc_types = dict(bc_types or {})
     if groups:
@@ -274,7 +274,7 @@ class _Params:
         return self.p.get(name)
 
 
-def mesh_from_volume_mesh(volume_mesh) -> Mesh:
+def mesh_from_volume_mesh(volume_mesh, topology="host") -> Mesh:
     """Mesh of a reference ``VolumeMeshData`` (src/autoflowcfd/grid/high_order/high_order_mesh.py:111-140
     reads the same attributes), with its boundary groups, periodic pairs and wall nodes."""
     nodes = volume_mesh.nodes.get_coordinates()
@@ -291,30 +291,31 @@ def mesh_from_volume_mesh(volume_mesh) -> Mesh:
             params = _Params({n: bm.get_parameters(n) for n in (bc_types or {})})
         else:
             params = _Params(getattr(bm, "parameters", None))
-    return _mesh_from_parts(nodes, prism, tets, groups, bc_types, params)
+    return _mesh_from_parts(nodes, prism, tets, groups, bc_types, params, topology)
 
 
-def mesh_from_high_order_mesh(hom, parameters=None) -> Mesh:
+def mesh_from_high_order_mesh(hom, parameters=None, topology="host") -> Mesh:
     """Mesh of a reference ``HighOrderMesh`` from its orientation-fixed connectivity and boundary
     groups (high_order_mesh.py:86-93,154-175); when it carries its face connectivity, the face node
     triples are kept (meta["reference_face_node_ids"]) so that a provider indexed by its faces maps."""
     mesh = _mesh_from_parts(hom._node_coords, hom._fixed_prism_conn, hom._fixed_tet_conn,
                             getattr(hom, "boundary_groups", None), getattr(hom, "boundary_bc_types", None),
-                            _Params(parameters))
+                            _Params(parameters), topology)
     fc = getattr(hom, "face_connectivity", None)
     if fc is not None and getattr(fc, "face_node_ids", None) is not None:
         mesh.meta["reference_face_node_ids"] = np.asarray(fc.face_node_ids)
     return mesh
 
 
-def as_mesh(obj) -> Mesh:
-    """This package's Mesh from a Mesh, a reference VolumeMeshData or a reference HighOrderMesh."""
+def as_mesh(obj, topology="host") -> Mesh:
+    """This package's Mesh from a Mesh, a reference VolumeMeshData or a reference HighOrderMesh
+    (face topology built by ``topology``: "host" or "gpu", the same records)."""
     if isinstance(obj, Mesh):
         return obj
     if hasattr(obj, "_node_coords"):
-        return mesh_from_high_order_mesh(obj)
+        return mesh_from_high_order_mesh(obj, topology=topology)
     if hasattr(obj, "nodes") and hasattr(getattr(obj, "nodes"), "get_coordinates"):
-        return mesh_from_volume_mesh(obj)
+        return mesh_from_volume_mesh(obj, topology=topology)
     raise TypeError(f"cannot read a mesh from {type(obj).__name__}")
 
 

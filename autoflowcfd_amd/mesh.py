@@ -153,7 +153,7 @@ def graded_coords(n: int, length: float, ratio: Optional[float]) -> np.ndarray:
 
 def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
             displacement=0.0, seed=0, permute=True, bc_by_plane=None,
-            periodic=False, extrude="y") -> Mesh:
+            periodic=False, extrude="y", topology="host") -> Mesh:
     """Structured hex box split into tets or extruded prisms.
 
     Args:
@@ -169,6 +169,7 @@ def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
         extrude: prism extrusion axis, "y" (wall-normal, tests/validation/_channel_mesh.py:159-229)
             or "x" (along the periodic direction, tests/validation/_periodic_mesh.py:22-60: the
             x planes are then single triangle caps).
+        topology: "host" (NumPy) or "gpu" (afr_face_topology): the same records bit for bit.
     """
     rng = np.random.default_rng(seed)
     xs = np.linspace(0.0, Lx, nx + 1)
@@ -242,7 +243,7 @@ def hex_box(nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, split="tet", grading_y=None,
     mesh = Mesh(nodes=nodes, prism_conn=prisms.astype(np.int64), tet_conn=tets.astype(np.int64))
     mesh.meta.update(dict(nx=nx, ny=ny, nz=nz, L=(Lx, Ly, Lz), split=split, seed=seed,
                           grading_y=grading_y, displacement=displacement, extrude=extrude))
-    build_faces(mesh, bc_by_plane or {}, (Lx, Ly, Lz), periodic=periodic)
+    build_faces(mesh, bc_by_plane or {}, (Lx, Ly, Lz), periodic=periodic, topology=topology)
     return mesh
 
 
@@ -306,7 +307,18 @@ def _resolve_codes(mesh: Mesh, cells: np.ndarray, tri: np.ndarray) -> np.ndarray
     return codes
 
 
-def mesh_from_cells(nodes, prism_conn=None, tet_conn=None, default_bc=BC_EXTRAP) -> Mesh:
+def topology_of(mesh: Mesh, topology="host") -> FaceRecords:
+    """The face records of ``mesh``: "host" builds them with NumPy (build_topology), "gpu" on the
+    device (topology.build_topology_gpu, the same records bit for bit)."""
+    if topology == "gpu":
+        from .topology import build_topology_gpu
+        return build_topology_gpu(mesh)
+    if topology != "host":
+        raise ValueError(f"topology {topology!r}: 'host' or 'gpu'")
+    return build_topology(mesh)
+
+
+def mesh_from_cells(nodes, prism_conn=None, tet_conn=None, default_bc=BC_EXTRAP, topology="host") -> Mesh:
     """Mesh from arbitrary prism/tet connectivity; every boundary record gets ``default_bc``."""
     prisms = np.zeros((0, 6), np.int64) if prism_conn is None else np.asarray(prism_conn, np.int64)
     tets = np.zeros((0, 4), np.int64) if tet_conn is None else np.asarray(tet_conn, np.int64)
@@ -314,7 +326,7 @@ def mesh_from_cells(nodes, prism_conn=None, tet_conn=None, default_bc=BC_EXTRAP)
     prisms = fix_prism_orientation(prisms, nodes) if len(prisms) else prisms
     tets = fix_tet_orientation(tets, nodes) if len(tets) else tets
     mesh = Mesh(nodes=nodes, prism_conn=prisms, tet_conn=tets)
-    fr = build_topology(mesh)
+    fr = topology_of(mesh, topology)
     fr.bc[fr.is_boundary] = default_bc
     mesh.faces = fr
     mesh.wall_nodes = np.zeros(0, np.int64)
@@ -377,10 +389,10 @@ def build_topology(mesh: Mesh) -> FaceRecords:
                        translation=np.zeros((n_faces, 3)), bc=np.full(n_faces, BC_INTERIOR, dtype=np.int64))
 
 
-def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_rel=1e-7):
+def build_faces(mesh: Mesh, bc_by_plane: Dict[str, int], L, periodic=False, tol_rel=1e-7, topology="host"):
     """Topology plus face-exact box-plane boundary classification and periodic pairing
     (``periodic``: True for all three axes, or a string of axis letters)."""
-    fr = build_topology(mesh)
+    fr = topology_of(mesh, topology)
     bnd = np.flatnonzero(fr.is_boundary)
     Lx, Ly, Lz = L
     tol = tol_rel * max(Lx, Ly, Lz, 1.0)

@@ -82,7 +82,7 @@ class GPUFRSolver:
         self.cfl = cfl
         self.freestream = {"rho_inf": rho_inf, "vel_inf": vel_inf, "p_inf": p_inf}
         Q_free = np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf])
-        mesh = as_mesh(mesh)
+        mesh = as_mesh(mesh, topology="gpu")
         self.mesh = mesh
         provider = boundary_ghost_provider if hasattr(boundary_ghost_provider, "group_code") else None
         overrides = {} if provider is not None else dict(boundary_ghost_provider or {})

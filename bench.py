@@ -37,11 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def wall_box_mesh(n, seed=0):
+def wall_box_mesh(n, seed=0, topology="host"):
     from autoflowcfd_amd.mesh import (BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, hex_box)
     bc = {"y0": BC_WALL, "x0": BC_INLET, "x1": BC_OUTLET, "y1": BC_SYMMETRY, "z0": BC_SYMMETRY, "z1": BC_SYMMETRY}
     return hex_box(n, n, n, 4.0 * L_UNIT, 1.0 * L_UNIT, 1.0 * L_UNIT, split="prism", grading_y=1.1, seed=seed,
-                   bc_by_plane=bc)
+                   bc_by_plane=bc, topology=topology)
 
 
 def freestream(mach=0.15):
@@ -429,7 +429,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     t0 = time.time()
-    mesh = wall_box_mesh(args.n)
+    mesh = wall_box_mesh(args.n, topology="gpu")
     log(f"[bench] mesh {mesh.n_cells} prisms, {mesh.faces.n_faces} face records ({time.time() - t0:.1f}s)")
     if args.storage == "locality":
         # the solver stores cells in the Morton order of their centroids (the input order is random)

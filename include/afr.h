@@ -414,6 +414,35 @@ int afr_p0_ssp_rk3_step(const AfrP0Desc *desc, void *U, const void *mu_t, double
 int afr_interpolate_order(const void *X_old, void *X_new, const double *W, int64_t n_cells, int32_t ns_old,
                           int32_t ns_new, int32_t n_vars, int32_t dtype, void *stream);
 
+/* Face topology of a prism / tet mesh on the GPU, as FaceExtractor.extract_faces_mixed
+ * (src/autoflowcfd/grid/mesh_gen/extraction/face_extractor.py:296-376) with the cube-face codes of
+ * src/autoflowcfd/grid/connectivity/face_connectivity.py:53-231: every prism contributes 8
+ * triangles (its quads split on the global-node-id diagonal rule, face_extraction_kernels.py:247-322),
+ * every tet 4; the records are the distinct sorted node triples in ascending order, owner = the
+ * first triangle in cell order, neighbour = the second (-1 on the boundary); normal (unit, away from
+ * the owner's vertex centroid), area and centre in FP64 over the sorted triple
+ * (face_geometry_finalize.py:92-160).  Inputs: nodes (n_nodes, 3) FP64, prism_conn (n_prism, 6),
+ * tet_conn (n_tet, 4) int32 node ids.  Outputs, each with room for 8 n_prism + 4 n_tet records:
+ * owner, neighbor, owner_code, neighbor_code (int32), node_ids (3 per record, sorted), normal (3),
+ * area, center (3).  n_faces[0] (device) = the record count; status[0] (device) = 0, or bit 0 set
+ * when a triple is shared by more than two cells, bit 1 when a side matches no cube face or
+ * several.  work: afr_face_topology_workspace bytes of device scratch. */
+typedef struct AfrTopoDesc {
+    int32_t abi_version;
+    int32_t pad_;
+    int64_t n_nodes, n_prism, n_tet;
+    const double *nodes;
+    const int32_t *prism_conn, *tet_conn;
+    void *work;
+    size_t work_bytes;
+    int32_t *owner, *neighbor, *owner_code, *neighbor_code, *node_ids;
+    double *normal, *area, *center;
+    int32_t *n_faces, *status;
+} AfrTopoDesc;
+
+int afr_face_topology_workspace(int64_t n_prism, int64_t n_tet, size_t *bytes);
+int afr_face_topology(const AfrTopoDesc *desc, void *stream);
+
 /* HIP graph of afr_* calls (no reference counterpart: the reference launches every kernel of a
  * pseudo-step from Python, src/autoflowcfd/core/gpu/gpu_solver.py:step).  afr_graph_begin starts
  * capturing the work this host thread issues on `stream` (thread-local capture: other threads
