@@ -71,6 +71,19 @@ class AfrTopoDesc(ctypes.Structure):
     ]
 
 
+class AfrSlotDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("pad_", ctypes.c_int32), ("n_prism", ctypes.c_int64),
+        ("n_tet", ctypes.c_int64), ("n_records", ctypes.c_int64),
+        ("nodes", c_vp), ("prism_conn", c_vp), ("tet_conn", c_vp),
+        ("owner", c_vp), ("neighbor", c_vp), ("owner_code", c_vp), ("neighbor_code", c_vp), ("bc", c_vp),
+        ("bc_row", c_vp), ("normal", c_vp), ("translation", c_vp), ("work", c_vp), ("work_bytes", ctypes.c_size_t),
+        ("nb", c_vp), ("blk", c_vp), ("slot_bc", c_vp), ("slot_bc_row", c_vp), ("tn", c_vp), ("trans", c_vp),
+        ("other", c_vp), ("rec", c_vp), ("slot_flags", c_vp), ("rep", c_vp), ("run_start", c_vp),
+        ("sorted_slot", c_vp), ("counts", c_vp), ("status", c_vp),
+    ]
+
+
 # every function declared in include/afr.h: name -> (restype, argtypes)
 SIGNATURES = {
     "afr_version": (ctypes.c_int, []),
@@ -101,6 +114,8 @@ SIGNATURES = {
                                            c_vp, c_vp]),
     "afr_face_topology_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "afr_face_topology": (ctypes.c_int, [ctypes.POINTER(AfrTopoDesc), c_vp]),
+    "afr_face_slots_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "afr_face_slots": (ctypes.c_int, [ctypes.POINTER(AfrSlotDesc), c_vp]),
     "afr_interpolate_order": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libafr.so")
-SOURCES = ["afr_residual.hip", "afr_prism.hip", "afr_viscous.hip", "afr_step.hip", "afr_sst.hip", "afr_forces.hip", "afr_geometry.hip", "afr_outlier.hip", "afr_p0.hip", "afr_topology.hip"]
+SOURCES = ["afr_residual.hip", "afr_prism.hip", "afr_viscous.hip", "afr_step.hip", "afr_sst.hip", "afr_forces.hip", "afr_geometry.hip", "afr_outlier.hip", "afr_p0.hip", "afr_topology.hip", "afr_slots.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

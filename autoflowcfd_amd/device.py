@@ -160,7 +160,13 @@ class DeviceMesh:
         C = mesh.n_cells
         self.n_cells, self.ns, self.nfp, self.nf = C, T.ns, T.nfp, T.nf
         on_gpu = metrics == "gpu"
-        sl = slots if slots is not None else build_slots(mesh, T, with_geometry=not on_gpu)
+        if slots is not None:
+            sl = slots
+        elif on_gpu:
+            from .slots_gpu import build_slots_gpu
+            sl = build_slots_gpu(mesh, T, device=self.device)
+        else:
+            sl = build_slots(mesh, T, with_geometry=True)
         self.slots = sl
         self._validate(sl, C, T, mesh.n_prism)
 

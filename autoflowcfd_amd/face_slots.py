@@ -167,6 +167,42 @@ def _quad_lower_mask(mesh, cell, code, tri_nodes, n1):
     return low if is_lower else ~low
 
 
+def _own_fp_phys(mesh, T, cells, codes):
+    """Physical positions (K, nfp, 3) of the flux points of cube face ``codes`` of ``cells``."""
+    out = np.zeros((len(cells), T.nfp, 3))
+    for code in np.unique(codes):
+        sel = np.flatnonzero(codes == code)
+        ref = face_pts(T.n1, int(CUBE_FACE_AXIS[code]), float(CUBE_FACE_SIDE[code]))
+        for is_prism in (True, False):
+            ss = sel[(cells[sel] < mesh.n_prism) == is_prism]
+            if len(ss):
+                out[ss] = map_ref(is_prism, ref, _cells_nodes(mesh, cells[ss]))
+    return out
+
+
+def _rows_for(mesh, T, oth_cells, oth_codes, targets):
+    """Interpolation rows (K, nfp, ns) of the other cells at the located targets
+    (face_flux_points.py:164-236: the other cell's basis at the located points, times V^-1)."""
+    nfp = T.nfp
+    out = np.zeros((len(oth_cells), nfp, T.ns))
+    for is_prism in (True, False):
+        ss = np.flatnonzero((oth_cells < mesh.n_prism) == is_prism)
+        if len(ss) == 0:
+            continue
+        free = locate(mesh, oth_cells[ss], oth_codes[ss], targets[ss])
+        for code in np.unique(oth_codes[ss]):
+            s2 = ss[oth_codes[ss] == code]
+            ax = int(CUBE_FACE_AXIS[code])
+            other = [d for d in range(3) if d != ax]
+            abc = np.empty((len(s2), nfp, 3))
+            abc[..., ax] = CUBE_FACE_SIDE[code]
+            f2 = free[np.searchsorted(ss, s2)]
+            abc[..., other[0]] = f2[..., 0]
+            abc[..., other[1]] = f2[..., 1]
+            out[s2] = T.interp[0 if is_prism else 1].at(abc.reshape(-1, 3)).reshape(len(s2), nfp, T.ns)
+    return out
+
+
 def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slots:
     """with_geometry=False leaves adjrow/fallback to the device metric precompute."""
     fr = mesh.faces
@@ -259,40 +295,14 @@ def build_slots(mesh, T, check_samples=4, rng_seed=0, with_geometry=True) -> Slo
     blk = np.zeros((C, NSLOT, 2), dtype=np.int32)
     mask = np.zeros((C, NSLOT), dtype=np.int32)
 
-    # own FP physical positions (only needed for representatives / split slots)
     def own_fp_phys(cells, codes):
-        out = np.zeros((len(cells), nfp, 3))
-        for code in np.unique(codes):
-            sel = np.flatnonzero(codes == code)
-            ref = face_pts(n1, int(CUBE_FACE_AXIS[code]), float(CUBE_FACE_SIDE[code]))
-            for is_prism in (True, False):
-                ss = sel[(cells[sel] < mesh.n_prism) == is_prism]
-                if len(ss):
-                    out[ss] = map_ref(is_prism, ref, _cells_nodes(mesh, cells[ss]))
-        return out
+        return _own_fp_phys(mesh, T, cells, codes)
 
     rows_list = []
     n_blocks = 0
 
     def rows_for(oth_cells, oth_codes, targets):
-        """Interpolation rows (K,nfp,ns) of the other cells at the located targets."""
-        out = np.zeros((len(oth_cells), nfp, T.ns))
-        for is_prism in (True, False):
-            ss = np.flatnonzero((oth_cells < mesh.n_prism) == is_prism)
-            if len(ss) == 0:
-                continue
-            free = locate(mesh, oth_cells[ss], oth_codes[ss], targets[ss])
-            for code in np.unique(oth_codes[ss]):
-                s2 = ss[oth_codes[ss] == code]
-                ax = int(CUBE_FACE_AXIS[code])
-                other = [d for d in range(3) if d != ax]
-                abc = np.empty((len(s2), nfp, 3))
-                abc[..., ax] = CUBE_FACE_SIDE[code]
-                f2 = free[np.searchsorted(ss, s2)]
-                abc[..., other[0]] = f2[..., 0]
-                abc[..., other[1]] = f2[..., 1]
-                out[s2] = T.interp[0 if is_prism else 1].at(abc.reshape(-1, 3)).reshape(len(s2), nfp, T.ns)
-        return out, None
+        return _rows_for(mesh, T, oth_cells, oth_codes, targets), None
 
     # ---- keyed (single-source) slots
     keyed = interior & ~split

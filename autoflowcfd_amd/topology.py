@@ -53,7 +53,7 @@ def build_topology_gpu(mesh, device=None) -> FaceRecords:
     with torch.cuda.device(dv):
         stream = ctypes.c_void_p(torch.cuda.current_stream(dv).cuda_stream)
         _lib.check(lib.afr_face_topology(ctypes.byref(d), stream), "afr_face_topology")
-        torch.cuda.synchronize(dv)
+        torch.cuda.current_stream(dv).synchronize()
     status = int(out["status"].item())
     if status & 1:
         raise RuntimeError("face shared by more than two cells")

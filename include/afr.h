@@ -443,6 +443,39 @@ typedef struct AfrTopoDesc {
 int afr_face_topology_workspace(int64_t n_prism, int64_t n_tet, size_t *bytes);
 int afr_face_topology(const AfrTopoDesc *desc, void *stream);
 
+/* Cell-centric face slots on the GPU: the tables of the slot form of the reference's flat face
+ * geometry (src/autoflowcfd/fr/face_flux_points_merge.py:83-124, the primary side of every cube
+ * face; face_flux_points.py:164-236, the cross-interpolation rows keyed by the face correspondence;
+ * face_flux_points_data.py:145-217, prism quads split between two cells).  Inputs: the mesh
+ * (nodes FP64, prism_conn, tet_conn int32) and its face records (owner, neighbor, owner_code,
+ * neighbor_code, bc, bc_row (may be NULL) int32; normal, translation (3 per record) FP64).  Outputs
+ * per slot (C = n_prism + n_tet cells x 5 slots): nb (2: source cells, -1 none), blk (2: row block
+ * of a keyed slot's first source, 0 otherwise; split slots are completed by the caller), slot_bc
+ * (BC code, -1 interior, -2 unused), slot_bc_row (may be NULL), tn (3, role-signed true normal),
+ * trans (3, role-signed periodic translation), other (4: other cell 0, its face code, other cell 1,
+ * its face code), rec (the record), slot_flags (bit 0 keyed, bit 1 split); per distinct key (row
+ * block): rep (its first slot) and run_start (its first position in sorted_slot); sorted_slot
+ * (every slot, keyed slots by ascending key first); counts[0] = distinct keys, counts[1] = keyed
+ * slots; status[0] = 0, bit 0 when a real cube face is not the primary side of exactly one record,
+ * bit 1 when the corners of a periodic pair do not coincide.  work: afr_face_slots_workspace bytes. */
+typedef struct AfrSlotDesc {
+    int32_t abi_version;
+    int32_t pad_;
+    int64_t n_prism, n_tet, n_records;
+    const double *nodes;
+    const int32_t *prism_conn, *tet_conn;
+    const int32_t *owner, *neighbor, *owner_code, *neighbor_code, *bc, *bc_row;
+    const double *normal, *translation;
+    void *work;
+    size_t work_bytes;
+    int32_t *nb, *blk, *slot_bc, *slot_bc_row;
+    double *tn, *trans;
+    int32_t *other, *rec, *slot_flags, *rep, *run_start, *sorted_slot, *counts, *status;
+} AfrSlotDesc;
+
+int afr_face_slots_workspace(int64_t n_cells, size_t *bytes);
+int afr_face_slots(const AfrSlotDesc *desc, void *stream);
+
 /* HIP graph of afr_* calls (no reference counterpart: the reference launches every kernel of a
  * pseudo-step from Python, src/autoflowcfd/core/gpu/gpu_solver.py:step).  afr_graph_begin starts
  * capturing the work this host thread issues on `stream` (thread-local capture: other threads

@@ -134,13 +134,15 @@ def test_abi_struct_layouts_match_ctypes(tmp_path):
     fields_g = [f[0] for f in _lib.AfrGeomDesc._fields_]
     fields_p = [f[0] for f in _lib.AfrP0Desc._fields_]
     fields_t = [f[0] for f in _lib.AfrTopoDesc._fields_]
+    fields_s = [f[0] for f in _lib.AfrSlotDesc._fields_]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "afr.h"', 'int main(void) {',
-             'printf("%zu %zu %zu %zu\\n", sizeof(AfrMeshDesc), sizeof(AfrGeomDesc), sizeof(AfrP0Desc), '
-             'sizeof(AfrTopoDesc));']
+             'printf("%zu %zu %zu %zu %zu\\n", sizeof(AfrMeshDesc), sizeof(AfrGeomDesc), sizeof(AfrP0Desc), '
+             'sizeof(AfrTopoDesc), sizeof(AfrSlotDesc));']
     lines += [f'printf("%zu\\n", offsetof(AfrMeshDesc, {f}));' for f in fields_m]
     lines += [f'printf("%zu\\n", offsetof(AfrGeomDesc, {f}));' for f in fields_g]
     lines += [f'printf("%zu\\n", offsetof(AfrP0Desc, {f}));' for f in fields_p]
     lines += [f'printf("%zu\\n", offsetof(AfrTopoDesc, {f}));' for f in fields_t]
+    lines += [f'printf("%zu\\n", offsetof(AfrSlotDesc, {f}));' for f in fields_s]
     lines += ['return 0; }']
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
@@ -149,15 +151,14 @@ def test_abi_struct_layouts_match_ctypes(tmp_path):
     vals = [int(x) for x in out]
     assert vals[0] == ctypes.sizeof(_lib.AfrMeshDesc) and vals[1] == ctypes.sizeof(_lib.AfrGeomDesc)
     assert vals[2] == ctypes.sizeof(_lib.AfrP0Desc) and vals[3] == ctypes.sizeof(_lib.AfrTopoDesc)
+    assert vals[4] == ctypes.sizeof(_lib.AfrSlotDesc)
     off_m = [getattr(_lib.AfrMeshDesc, f).offset for f in fields_m]
     off_g = [getattr(_lib.AfrGeomDesc, f).offset for f in fields_g]
     off_p = [getattr(_lib.AfrP0Desc, f).offset for f in fields_p]
     off_t = [getattr(_lib.AfrTopoDesc, f).offset for f in fields_t]
-    n_m, n_g, n_p = len(fields_m), len(fields_g), len(fields_p)
-    assert vals[4:4 + n_m] == off_m
-    assert vals[4 + n_m:4 + n_m + n_g] == off_g
-    assert vals[4 + n_m + n_g:4 + n_m + n_g + n_p] == off_p
-    assert vals[4 + n_m + n_g + n_p:] == off_t
+    off_s = [getattr(_lib.AfrSlotDesc, f).offset for f in fields_s]
+    offs = off_m + off_g + off_p + off_t + off_s
+    assert vals[5:] == offs
 
 
 def test_error_paths_without_gpu():
