@@ -76,10 +76,16 @@ def free_stream_coefficients(F, Q_free, reference_area, include_viscous=True):
 
 
 def case_slots(meshes, orders, host_workers=0):
-    """Slot tables of every case in the locality order GPUFRSolver stores it (face_slots.slots_for),
-    serially or in ``host_workers`` spawned processes (tables are deterministic: the same either way)."""
+    """Slot tables of every case in the locality order GPUFRSolver stores it: on the GPU
+    (slots_gpu.build_slots_gpu) by default, or with NumPy (face_slots.slots_for) in ``host_workers``
+    spawned processes; the tables are the same bit for bit either way."""
     from .face_slots import slots_for
-    if host_workers and len(meshes) > 1:
+    if not host_workers:
+        from .fr_tables import tables
+        from .mesh import locality_renumbered
+        from .slots_gpu import build_slots_gpu
+        return [build_slots_gpu(locality_renumbered(m)[0], tables(p)) for m, p in zip(meshes, orders)]
+    if len(meshes) > 1:
         import multiprocessing
         from concurrent.futures import ProcessPoolExecutor
         with ProcessPoolExecutor(max_workers=host_workers, mp_context=multiprocessing.get_context("spawn")) as pool:
@@ -92,8 +98,8 @@ class CaseBatch:
 
     def __init__(self, cases, order=2, dtype=torch.float32, turb_model="SST", mu_molecular=1.8e-5, Pr=0.72,
                  Pr_t=0.9, device_id=0, ref_state=None, slots_list=None, host_workers=0):
-        """slots_list: each case's slot tables at this order (face_slots.slots_for), else built here,
-        in ``host_workers`` spawned processes when > 0 (NumPy only, never the GPU)."""
+        """slots_list: each case's slot tables at this order, else built here: on the GPU, or in
+        ``host_workers`` spawned processes when > 0 (NumPy only, never the GPU)."""
         if not torch.cuda.is_available():
             raise RuntimeError("CaseBatch needs a visible GPU (HIP device)")
         if not cases:

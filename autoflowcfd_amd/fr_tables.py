@@ -13,6 +13,8 @@ Q_fine = c2f @ Q and div = Op @ F~_fine.
 
 from dataclasses import dataclass
 
+import threading
+
 import numpy as np
 from scipy.linalg import lu_factor, lu_solve
 
@@ -146,11 +148,21 @@ class Tables:
 
 
 _CACHE = {}
+# Host work that calls LAPACK (the table generation, the location of flux points) runs under this
+# lock: solves set up on several host threads would otherwise enter the BLAS library at once.
+HOST_LOCK = threading.RLock()
 
 
 def tables(order: int) -> Tables:
     if order in _CACHE:
         return _CACHE[order]
+    with HOST_LOCK:
+        if order not in _CACHE:
+            _CACHE[order] = _make_tables(order)
+    return _CACHE[order]
+
+
+def _make_tables(order: int) -> Tables:
     n1 = order + 1
     sps = tensor_pts(n1)
     over = min(2 * order, OVER_MAX)
@@ -197,7 +209,6 @@ def tables(order: int) -> Tables:
     t = Tables(order=order, n1=n1, ns=n1 ** 3, nf=len(fine), nfp=n1 * n1, D=D, E=E, c2f=c2f, Dfine=Dfine,
                f2c=f2c, Op=Op, filt=filt, gL=gL, gR=gR, fp_of_sp=fp_of, coord_of_sp=co_of, w_sps=W,
                sps=sps, fine=fine, interp=tuple(interps))
-    _CACHE[order] = t
     return t
 
 

@@ -154,7 +154,7 @@ class SolveServer:
     """Continuous batching of steady solves on one GPU (see the module docstring)."""
 
     def __init__(self, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, max_active=8, hbm_budget=None,
-                 use_graph=True, clock=time.perf_counter, host_workers=2, batched=False, ref_state=None,
+                 use_graph=True, clock=time.perf_counter, host_workers=0, batched=False, ref_state=None,
                  setup_threads=3):
         """batched: step the resident solves of each order as ONE batched mesh assembled on the device
         (batch.ResidentBatch), rebuilt whenever solves join or retire, instead of one graph per solve

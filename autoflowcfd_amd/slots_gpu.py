@@ -17,6 +17,7 @@ import torch
 
 from . import _lib
 from .face_slots import NSLOT, SLOT_CODES, Slots, _own_fp_phys, _quad_lower_mask, _rows_for
+from .fr_tables import HOST_LOCK
 
 
 def build_slots_gpu(mesh, T, check_samples=4, device=None) -> Slots:
@@ -83,11 +84,21 @@ def build_slots_gpu(mesh, T, check_samples=4, device=None) -> Slots:
     bc = out["slot_bc"].cpu().numpy()
     tn = out["tn"].cpu().numpy()
     bc_row = None if out["slot_bc_row"] is None else out["slot_bc_row"].cpu().numpy()
-    flags = out["slot_flags"]
     mask = np.zeros((C, NSLOT), dtype=np.int32)
     own_code = np.full((C, NSLOT), -1, dtype=np.int64)
     own_code[:P] = np.array(SLOT_CODES[0])
     own_code[P:, :4] = np.array(SLOT_CODES[1])
+    with HOST_LOCK:
+        rows, n_blocks, n_split = _host_rows(mesh, T, out, n, n_blocks, n_keyed, own_code, blk, mask, check_samples,
+                                             dv)
+    return Slots(nb=nb, blk=blk, mask=mask, bc=bc, tn=tn, adjrow=None, fallback=None, rows=rows,
+                 n_blocks=n_blocks, keyed_slots=n_keyed, split_slots=n_split, bc_row=bc_row)
+
+
+def _host_rows(mesh, T, out, n, n_blocks, n_keyed, own_code, blk, mask, check_samples, dv):
+    """Interpolation rows of the block representatives and of the split slots (host, LAPACK)."""
+    fr, nfp = mesh.faces, T.nfp
+    flags = out["slot_flags"]
     rows_list = []
     if n_keyed:
         rep = out["rep"][:n_blocks].long()
@@ -135,5 +146,4 @@ def build_slots_gpu(mesh, T, check_samples=4, device=None) -> Slots:
             mask[c, s] = int(np.sum((~m0).astype(np.int64) << np.arange(nfp)))
         n_blocks = nb_blocks
     rows = np.concatenate(rows_list, axis=0) if rows_list else np.zeros((nfp, T.ns))
-    return Slots(nb=nb, blk=blk, mask=mask, bc=bc, tn=tn, adjrow=None, fallback=None, rows=rows,
-                 n_blocks=n_blocks, keyed_slots=n_keyed, split_slots=n_split, bc_row=bc_row)
+    return rows, n_blocks, n_split

@@ -401,11 +401,13 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-active", type=int, default=8, help="resident solves at most")
     ap.add_argument("--serve-seed", type=int, default=1)
-    ap.add_argument("--serve-host-workers", type=int, default=2, help="processes building host tables")
+    ap.add_argument("--serve-host-workers", type=int, default=0,
+                    help="processes building the slot tables with NumPy (0: on the GPU)")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-hex", default="2e3,2e4", help="hex-count range of the batched cases (lo,hi)")
-    ap.add_argument("--batch-host-workers", type=int, default=0, help="processes building the batch's host tables")
+    ap.add_argument("--batch-host-workers", type=int, default=0,
+                    help="processes building the slot tables with NumPy (0: on the GPU)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
     ap.add_argument("--tgv-p3", type=int, default=0,
                     help="run only the P3 Taylor-Green vortex leg on N^3 hex x 6 tets (prints its JSON)")
