@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import DeviceMesh
+from .device import DeviceMesh, record_incidence
 from .geometry import det_adj, jacobian
 from .mesh import CUBE_FACE_AXIS, CUBE_FACE_SIDE
 
@@ -78,16 +78,7 @@ class P0Mesh:
         side = CUBE_FACE_SIDE[fr.owner_code] > 0
         flags = primary.astype(np.int64) | (axis << 1) | (side.astype(np.int64) << 3)
         geom = np.concatenate([fr.normal, fr.area[:, None]], axis=1)
-        inc = np.full((C, 8), -1, dtype=np.int64)
-        fill = np.zeros(C, dtype=np.int64)
-        rid = np.arange(F, dtype=np.int64)
-        nbr = np.flatnonzero(~bnd)
-        for cells, codes in ((fr.owner, rid * 16 + fr.owner_code), (fr.neighbor[nbr], nbr * 16 + 8 + fr.neighbor_code[nbr])):
-            for c, e in zip(cells, codes):
-                if fill[c] >= 8:
-                    raise ValueError("more than 8 face records on one cell")
-                inc[c, fill[c]] = e
-                fill[c] += 1
+        inc = record_incidence(C, fr)
         if F * 16 >= 2 ** 31:
             raise ValueError("too many face records for the int32 incidence encoding")
         ghost = np.full(C, -1, dtype=np.int64)

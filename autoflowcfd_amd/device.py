@@ -141,6 +141,27 @@ class LibGraph:
             pass
 
 
+def record_incidence(C, fr, width=8):
+    """(C, width) int64 incidence of every cell: record * 16 + role * 8 + the cell's face code
+    (role 1: the cell is the neighbour), the owner entries in record order, then the neighbour
+    entries in record order, -1 padded."""
+    F = fr.n_faces
+    rid = np.arange(F, dtype=np.int64)
+    nb = np.flatnonzero(fr.neighbor >= 0)
+    cells = np.concatenate([fr.owner, fr.neighbor[nb]]).astype(np.int64)
+    codes = np.concatenate([rid * 16 + fr.owner_code, nb * 16 + 8 + fr.neighbor_code[nb]])
+    order = np.argsort(cells, kind="stable")
+    cs = cells[order]
+    count = np.bincount(cs, minlength=C)
+    if count.size and count.max() > width:
+        raise ValueError(f"more than {width} face records on one cell")
+    start = np.concatenate([[0], np.cumsum(count)[:-1]])
+    slot = np.arange(len(cs)) - start[cs]
+    inc = np.full((C, width), -1, dtype=np.int64)
+    inc[cs, slot] = codes[order]
+    return inc
+
+
 class DeviceMesh:
     """One mesh at one order on one GPU."""
 
@@ -521,11 +542,16 @@ class DeviceMesh:
             primary[qi[order[~first]]] = False
         return primary
 
+    def _primary(self):
+        if getattr(self, "_primary_cache", None) is None:
+            self._primary_cache = self.owner_primary(self.mesh)
+        return self._primary_cache
+
     def _build_wall_records(self, dev, devi):
         """Owner-primary WALL records of the force integral (fr_coefficients.py:108-112)."""
         from .mesh import is_wall
         fr = self.mesh.faces
-        sel = np.flatnonzero(is_wall(fr.bc) & self.owner_primary(self.mesh))
+        sel = np.flatnonzero(is_wall(fr.bc) & self._primary())
         self.n_wall_rec = len(sel)
         if not len(sel):
             return
@@ -585,7 +611,7 @@ class DeviceMesh:
         fr = mesh.faces
         F = fr.n_faces
         bnd = fr.neighbor < 0
-        primary = self.owner_primary(mesh)
+        primary = self._primary()
         flags = ((~bnd & primary).astype(np.int32)) | (is_wall(fr.bc).astype(np.int32) << 1)
         code_to_slot = np.full(6, -1, dtype=np.int64)
         for si, code in enumerate(SLOT_CODES[0]):
@@ -593,18 +619,7 @@ class DeviceMesh:
         oslot = code_to_slot[fr.owner_code]
         if np.any(oslot < 0):
             raise ValueError("face record with an unknown owner face code")
-        inc = np.full((C, 8), -1, dtype=np.int64)
-        fill = np.zeros(C, dtype=np.int64)
-        rid = np.arange(F, dtype=np.int64)
-        entries = [(fr.owner, rid * 16 + fr.owner_code)]
-        nb = np.flatnonzero(~bnd)
-        entries.append((fr.neighbor[nb], nb * 16 + 8 + fr.neighbor_code[nb]))
-        for cells, codes in entries:
-            for c, e in zip(cells, codes):
-                if fill[c] >= 8:
-                    raise ValueError("more than 8 face records on one cell")
-                inc[c, fill[c]] = e
-                fill[c] += 1
+        inc = record_incidence(C, fr)
         if F * 16 >= 2 ** 31:
             raise ValueError("too many face records for the int32 incidence encoding")
         # records renumbered in the locality order of their owners (prisms: the Morton schedule,
@@ -657,7 +672,7 @@ class DeviceMesh:
                                                   ctypes.c_void_p(self.d_wall[c0:c1].data_ptr()),
                                                   0 if dt == torch.float32 else 1, self._stream()),
                        "afr_wall_distance")
-        torch.cuda.synchronize(self.device)
+        torch.cuda.current_stream(self.device).synchronize()
 
     def sst_fields(self, k0, omega0):
         """Uniform k, omega (C, ns) and zero nu_t / mu_t of the SST model."""
@@ -734,7 +749,7 @@ class DeviceMesh:
             self.sps_xyz = torch.empty((C, T.ns, 3), dtype=f64, device=dv)
             g.sps_xyz = p(self.sps_xyz)
         _lib.check(self.lib.afr_metrics(ctypes.byref(g), self._stream()), "afr_metrics")
-        torch.cuda.synchronize(dv)
+        torch.cuda.current_stream(dv).synchronize()
         if int(bad.item()) != 0:
             raise ValueError("non-positive Jacobian determinant (mesh distortion)")
         self.volume = vol.cpu().numpy()
