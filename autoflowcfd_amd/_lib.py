@@ -5,7 +5,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafr.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_vp = ctypes.c_void_p
 
@@ -34,6 +34,7 @@ class AfrMeshDesc(ctypes.Structure):
         ("work_pack", c_vp), ("work_nbp", c_vp), ("work_tvol", c_vp), ("work_tsrc", c_vp), ("work_crec", c_vp),
         ("n_wall_rec", ctypes.c_int64), ("wrec_owner", c_vp), ("wrec_oslot", c_vp), ("wrec_geom", c_vp),
         ("w1d", c_vp), ("work_fpart", c_vp), ("grid_cap", ctypes.c_int32 * 8),
+        ("variant", ctypes.c_int32 * 8),
     ]
 
 
@@ -130,6 +131,8 @@ SIGNATURES = {
     "afr_forces_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, c_vp, c_vp]),
     "afr_mesh_set_grid": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32]),
     "afr_mesh_get_grid": (ctypes.c_int, [c_vp, ctypes.c_int32]),
+    "afr_mesh_set_variant": (ctypes.c_int, [c_vp, ctypes.c_int32, ctypes.c_int32]),
+    "afr_mesh_get_variant": (ctypes.c_int, [c_vp, ctypes.c_int32]),
     "afr_graph_begin": (ctypes.c_int, [c_vp]),
     "afr_graph_end": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "afr_graph_launch": (ctypes.c_int, [c_vp, c_vp]),

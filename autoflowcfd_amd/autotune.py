@@ -1,12 +1,15 @@
-"""Autotuner of the persistent grid sizes of the residual kernels on one mesh.
+"""Autotuner of the launch shapes of the residual kernels on one mesh.
 
 Every residual kernel walks its cells with a persistent grid; how many workgroups that grid
-has trades latency hiding against tail effects and L2 sharing, and the best value depends on
-the kernel, the order, the precision and the mesh size.  ``autotune`` times each kernel's phase
-with HIP events on the launch stream for a set of candidate grid sizes (results do not depend on
-the grid: only the time does) and leaves the fastest one set on the mesh
-(``afr_mesh_set_grid``).  Choices are cached per (device, order, dtype, kernel, size bucket) in
-memory and, when ``cache_path`` is given, in a JSON file.
+has trades latency hiding against tail effects and L2 sharing, and two kernels also come in two
+tile shapes (``VARIANTS``: the prism inviscid kernel with or without its five slot sources staged
+in LDS, the SST cell kernel with two prisms or one cell per wavefront, which trade LDS and lanes
+against occupancy).  The best choice depends on the kernel, the order, the precision and the
+mesh size.  ``autotune`` times each kernel's phase with HIP events on the launch stream for every
+(tile variant, grid size) candidate (results do not depend on either: only the time does) and
+leaves the fastest set on the mesh (``afr_mesh_set_variant``, ``afr_mesh_set_grid``).  Choices
+are cached per (device, order, dtype, kernel, size bucket) in memory and, when ``cache_path`` is
+given, in a JSON file.
 """
 
 import json
@@ -20,6 +23,7 @@ from . import _lib
 KERNELS = {"prism_inviscid": 0, "tet_inviscid": 1, "visc_volume": 2, "visc_iface": 3, "sst_cell": 4,
            "sst_face": 5, "sst_update": 6}
 CANDIDATES = (0, 256, 512, 1024, 2048, 4096, 8192, 16384)   # workgroups; 0 = built-in default
+VARIANTS = {"prism_inviscid": (0, 1), "sst_cell": (0, 1)}   # tile shapes (afr_mesh_set_variant)
 
 _memo = {}
 
@@ -36,6 +40,23 @@ def set_grid(dm, name, cap):
 
 def get_grid(dm, name):
     return int(dm.lib.afr_mesh_get_grid(dm.handle, KERNELS[name]))
+
+
+def set_variant(dm, name, variant):
+    _lib.check(dm.lib.afr_mesh_set_variant(dm.handle, KERNELS[name], int(variant)), "afr_mesh_set_variant")
+
+
+def get_variant(dm, name):
+    return int(dm.lib.afr_mesh_get_variant(dm.handle, KERNELS[name]))
+
+
+def _apply(dm, name, choice):
+    """choice: {"grid": g, "variant": v} (or a bare grid size)."""
+    if not isinstance(choice, dict):
+        choice = {"grid": int(choice), "variant": 0}
+    set_variant(dm, name, choice.get("variant", 0))
+    set_grid(dm, name, choice["grid"])
+    return choice
 
 
 def _time(fn, reps):
@@ -81,19 +102,20 @@ def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CA
     for name, fn in phases.items():
         key = _key(dm, name)
         if key in _memo or key in cache:
-            cap = _memo.get(key, cache.get(key))
-            set_grid(dm, name, cap)
-            out[name] = {"grid": cap, "ms": None, "cached": True}
+            choice = _apply(dm, name, _memo.get(key, cache.get(key)))
+            out[name] = dict(choice, ms=None, cached=True)
             continue
         times = {}
-        for cap in candidates:
-            set_grid(dm, name, cap)
-            times[cap] = _time(fn, reps)
-        best = min(times, key=times.get)
-        set_grid(dm, name, best)
-        _memo[key] = best
-        cache[key] = best
-        out[name] = {"grid": best, "ms": times}
+        for var in VARIANTS.get(name, (0,)):
+            set_variant(dm, name, var)
+            for cap in candidates:
+                set_grid(dm, name, cap)
+                times[(var, cap)] = _time(fn, reps)
+        var, cap = min(times, key=times.get)
+        choice = _apply(dm, name, {"grid": cap, "variant": var})
+        _memo[key] = choice
+        cache[key] = choice
+        out[name] = dict(choice, ms={f"v{v}/g{g}": t for (v, g), t in times.items()})
     dm.viscous_volume(U, mu_t, mu=mu)
     torch.cuda.synchronize(dm.device)
     if cache_path:

@@ -160,6 +160,19 @@ class CaseBatch:
         return self.dm.ssp_rk3_step(self.U, None, mu=self.mu, Pr=self.Pr, Pr_t=self.Pr_t, n_vars_total=self.n_vars,
                                     norm=self.norms)
 
+    def tune(self, reps=1, candidates=None):
+        """Autotune this batch's kernels (tile variant and persistent grid per kernel, for its order,
+        precision and size) on copies of the state: the batch's own fields are not touched, and
+        results do not depend on the choice.  Call before capture_graph (a graph keeps the launch
+        shapes it was captured with)."""
+        from .autotune import CANDIDATES, autotune
+        U = self.U.clone()
+        sst = None
+        if self.turb == "SST":
+            sst = tuple(x.clone() for x in (self.k, self.omega, self.nu_t, self.mu_t))
+        return autotune(self.dm, U, None if sst is None else sst[3], mu=self.mu, sst_fields=sst, reps=reps,
+                        candidates=candidates or CANDIDATES)
+
     def capture_graph(self):
         """Record one batched pseudo-step into a HIP graph (after one eager warm-up step, counted)."""
         torch.cuda.synchronize(self.device)
@@ -278,6 +291,10 @@ class MixedBatch:
         self.groups[orders[0]]._launch_step()
         for p in orders[1:]:
             cur.wait_stream(self._side[p])
+
+    def tune(self, reps=1, candidates=None):
+        """Autotune every order group on its own mesh (CaseBatch.tune); {order: choices}."""
+        return {p: b.tune(reps, candidates) for p, b in self.groups.items()}
 
     def capture_graph(self):
         """One eager warm-up step (counted), then record one mixed pseudo-step into a HIP graph."""

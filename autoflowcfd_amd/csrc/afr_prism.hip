@@ -68,16 +68,13 @@ __device__ __forceinline__ void st4(real* p, typename V4T<real>::type v) {
 constexpr int r4(int n) { return (n + 3) / 4 * 4; }
 constexpr int r64(int n) { return (n + 63) / 64 * 64; }
 
-// 1: the five slot source cells' dQ are copied into LDS with the own cell's; 0: only the own
-// cell is staged and the interface lanes read the sources from global memory.  0 frees
-// 5 x 540 B of LDS per wave at P2 FP32 (29 instead of 39 KB per workgroup, 5 instead of 4 waves
-// per SIMD): measured 3.08 -> 2.76 ms on the 82^3 P2 mesh.
-#ifndef AFR_PRISM_STAGE_NB
-#define AFR_PRISM_STAGE_NB 0
-#endif
-
+// STAGE (the kernel's tile variant) 1: the five slot source cells' dQ are copied into LDS with
+// the own cell's; 0: only the own cell is staged and the interface lanes read the sources from
+// global memory.  0 frees 5 x 540 B of LDS per wave at P2 FP32 (29 instead of 39 KB per
+// workgroup, 5 instead of 4 waves per SIMD): measured 3.08 -> 2.76 ms on the 82^3 P2 mesh; the
+// autotuner picks per mesh.
 // Padded LDS layout of the prism kernel (units: reals).
-template <int P, typename real> struct PL {
+template <int P, typename real, int STAGE> struct PL {
     using K = PK<P>;
     static constexpr int N1 = K::N1, NAB = K::NAB, NF1 = K::NF1, NABF = K::NABF, NF = K::NF, NS = K::NS;
     static constexpr int NFP = K::NFP, NSV = NS * 5;
@@ -95,7 +92,7 @@ template <int P, typename real> struct PL {
     static constexpr int T_G = T_EC + 2 * N1;                  // [2][N1] gL, gR
     static constexpr int T_N = r4(T_G + 2 * N1);
     // per-wave buffers
-    static constexpr int QN_DW = r64((AFR_PRISM_STAGE_NB ? 6 : 1) * NSV * WPR);  // own cell (+ 5 slot sources), raw dwords
+    static constexpr int QN_DW = r64((STAGE ? 6 : 1) * NSV * WPR);  // own cell (+ 5 slot sources), raw dwords
     static constexpr int W_QN = 0;
     static constexpr int W_XY = W_QN + QN_DW / WPR;            // X [NF1][NX] | Y [NAB][5][4]
     static constexpr int XY_N = r4(NF1 * NX > NAB * 20 ? NF1 * NX : NAB * 20);
@@ -154,9 +151,9 @@ __device__ __forceinline__ void dma_cells(const uint32_t* src, const int64_t (&b
     DmaCells<0, NCH, CELL_DW>::run(src, base, dst, t);
 }
 
-template <int P, int KMAX, typename real>
+template <int P, int KMAX, typename real, int STAGE>
 __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_prism_inviscid(PrismArgs<real> a) {
-    using L = PL<P, real>;
+    using L = PL<P, real, STAGE>;
     using v4 = typename V4T<real>::type;
     constexpr int N1 = L::N1, NAB = L::NAB, NF1 = L::NF1, NABF = L::NABF, NF = L::NF, NS = L::NS;
     constexpr int NFP = L::NFP, NSV = L::NSV, NABF4 = L::NABF4, NABP4 = L::NABP4, NG = L::NG, NX = L::NX;
@@ -415,7 +412,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
                     rw[k] = a.rows_w[(int64_t)row * KMAX + k];
                 }
                 real dqn[5] = {0, 0, 0, 0, 0};
-                if (AFR_PRISM_STAGE_NB && src == 0) {  // primary source: staged in LDS
+                if (STAGE && src == 0) {  // primary source: staged in LDS
                     const real* qs = sQ + (1 + slot) * NSV;
 #pragma unroll
                     for (int k = 0; k < KMAX; ++k)
@@ -556,12 +553,16 @@ int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
     const dim3 grid((unsigned)(g > 0 ? g : 1)), block(64 * kWaves);
     // interpolation rows: Kronecker-sparse (prism sources: P2 <= 9, P1 <= 4 non-zeros) or dense (tet sources)
     constexpr int KS = (P + 1) * (P + 1), NS = PK<P>::NS;
-    if (a.knz == KS)
-        hipLaunchKernelGGL((k_prism_inviscid<P, KS, real>), grid, block, 0, st, a);
-    else if (a.knz == NS)
-        hipLaunchKernelGGL((k_prism_inviscid<P, NS, real>), grid, block, 0, st, a);
-    else
+    if (a.variant < 0 || a.variant > 1) return -1;
+    if (a.knz == KS) {
+        if (a.variant) hipLaunchKernelGGL((k_prism_inviscid<P, KS, real, 1>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_prism_inviscid<P, KS, real, 0>), grid, block, 0, st, a);
+    } else if (a.knz == NS) {
+        if (a.variant) hipLaunchKernelGGL((k_prism_inviscid<P, NS, real, 1>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_prism_inviscid<P, NS, real, 0>), grid, block, 0, st, a);
+    } else {
         return -1;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -38,6 +38,7 @@ struct PrismArgs {
     int64_t n_cells;  // prisms occupy cells [0, n_cells)
     const int* schedule;  // (n_cells) processing order of the prisms (locality order)
     int grid_cap = 0;     // persistent grid size (workgroups); 0: built-in default
+    int variant = 0;      // 0: own cell staged in LDS; 1: own cell and slot sources staged
     const real* dQ;
     const real* umean;
     real* R;

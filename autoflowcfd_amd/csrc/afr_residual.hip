@@ -370,6 +370,7 @@ int launch_inviscid(const AfrMeshDesc& d, const real* Q, const real* umean, real
         p.n_cells = d.n_prism;
         p.schedule = d.prism_schedule;
         p.grid_cap = d.grid_cap[AFR_K_PRISM_INVISCID];
+        p.variant = d.variant[AFR_K_PRISM_INVISCID];
         p.dQ = Q;
         p.umean = umean;
         p.R = R;
@@ -513,7 +514,7 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
     m->case_wall_off = std::move(wo);
     // the host arrays are the caller's: only the copies are kept
     m->d.case_prism_off = m->d.case_tet_off = m->d.case_wall_off = nullptr;
-    for (int k = 0; k < 8; ++k) m->d.grid_cap[k] = 0;
+    for (int k = 0; k < 8; ++k) m->d.grid_cap[k] = m->d.variant[k] = 0;
     *out = m;
     return 0;
 }
@@ -530,6 +531,20 @@ int afr_mesh_set_grid(AfrMesh* mesh, int32_t kernel, int32_t grid_cap) {
 int afr_mesh_get_grid(const AfrMesh* mesh, int32_t kernel) {
     if (!mesh || kernel < 0 || kernel > AFR_K_SST_UPDATE) return fail("afr_mesh_get_grid: bad argument");
     return mesh->d.grid_cap[kernel];
+}
+
+int afr_mesh_set_variant(AfrMesh* mesh, int32_t kernel, int32_t variant) {
+    if (!mesh) return fail("afr_mesh_set_variant: null mesh");
+    if (kernel < 0 || kernel > AFR_K_SST_UPDATE) return fail("afr_mesh_set_variant: unknown kernel id");
+    const int n_var = (kernel == AFR_K_PRISM_INVISCID || kernel == AFR_K_SST_CELL) ? 2 : 1;
+    if (variant < 0 || variant >= n_var) return fail("afr_mesh_set_variant: no such variant of this kernel");
+    mesh->d.variant[kernel] = variant;
+    return 0;
+}
+
+int afr_mesh_get_variant(const AfrMesh* mesh, int32_t kernel) {
+    if (!mesh || kernel < 0 || kernel > AFR_K_SST_UPDATE) return fail("afr_mesh_get_variant: bad argument");
+    return mesh->d.variant[kernel];
 }
 
 int afr_mesh_destroy(AfrMesh* mesh) {
@@ -1029,6 +1044,7 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
             b.Esp_i = d.esp_i[ct];
             b.schedule = ct == 0 ? d.prism_schedule : nullptr;
             b.grid_cap = d.grid_cap[phase == 0 ? AFR_K_SST_CELL : AFR_K_SST_UPDATE];
+            b.variant = phase == 0 ? d.variant[AFR_K_SST_CELL] : 0;
             if (b.n_cells <= 0) continue;
             int rc;
             if (phase == 0)

@@ -131,22 +131,20 @@ template <int P, int CT> struct SW {
     static constexpr int KE = CT == 0 ? N1 * N1 : NS;
 };
 
-// cells per wavefront of the cell kernel: two half-waves of 32 lanes when a cell has at most
-// 32 solution points, so the per-point phases (FP64 sources) keep 54 of 64 lanes busy at P2
-#ifndef AFR_SST_PAIR
-#define AFR_SST_PAIR 1
-#endif
-template <int P, int CT> struct SstCellPlan {
+// cells per wavefront of the cell kernel (its tile variant, PAIR): two half-waves of 32 lanes when
+// a cell has at most 32 solution points, so the per-point phases (FP64 sources) keep 54 of 64
+// lanes busy at P2 (PAIR 1), or one cell per wavefront (PAIR 0); the autotuner picks per mesh
+template <int P, int CT, int PAIR> struct SstCellPlan {
     static constexpr int NS = (P + 1) * (P + 1) * (P + 1);
-    static constexpr int CPW = (AFR_SST_PAIR && CT == 0 && NS <= 32) ? 2 : 1;   // cells per wave (prisms)
+    static constexpr int CPW = (PAIR && CT == 0 && NS <= 32) ? 2 : 1;   // cells per wave (prisms)
     static constexpr int LW = 64 / CPW;                             // lanes per cell
 };
 
-template <int P, int CT, typename real>
+template <int P, int CT, typename real, int PAIR>
 __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_sst_cell(SstArgs<real> a) {
     using O = SO<P, CT>;
     using W = SW<P, CT>;
-    using L = SstCellPlan<P, CT>;
+    using L = SstCellPlan<P, CT, PAIR>;
     constexpr int NS = O::NS, WM = W::WM, KE = W::KE, CPW = L::CPW, LW = L::LW;
     constexpr int WD[3] = {W::W0, W::W1, W::W2};
     // per-SP record (12): rho, u, v, w, k, omega, Gk, Gw (the face-trace fields, contiguous),
@@ -578,14 +576,22 @@ __global__ void __launch_bounds__(256) k_wall_distance(const double* __restrict_
     }
 }
 
+template <int P, int CT, typename real, int PAIR>
+int launch_sst_cell_t(const SstArgs<real>& a, hipStream_t st) {
+    constexpr int CPW = SstCellPlan<P, CT, PAIR>::CPW;
+    const int64_t units = (a.n_cells + CPW - 1) / CPW;
+    hipLaunchKernelGGL((k_sst_cell<P, CT, real, PAIR>),
+                       dim3((unsigned)walk_grid(units, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)), dim3(64 * kWaves),
+                       0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <int P, int CT, typename real>
 int launch_sst_cell(const SstArgs<real>& a, hipStream_t st) {
     if (a.n_cells <= 0) return 0;
-    constexpr int CPW = SstCellPlan<P, CT>::CPW;
-    const int64_t units = (a.n_cells + CPW - 1) / CPW;
-    hipLaunchKernelGGL((k_sst_cell<P, CT, real>), dim3((unsigned)walk_grid(units, kWaves, a.grid_cap > 0 ? a.grid_cap : 4096)),
-                       dim3(64 * kWaves), 0, st, a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (a.variant == 0) return launch_sst_cell_t<P, CT, real, 1>(a, st);
+    if (a.variant == 1) return launch_sst_cell_t<P, CT, real, 0>(a, st);
+    return -1;
 }
 
 template <int P, typename real>

@@ -54,6 +54,7 @@ struct SstArgs {
     const int* schedule;          // null, or the locality order of the cells of this type
     int grid_cap = 0;     // persistent grid size (workgroups); 0: built-in default
     int grid_cap_face = 0;  // same for the face-record kernel
+    int variant = 0;        // cell kernel: 0 two prisms per wavefront, 1 one cell per wavefront
 };
 
 template <int P, int CT, typename real>

@@ -260,6 +260,9 @@ def batched_leg(args):
     t0 = time.perf_counter()
     b = MixedBatch([r.case for r in reqs], [r.order for r in reqs], dtype=torch.float32, turb_model="SST",
                    ref_state=ref, host_workers=args.batch_host_workers)
+    t_tune = time.perf_counter()
+    tuned = b.tune() if args.batch_tune else None
+    t_tune = time.perf_counter() - t_tune
     t_setup = time.perf_counter() - t0
     b.capture_graph()                                       # one eager warm-up step, counted
     t1 = time.perf_counter()
@@ -269,7 +272,8 @@ def batched_leg(args):
     t_steps = time.perf_counter() - t1
     coef = b.coefficients()
     t_batch = time.perf_counter() - t0
-    log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s (set-up {t_setup:.2f}s, "
+    log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s (set-up {t_setup:.2f}s incl. autotuning "
+        f"{t_tune:.2f}s, "
         f"{t_steps / max(args.batch_steps - 1, 1) * 1e3:.2f} ms per mixed pseudo-step)")
     dcd = max(abs(coef[r.rid]["Cd"] - r.coefficients["Cd"]) for r in plain)
     del b
@@ -280,6 +284,9 @@ def batched_leg(args):
             "solves_per_hour": len(reqs) / t_batch * 3600.0, "plain_solves_per_hour": len(reqs) / t_plain * 3600.0,
             "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd,
             "ms_per_mixed_pseudo_step": t_steps / max(args.batch_steps - 1, 1) * 1e3,
+            "autotune_s": t_tune,
+            "autotuned": None if tuned is None else {f"P{p}": {k: {"grid": v["grid"], "variant": v["variant"]}
+                                                                for k, v in ch.items()} for p, ch in tuned.items()},
             "solution_points": n_sp}
 
 
@@ -390,7 +397,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in persistent grid sizes")
     ap.add_argument("--storage", choices=("locality", "input"), default="locality",
                     help="device cell storage order: Morton order of the centroids, or the mesh's own order")
-    ap.add_argument("--grids", default="", help="explicit grid sizes instead of autotuning, e.g. visc_iface=16384,...")
+    ap.add_argument("--grids", default="", help="explicit launch shapes instead of autotuning: kernel=grid[:variant],...")
     ap.add_argument("--serve-only", action="store_true", help="run only the served-stream leg (prints its JSON)")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-hex", default="2e3,2e4", help="hex-count range of the served requests (lo,hi)")
@@ -406,6 +413,7 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-hex", default="2e3,2e4", help="hex-count range of the batched cases (lo,hi)")
+    ap.add_argument("--batch-tune", type=int, default=1, help="autotune the batch's launch shapes (1) or not (0)")
     ap.add_argument("--batch-host-workers", type=int, default=0,
                     help="processes building the slot tables with NumPy (0: on the GPU)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
@@ -465,15 +473,19 @@ def main():
     if args.grids:
         from autoflowcfd_amd.autotune import set_grid
         tuned = {}
+        from autoflowcfd_amd.autotune import set_variant
         for item in args.grids.split(","):
-            name, cap = item.split("=")
+            name, val = item.split("=")
+            cap, _, var = val.partition(":")
+            set_variant(dm, name.strip(), int(var or 0))
             set_grid(dm, name.strip(), int(cap))
-            tuned[name.strip()] = {"grid": int(cap)}
+            tuned[name.strip()] = {"grid": int(cap), "variant": int(var or 0)}
         log("[bench] grids set: " + args.grids)
     elif not args.no_autotune:
         from autoflowcfd_amd.autotune import autotune
         tuned = autotune(dm, U, mut, mu=MU, sst_fields=(k, w, nut, mut))
-        log("[bench] autotuned grids: " + ", ".join(f"{n} {v['grid']}" for n, v in tuned.items()))
+        log("[bench] autotuned grids: " + ", ".join(f"{n} {v['grid']}" + (f" (variant {v['variant']})" if v.get('variant') else "")
+                                                    for n, v in tuned.items()))
         for _ in range(args.warmup):
             residual_step()
         torch.cuda.synchronize()
@@ -572,6 +584,7 @@ def main():
         "phase_ms": {n: float(t) for n, t in zip(names, ph)},
         "ms_per_rk3_pseudo_step": ms_rk,
         "autotuned_grids": None if tuned is None else {n: v["grid"] for n, v in tuned.items()},
+        "autotuned_variants": None if tuned is None else {n: v.get("variant", 0) for n, v in tuned.items()},
     }
     if not args.no_serve:
         del U, R, rates, k, w, nut, mut, dm

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define AFR_ABI_VERSION 4
+#define AFR_ABI_VERSION 5
 
 enum { AFR_F32 = 0, AFR_F64 = 1 };
 
@@ -163,6 +163,9 @@ typedef struct AfrMeshDesc {
     /* persistent-grid sizes (workgroups) per kernel, indexed by AfrKernel; 0 = built-in default.
      * Set by afr_mesh_set_grid (the autotuner), not read from the caller's descriptor. */
     int32_t grid_cap[8];
+    /* kernel variants (tile shapes) per kernel, indexed by AfrKernel; 0 = the built-in default.
+     * Set by afr_mesh_set_variant (the autotuner), not read from the caller's descriptor. */
+    int32_t variant[8];
 } AfrMeshDesc;
 
 /* Kernels whose persistent grid size is tunable (afr_mesh_set_grid). */
@@ -353,6 +356,15 @@ int afr_suppress_residual_outliers(void *R, const void *U, int64_t n_cells, int3
  * reference counterpart (the reference launches CuPy kernels with fixed grids). */
 int afr_mesh_set_grid(AfrMesh *mesh, int32_t kernel, int32_t grid_cap);
 int afr_mesh_get_grid(const AfrMesh *mesh, int32_t kernel);
+
+/* Tile shape of one kernel on this mesh (no reference counterpart; picked by the autotuner with
+ * the grid size).  AFR_K_PRISM_INVISCID: 0 = the own cell's state staged in LDS, the slot sources
+ * read from global memory; 1 = the own cell and its five slot sources staged in LDS (more LDS per
+ * wavefront, fewer waves per SIMD).  AFR_K_SST_CELL: 0 = two prisms per wavefront (half-waves of
+ * 32 lanes, P <= 2 prisms), 1 = one cell per wavefront.  Other kernels: 0 only.  Every variant
+ * computes the same values bit for bit. */
+int afr_mesh_set_variant(AfrMesh *mesh, int32_t kernel, int32_t variant);
+int afr_mesh_get_variant(const AfrMesh *mesh, int32_t kernel);
 
 /* Per-case RMS norms of a batched mesh (cell_case set): out[i] = sqrt(sum over the cells of
  * case i of R^2 / (cells_i ns n_vars_total)); out: n_cases doubles (device).  Each case is
