@@ -59,6 +59,7 @@ class AfrP0Desc(ctypes.Structure):
         ("rec_owner", c_vp), ("rec_nb", c_vp), ("rec_bc", c_vp), ("rec_flags", c_vp), ("rec_geom", c_vp),
         ("cell_inc", c_vp), ("cell_ghost_rec", c_vp), ("bc_params", c_vp),
         ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
+        ("rec_src", c_vp), ("rec_w", c_vp), ("d_wall", c_vp), ("work_sst", c_vp),
     ]
 
 
@@ -117,6 +118,8 @@ SIGNATURES = {
     "afr_face_topology": (ctypes.c_int, [ctypes.POINTER(AfrTopoDesc), c_vp]),
     "afr_face_slots_workspace": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     "afr_face_slots": (ctypes.c_int, [ctypes.POINTER(AfrSlotDesc), c_vp]),
+    "afr_p0_sst_step": (ctypes.c_int, [ctypes.POINTER(AfrP0Desc), c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double,
+                                       ctypes.c_int32, c_vp, c_vp]),
     "afr_interpolate_order": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32, c_vp]),
     "afr_ssp_rk3_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,

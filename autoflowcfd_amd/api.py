@@ -6,14 +6,10 @@ autoflowcfd_amd.meshio.load_mesh.  With SST the wall distance is taken to the wa
 WALL groups (src/autoflowcfd/cli/solve_wall_distance.py:30-60).  Returns a SolverResult; the solver
 is kept on the returned object as ``result.solver`` for force / state queries.
 
-For P >= 2 the laminar solve runs the order continuation P0 -> P1 -> .. -> P, the reference's
-default in FRSolver.solve (src/autoflowcfd/core/fr_solver/solver.py:236,292); order_continuation=False
-solves at the target order from the free stream.  The P0 phase of the SST transport is not built:
-an SST solve runs at the target order from the free stream, and order_continuation=True with SST
-raises.
+For P >= 2 the solve runs the order continuation P0 -> P1 -> .. -> P (laminar and SST), the
+reference's default in FRSolver.solve (src/autoflowcfd/core/fr_solver/solver.py:236,292);
+order_continuation=False solves at the target order from the free stream.
 """
-
-import warnings
 
 from .solver import GPUFRSolver, SolverResult
 
@@ -33,12 +29,8 @@ def run_steady(volume_mesh, backend: str = "gpu", order: int = 2, turbulence_mod
     if tm is None:
         raise NotImplementedError(f"turbulence model {turbulence_model!r}: this backend implements none and sst")
     cont = kwargs.pop("order_continuation", None)
-    if cont and tm == "SST":
-        raise NotImplementedError("order continuation with SST: the P0 phase of the SST transport is not built")
     if cont is None:
-        cont = order >= 2 and tm == "NONE"
-        if order >= 2 and tm == "SST":
-            warnings.warn("SST solve at P>=2 runs without order continuation (its P0 SST phase is not built)")
+        cont = order >= 2
 
     def make(p):
         return GPUFRSolver(volume_mesh, order=p, rho_inf=rho_inf, vel_inf=vel_inf, p_inf=p_inf, turb_model=tm,

@@ -12,9 +12,12 @@ reference's default for P >= 2 in FRSolver.solve, src/autoflowcfd/core/fr_solver
   * each phase runs at most max_iter // (P + 1) pseudo-steps, converges below tol 10^(P - p), and
     hands over to the next order after at least 20 steps once its residual fell 100x.
 
-Laminar path (turbulence model NONE); the P0 phase of the SST transport is not built, so an SST
-solve raises instead of silently starting at P1.  All orders share the locality cell order the
-solvers store, so the interpolation is one kernel on resident arrays.
+With SST the P0 phase also runs the turbulence step with the one-point operators
+(afr_p0_sst_step), from the reference's reset fields k = 1e-6, omega = 1, nu_t = 0 and the per-cell
+mean of the target order's wall distance (order_continuation.py:241-272); k, omega, nu_t and the
+wall distance are interpolated with the state at every transition (order_continuation.py:118-140),
+so every phase, the target one included, sees that cell-constant wall distance.  All orders share
+the locality cell order the solvers store, so the interpolation is one kernel on resident arrays.
 """
 
 import ctypes
@@ -25,7 +28,7 @@ import torch
 from . import _lib
 from .device import DeviceMesh, record_incidence
 from .geometry import det_adj, jacobian
-from .mesh import CUBE_FACE_AXIS, CUBE_FACE_SIDE
+from .mesh import CUBE_FACE_AXIS, CUBE_FACE_SIDE, PRISM_CUBE_FACES, is_wall
 
 GAMMA = 1.4
 
@@ -49,10 +52,70 @@ def interp_matrix(old_order, new_order):
     return np.kron(np.kron(W1, W1), W1)
 
 
-class P0Mesh:
-    """Device arrays of the P0 phase on one mesh (in the order of ``mesh``)."""
+def _quad_lower(mesh, cells, codes, tri_nodes):
+    """Whether each record's triangle is the lower half of its owner's prism quad (the half that
+    holds the quad's first corner and the diagonal of the triangulation rule,
+    face_flux_points_data.py:145-217)."""
+    cn = mesh.prism_conn[cells]
+    vs = np.argsort(cn[:, :3], axis=1, kind="stable")
+    ws = vs + 3
+    quad = np.array([PRISM_CUBE_FACES[c] for c in (0, 1, 2)])[codes]
+    e0, e1 = quad[:, 0], quad[:, 1]
 
-    def __init__(self, mesh, cell_volume, bc_params, dtype=torch.float32, device="cuda"):
+    def same(a, b, c, d):
+        return ((a == c) & (b == d)) | ((a == d) & (b == c))
+    d01 = same(e0, e1, vs[:, 0], vs[:, 1])
+    d12 = same(e0, e1, vs[:, 1], vs[:, 2])
+    da = np.where(d01, vs[:, 0], np.where(d12, vs[:, 1], vs[:, 0]))
+    db = np.where(d01, ws[:, 1], np.where(d12, ws[:, 2], ws[:, 2]))
+    i0, i1, i2, i3 = quad.T
+    std = same(da, db, i0, i2)
+    loc = np.where(std[:, None], np.stack([i0, i1, i2], 1), np.stack([i0, i1, i3], 1))
+    lower = np.sort(np.take_along_axis(cn, loc, 1), axis=1)
+    return np.all(lower == np.sort(tri_nodes, axis=1), axis=1)
+
+
+def p0_record_sources(mesh):
+    """(F, 2) source cells and (F, 2) weights of every record's owner side at the one P0 flux point
+    (face_flux_points_merge.py:83-124 with the split of face_flux_points_data.py:145-217): a plain
+    interior record takes its neighbour; the primary record of a prism quad group takes the
+    group's first neighbour with weight 1 when its triangle is the quad's lower half (the centre
+    point lies on the diagonal, which belongs to the lower half) and the second record's neighbour
+    with the complementary weight; boundary and non-primary records have none (-1)."""
+    fr = mesh.faces
+    F, P = fr.n_faces, mesh.n_prism
+    src = np.full((F, 2), -1, dtype=np.int64)
+    wgt = np.zeros((F, 2))
+    inner = fr.neighbor >= 0
+    quad = inner & (fr.owner < P) & np.isin(fr.owner_code, (0, 1, 2))
+    plain = np.flatnonzero(inner & ~quad)
+    src[plain, 0] = fr.neighbor[plain]
+    wgt[plain, 0] = 1.0
+    qi = np.flatnonzero(quad)
+    if len(qi):
+        key = fr.owner[qi] * 8 + fr.owner_code[qi]
+        order = np.lexsort((qi, key))
+        qs, ks = qi[order], key[order]
+        start = np.flatnonzero(np.concatenate([[True], ks[1:] != ks[:-1]]))
+        size = np.diff(np.append(start, len(qs)))
+        if np.any(size > 2):
+            raise ValueError("more than two records on one prism quad")
+        g0 = qs[start]
+        low = _quad_lower(mesh, fr.owner[g0], fr.owner_code[g0], fr.node_ids[g0]).astype(np.float64)
+        src[g0, 0] = fr.neighbor[g0]
+        wgt[g0, 0] = low
+        two = size == 2
+        g1 = qs[start[two] + 1]
+        src[g0[two], 1] = fr.neighbor[g1]
+        wgt[g0[two], 1] = 1.0 - low[two]
+    return src, wgt
+
+
+class P0Mesh:
+    """Device arrays of the P0 phase on one mesh (in the order of ``mesh``); with ``d_wall`` (C)
+    also those of the P0 SST step."""
+
+    def __init__(self, mesh, cell_volume, bc_params, dtype=torch.float32, device="cuda", d_wall=None):
         self.lib = _lib.load()
         self.dtype, self.device = dtype, torch.device(device)
         if self.device.index is None:
@@ -76,7 +139,8 @@ class P0Mesh:
         primary = DeviceMesh.owner_primary(mesh)
         axis = CUBE_FACE_AXIS[fr.owner_code]
         side = CUBE_FACE_SIDE[fr.owner_code] > 0
-        flags = primary.astype(np.int64) | (axis << 1) | (side.astype(np.int64) << 3)
+        flags = (primary.astype(np.int64) | (axis << 1) | (side.astype(np.int64) << 3)
+                 | ((bnd & is_wall(fr.bc)).astype(np.int64) << 4))
         geom = np.concatenate([fr.normal, fr.area[:, None]], axis=1)
         inc = record_incidence(C, fr)
         if F * 16 >= 2 ** 31:
@@ -118,6 +182,13 @@ class P0Mesh:
         d.bc_params = p(self.bc_params)
         d.work_r, d.work_s1, d.work_s2 = p(self.work_r), p(self.work_s1), p(self.work_s2)
         d.work_dt, d.work_part = p(self.work_dt), p(self.work_part)
+        self.sst = d_wall is not None
+        if self.sst:
+            src, wgt = p0_record_sources(mesh)
+            self.rec_src, self.rec_w = i32(src), f64(wgt)
+            self.d_wall = torch.as_tensor(d_wall, dtype=dtype).reshape(C).to(dv).contiguous()
+            self.work_sst = torch.zeros((C, 8), dtype=torch.float64, device=dv)
+            d.rec_src, d.rec_w, d.d_wall, d.work_sst = p(self.rec_src), p(self.rec_w), p(self.d_wall), p(self.work_sst)
         self._desc = d
 
     def _stream(self):
@@ -145,6 +216,22 @@ class P0Mesh:
         return self.norm
 
 
+    def sst_step(self, U, k, omega, nu_t, mu_t, mu=1.8e-5, n_vars_total=7):
+        """One P0 pseudo-step with SST: k, omega, nu_t, mu_t (C,) and U updated in place; returns the
+        (1,) FP64 device norm."""
+        if not self.sst:
+            raise ValueError("P0Mesh built without a wall distance: no SST step")
+        self._check(U)
+        for x in (k, omega, nu_t, mu_t):
+            if x.numel() != self.n_cells or x.dtype != self.dtype or not x.is_contiguous() or x.device != self.device:
+                raise ValueError(f"P0 SST fields must be contiguous ({self.n_cells},) {self.dtype} tensors")
+        vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        _lib.check(self.lib.afr_p0_sst_step(ctypes.byref(self._desc), vp(U), vp(k), vp(omega), vp(nu_t), vp(mu_t),
+                                            float(mu), int(n_vars_total), vp(self.norm), self._stream()),
+                   "afr_p0_sst_step")
+        return self.norm
+
+
 def interpolate(X, old_order, new_order, out=None):
     """X (C, ns_old, nv) -> (C, ns_new, nv) on the device (afr_interpolate_order)."""
     lib = _lib.load()
@@ -167,18 +254,30 @@ def solve_with_order_continuation(make_solver, order, max_iter=1000, tol=1e-6, n
                                   history=None):
     """run_order_continuation (order_continuation.py:207-425) on the GPU.
 
-    make_solver(p) -> GPUFRSolver of order p (the same mesh, free stream and storage order).
-    Returns (solver of the target order holding the final state, SolverResult-like dict)."""
+    make_solver(p) -> GPUFRSolver of order p (the same mesh, free stream, turbulence model and
+    storage order).  Returns (solver of the target order holding the final state, SolverResult-like
+    dict)."""
     from .solver import _cons
     target = make_solver(order)
-    if target.turb_model_name != "NONE":
-        raise NotImplementedError("order continuation: the P0 phase of the SST transport is not built")
+    sst = target.turb_model_name == "SST"
     fs = target.freestream
-    C = target.dm.n_cells
+    dm = target.dm
+    C, dv, dt = dm.n_cells, target.device, dm.dtype
     Q_free = np.array([fs["rho_inf"], fs["vel_inf"], 0.0, 0.0, fs["p_inf"]])
-    p0 = P0Mesh(target.dm.mesh, target.dm.volume, target.dm.bc_params.cpu().numpy(), dtype=target.dm.dtype,
-                device=target.device)
-    U = torch.as_tensor(np.broadcast_to(_cons(Q_free), (C, 1, 5)).copy(), dtype=target.dm.dtype).to(target.device)
+    d0 = None
+    if sst:
+        # the reference's P0 wall distance: the mean over each cell's solution points
+        d0 = dm.d_wall.double().mean(dim=1).to(dt).contiguous()
+    p0 = P0Mesh(dm.mesh, dm.volume, dm.bc_params.cpu().numpy(), dtype=dt, device=dv, d_wall=d0)
+    U = torch.as_tensor(np.broadcast_to(_cons(Q_free), (C, 1, 5)).copy(), dtype=dt).to(dv)
+    if sst:
+        # turbulence fields reset to the model's defaults (order_continuation.py:245-259)
+        k = torch.full((C, 1), 1e-6, dtype=dt, device=dv)
+        w = torch.full((C, 1), 1.0, dtype=dt, device=dv)
+        nut = torch.zeros((C, 1), dtype=dt, device=dv)
+        mut = torch.zeros((C, 1), dtype=dt, device=dv)
+        dwall = d0.reshape(C, 1).clone()
+    n_vars = 7 if sst else 5
     orders = list(range(order + 1))
     phase_max = max_iter // len(orders)
     total, final, converged = 0, float("inf"), False
@@ -190,10 +289,25 @@ def solve_with_order_continuation(make_solver, order, max_iter=1000, tol=1e-6, n
             solver = target if p == order else make_solver(p)
             solver.U.copy_(U)
             U = solver.U
+            if sst:
+                ns = (p + 1) ** 3
+                k, w, nut, dwall = (interpolate(x.reshape(C, -1, 1), p - 1, p).reshape(C, ns)
+                                    for x in (k, w, nut, dwall))
+                solver.k.copy_(k)
+                solver.omega.copy_(w)
+                solver.nu_t.copy_(nut)
+                solver.mu_t.copy_(nut * torch.clamp(U[..., 0], min=1e-10))
+                solver.dm.d_wall.copy_(dwall)
+                k, w, nut = solver.k, solver.omega, solver.nu_t
         phase_tol = tol * 10.0 ** (order - p)
         h = []
         for i in range(phase_max):
-            res = float(p0.step(U, mu=target.mu_molecular).item()) if p == 0 else solver.step()
+            if p == 0:
+                n = p0.sst_step(U, k.view(C), w.view(C), nut.view(C), mut.view(C), mu=target.mu_molecular,
+                                n_vars_total=n_vars) if sst else p0.step(U, mu=target.mu_molecular)
+                res = float(n.item())
+            else:
+                res = solver.step()
             h.append(res)
             total += 1
             final = res

@@ -15,12 +15,23 @@
 //                    filter is the identity).
 //   k_interp_order   X_new[c][a][v] = sum_b W[a][b] X_old[c][b][v]: the tensor-product linear
 //                    interpolation of src/autoflowcfd/core/utils/order_continuation.py:14-57.
+//   k_p0_sst_point / k_p0_sst_transport / k_p0_sst_update
+//                    the SST turbulence step of the P0 phase (the FR turbulence step with the
+//                    one-point operators, src/autoflowcfd/core/fr_solver/turbulence.py:178-330): per
+//                    cell the point physics with zero gradients (sst.py:203-330), the transport
+//                    gathered over the cell's face records (transport.py:122-520,
+//                    transport_kernel.py:103-260: every record, owner-side extrapolation, neighbour
+//                    sources or the Neumann / Dirichlet-zero ghost, upwind convection, BR1-type
+//                    diffusion, distribution with g' = -1/2, +1/2 over det J; the two corrections are
+//                    summed separately and subtracted, as the reference does), then the
+//                    point-implicit update with floors (sst.py:335-479).
 #include <hip/hip_runtime.h>
 
 #include <string>
 
 #include "../../include/afr.h"
 #include "afr_physics.hpp"
+#include "afr_sst_point.hpp"
 #include "afr_step.hpp"
 
 using namespace afr;
@@ -159,6 +170,144 @@ __global__ void __launch_bounds__(kThreads) k_interp_order(const real* __restric
     }
 }
 
+
+using afr_sst::A1;
+using afr_sst::B1;
+using afr_sst::B2;
+using afr_sst::BSTAR;
+using afr_sst::KAP;
+using afr_sst::SK1;
+using afr_sst::SK2;
+using afr_sst::SW1;
+using afr_sst::SW2;
+using afr_sst::dmax;
+using afr_sst::dmin;
+using afr_sst::finite_or;
+
+// work_sst per cell: 0 Gamma_k, 1 Gamma_omega, 2 S_k / rho, 3 S_omega / rho, 4 beta, 5 nu_t, 6 T_k, 7 T_omega
+constexpr int kSstW = 8;
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_p0_sst_point(AfrP0Desc d, const real* __restrict__ U,
+                                                           const real* __restrict__ k, const real* __restrict__ w,
+                                                           double mu) {
+    const real* dw = (const real*)d.d_wall;
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
+         c += (int64_t)gridDim.x * kThreads) {
+        const double rho = dmax((double)U[c * 5], 1e-10);
+        const double kc = (double)k[c], wc = (double)w[c], dc = (double)dw[c];
+        // sources with zero gradients (S = 0, grad k . grad omega = 0), mu = 1.8e-5 as the reference's
+        const double musrc = 1.8e-5;
+        const double nu = musrc / rho;
+        const double ks = dmin(kc, 1e40), wsf = dmax(dmin(wc, 1e40), 1e-10);
+        const double CD = dmax(2.0 * rho * SW2 / wsf * 0.0, 1e-10);
+        const double F1 = afr_sst::blend_F1(ks, wsf, dc, nu, rho, CD);
+        const double F2 = afr_sst::blend_F2(ks, wsf, dc, nu);
+        const double beta = F1 * B1 + (1.0 - F1) * B2;
+        double nut = A1 * dmax(ks, 1e-10) / dmax(A1 * dmax(wsf, 1e-10), F2 * 1e-10);
+        nut = dmin(nut, 1.0e5 * musrc / rho);
+        nut = isfinite(nut) ? nut : 0.0;
+        const double Pk = dmin(nut * rho * 0.0, 10.0 * BSTAR * rho * ks * wsf);
+        const double Sk = finite_or(Pk - rho * BSTAR * ks * wsf, 0.0);
+        const double g1 = B1 / BSTAR - SW1 * KAP * KAP / sqrt(BSTAR);
+        const double g2 = B2 / BSTAR - SW2 * KAP * KAP / sqrt(BSTAR);
+        const double gam = F1 * g1 + (1.0 - F1) * g2;
+        const double Sw = finite_or(rho * gam * 0.0 - rho * beta * wsf * wsf + 2.0 * rho * (1.0 - F1) * SW2 / wsf * 0.0,
+                                    0.0);
+        // diffusivities with the transport's own F1 (molecular mu, raw k / omega)
+        const double CDt = dmax(2.0 * rho * SW2 / dmax(wc, 1e-10) * 0.0, 1e-10);
+        const double F1t = afr_sst::blend_F1(kc, wc, dc, mu / rho, rho, CDt);
+        const double rnut = rho * nut;
+        double* o = d.work_sst + c * kSstW;
+        o[0] = mu + (F1t * SK1 + (1.0 - F1t) * SK2) * rnut;
+        o[1] = mu + (F1t * SW1 + (1.0 - F1t) * SW2) * rnut;
+        o[2] = Sk / rho;
+        o[3] = Sw / rho;
+        o[4] = beta;
+        o[5] = nut;
+    }
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_p0_sst_transport(AfrP0Desc d, const real* __restrict__ U,
+                                                               const real* __restrict__ k,
+                                                               const real* __restrict__ w) {
+    const real* det = (const real*)d.det;
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
+         c += (int64_t)gridDim.x * kThreads) {
+        double conv[2] = {0.0, 0.0}, diff[2] = {0.0, 0.0};
+        const double dj = (double)det[c];
+        for (int q = 0; q < 8; ++q) {
+            const int e = d.cell_inc[c * 8 + q];
+            if (e < 0) break;
+            const int r = e >> 4, role = (e >> 3) & 1, code = e & 7;
+            const double g = (code & 1) ? 0.5 : -0.5;   // g'(0) of the side's Radau correction
+            const int64_t o = d.rec_owner[r];
+            const double ro = dmax((double)U[o * 5], 1e-10);
+            const double* n = d.rec_geom + (int64_t)r * 4;
+            const double vo[3] = {(double)U[o * 5 + 1] / ro, (double)U[o * 5 + 2] / ro, (double)U[o * 5 + 3] / ro};
+            const double mass = ro * vo[0] * n[0] + ro * vo[1] * n[1] + ro * vo[2] * n[2];
+            const int s0 = d.rec_src[r * 2], s1 = d.rec_src[r * 2 + 1];
+            const double w0 = d.rec_w[r * 2], w1 = d.rec_w[r * 2 + 1];
+            const bool none = s0 < 0 && s1 < 0;
+            const bool wall = (d.rec_flags[r] >> 4) & 1;
+            for (int f = 0; f < 2; ++f) {
+                const real* phi = f == 0 ? k : w;
+                const double po = (double)phi[o];
+                const double go = d.work_sst[o * kSstW + f];
+                double pn, gn;
+                if (none) {
+                    pn = (f == 0 && wall) ? -po : po;
+                    gn = go;
+                } else {
+                    pn = s0 >= 0 ? w0 * (double)phi[s0] : 0.0;
+                    gn = s0 >= 0 ? w0 * d.work_sst[(int64_t)s0 * kSstW + f] : 0.0;
+                    if (s1 >= 0) {
+                        pn += w1 * (double)phi[s1];
+                        gn += w1 * d.work_sst[(int64_t)s1 * kSstW + f];
+                    }
+                }
+                const double up = mass >= 0.0 ? po : pn;
+                const double cc = mass * (up - po);
+                const double cd = 0.5 * (go + gn) * (0.5 * (po + pn) - po);
+                if (role == 0) {
+                    conv[f] += -(g * cc) / dj;
+                    diff[f] += -(g * cd) / dj;
+                } else {
+                    conv[f] += (g * cc) / dj;
+                    diff[f] += (g * cd) / dj;
+                }
+            }
+        }
+        const double rho = dmax((double)U[c * 5], 1e-10);
+        for (int f = 0; f < 2; ++f) d.work_sst[c * kSstW + 6 + f] = finite_or((conv[f] - diff[f]) / rho, 0.0);
+    }
+}
+
+template <typename real>
+__global__ void __launch_bounds__(kThreads) k_p0_sst_update(AfrP0Desc d, const real* __restrict__ U, real* k, real* w,
+                                                            real* nu_t, real* mu_t, const real* __restrict__ dt) {
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
+         c += (int64_t)gridDim.x * kThreads) {
+        const double* o = d.work_sst + c * kSstW;
+        const double h = (double)dt[c];
+        const double kc = (double)k[c], wc = (double)w[c];
+        const double ws = dmax(wc, 1e-10);
+        double dk = finite_or(o[2] / (1.0 + h * BSTAR * ws), 0.0) + o[6];
+        double dw = finite_or(o[3] / (1.0 + h * o[4] * ws), 0.0) + o[7];
+        dk = finite_or(dk, 0.0);
+        dw = finite_or(dw, 0.0);
+        double kn = kc + h * dk, wn = wc + h * dw;
+        kn = isfinite(kn) ? kn : 1e-12;
+        wn = isfinite(wn) ? wn : 1e-12;
+        k[c] = (real)dmax(kn, 1e-12);
+        w[c] = (real)dmax(wn, 1e-12);
+        const double rho = dmax((double)U[c * 5], 1e-10);
+        nu_t[c] = (real)o[5];
+        mu_t[c] = (real)(rho * o[5]);
+    }
+}
+
 int grid_of(int64_t n) {
     int64_t b = (n + kThreads - 1) / kThreads;
     return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -185,17 +334,16 @@ int stage_t(const void* U0, const void* Ua, const void* R, const void* dt, void*
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// the residual norm and the three Shu-Osher stages of U with the local time step already in work_dt
 template <typename real>
-int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm, hipStream_t st) {
+int stages_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm,
+             hipStream_t st) {
     const int64_t C = d.n_cells;
     real* u = (real*)U;
     real* r = (real*)d.work_r;
     real* s1 = (real*)d.work_s1;
     real* s2 = (real*)d.work_s2;
     real* dt = (real*)d.work_dt;
-    if (afr_step::launch_dt_local<real>(u, (const real*)mu_t, (const real*)d.det, (const real*)d.mfs,
-                                        (const real*)d.vol1, dt, C, 1, mu, 0, 0, st))
-        return -1;
     if (residual_t<real>(d, u, mu_t, r, mu, st)) return -1;
     if (afr_step::launch_norm<real>(r, C * 5, (double)C * n_vars_total, d.work_part, afr_step::kNormParts, norm, st))
         return -1;
@@ -204,6 +352,33 @@ int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_
     if (stage_t<real>(u, s1, r, dt, s2, C, 0.75, 0.25, 0.25, st)) return -1;
     if (residual_t<real>(d, s2, mu_t, r, mu, st)) return -1;
     return stage_t<real>(u, s2, r, dt, u, C, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, st);
+}
+
+template <typename real>
+int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm, hipStream_t st) {
+    if (afr_step::launch_dt_local<real>((const real*)U, (const real*)mu_t, (const real*)d.det, (const real*)d.mfs,
+                                        (const real*)d.vol1, (real*)d.work_dt, d.n_cells, 1, mu, 0, 0, st))
+        return -1;
+    return stages_t<real>(d, U, mu_t, mu, n_vars_total, norm, st);
+}
+
+template <typename real>
+int sst_step_t(const AfrP0Desc& d, void* U, void* k, void* w, void* nu_t, void* mu_t, double mu, int n_vars_total,
+               double* norm, hipStream_t st) {
+    const int g = grid_of(d.n_cells);
+    real* dt = (real*)d.work_dt;
+    // local time step with the previous step's eddy viscosity (nu_t held, mu_eff = mu + rho nu_t)
+    if (afr_step::launch_dt_local<real>((const real*)U, (const real*)nu_t, (const real*)d.det, (const real*)d.mfs,
+                                        (const real*)d.vol1, dt, d.n_cells, 1, mu, 0, 1, st))
+        return -1;
+    hipLaunchKernelGGL((k_p0_sst_point<real>), dim3(g), dim3(kThreads), 0, st, d, (const real*)U, (const real*)k,
+                       (const real*)w, mu);
+    hipLaunchKernelGGL((k_p0_sst_transport<real>), dim3(g), dim3(kThreads), 0, st, d, (const real*)U,
+                       (const real*)k, (const real*)w);
+    hipLaunchKernelGGL((k_p0_sst_update<real>), dim3(g), dim3(kThreads), 0, st, d, (const real*)U, (real*)k,
+                       (real*)w, (real*)nu_t, (real*)mu_t, (const real*)dt);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return stages_t<real>(d, U, mu_t, mu, n_vars_total, norm, st);
 }
 
 }  // namespace
@@ -233,6 +408,20 @@ int afr_p0_ssp_rk3_step(const AfrP0Desc* d, void* U, const void* mu_t, double mu
     const int rc = d->dtype == AFR_F64 ? step_t<double>(*d, U, mu_t, mu, n_vars_total, norm, st)
                                        : step_t<float>(*d, U, mu_t, mu, n_vars_total, norm, st);
     if (rc) afr_set_error("afr_p0_ssp_rk3_step: launch failed");
+    return rc;
+}
+
+int afr_p0_sst_step(const AfrP0Desc* d, void* U, void* k, void* omega, void* nu_t, void* mu_t, double mu,
+                    int32_t n_vars_total, double* norm, void* stream) {
+    if (!desc_ok(d) || !U || !k || !omega || !nu_t || !mu_t || !norm || !d->work_r || !d->work_s1 || !d->work_s2 ||
+        !d->work_dt || !d->work_part || !d->rec_src || !d->rec_w || !d->d_wall || !d->work_sst || n_vars_total < 5) {
+        afr_set_error("afr_p0_sst_step: bad descriptor (SST arrays), work buffer or argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = d->dtype == AFR_F64 ? sst_step_t<double>(*d, U, k, omega, nu_t, mu_t, mu, n_vars_total, norm, st)
+                                       : sst_step_t<float>(*d, U, k, omega, nu_t, mu_t, mu, n_vars_total, norm, st);
+    if (rc) afr_set_error("afr_p0_sst_step: launch failed");
     return rc;
 }
 

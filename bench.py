@@ -534,7 +534,11 @@ def main():
     # per-phase rooflines (prism-only mesh: one launch per phase, three for SST), timed with HIP
     # events on the launch stream; PMC traffic from the committed rocprofv3 summary
     P, knz = args.order, dm.rows_knz
-    sst_kernels = [f"afr_sst::k_sst_cell<{P}, 0, float>", f"afr_sst::k_sst_face<{P}, {knz}, float>",
+    # kernel names as the profiler prints them, with the tile variant in force (afr_mesh_set_variant:
+    # prism variant v stages the slot sources when v = 1; SST cell variant v pairs prisms when v = 0)
+    from autoflowcfd_amd.autotune import get_variant
+    v_inv, v_sst = get_variant(dm, "prism_inviscid"), get_variant(dm, "sst_cell")
+    sst_kernels = [f"afr_sst::k_sst_cell<{P}, 0, float, {1 - v_sst}>", f"afr_sst::k_sst_face<{P}, {knz}, float>",
                    f"afr_sst::k_sst_update<{P}, 0, float>"]
     kr = {}
     for kname, phase, b, parts in (
@@ -542,7 +546,7 @@ def main():
             (" + ".join(sst_kernels), "sst", algorithmic_bytes_sst(n_sp, mesh.n_cells, fr.n_faces), sst_kernels),
             (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
              algorithmic_bytes_visc_volume(n_sp, mesh.n_cells), None),
-            (f"afr_prism::k_prism_inviscid<{P}, {knz}, float>", "inviscid", b_inv, None),
+            (f"afr_prism::k_prism_inviscid<{P}, {knz}, float, {v_inv}>", "inviscid", b_inv, None),
             (f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>", "visc_interface",
              algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp), None)):
         t = float(ph[names.index(phase)]) * 1e-3

@@ -407,6 +407,14 @@ typedef struct AfrP0Desc {
     const double *bc_params;
     void *work_r, *work_s1, *work_s2, *work_dt;
     double *work_part;
+    /* SST (afr_p0_sst_step only, else NULL): per record the neighbour sources of its owner side
+     * at the one flux point (rec_src: two cells, -1 none; rec_w: their FP64 weights, 0 or 1: a prism
+     * quad's point lies on its diagonal and goes to the half the reference assigns it); flags bit 4
+     * marks WALL records; d_wall (C) the wall distance; work_sst (C, 8) FP64 scratch. */
+    const int32_t *rec_src;
+    const double *rec_w;
+    const void *d_wall;
+    double *work_sst;
 } AfrP0Desc;
 
 /* P0 residual of the conserved state U (C, 1, 5) into R: the finite-volume inviscid residual of
@@ -420,6 +428,19 @@ int afr_p0_residual(const AfrP0Desc *desc, const void *U, const void *mu_t, doub
  * positivity; the P0 filter is the identity) and the RMS norm of R0 into norm[0] (device). */
 int afr_p0_ssp_rk3_step(const AfrP0Desc *desc, void *U, const void *mu_t, double mu, int32_t n_vars_total,
                         double *norm, void *stream);
+
+/* One P0 pseudo-step with the SST model (src/autoflowcfd/core/fr_solver/step.py:15-221 on the P0
+ * phase of the order continuation): the local time step with the previous step's eddy viscosity
+ * (nu_t in, mu_eff = mu + rho nu_t); the SST turbulence step with the one-point operators
+ * (zero gradients: sources from destruction only, nu_t = a1 k / max(a1 omega, F2 1e-10) with the
+ * viscosity-ratio cap; transport from the face records alone: upwind convection and the BR1-type
+ * diffusion correction distributed with g' = -1/2, +1/2 over det J, Neumann ghost on records without
+ * neighbour sources, Dirichlet-zero k on WALL records; point-implicit update with floors),
+ * src/autoflowcfd/core/turbulence/sst.py:203-479 and transport.py:122-520 at P0; then the
+ * SSP-RK3 stages of U with mu_t = rho nu_t frozen.  k, omega, nu_t, mu_t (C) updated in place;
+ * norm[0] = RMS of R(U) over C n_vars_total entries. */
+int afr_p0_sst_step(const AfrP0Desc *desc, void *U, void *k, void *omega, void *nu_t, void *mu_t, double mu,
+                    int32_t n_vars_total, double *norm, void *stream);
 
 /* X_new (C, ns_new, n_vars) = W (ns_new, ns_old, FP64, device) applied to X_old (C, ns_old, n_vars)
  * per cell: the interpolation between orders of order_continuation.py:14-57.  ns <= 64. */

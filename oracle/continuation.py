@@ -19,7 +19,13 @@ Restates src/autoflowcfd/core/utils/order_continuation.py:60-425 for the laminar
     fill_value None, as there);
   * ``run``: the phases of run_order_continuation (order_continuation.py:207-425): P0 from the
     uniform free stream, each phase max_iter // (P + 1) steps, converged below tol 10^(P - p),
-    advanced after >= 20 steps once the residual fell 100x, interpolated to the next order.
+    advanced after >= 20 steps once the residual fell 100x, interpolated to the next order;
+  * with SST (``turbulence=True``): ``p0_sst_step`` runs the general turbulence step
+    (oracle.sst.turbulence_step, turbulence.py:178-330) with the one-point operators
+    (generate_fr_operators(0), as order_continuation.py:276 sets them) before the P0 stages with
+    mu_t = rho nu_t; the fields start from k = 1e-6, omega = 1, nu_t = 0 and the wall distance is
+    each cell's mean over the target order's solution points (order_continuation.py:241-272); k,
+    omega, nu_t and the wall distance are interpolated with the state (order_continuation.py:118-140).
 
 P0 geometry: the single solution point is the centre (0, 0, 0) of the collapsed reference cube
 (GL with one point); det J and adj J there, metric flux scale sum_m |adj row m|.
@@ -80,9 +86,10 @@ def p0_inviscid_residual(U, mesh, g0, bc_params):
     return R[:, None, :]
 
 
-def p0_viscous_residual(U, mesh, g0, fl, bc_params, mu):
+def p0_viscous_residual(U, mesh, g0, fl, bc_params, mu, mu_t=None):
     """(C, 1, 5); the BR1 viscous residual at one point (viscous_p0_kernel.py:26-285): only the
-    boundary interior-penalty term of owner-primary boundary records survives."""
+    boundary interior-penalty term of owner-primary boundary records survives, with the owner's
+    mu + mu_t (viscous_p0_kernel.py:172)."""
     Q = conserved_to_primitive(U[..., :5])[:, 0, :]
     C = mesh.n_cells
     R = np.zeros((C, 5))
@@ -95,7 +102,8 @@ def p0_viscous_residual(U, mesh, g0, fl, bc_params, mu):
     ar = g0.adj[cells, 0, axis, :]
     mag = np.sqrt(np.sum(ar * ar, axis=-1))
     h = np.maximum(np.abs(g0.det[cells, 0]) ** (1.0 / 3.0), 1e-300)
-    eta = C_IP * mu / h
+    mut = np.zeros(len(sel)) if mu_t is None else np.asarray(mu_t).reshape(mesh.n_cells)[cells]
+    eta = C_IP * (mu + mut) / h
     pen = -(eta * mag * side)[:, None] * (Qo[:, 1:4] - Qg[:, 1:4])
     gp = np.where(side < 0, -0.5, 0.5)                   # g_L'(0), g_R'(0) of the degree-1 Radau corrections
     contrib = np.zeros((len(sel), 5))
@@ -131,6 +139,45 @@ def p0_step(U, mesh, g0, fl, bc_params, mu, viscous=True):
     return ssp_rk3(U, rhs, dt, SimpleNamespace(filt=np.ones((2, 1, 1))), mesh.n_prism, R0=R0), residual_norm(R0)
 
 
+def p0_sst_step(U, k, w, nut, mesh, g0, g0fr, ops0, fl0, d0, bc_params, mu):
+    """One P0 pseudo-step with SST (step.py:15-221 at P0): dt with the previous eddy viscosity, the
+    turbulence step with the one-point operators, the P0 stages with mu_t = rho nu_t.  Returns
+    (U, k, w, nu_t, RMS norm of R(U) over 7 variables); k, w, nu_t (C, 1)."""
+    from types import SimpleNamespace
+    from . import sst
+    from .residual import BC_WALL
+    from .time_step import local_time_step, ssp_rk3
+    Q = update_primitives(U)
+    dt = local_time_step(Q, g0fr, 0, mu, Q[..., 0] * nut)
+    k, w, nut, _, _ = sst.turbulence_step(U, k, w, g0fr, ops0, fl0, d0, dt, mu, fl0.bc == BC_WALL)
+    mut = Q[..., 0] * nut
+
+    def rhs(X):
+        return p0_inviscid_residual(X, mesh, g0, bc_params) + p0_viscous_residual(X, mesh, g0, fl0, bc_params, mu,
+                                                                                 mut)
+    R0 = rhs(U)
+    U = ssp_rk3(U, rhs, dt, SimpleNamespace(filt=np.ones((2, 1, 1))), mesh.n_prism, R0=R0)
+    return U, k, w, nut, residual_norm(R0, n_vars=7)
+
+
+def sst_step(U, k, w, nut, geom, ops, fl, d, bc_params, order, mu):
+    """One SST pseudo-step at order >= 1 (step.py:15-221): (U, k, w, nu_t, norm over 7 variables)."""
+    from . import sst
+    from .residual import BC_WALL
+    from .time_step import local_time_step, ssp_rk3
+    Q = update_primitives(U)
+    dt = local_time_step(Q, geom, order, mu, Q[..., 0] * nut)
+    k, w, nut, _, _ = sst.turbulence_step(U, k, w, geom, ops, fl, d, dt, mu, fl.bc == BC_WALL)
+    mut = Q[..., 0] * nut
+
+    def rhs(X):
+        return inviscid_residual(X, geom, ops, fl, bc_params) + viscous_residual(X, geom, ops, fl, bc_params, mu,
+                                                                               mu_t=mut)
+    R0 = rhs(U)
+    U = ssp_rk3(U, rhs, dt, ops, geom.n_prism, R0=R0)
+    return U, k, w, nut, residual_norm(R0, n_vars=7)
+
+
 def interp_matrix(old_order, new_order):
     """(ns_new, ns_old) linear interpolation / extrapolation on the tensor GL points
     (order_continuation.py:14-57)."""
@@ -154,31 +201,47 @@ def interp_matrix(old_order, new_order):
     return W
 
 
-def run(mesh, target_order, bc_params, U_free, mu, max_iter, tol, fr_ops, record=None):
-    """Phases P0 .. target_order of run_order_continuation (order_continuation.py:207-425) on the
-    laminar Navier-Stokes path.  Returns (U at the target order, per-phase residual histories)."""
+def run(mesh, target_order, bc_params, U_free, mu, max_iter, tol, fr_ops, record=None, turbulence=False):
+    """Phases P0 .. target_order of run_order_continuation (order_continuation.py:207-425), laminar
+    or (turbulence=True) SST.  Returns (U at the target order, per-phase residual histories), and
+    with SST also the final (k, w, nu_t)."""
+    from . import sst
     ops_t = fr_ops.generate_operators(target_order)
     g_t = Geometry(mesh, ops_t)
     g0 = P0Geometry(mesh, g_t)
     fl0 = build_flat_faces(mesh, ops_t)      # owner-primary flags and axes do not depend on the order
-    U = np.broadcast_to(np.asarray(U_free, dtype=np.float64), (mesh.n_cells, 1, 5)).copy()
+    C = mesh.n_cells
+    U = np.broadcast_to(np.asarray(U_free, dtype=np.float64), (C, 1, 5)).copy()
+    if turbulence:
+        ops0 = fr_ops.generate_operators(0)
+        g0fr, fl0fr = Geometry(mesh, ops0), build_flat_faces(mesh, ops0)
+        d = np.mean(sst.wall_distance(g_t.sps_xyz, mesh.nodes[mesh.wall_nodes]), axis=1, keepdims=True)
+        k, w, nut = np.full((C, 1), 1e-6), np.full((C, 1), 1.0), np.zeros((C, 1))
     orders = list(range(0, target_order + 1))
     phase_max = max_iter // len(orders)
     hist = {}
     for p in orders:
         if p > 0:
-            U = np.einsum("ab,cbv->cav", interp_matrix(p - 1, p), U)
-        if p > 0:
+            W = interp_matrix(p - 1, p)
+            U = np.einsum("ab,cbv->cav", W, U)
+            if turbulence:
+                k, w, nut, d = (np.einsum("ab,cb->ca", W, x) for x in (k, w, nut, d))
             ops = fr_ops.generate_operators(p)
             g, fl = Geometry(mesh, ops), build_flat_faces(mesh, ops)
         phase_tol = tol * 10.0 ** (target_order - p)
         h = []
         for i in range(phase_max):
             if p == 0:
-                U, res = p0_step(U, mesh, g0, fl0, bc_params, mu)
+                if turbulence:
+                    U, k, w, nut, res = p0_sst_step(U, k, w, nut, mesh, g0, g0fr, ops0, fl0fr, d, bc_params, mu)
+                else:
+                    U, res = p0_step(U, mesh, g0, fl0, bc_params, mu)
             else:
-                from .time_step import ns_step
-                U, res, _ = ns_step(U, g, ops, fl, bc_params, p, mu=mu)
+                if turbulence:
+                    U, k, w, nut, res = sst_step(U, k, w, nut, g, ops, fl, d, bc_params, p, mu)
+                else:
+                    from .time_step import ns_step
+                    U, res, _ = ns_step(U, g, ops, fl, bc_params, p, mu=mu)
             h.append(res)
             if res < phase_tol:
                 break
@@ -187,8 +250,11 @@ def run(mesh, target_order, bc_params, U_free, mu, max_iter, tol, fr_ops, record
         hist[p] = h
         if p == target_order and h[-1] < phase_tol:
             break
+    if turbulence:
+        return U, hist, (k, w, nut)
     return U, hist
 
 
 __all__ = ["P0Geometry", "p0_inviscid_residual", "p0_viscous_residual", "p0_local_time_step", "p0_step",
+           "p0_sst_step", "sst_step",
            "interp_matrix", "run", "ausm_up", "inviscid_residual", "viscous_residual", "update_primitives"]

@@ -317,11 +317,14 @@ _CACHE: Dict[int, Operators] = {}
 
 
 def generate_operators(order: int) -> Operators:
-    """src/autoflowcfd/fr/operators.py:105-206 for order >= 1."""
+    """src/autoflowcfd/fr/operators.py:105-206.  Order 0 gives the one-point operators the order
+    continuation runs its P0 phase's turbulence step with (order_continuation.py:241,276:
+    generate_fr_operators(0)): zero derivatives, unit extrapolation and filter, g' = -1/2, +1/2;
+    the P0 mean-flow residual is the finite-volume one (oracle.continuation)."""
     if order in _CACHE:
         return _CACHE[order]
-    if order < 1:
-        raise ValueError("the FR path needs order >= 1 (P0 is the FV path)")
+    if order < 0:
+        raise ValueError("negative order")
     n1d = order + 1
     ref = tensor_points(n1d)
     types = ("prism", "tet")

@@ -163,5 +163,97 @@ def test_run_steady_runs_the_continuation_for_laminar_p2():
     s = GPUFRSolver(mesh, order=2, **kw)
     ref = s.solve(max_iter=12, tol=1e-12, output_interval=0)
     assert direct.iterations == 12 and direct.final_residual == ref["final_residual"]
-    with pytest.raises(NotImplementedError):
-        run_steady(mesh, order=2, turbulence_model="sst", max_iter=2, order_continuation=True, **kw)
+    # SST at P2: the continuation by default too, as FRSolver.solve
+    res_sst = run_steady(mesh, order=2, turbulence_model="sst", max_iter=6, tol=1e-12, **kw)
+    _, out_sst = solve_with_order_continuation(lambda p: GPUFRSolver(mesh, order=p, turb_model="SST", **kw), 2,
+                                               max_iter=6, tol=1e-12)
+    assert res_sst.iterations == out_sst["iterations"] == 6
+    assert res_sst.final_residual == out_sst["final_residual"]
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_p0_sst_step_matches_oracle(prec, tol):
+    """One P0 pseudo-step with SST (afr_p0_sst_step) against the oracle's general turbulence step
+    with the one-point operators and the P0 stages: k, omega, nu_t, the state and the norm, on a
+    perturbed wall-box state with non-uniform k, omega (so that convection, diffusion, the WALL
+    mirror and the Neumann records all act)."""
+    import torch
+    from autoflowcfd_amd.continuation import P0Mesh
+    from autoflowcfd_amd.device import bc_params_table
+    from oracle import continuation as OC
+    from oracle import fr_ops, sst
+    from oracle.faces import build_flat_faces
+    from oracle.geometry import Geometry
+    print(f"[gpu] P0 SST step {prec} vs oracle", flush=True)
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    mesh = wall_box(seed=7)
+    Q0 = freestream(0.2, 6.0)
+    bcp = bc_params_table(Q0)
+    bcd = {c: bcp[c] for c in range(8)}
+    ops_t, ops0 = fr_ops.generate_operators(2), fr_ops.generate_operators(0)
+    g_t = Geometry(mesh, ops_t)
+    g0 = OC.P0Geometry(mesh, g_t)
+    g0fr, fl0fr = Geometry(mesh, ops0), build_flat_faces(mesh, ops0)
+    d = np.mean(sst.wall_distance(g_t.sps_xyz, mesh.nodes[mesh.wall_nodes]), axis=1, keepdims=True)
+    U = _state(mesh, Q0, seed=4)
+    rng = np.random.default_rng(5)
+    k = 0.4 * (1.0 + 0.3 * rng.random((mesh.n_cells, 1)))
+    w = 2600.0 * (1.0 + 0.3 * rng.random((mesh.n_cells, 1)))
+    nut = 1e-5 * (1.0 + rng.random((mesh.n_cells, 1)))
+    if dtype == torch.float32:
+        U, k, w, nut = (x.astype(np.float32).astype(np.float64) for x in (U, k, w, nut))
+    mu = 1.8e-5
+    U_ref, k_ref, w_ref, nut_ref, res_ref = OC.p0_sst_step(U, k, w, nut, mesh, g0, g0fr, ops0, fl0fr, d, bcd, mu)
+    dev = P0Mesh(mesh, g_t.volume, bcp, dtype=dtype, device="cuda", d_wall=d[:, 0])
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).cuda().contiguous()  # noqa: E731
+    Ud, kd, wd, nutd = t(U), t(k[:, 0]), t(w[:, 0]), t(nut[:, 0])
+    mutd = torch.zeros_like(kd)
+    res = float(dev.sst_step(Ud, kd, wd, nutd, mutd, mu=mu).item())
+    e = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
+    got = {n: x.double().cpu().numpy() for n, x in (("k", kd), ("w", wd), ("nut", nutd))}
+    print(f"P0 SST {prec}: norm {res:.12e} / {res_ref:.12e}; k {e(got['k'], k_ref[:, 0]):.2e} "
+          f"w {e(got['w'], w_ref[:, 0]):.2e} nut {e(got['nut'], nut_ref[:, 0]):.2e}")
+    assert abs(res - res_ref) <= tol * res_ref
+    assert e(got["nut"], nut_ref[:, 0]) < tol
+    # the increments of k, omega and U (the fields themselves hide them)
+    assert np.linalg.norm(got["k"] - k_ref[:, 0]) <= tol * np.linalg.norm(k_ref[:, 0] - k[:, 0]) * 10
+    assert np.linalg.norm(got["w"] - w_ref[:, 0]) <= tol * np.linalg.norm(w_ref[:, 0] - w[:, 0]) * 10
+    Un = Ud.double().cpu().numpy()
+    assert np.linalg.norm(Un - U_ref) <= (1e-9 if prec == "f64" else 1e-3) * np.linalg.norm(U_ref - U)
+
+
+def test_sst_order_continuation_matches_oracle_run():
+    """The SST continuation P0 -> P1 -> P2 in FP64 against the oracle's run: per-phase histories,
+    the final state and k, omega (reset fields at P0, cell-mean wall distance carried through)."""
+    import torch
+    from autoflowcfd_amd.continuation import solve_with_order_continuation
+    from autoflowcfd_amd.device import bc_params_table
+    from autoflowcfd_amd.solver import GPUFRSolver
+    from oracle import continuation as OC
+    from oracle import fr_ops
+    print("[gpu] SST order continuation P0 -> P2 f64 vs oracle run", flush=True)
+    mesh = wall_box(n=(3, 2, 2), seed=4)
+    Q0 = freestream(0.15, 0.0)
+    mu = 1.8e-5
+    bcp = bc_params_table(Q0)
+    U_ref, hist_ref, (k_ref, w_ref, nut_ref) = OC.run(mesh, 2, {c: bcp[c] for c in range(8)}, prim_to_cons(Q0), mu,
+                                                      max_iter=30, tol=1e-12, fr_ops=fr_ops, turbulence=True)
+
+    def make(p):
+        return GPUFRSolver(mesh, order=p, rho_inf=Q0[0], vel_inf=Q0[1], p_inf=Q0[4], mu_molecular=mu,
+                           dtype=torch.float64, turb_model="SST")
+    hist = {}
+    s, out = solve_with_order_continuation(make, 2, max_iter=30, tol=1e-12, history=hist)
+    for p in hist_ref:
+        a, b = np.array(hist[p]), np.array(hist_ref[p])
+        d = np.max(np.abs(a - b) / b) if len(a) == len(b) else np.inf
+        print(f"SST P{p}: {len(a)} steps, first {a[0]:.10e} / {b[0]:.10e}, last {a[-1]:.10e} / {b[-1]:.10e}, "
+              f"max rel {d:.2e}")
+        assert len(a) == len(b)
+        assert d < 5e-6          # round-off of the free-stream start, amplified (as the laminar run)
+    st = s.get_state_cpu()
+    rel = np.linalg.norm(st["U"][..., :5] - U_ref) / np.linalg.norm(U_ref)
+    kq = st["Q"][..., 5]
+    rk = np.linalg.norm(kq - k_ref) / np.linalg.norm(k_ref)
+    print(f"final state rel {rel:.3e}, k rel {rk:.3e}")
+    assert rel < 1e-10 and rk < 1e-6

@@ -141,3 +141,37 @@ def test_p0_transient_amplifies_round_off():
     rel = np.abs(hs[1] - hs[0]) / hs[0]
     assert rel[0] < 1e-10
     assert rel[-1] > 1e-8
+
+
+@pytest.mark.parametrize("kind", ["wallbox", "mixed", "rotated", "tets"])
+def test_p0_record_sources_equal_the_oracle_flat_faces(kind):
+    """The product's P0 neighbour sources (continuation.p0_record_sources: one flux point per record,
+    a prism quad's centre on its diagonal) equal the oracle's flat faces built with the one-point
+    operators: source cells, and weights equal to the oracle's 1 x 1 interpolation rows."""
+    from autoflowcfd_amd.continuation import p0_record_sources
+    from cases import mixed_mesh, rotated_prism_box, tet_box
+    mesh = {"wallbox": lambda: wall_box(seed=3), "mixed": mixed_mesh, "rotated": lambda: rotated_prism_box(seed=1),
+            "tets": lambda: tet_box(3, seed=2)}[kind]()
+    fl = build_flat_faces(mesh, fr_ops.generate_operators(0))
+    src, wgt = p0_record_sources(mesh)
+    assert np.array_equal(src[:, 0], fl.nb_src0_cell)
+    assert np.allclose(wgt[:, 0], fl.nb_src0_mat[:, 0, 0], rtol=0, atol=1e-14)
+    s1 = np.full(mesh.faces.n_faces, -1)
+    w1 = np.zeros(mesh.faces.n_faces)
+    has = fl.nb_src1_idx >= 0
+    s1[has] = fl.nb_src1_cell[fl.nb_src1_idx[has]]
+    w1[has] = fl.nb_src1_mat[fl.nb_src1_idx[has], 0, 0]
+    assert np.array_equal(src[:, 1], s1)
+    assert np.allclose(wgt[:, 1], w1, rtol=0, atol=1e-14)
+    if kind in ("wallbox", "rotated"):
+        assert has.any() and (wgt[:, 0] == 0).any()      # quads whose centre goes to the second source
+
+
+def test_p0_operators_are_the_one_point_closed_forms():
+    """generate_operators(0): zero derivatives, unit extrapolation and filter, g' = -1/2, +1/2 (the
+    degree-1 Radau corrections (1 -+ xi) / 2 differentiated)."""
+    ops = fr_ops.generate_operators(0)
+    assert ops.n_sps == 1 and ops.n_fp == 1
+    assert np.abs(ops.D).max() == 0.0
+    assert np.allclose(ops.E, 1.0) and np.allclose(ops.filt, 1.0)
+    assert np.allclose(ops.g_left, -0.5) and np.allclose(ops.g_right, 0.5)
