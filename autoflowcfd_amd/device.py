@@ -223,7 +223,14 @@ class DeviceMesh:
         self.prism_tab = dev(prism_factors(order).packed())
         # prisms are processed in Morton order of their centroids (L2 reuse of source cells)
         cen = mesh.nodes[mesh.prism_conn].mean(axis=1) if mesh.n_prism else np.zeros((0, 3))
-        self.prism_schedule = devi(morton_order(cen))
+        if cell_case is not None and "prism_off" in mesh.meta and mesh.n_prism:
+            # a batch walks case by case (each case in its own Morton order): the cases' boxes overlap
+            # in space, and one Morton order over all of them would interleave the cases' cells
+            po = np.asarray(mesh.meta["prism_off"], dtype=np.int64)
+            sched = np.concatenate([po[i] + morton_order(cen[po[i]:po[i + 1]]) for i in range(len(po) - 1)])
+        else:
+            sched = morton_order(cen)
+        self.prism_schedule = devi(sched)
         self.E = []
         self.E64 = []
         for ct in (0, 1):

@@ -47,6 +47,7 @@ class SolveRequest:
     arrival: float = 0.0
     max_iter: int = 2000
     rel_tol: float = 1e-6
+    stop_nonfinite: bool = True     # False: a fixed step count runs on even through a non-finite residual
 
 
 @dataclass
@@ -72,7 +73,7 @@ class SolveResult:
 def _stop(hist, it, req):
     """(stop, converged) after a step: relative drop, non-finite residual or iteration cap."""
     r = hist[-1]
-    if not np.isfinite(r):
+    if not np.isfinite(r) and req.stop_nonfinite:
         return True, False
     if hist[0] > 0 and r / hist[0] < req.rel_tol:
         return True, True
@@ -477,12 +478,13 @@ def latency_stats(results):
 
 def synthetic_requests(n_requests, seed=0, hex_range=(2e3, 2e4), orders=(1, 2), mach_range=(0.05, 0.3),
                        yaw_range=(0.0, 10.0), aspect_range=(2.0, 6.0), rate=4.0, max_iter=2000, rel_tol=1e-6,
-                       length=0.2879, rho=1.225, p_inf=101325.0):
+                       length=0.2879, rho=1.225, p_inf=101325.0, topology="host", stop_nonfinite=True):
     """A seeded stream of steady wall-box solves: hex count log-uniform over hex_range, aspect
     ratio (box length / height) uniform, y-graded wall boxes (ratio 1.1, no-slip y = 0, inlet,
     outlet, symmetry) split into prisms, order drawn uniformly from ``orders``, Mach and yaw
     uniform (free stream (U cos yaw, 0, U sin yaw)), Poisson arrivals at ``rate`` per second.
-    k, omega of every case from 1% turbulence intensity and viscosity ratio 10."""
+    k, omega of every case from 1% turbulence intensity and viscosity ratio 10.  topology: "host" or
+    "gpu" face topology of the meshes (the same records)."""
     from .mesh import BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, hex_box
     rng = np.random.default_rng(seed)
     bc = {"y0": BC_WALL, "x0": BC_INLET, "x1": BC_OUTLET, "y1": BC_SYMMETRY, "z0": BC_SYMMETRY, "z1": BC_SYMMETRY}
@@ -501,9 +503,10 @@ def synthetic_requests(n_requests, seed=0, hex_range=(2e3, 2e4), orders=(1, 2), 
         Q = np.array([rho, U * np.cos(yaw), 0.0, U * np.sin(yaw), p_inf])
         k = 1.5 * (0.01 * U) ** 2
         mesh = hex_box(nx, m, m, ar * length, length, length, split="prism", grading_y=1.1, seed=seed * 1000 + j,
-                       bc_by_plane=bc)
+                       bc_by_plane=bc, topology=topology)
         case = Case(mesh=mesh, Q_free=Q, k_inf=k, omega_inf=rho * k / (10 * 1.8e-5), reference_area=length * length,
                     name=f"wallbox{nx}x{m}x{m}")
-        out.append(SolveRequest(rid=j, case=case, order=order, arrival=t, max_iter=max_iter, rel_tol=rel_tol))
+        out.append(SolveRequest(rid=j, case=case, order=order, arrival=t, max_iter=max_iter, rel_tol=rel_tol,
+                                stop_nonfinite=stop_nonfinite))
         t += float(rng.exponential(1.0 / rate))
     return out

@@ -198,7 +198,7 @@ def served_leg(args):
     lo, hi = (float(x) for x in args.serve_hex.split(","))
     turb = args.serve_turb.upper()
     reqs = synthetic_requests(args.serve_requests, seed=args.serve_seed, hex_range=(lo, hi), rate=args.serve_rate,
-                              max_iter=args.serve_max_iter, rel_tol=1e-6)
+                              max_iter=args.serve_max_iter, rel_tol=1e-6, topology="gpu")
     n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
     log(f"[bench] served leg: {len(reqs)} {turb} requests, {n_sp} solution points in total")
     batched = args.serve_mode == "batched"
@@ -223,7 +223,7 @@ def served_leg(args):
     busy = [r.finished - r.admitted - r.setup_s for r in plain]
     log(f"[bench] plain: mean set-up {np.mean([r.setup_s for r in plain]):.3f}s, mean stepping {np.mean(busy):.3f}s "
         f"per solve; served: mean set-up {np.mean([r.setup_s for r in served]):.3f}s")
-    dcd = max(abs(a.coefficients.get("Cd", 0.0) - b.coefficients.get("Cd", 0.0)) for a, b in zip(served, plain))
+    dcd = max_abs_diff([a.coefficients.get("Cd", 0.0) for a in served], [b.coefficients.get("Cd", 0.0) for b in plain])
     same = all(a.residual_history == b.residual_history for a, b in zip(served, plain))
     return {
         "workload": f"{len(reqs)} steady {'laminar NS' if turb == 'NONE' else 'SST RANS'} solves, graded wall boxes of "
@@ -238,6 +238,17 @@ def served_leg(args):
     }
 
 
+def max_abs_diff(a, b):
+    """Largest |a_i - b_i|, 0 where both are the same non-finite value, inf where only one is."""
+    out = 0.0
+    for x, y in zip(a, b):
+        if np.isfinite(x) and np.isfinite(y):
+            out = max(out, abs(x - y))
+        elif not (np.isnan(x) and np.isnan(y)) and x != y:
+            out = float("inf")
+    return out
+
+
 def batched_leg(args):
     """Batched multi-case form vs the plain one-at-a-time path on the same seeded SST cases (fixed
     step count, set-up included in both): P1 and P2 cases in one MixedBatch (one HIP graph per
@@ -248,7 +259,7 @@ def batched_leg(args):
     from autoflowcfd_amd.serve import solve_alone, synthetic_requests
     lo, hi = (float(x) for x in args.batch_hex.split(","))
     reqs = synthetic_requests(args.batch_cases, seed=args.serve_seed + 7, hex_range=(lo, hi), rate=1e9,
-                              max_iter=args.batch_steps, rel_tol=0.0)
+                              max_iter=args.batch_steps, rel_tol=0.0, topology="gpu", stop_nonfinite=False)
     ref = static_ref_state([r.case for r in reqs])
     n_sp = sum(r.case.mesh.n_cells * (r.order + 1) ** 3 for r in reqs)
     log(f"[bench] batched leg: {len(reqs)} SST cases, {n_sp} solution points, {args.batch_steps} pseudo-steps each")
@@ -275,7 +286,10 @@ def batched_leg(args):
     log(f"[bench] plain {t_plain:.2f}s, batched {t_batch:.2f}s (set-up {t_setup:.2f}s incl. autotuning "
         f"{t_tune:.2f}s, "
         f"{t_steps / max(args.batch_steps - 1, 1) * 1e3:.2f} ms per mixed pseudo-step)")
-    dcd = max(abs(coef[r.rid]["Cd"] - r.coefficients["Cd"]) for r in plain)
+    if any(r.iterations != args.batch_steps for r in plain):
+        raise RuntimeError("a plain solve did not run the fixed step count")
+    dcd = max_abs_diff([coef[r.rid]["Cd"] for r in plain], [r.coefficients["Cd"] for r in plain])
+    nonfinite = int(sum(not np.isfinite(r.final_residual) for r in plain))
     del b
     torch.cuda.empty_cache()
     return {"workload": f"{len(reqs)} SST cases, graded wall boxes of {lo:g}-{hi:g} hex (log-uniform, aspect 2-6) "
@@ -283,6 +297,7 @@ def batched_leg(args):
                         f"Mach 0.05-0.3, yaw 0-10 deg, {args.batch_steps} pseudo-steps each, set-up included",
             "solves_per_hour": len(reqs) / t_batch * 3600.0, "plain_solves_per_hour": len(reqs) / t_plain * 3600.0,
             "speedup_vs_plain": t_plain / t_batch, "max_abs_dCd_vs_plain": dcd,
+            "cases_nonfinite_at_end": nonfinite,
             "ms_per_mixed_pseudo_step": t_steps / max(args.batch_steps - 1, 1) * 1e3,
             "autotune_s": t_tune,
             "autotuned": None if tuned is None else {f"P{p}": {k: {"grid": v["grid"], "variant": v["variant"]}
@@ -417,6 +432,7 @@ def main():
     ap.add_argument("--batch-host-workers", type=int, default=0,
                     help="processes building the slot tables with NumPy (0: on the GPU)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-case leg")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-case leg (prints its JSON)")
     ap.add_argument("--tgv-p3", type=int, default=0,
                     help="run only the P3 Taylor-Green vortex leg on N^3 hex x 6 tets (prints its JSON)")
     ap.add_argument("--tgv-steps", type=int, default=200)
@@ -431,6 +447,9 @@ def main():
 
     if args.tgv_p3:
         print(json.dumps(tgv_p3_leg(args)), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(batched_leg(args)), flush=True)
         return
     if args.serve_only:
         out = served_leg(args)
