@@ -71,12 +71,16 @@ def _time(fn, reps):
     return ev[0].elapsed_time(ev[1]) / reps
 
 
-def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CANDIDATES, cache_path=None):
+def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CANDIDATES, cache_path=None,
+             reuse=True):
     """Pick the grid size of every residual kernel present on this mesh.
 
     U: the state to time on (device, resident); mu_t: eddy viscosity or None; sst_fields:
-    (k, omega, nu_t, mu_t) to tune the SST kernels (None skips them).  Returns
-    {kernel: {"grid": chosen, "ms": {candidate: time}}}; the choice stays set on the mesh."""
+    (k, omega, nu_t, mu_t) to tune the SST kernels (None skips them); reuse: take a choice
+    already made in this process or stored in ``cache_path`` for the same key instead of timing
+    (False times every candidate again and replaces the stored choice).  Returns
+    {kernel: {"grid": chosen, "variant": v, "ms": {candidate: time} or None when reused}}; the
+    choice stays set on the mesh."""
     cache = {}
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
@@ -101,7 +105,7 @@ def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CA
     out = {}
     for name, fn in phases.items():
         key = _key(dm, name)
-        if key in _memo or key in cache:
+        if reuse and (key in _memo or key in cache):
             choice = _apply(dm, name, _memo.get(key, cache.get(key)))
             out[name] = dict(choice, ms=None, cached=True)
             continue

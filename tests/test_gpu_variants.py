@@ -87,7 +87,7 @@ def test_autotune_over_variants_keeps_results():
     ref = dm.residual(U, mut, torch.empty_like(U)).clone()
     # the SST phase writes nu_t and mu_t: tune on copies (as CaseBatch.tune does)
     fields = tuple(x.clone() for x in (k, w, nut, mut))
-    res = autotune(dm, U, fields[3], reps=1, candidates=(0, 1024), sst_fields=fields)
+    res = autotune(dm, U, fields[3], reps=1, candidates=(0, 1024), sst_fields=fields, reuse=False)
     assert {"prism_inviscid", "sst_cell"} <= set(res)
     assert res["prism_inviscid"]["variant"] == get_variant(dm, "prism_inviscid")
     assert res["sst_cell"]["variant"] == get_variant(dm, "sst_cell")
