@@ -5,7 +5,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafr.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 c_vp = ctypes.c_void_p
 
@@ -35,6 +35,7 @@ class AfrMeshDesc(ctypes.Structure):
         ("n_wall_rec", ctypes.c_int64), ("wrec_owner", c_vp), ("wrec_oslot", c_vp), ("wrec_geom", c_vp),
         ("w1d", c_vp), ("work_fpart", c_vp), ("grid_cap", ctypes.c_int32 * 8),
         ("variant", ctypes.c_int32 * 8),
+        ("state_ref", c_vp),
     ]
 
 
@@ -60,6 +61,7 @@ class AfrP0Desc(ctypes.Structure):
         ("cell_inc", c_vp), ("cell_ghost_rec", c_vp), ("bc_params", c_vp),
         ("work_r", c_vp), ("work_s1", c_vp), ("work_s2", c_vp), ("work_dt", c_vp), ("work_part", c_vp),
         ("rec_src", c_vp), ("rec_w", c_vp), ("d_wall", c_vp), ("work_sst", c_vp),
+        ("state_ref", c_vp),
     ]
 
 

@@ -97,9 +97,12 @@ class CaseBatch:
     """Many independent steady cases of one order on one GPU, one launch per kernel per stage."""
 
     def __init__(self, cases, order=2, dtype=torch.float32, turb_model="SST", mu_molecular=1.8e-5, Pr=0.72,
-                 Pr_t=0.9, device_id=0, ref_state=None, slots_list=None, host_workers=0):
+                 Pr_t=0.9, device_id=0, ref_state=None, slots_list=None, host_workers=0, offset_storage=None):
         """slots_list: each case's slot tables at this order, else built here: on the GPU, or in
-        ``host_workers`` spawned processes when > 0 (NumPy only, never the GPU)."""
+        ``host_workers`` spawned processes when > 0 (NumPy only, never the GPU).
+        offset_storage: store each case's state as its offset from the case's free stream
+        (DeviceMesh state_ref); None: with FP32 storage.  ``U`` then holds the offsets;
+        case_state / set_case_state take and return the conserved state."""
         if not torch.cuda.is_available():
             raise RuntimeError("CaseBatch needs a visible GPU (HIP device)")
         if not cases:
@@ -129,12 +132,15 @@ class CaseBatch:
         if slots_list is None:
             slots_list = case_slots([c.mesh for c in self.cases], [order] * self.n_cases, host_workers)
         slots = concat_slots(list(slots_list), [p.n_prism for p in parts], [p.n_tet for p in parts], T.nfp)
-        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=self.ref_state,
-                             slots=slots, turbulence="SST" if self.turb == "SST" else None, cell_case=cell_case)
-        ns = self.dm.ns
         Ucase = cons(np.array([c.Q_free for c in self.cases], dtype=np.float64))       # (n_cases, 5)
+        if offset_storage is None:
+            offset_storage = dtype == torch.float32
+        self.dm = DeviceMesh(mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=self.ref_state,
+                             slots=slots, turbulence="SST" if self.turb == "SST" else None, cell_case=cell_case,
+                             state_ref=Ucase if offset_storage else None)
+        ns = self.dm.ns
         U0 = np.ascontiguousarray(np.broadcast_to(Ucase[cell_case][:, None, :], (mesh.n_cells, ns, 5)))
-        self.U = torch.as_tensor(U0, dtype=dtype).to(self.device).contiguous()
+        self.U = self.dm.to_storage(torch.as_tensor(U0))
         if self.turb == "SST":
             kc = np.array([c.k_inf for c in self.cases])[cell_case]
             wc = np.array([c.omega_inf for c in self.cases])[cell_case]
@@ -222,7 +228,7 @@ class CaseBatch:
         """Conserved state (C_i, ns, 5) of case i and, with SST, k and omega (host, FP64), in the
         case mesh's own cell order."""
         idx = torch.as_tensor(self.case_cells(i), device=self.device)
-        out = {"U": self.U.index_select(0, idx).double().cpu().numpy()}
+        out = {"U": (self.U.index_select(0, idx).double() + self._case_offset(i)).cpu().numpy()}
         if self.turb == "SST":
             out["k"] = self.k.index_select(0, idx).double().cpu().numpy()
             out["omega"] = self.omega.index_select(0, idx).double().cpu().numpy()
@@ -231,7 +237,14 @@ class CaseBatch:
     def set_case_state(self, i, U):
         """Set the conserved state (C_i, ns, 5) of case i (the case mesh's own cell order)."""
         idx = torch.as_tensor(self.case_cells(i), device=self.device)
-        self.U.index_copy_(0, idx, torch.as_tensor(np.ascontiguousarray(U), dtype=self.dtype).to(self.device))
+        X = torch.as_tensor(np.ascontiguousarray(U, dtype=np.float64)).to(self.device) - self._case_offset(i)
+        self.U.index_copy_(0, idx, X.to(self.dtype))
+
+    def _case_offset(self, i):
+        """(5,) FP64 device offset of case i's stored state (zeros without offsets)."""
+        if self.dm.state_ref is None:
+            return torch.zeros((5,), dtype=torch.float64, device=self.device)
+        return self.dm.state_ref_dev[i]
 
     def wall_forces(self):
         """(n_cases, 6) FP64 host array: pressure force, viscous force of every case."""

@@ -113,9 +113,11 @@ def p0_record_sources(mesh):
 
 class P0Mesh:
     """Device arrays of the P0 phase on one mesh (in the order of ``mesh``); with ``d_wall`` (C)
-    also those of the P0 SST step."""
+    also those of the P0 SST step.  state_ref: None, or the (5,) conserved offset of the stored
+    state (every state this mesh takes then holds U - state_ref, as DeviceMesh's)."""
 
-    def __init__(self, mesh, cell_volume, bc_params, dtype=torch.float32, device="cuda", d_wall=None):
+    def __init__(self, mesh, cell_volume, bc_params, dtype=torch.float32, device="cuda", d_wall=None,
+                 state_ref=None):
         self.lib = _lib.load()
         self.dtype, self.device = dtype, torch.device(device)
         if self.device.index is None:
@@ -189,7 +191,23 @@ class P0Mesh:
             self.d_wall = torch.as_tensor(d_wall, dtype=dtype).reshape(C).to(dv).contiguous()
             self.work_sst = torch.zeros((C, 8), dtype=torch.float64, device=dv)
             d.rec_src, d.rec_w, d.d_wall, d.work_sst = p(self.rec_src), p(self.rec_w), p(self.d_wall), p(self.work_sst)
+        self.state_ref = None if state_ref is None else np.asarray(state_ref, dtype=np.float64).reshape(5)
+        if self.state_ref is not None:
+            self.state_ref_dev = f64(self.state_ref)
+            d.state_ref = p(self.state_ref_dev)
         self._desc = d
+
+    def to_storage(self, U_abs):
+        """Stored form (this mesh's precision) of a conserved (C, 1, 5) state."""
+        X = torch.as_tensor(U_abs, device=self.device).double()
+        if self.state_ref is not None:
+            X = X - self.state_ref_dev
+        return X.to(self.dtype).contiguous()
+
+    def to_absolute(self, U):
+        """Conserved state (FP64) of a state stored by this mesh."""
+        X = U.double()
+        return X if self.state_ref is None else X + self.state_ref_dev
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -268,8 +286,10 @@ def solve_with_order_continuation(make_solver, order, max_iter=1000, tol=1e-6, n
     if sst:
         # the reference's P0 wall distance: the mean over each cell's solution points
         d0 = dm.d_wall.double().mean(dim=1).to(dt).contiguous()
-    p0 = P0Mesh(dm.mesh, dm.volume, dm.bc_params.cpu().numpy(), dtype=dt, device=dv, d_wall=d0)
-    U = torch.as_tensor(np.broadcast_to(_cons(Q_free), (C, 1, 5)).copy(), dtype=dt).to(dv)
+    # the P0 state is stored as the target's is (offset from the same reference, or the state itself)
+    p0 = P0Mesh(dm.mesh, dm.volume, dm.bc_params.cpu().numpy(), dtype=dt, device=dv, d_wall=d0,
+                state_ref=None if dm.state_ref is None else dm.state_ref[0])
+    U = p0.to_storage(torch.as_tensor(np.broadcast_to(_cons(Q_free), (C, 1, 5)).copy()))
     if sst:
         # turbulence fields reset to the model's defaults (order_continuation.py:245-259)
         k = torch.full((C, 1), 1e-6, dtype=dt, device=dv)
@@ -287,6 +307,9 @@ def solve_with_order_continuation(make_solver, order, max_iter=1000, tol=1e-6, n
         if p > 0:
             U = interpolate(U, p - 1, p)
             solver = target if p == order else make_solver(p)
+            sr = solver.dm.state_ref
+            if (sr is None) != (p0.state_ref is None) or (sr is not None and not np.array_equal(sr[0], p0.state_ref)):
+                raise ValueError("make_solver must store the state of every order the same way (offset_storage)")
             solver.U.copy_(U)
             U = solver.U
             if sst:
@@ -296,7 +319,8 @@ def solve_with_order_continuation(make_solver, order, max_iter=1000, tol=1e-6, n
                 solver.k.copy_(k)
                 solver.omega.copy_(w)
                 solver.nu_t.copy_(nut)
-                solver.mu_t.copy_(nut * torch.clamp(U[..., 0], min=1e-10))
+                rho = torch.clamp(solver.dm.to_absolute(U)[..., 0], min=1e-10).to(dt)
+                solver.mu_t.copy_(nut * rho)
                 solver.dm.d_wall.copy_(dwall)
                 k, w, nut = solver.k, solver.omega, solver.nu_t
         phase_tol = tol * 10.0 ** (order - p)

@@ -42,14 +42,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_wall_forces(ForceArgs<real> a) 
         const real* E = (is_prism ? a.E0 : a.E1) + slot * NFP * NS;
         if (t < NS) {
             const int64_t i = c * NS + t;
-            const double rho = fmax((double)a.U[i * 5 + 0], 1e-10);
-            const double u = (double)a.U[i * 5 + 1] / rho, v = (double)a.U[i * 5 + 2] / rho,
-                         w = (double)a.U[i * 5 + 3] / rho;
+            double u5[5];
+            afr::load_state(a.U, i, a.so.of(c), u5);
+            const double rho = fmax(u5[0], 1e-10);
+            const double u = u5[1] / rho, v = u5[2] / rho, w = u5[3] / rho;
             const double ke = 0.5 * (u * u + v * v + w * w);
             q[t * 4 + 0] = u;
             q[t * 4 + 1] = v;
             q[t * 4 + 2] = w;
-            q[t * 4 + 3] = fmax(((double)a.U[i * 5 + 4] - rho * ke) * 0.4, 1.0);
+            q[t * 4 + 3] = fmax((u5[4] - rho * ke) * 0.4, 1.0);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

@@ -3,12 +3,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "afr_physics.hpp"
+
 namespace afr_forces {
 
 template <typename real>
 struct ForceArgs {
     int64_t n_wall, n_prism;
-    const real* U;
+    const real* U;          // offset from so's table when it has one
+    afr::StateOff so;
     const real* mu_t;       // may be null
     const real* det;
     const real* adj;

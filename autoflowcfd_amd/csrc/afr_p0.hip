@@ -43,11 +43,11 @@ namespace {
 constexpr int kThreads = 256;
 constexpr double kCip = 4.0;   // interior-penalty coefficient, viscous_flux.py:183-192
 
+// primitive state of cell c; off: the conserved offset of the stored state (null: absolute)
 template <typename real>
-__device__ __forceinline__ void prim_at(const real* U, int64_t c, double q[5]) {
+__device__ __forceinline__ void prim_at(const real* U, int64_t c, const double* off, double q[5]) {
     double u[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) u[v] = (double)U[c * 5 + v];
+    load_state(U, c, off, u);
     cons2prim(u, q);
 }
 
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kThreads) k_p0_residual(AfrP0Desc d, const rea
     for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
          c += (int64_t)gridDim.x * kThreads) {
         double qc[5];
-        prim_at(U, c, qc);
+        prim_at(U, c, d.state_ref, qc);
         double res[5] = {0, 0, 0, 0, 0};
         // the ghost state the cell keeps: that of its last boundary record
         double qg[5];
@@ -84,13 +84,13 @@ __global__ void __launch_bounds__(kThreads) k_p0_residual(AfrP0Desc d, const rea
             if (role == 0) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) qo[v] = qc[v];
-                if (nb >= 0) prim_at(U, nb, qn);
+                if (nb >= 0) prim_at(U, nb, d.state_ref, qn);
                 else {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) qn[v] = qg[v];
                 }
             } else {
-                prim_at(U, d.rec_owner[r], qo);
+                prim_at(U, d.rec_owner[r], d.state_ref, qo);
 #pragma unroll
                 for (int v = 0; v < 5; ++v) qn[v] = qc[v];
             }
@@ -125,34 +125,42 @@ __global__ void __launch_bounds__(kThreads) k_p0_residual(AfrP0Desc d, const rea
 template <typename real>
 __global__ void __launch_bounds__(kThreads) k_p0_stage(const real* U0, const real* Ua, const real* __restrict__ R,
                                                        const real* __restrict__ dt, real* U_out, int64_t n,
-                                                       double c0, double c1, double c2, double p_floor) {
+                                                       const double* off, double c0, double c1, double c2,
+                                                       double p_floor) {
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const double d = (double)dt[i];
-        double q[5];
+        double q[5], qa[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
             if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
             q[v] = (double)(real)val;
+            qa[v] = off ? q[v] + off[v] : q[v];
         }
-        // positivity (src/autoflowcfd/core/time_integration/positivity.py:13-45)
-        const double rho = q[0] > 1e-6 ? q[0] : 1e-6;
-        double vx = q[1] / rho, vy = q[2] / rho, vz = q[3] / rho;
+        // positivity (src/autoflowcfd/core/time_integration/positivity.py:13-45) on the absolute
+        // state; a projected component is stored as its value less the offset, the others as they were
+        const bool low_rho = !(qa[0] > 1e-6);
+        const double rho = low_rho ? 1e-6 : qa[0];
+        double vx = qa[1] / rho, vy = qa[2] / rho, vz = qa[3] / rho;
         const double vm = sqrt(vx * vx + vy * vy + vz * vz);
-        double m1 = q[1], m2 = q[2], m3 = q[3];
-        if (vm > 1e4) {
+        const bool fast = vm > 1e4;
+        double m1 = qa[1], m2 = qa[2], m3 = qa[3];
+        if (fast) {
             const double f = 1e4 / vm;
             vx *= f; vy *= f; vz *= f;
             m1 = rho * vx; m2 = rho * vy; m3 = rho * vz;
         }
         const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-        double E = q[4];
-        if (0.4 * (E - ke) < p_floor) E = p_floor / 0.4 + ke;
-        U_out[i * 5 + 0] = (real)rho;
-        U_out[i * 5 + 1] = (real)m1;
-        U_out[i * 5 + 2] = (real)m2;
-        U_out[i * 5 + 3] = (real)m3;
-        U_out[i * 5 + 4] = (real)E;
+        double E = qa[4];
+        const bool low_p = 0.4 * (E - ke) < p_floor;
+        if (low_p) E = p_floor / 0.4 + ke;
+        const double o0 = off ? off[0] : 0.0, o1 = off ? off[1] : 0.0, o2 = off ? off[2] : 0.0;
+        const double o3 = off ? off[3] : 0.0, o4 = off ? off[4] : 0.0;
+        U_out[i * 5 + 0] = low_rho ? (real)(rho - o0) : (real)q[0];
+        U_out[i * 5 + 1] = fast ? (real)(m1 - o1) : (real)q[1];
+        U_out[i * 5 + 2] = fast ? (real)(m2 - o2) : (real)q[2];
+        U_out[i * 5 + 3] = fast ? (real)(m3 - o3) : (real)q[3];
+        U_out[i * 5 + 4] = low_p ? (real)(E - o4) : (real)q[4];
     }
 }
 
@@ -194,7 +202,7 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_point(AfrP0Desc d, const re
     const real* dw = (const real*)d.d_wall;
     for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < d.n_cells;
          c += (int64_t)gridDim.x * kThreads) {
-        const double rho = dmax((double)U[c * 5], 1e-10);
+        const double rho = dmax((double)U[c * 5] + (d.state_ref ? d.state_ref[0] : 0.0), 1e-10);
         const double kc = (double)k[c], wc = (double)w[c], dc = (double)dw[c];
         // sources with zero gradients (S = 0, grad k . grad omega = 0), mu = 1.8e-5 as the reference's
         const double musrc = 1.8e-5;
@@ -243,9 +251,11 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_transport(AfrP0Desc d, cons
             const int r = e >> 4, role = (e >> 3) & 1, code = e & 7;
             const double g = (code & 1) ? 0.5 : -0.5;   // g'(0) of the side's Radau correction
             const int64_t o = d.rec_owner[r];
-            const double ro = dmax((double)U[o * 5], 1e-10);
+            double uo[5];
+            load_state(U, o, d.state_ref, uo);
+            const double ro = dmax(uo[0], 1e-10);
             const double* n = d.rec_geom + (int64_t)r * 4;
-            const double vo[3] = {(double)U[o * 5 + 1] / ro, (double)U[o * 5 + 2] / ro, (double)U[o * 5 + 3] / ro};
+            const double vo[3] = {uo[1] / ro, uo[2] / ro, uo[3] / ro};
             const double mass = ro * vo[0] * n[0] + ro * vo[1] * n[1] + ro * vo[2] * n[2];
             const int s0 = d.rec_src[r * 2], s1 = d.rec_src[r * 2 + 1];
             const double w0 = d.rec_w[r * 2], w1 = d.rec_w[r * 2 + 1];
@@ -279,7 +289,7 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_transport(AfrP0Desc d, cons
                 }
             }
         }
-        const double rho = dmax((double)U[c * 5], 1e-10);
+        const double rho = dmax((double)U[c * 5] + (d.state_ref ? d.state_ref[0] : 0.0), 1e-10);
         for (int f = 0; f < 2; ++f) d.work_sst[c * kSstW + 6 + f] = finite_or((conv[f] - diff[f]) / rho, 0.0);
     }
 }
@@ -302,7 +312,7 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_update(AfrP0Desc d, const r
         wn = isfinite(wn) ? wn : 1e-12;
         k[c] = (real)dmax(kn, 1e-12);
         w[c] = (real)dmax(wn, 1e-12);
-        const double rho = dmax((double)U[c * 5], 1e-10);
+        const double rho = dmax((double)U[c * 5] + (d.state_ref ? d.state_ref[0] : 0.0), 1e-10);
         nu_t[c] = (real)o[5];
         mu_t[c] = (real)(rho * o[5]);
     }
@@ -327,10 +337,11 @@ int residual_t(const AfrP0Desc& d, const void* U, const void* mu_t, void* R, dou
 }
 
 template <typename real>
-int stage_t(const void* U0, const void* Ua, const void* R, const void* dt, void* out, int64_t n, double c0, double c1,
-            double c2, hipStream_t st) {
+int stage_t(const void* U0, const void* Ua, const void* R, const void* dt, void* out, int64_t n, const double* off,
+            double c0, double c1, double c2, hipStream_t st) {
     hipLaunchKernelGGL((k_p0_stage<real>), dim3(grid_of(n)), dim3(kThreads), 0, st, (const real*)U0,
-                       (const real*)(Ua ? Ua : U0), (const real*)R, (const real*)dt, (real*)out, n, c0, c1, c2, 1.0);
+                       (const real*)(Ua ? Ua : U0), (const real*)R, (const real*)dt, (real*)out, n, off, c0, c1, c2,
+                       1.0);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -347,16 +358,17 @@ int stages_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_var
     if (residual_t<real>(d, u, mu_t, r, mu, st)) return -1;
     if (afr_step::launch_norm<real>(r, C * 5, (double)C * n_vars_total, d.work_part, afr_step::kNormParts, norm, st))
         return -1;
-    if (stage_t<real>(u, nullptr, r, dt, s1, C, 1.0, 0.0, 1.0, st)) return -1;
+    if (stage_t<real>(u, nullptr, r, dt, s1, C, d.state_ref, 1.0, 0.0, 1.0, st)) return -1;
     if (residual_t<real>(d, s1, mu_t, r, mu, st)) return -1;
-    if (stage_t<real>(u, s1, r, dt, s2, C, 0.75, 0.25, 0.25, st)) return -1;
+    if (stage_t<real>(u, s1, r, dt, s2, C, d.state_ref, 0.75, 0.25, 0.25, st)) return -1;
     if (residual_t<real>(d, s2, mu_t, r, mu, st)) return -1;
-    return stage_t<real>(u, s2, r, dt, u, C, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, st);
+    return stage_t<real>(u, s2, r, dt, u, C, d.state_ref, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, st);
 }
 
 template <typename real>
 int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm, hipStream_t st) {
-    if (afr_step::launch_dt_local<real>((const real*)U, (const real*)mu_t, (const real*)d.det, (const real*)d.mfs,
+    if (afr_step::launch_dt_local<real>((const real*)U, StateOff{d.state_ref, nullptr}, (const real*)mu_t,
+                                        (const real*)d.det, (const real*)d.mfs,
                                         (const real*)d.vol1, (real*)d.work_dt, d.n_cells, 1, mu, 0, 0, st))
         return -1;
     return stages_t<real>(d, U, mu_t, mu, n_vars_total, norm, st);
@@ -368,7 +380,8 @@ int sst_step_t(const AfrP0Desc& d, void* U, void* k, void* w, void* nu_t, void* 
     const int g = grid_of(d.n_cells);
     real* dt = (real*)d.work_dt;
     // local time step with the previous step's eddy viscosity (nu_t held, mu_eff = mu + rho nu_t)
-    if (afr_step::launch_dt_local<real>((const real*)U, (const real*)nu_t, (const real*)d.det, (const real*)d.mfs,
+    if (afr_step::launch_dt_local<real>((const real*)U, StateOff{d.state_ref, nullptr}, (const real*)nu_t,
+                                        (const real*)d.det, (const real*)d.mfs,
                                         (const real*)d.vol1, dt, d.n_cells, 1, mu, 0, 1, st))
         return -1;
     hipLaunchKernelGGL((k_p0_sst_point<real>), dim3(g), dim3(kThreads), 0, st, d, (const real*)U, (const real*)k,

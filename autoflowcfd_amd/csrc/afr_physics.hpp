@@ -80,6 +80,28 @@ __host__ __device__ __forceinline__ float recip(float x) {
 template <typename real>
 __host__ __device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
 
+// Stored state -> conserved state.  A mesh may store every state as the offset from a per-case
+// conserved reference (sref, n_cases x 5, FP64; the case's free stream): the kernels add the offset
+// back in FP64 when they load a point.  Without a table the storage holds the state itself and the
+// load is a plain widening (bit for bit the same values as before the offset existed).
+struct StateOff {
+    const double* sref;    // null: absolute storage
+    const int* cell_case;  // null: one case
+    __device__ __forceinline__ const double* of(int64_t cell) const {
+        return sref ? sref + 5 * (int64_t)(cell_case ? cell_case[cell] : 0) : nullptr;
+    }
+};
+
+template <typename real>
+__device__ __forceinline__ void load_state(const real* U, int64_t i, const double* off, double u[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = (double)U[i * 5 + v];
+    if (off) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) u[v] += off[v];
+    }
+}
+
 // conserved (rho, rho u, rho v, rho w, rho E) -> primitive (rho, u, v, w, p); rho floor 1e-10, no p floor
 template <typename real>
 __host__ __device__ __forceinline__ void cons2prim(const real U[5], real Q[5]) {

@@ -510,7 +510,8 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_PRISM) k_pris
 // dQ = primitive(U) - ref and umean[c][v] = mean_s |U[c][s][v]| (outlier field reference)
 template <int NS, typename real>
 __global__ void __launch_bounds__(256) k_prim_umean(const real* __restrict__ U, real* __restrict__ dQ,
-                                                    real* __restrict__ umean, int64_t n_cells, Ref5 ref) {
+                                                    real* __restrict__ umean, int64_t n_cells, Ref5 ref,
+                                                    StateOff so) {
     constexpr int CPB = 256 / NS;
     __shared__ real s_abs[CPB * NS * 5];
     const int t = threadIdx.x;
@@ -520,13 +521,12 @@ __global__ void __launch_bounds__(256) k_prim_umean(const real* __restrict__ U, 
         if (ok) {
             const int64_t i = (c0 + c) * NS + s;
             double u[5], q[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) u[v] = (double)U[i * 5 + v];
+            load_state(U, i, so.of(c0 + c), u);
             cons2prim(u, q);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 dQ[i * 5 + v] = (real)(q[v] - ref.q[v]);
-                const real x = U[i * 5 + v];
+                const real x = (real)u[v];
                 s_abs[(c * NS + s) * 5 + v] = x < real(0) ? -x : x;
             }
         }
@@ -567,13 +567,14 @@ int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st) {
 }
 
 template <int NS, typename real>
-int launch_prim_umean(const real* U, real* dQ, real* umean, int64_t n, const double* ref, hipStream_t st) {
+int launch_prim_umean(const real* U, real* dQ, real* umean, int64_t n, const double* ref, StateOff so,
+                      hipStream_t st) {
     Ref5 r;
     for (int v = 0; v < 5; ++v) r.q[v] = ref ? ref[v] : 0.0;
     constexpr int CPB = 256 / NS;
     int64_t blocks = (n + CPB - 1) / CPB;
     int grid = (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
-    hipLaunchKernelGGL((k_prim_umean<NS, real>), dim3(grid), dim3(256), 0, st, U, dQ, umean, n, r);
+    hipLaunchKernelGGL((k_prim_umean<NS, real>), dim3(grid), dim3(256), 0, st, U, dQ, umean, n, r, so);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -581,11 +582,17 @@ template int launch_prism_inviscid<1, float>(const PrismArgs<float>&, hipStream_
 template int launch_prism_inviscid<2, float>(const PrismArgs<float>&, hipStream_t);
 template int launch_prism_inviscid<1, double>(const PrismArgs<double>&, hipStream_t);
 template int launch_prism_inviscid<2, double>(const PrismArgs<double>&, hipStream_t);
-template int launch_prim_umean<8, float>(const float*, float*, float*, int64_t, const double*, hipStream_t);
-template int launch_prim_umean<27, float>(const float*, float*, float*, int64_t, const double*, hipStream_t);
-template int launch_prim_umean<64, float>(const float*, float*, float*, int64_t, const double*, hipStream_t);
-template int launch_prim_umean<8, double>(const double*, double*, double*, int64_t, const double*, hipStream_t);
-template int launch_prim_umean<27, double>(const double*, double*, double*, int64_t, const double*, hipStream_t);
-template int launch_prim_umean<64, double>(const double*, double*, double*, int64_t, const double*, hipStream_t);
+template int launch_prim_umean<8, float>(const float*, float*, float*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
+template int launch_prim_umean<27, float>(const float*, float*, float*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
+template int launch_prim_umean<64, float>(const float*, float*, float*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
+template int launch_prim_umean<8, double>(const double*, double*, double*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
+template int launch_prim_umean<27, double>(const double*, double*, double*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
+template int launch_prim_umean<64, double>(const double*, double*, double*, int64_t, const double*, StateOff,
+                                                 hipStream_t);
 
 }  // namespace afr_prism

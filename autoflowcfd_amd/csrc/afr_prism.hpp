@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "afr_physics.hpp"
+
 struct Ref5 {
     double q[5];
 };
@@ -73,6 +75,7 @@ template <int P, typename real>
 int launch_prism_inviscid(const PrismArgs<real>& a, hipStream_t st);
 
 template <int NS, typename real>
-int launch_prim_umean(const real* U, real* dQ, real* umean, int64_t n, const double* ref, hipStream_t st);
+int launch_prim_umean(const real* U, real* dQ, real* umean, int64_t n, const double* ref, afr::StateOff so,
+                      hipStream_t st);
 
 }  // namespace afr_prism

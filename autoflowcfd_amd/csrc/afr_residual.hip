@@ -65,6 +65,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kSlots = 5;
 
+// the mesh's state storage: offsets from its per-case reference table, or the state itself
+StateOff state_off(const AfrMeshDesc& d) { return StateOff{d.state_ref, d.cell_case}; }
+
 template <int P> struct Ord {
     static constexpr int N1 = P + 1;
     static constexpr int NS = N1 * N1 * N1;
@@ -610,16 +613,16 @@ int afr_prepare_state(const AfrMesh* mesh, const void* U, void* dQ, void* umean,
         auto* u = (const float*)U;
         auto* q = (float*)dQ;
         auto* m = (float*)umean;
-        rc = ns == 8    ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, st)
-             : ns == 27 ? afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, st)
-                        : afr_prism::launch_prim_umean<64, float>(u, q, m, d.n_cells, d.ref_state, st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
+                        : afr_prism::launch_prim_umean<64, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st);
     } else {
         auto* u = (const double*)U;
         auto* q = (double*)dQ;
         auto* m = (double*)umean;
-        rc = ns == 8    ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, st)
-             : ns == 27 ? afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, st)
-                        : afr_prism::launch_prim_umean<64, double>(u, q, m, d.n_cells, d.ref_state, st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
+                        : afr_prism::launch_prim_umean<64, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st);
     }
     return rc ? fail("k_prim_umean launch failed") : 0;
 }
@@ -664,6 +667,7 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
         if (a.n_cells <= 0) continue;
         a.U = (const real*)U;
+        a.so = state_off(d);
         a.mu_t = (const real*)mu_t;
         a.dQ = (real*)d.work_q;
         a.umean = (real*)d.work_umean;
@@ -789,10 +793,10 @@ static int local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t,
     if (!d.metric_flux_scale || !d.volume) return fail("afr_local_time_step: mesh has no metric_flux_scale/volume");
     hipStream_t st = (hipStream_t)stream;
     const int rc = d.dtype == AFR_F32
-                       ? afr_step::launch_dt_local<float>((const float*)U, (const float*)mu_t, (const float*)d.det,
+                       ? afr_step::launch_dt_local<float>((const float*)U, state_off(d), (const float*)mu_t, (const float*)d.det,
                                                           (const float*)d.metric_flux_scale, (const float*)d.volume,
                                                           (float*)dt, d.n_cells, ns_of(d), mu, d.order, nut_mode, st)
-                       : afr_step::launch_dt_local<double>((const double*)U, (const double*)mu_t, (const double*)d.det,
+                       : afr_step::launch_dt_local<double>((const double*)U, state_off(d), (const double*)mu_t, (const double*)d.det,
                                                            (const double*)d.metric_flux_scale,
                                                            (const double*)d.volume, (double*)dt, d.n_cells, ns_of(d),
                                                            mu, d.order, nut_mode, st);
@@ -816,12 +820,12 @@ int afr_rk_stage(const AfrMesh* mesh, const void* U0, const void* Ua, const void
     const void* ua = Ua ? Ua : U0;
     const int rc = d.dtype == AFR_F32
                        ? afr_step::launch_rk_stage<float>(ns_of(d), (const float*)U0, (const float*)ua, (const float*)R,
-                                                          (const float*)dt, (float*)U_out, (const float*)d.filt[0],
+                                                          (const float*)dt, (float*)U_out, state_off(d), (const float*)d.filt[0],
                                                           (const float*)d.filt[1], d.n_cells, d.n_prism, c0, c1, c2,
                                                           1.0, st)
                        : afr_step::launch_rk_stage<double>(ns_of(d), (const double*)U0, (const double*)ua,
                                                            (const double*)R, (const double*)dt, (double*)U_out,
-                                                           (const double*)d.filt[0], (const double*)d.filt[1],
+                                                           state_off(d), (const double*)d.filt[0], (const double*)d.filt[1],
                                                            d.n_cells, d.n_prism, c0, c1, c2, 1.0, st);
     return rc ? fail("k_rk_stage launch failed") : 0;
 }
@@ -991,6 +995,7 @@ int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* d
     afr_sst::SstArgs<real> a;
     a.n_records = d.n_records;
     a.U = (const real*)U;
+    a.so = state_off(d);
     a.k = (real*)k;
     a.w = (real*)w;
     a.d_wall = (const real*)dw;
@@ -1097,6 +1102,7 @@ static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, dou
         a.n_wall = d.n_wall_rec;
         a.n_prism = d.n_prism;
         a.U = (const real*)U;
+        a.so = state_off(d);
         a.mu_t = (const real*)mu_t;
         a.det = (const real*)d.det;
         a.adj = (const real*)d.adj;

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "afr_physics.hpp"
+
 namespace afr_sst {
 
 constexpr int kPack = 8;  // face pack: rho, u, v, w, k, omega, Gamma_k, Gamma_omega
@@ -12,7 +14,8 @@ template <typename real>
 struct SstArgs {
     int64_t cell_begin, n_cells;  // cells of one type
     int64_t n_records;
-    const real* U;                // (C, NS, 5)
+    const real* U;                // (C, NS, 5) (offset from so's table when it has one)
+    afr::StateOff so;
     real* k;                      // (C, NS) in/out
     real* w;                      // (C, NS) in/out
     const real* d_wall;           // (C, NS)

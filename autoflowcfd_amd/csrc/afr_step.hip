@@ -21,7 +21,8 @@ namespace afr_step {
 constexpr int kThreads = 256;
 
 template <typename real>
-__global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ U, const real* __restrict__ mu_t,
+__global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ U, StateOff so,
+                                                       const real* __restrict__ mu_t,
                                                        const real* __restrict__ det, const real* __restrict__ mfs,
                                                        const real* __restrict__ vol, real* __restrict__ dt,
                                                        int64_t n_cells, int ns, double mu, int order, int nut_mode) {
@@ -30,8 +31,7 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = i / ns;
         double u5[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) u5[v] = (double)U[i * 5 + v];
+        load_state(U, i, so.of(c), u5);
         const double rho = u5[0] > 1e-10 ? u5[0] : 1e-10;
         const double ir = 1.0 / rho;
         const double u = u5[1] * ir, v = u5[2] * ir, w = u5[3] * ir;
@@ -60,10 +60,14 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
 // U_out = c0 U0 + c1 Ua + c2 dt R, then positivity (rho >= 1e-6, |u| <= 1e4,
 // p >= p_floor) per solution point and the modal filter per cell.  One
 // wavefront per cell; Ua may alias U_out, U0 may alias U_out (each cell is
-// read completely before it is written).
+// read completely before it is written).  With a state offset the three states
+// are offsets from the cell's reference: the stage combination is the same (c0 +
+// c1 = 1), the positivity test sees the absolute state, a projected component
+// is stored back as its offset, and the filter (which keeps constants) acts on
+// the offset.
 template <int NS, typename real>
 __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const real* Ua, const real* __restrict__ R,
-                                                       const real* __restrict__ dt, real* U_out,
+                                                       const real* __restrict__ dt, real* U_out, StateOff so,
                                                        const real* __restrict__ filt0, const real* __restrict__ filt1,
                                                        int64_t n_cells, int64_t n_prism, double c0, double c1,
                                                        double c2, double p_floor) {
@@ -81,30 +85,39 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
         if (t < NS) {
             const int64_t i = cell * NS + t;
             const double d = (double)dt[i];
-            double q[5];
+            double q[5], qa[5];
+            const double* off = so.of(cell);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
                 if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
                 q[v] = (double)(real)val;  // stage value in the storage precision
+                qa[v] = off ? q[v] + off[v] : q[v];
             }
-            const double rho = q[0] > 1e-6 ? q[0] : 1e-6;
-            double vx = q[1] / rho, vy = q[2] / rho, vz = q[3] / rho;
+            const bool low_rho = !(qa[0] > 1e-6);
+            const double rho = low_rho ? 1e-6 : qa[0];
+            double vx = qa[1] / rho, vy = qa[2] / rho, vz = qa[3] / rho;
             const double vm = sqrt(vx * vx + vy * vy + vz * vz);
-            double m1 = q[1], m2 = q[2], m3 = q[3];
-            if (vm > 1e4) {
+            const bool fast = vm > 1e4;
+            double m1 = qa[1], m2 = qa[2], m3 = qa[3];
+            if (fast) {
                 const double f = 1e4 / vm;
                 vx *= f; vy *= f; vz *= f;
                 m1 = rho * vx; m2 = rho * vy; m3 = rho * vz;
             }
             const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-            double E = q[4];
-            if (0.4 * (E - ke) < p_floor) E = p_floor / 0.4 + ke;
-            x[t * 5 + 0] = (real)rho;
-            x[t * 5 + 1] = (real)m1;
-            x[t * 5 + 2] = (real)m2;
-            x[t * 5 + 3] = (real)m3;
-            x[t * 5 + 4] = (real)E;
+            double E = qa[4];
+            const bool low_p = 0.4 * (E - ke) < p_floor;
+            if (low_p) E = p_floor / 0.4 + ke;
+            // a component the projection left alone is stored as it was (exact); a projected one
+            // as its (absolute) value less the offset
+            const double o0 = off ? off[0] : 0.0, o1 = off ? off[1] : 0.0, o2 = off ? off[2] : 0.0;
+            const double o3 = off ? off[3] : 0.0, o4 = off ? off[4] : 0.0;
+            x[t * 5 + 0] = low_rho ? (real)(rho - o0) : (real)q[0];
+            x[t * 5 + 1] = fast ? (real)(m1 - o1) : (real)q[1];
+            x[t * 5 + 2] = fast ? (real)(m2 - o2) : (real)q[2];
+            x[t * 5 + 3] = fast ? (real)(m3 - o3) : (real)q[3];
+            x[t * 5 + 4] = low_p ? (real)(E - o4) : (real)q[4];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -163,27 +176,28 @@ static int grid_for(int64_t n, int per) {
 }
 
 template <typename real>
-int launch_dt_local(const real* U, const real* mu_t, const real* det, const real* mfs, const real* vol, real* dt,
-                    int64_t n_cells, int ns, double mu, int order, int nut_mode, hipStream_t st) {
-    hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U, mu_t,
-                       det, mfs, vol, dt, n_cells, ns, mu, order, nut_mode);
+int launch_dt_local(const real* U, StateOff so, const real* mu_t, const real* det, const real* mfs,
+                    const real* vol, real* dt, int64_t n_cells, int ns, double mu, int order, int nut_mode,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U, so,
+                       mu_t, det, mfs, vol, dt, n_cells, ns, mu, order, nut_mode);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <typename real>
 int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const real* dt, real* U_out,
-                    const real* filt0, const real* filt1, int64_t n_cells, int64_t n_prism, double c0, double c1,
-                    double c2, double p_floor, hipStream_t st) {
+                    StateOff so, const real* filt0, const real* filt1, int64_t n_cells, int64_t n_prism,
+                    double c0, double c1, double c2, double p_floor, hipStream_t st) {
     const int grid = grid_for(n_cells, kThreads / 64);
     if (ns == 8)
-        hipLaunchKernelGGL((k_rk_stage<8, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
-                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+        hipLaunchKernelGGL((k_rk_stage<8, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, so,
+                           filt0, filt1, n_cells, n_prism, c0, c1, c2, p_floor);
     else if (ns == 27)
-        hipLaunchKernelGGL((k_rk_stage<27, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
-                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+        hipLaunchKernelGGL((k_rk_stage<27, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, so,
+                           filt0, filt1, n_cells, n_prism, c0, c1, c2, p_floor);
     else if (ns == 64)
-        hipLaunchKernelGGL((k_rk_stage<64, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, filt0,
-                           filt1, n_cells, n_prism, c0, c1, c2, p_floor);
+        hipLaunchKernelGGL((k_rk_stage<64, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, so,
+                           filt0, filt1, n_cells, n_prism, c0, c1, c2, p_floor);
     else
         return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -219,15 +233,16 @@ int launch_norm2(const real* x, int64_t a0, int64_t n0, int64_t a1, int64_t n1, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template int launch_dt_local<float>(const float*, const float*, const float*, const float*, const float*, float*,
-                                    int64_t, int, double, int, int, hipStream_t);
-template int launch_dt_local<double>(const double*, const double*, const double*, const double*, const double*,
-                                     double*, int64_t, int, double, int, int, hipStream_t);
-template int launch_rk_stage<float>(int, const float*, const float*, const float*, const float*, float*, const float*,
-                                    const float*, int64_t, int64_t, double, double, double, double, hipStream_t);
+template int launch_dt_local<float>(const float*, StateOff, const float*, const float*, const float*, const float*,
+                                    float*, int64_t, int, double, int, int, hipStream_t);
+template int launch_dt_local<double>(const double*, StateOff, const double*, const double*, const double*,
+                                     const double*, double*, int64_t, int, double, int, int, hipStream_t);
+template int launch_rk_stage<float>(int, const float*, const float*, const float*, const float*, float*, StateOff,
+                                    const float*, const float*, int64_t, int64_t, double, double, double, double,
+                                    hipStream_t);
 template int launch_rk_stage<double>(int, const double*, const double*, const double*, const double*, double*,
-                                     const double*, const double*, int64_t, int64_t, double, double, double, double,
-                                     hipStream_t);
+                                     StateOff, const double*, const double*, int64_t, int64_t, double, double, double,
+                                     double, hipStream_t);
 template int launch_norm2<float>(const float*, int64_t, int64_t, int64_t, int64_t, double, double*, int, double*,
                                  hipStream_t);
 template int launch_norm2<double>(const double*, int64_t, int64_t, int64_t, int64_t, double, double*, int, double*,

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "afr_physics.hpp"
+
 namespace afr_visc {
 
 constexpr int kVP = 20;  // per-point viscous pack: dQ (5), grad u (9), grad T (3), mu_t (1), pad (2)
@@ -10,7 +12,8 @@ constexpr int kVP = 20;  // per-point viscous pack: dQ (5), grad u (9), grad T (
 template <typename real>
 struct ViscArgs {
     int64_t cell_begin, n_cells;   // cells [cell_begin, cell_begin + n_cells) of one type
-    const real* U;                 // (C, NS, 5) conserved
+    const real* U;                 // (C, NS, 5) conserved (offset from so's table when it has one)
+    afr::StateOff so;
     const real* mu_t;              // (C, NS) or null
     real* dQ;                      // (C, NS, 5) out: primitive - Q_ref
     real* umean;                   // (C, 5) out

@@ -166,12 +166,18 @@ class DeviceMesh:
     """One mesh at one order on one GPU."""
 
     def __init__(self, mesh, order, dtype=torch.float32, device="cuda", bc_params=None, ref_state=None,
-                 slots=None, metrics="gpu", turbulence=None, cell_case=None):
+                 slots=None, metrics="gpu", turbulence=None, cell_case=None, state_ref=None):
         """metrics: "gpu" (FP64 precompute on the device) or "host" (NumPy FP64, small meshes).
         turbulence: None or "SST" (face-record tables, wall distance and SST work buffers).
         cell_case: None, or the case index of every cell of a disjoint union of cases
         (mesh.concat_meshes); bc_params is then (n_cases, 8, 5) and the wall distance is taken
-        case by case (mesh.meta["wall_off"] splits mesh.wall_nodes)."""
+        case by case (mesh.meta["wall_off"] splits mesh.wall_nodes).
+        state_ref: None (every state this mesh takes holds the conserved state itself) or the
+        conserved offset of each case, (5,) or (n_cases, 5) FP64: every state this mesh takes or
+        writes (U, stage states) then holds U - state_ref[case of the cell], and the kernels add
+        the offset back in FP64 when they load a point (``to_storage`` / ``to_absolute``
+        convert).  With each case's free stream as the offset, FP32 storage holds the flow's
+        variation about the free stream instead of rounding rho E ~ 2.5e5 to 0.016."""
         self.lib = _lib.load()
         self.turbulence = turbulence
         self.sps_xyz = None
@@ -296,6 +302,7 @@ class DeviceMesh:
         elif bcp.shape != (8, 5):
             raise ValueError("bc_params must be (8, 5)")
         self.bc_params = dev(bcp, torch.float64)
+        self._set_state_ref(state_ref)
         # reference state of the perturbation form: the freestream by default
         if ref_state is None and bcp.ndim == 3:
             raise ValueError("a batched mesh needs an explicit common ref_state")
@@ -322,6 +329,38 @@ class DeviceMesh:
         self._create(int(sl.rows.shape[0]), mesh.n_prism,
                      mesh.meta["prism_off"] if self.cell_case is not None else None,
                      mesh.meta["tet_off"] if self.cell_case is not None else None)
+
+    def _set_state_ref(self, state_ref):
+        if state_ref is None:
+            self.state_ref, self.state_ref_dev = None, None
+            return
+        sr = np.asarray(state_ref, dtype=np.float64).reshape(-1, 5)
+        if sr.shape[0] != self.n_cases or not np.all(np.isfinite(sr)):
+            raise ValueError(f"state_ref must hold {self.n_cases} finite conserved states of 5 values")
+        self.state_ref = sr
+        self.state_ref_dev = torch.as_tensor(np.ascontiguousarray(sr)).to(self.device)
+
+    def cell_offset(self):
+        """(C, 1, 5) FP64 device tensor: the conserved offset of every cell's stored state (zeros
+        without a state_ref)."""
+        if self.state_ref is None:
+            return torch.zeros((self.n_cells, 1, 5), dtype=torch.float64, device=self.device)
+        if self.cell_case is None:
+            return self.state_ref_dev.view(1, 1, 5).expand(self.n_cells, 1, 5)
+        return self.state_ref_dev[self.cell_case.long()].view(self.n_cells, 1, 5)
+
+    def to_absolute(self, U):
+        """The conserved state (FP64, device) of a state U stored by this mesh."""
+        X = U[..., :5].double()
+        return X if self.state_ref is None else X + self.cell_offset()
+
+    def to_storage(self, U_abs):
+        """The stored form (this mesh's precision, device) of a conserved state (FP64 preferred:
+        an FP32 input has already lost what the offset keeps)."""
+        X = torch.as_tensor(U_abs, device=self.device)[..., :5].double()
+        if self.state_ref is not None:
+            X = X - self.cell_offset()
+        return X.to(self.dtype).contiguous()
 
     def _create(self, n_rows, n_prism, prism_off=None, tet_off=None):
         """Fill the AfrMeshDesc from this object's device arrays and create the library's mesh."""
@@ -362,6 +401,7 @@ class DeviceMesh:
                 d.case_wall_off = self.case_wall_off.ctypes.data_as(ctypes.c_void_p)
         for v in range(5):
             d.ref_state[v] = float(self.ref_state[v])
+        d.state_ref = p(self.state_ref_dev) if self.state_ref is not None else None
         d.work_q, d.work_grad, d.work_umean = p(self.work_q), p(self.work_grad), p(self.work_umean)
         d.work_rvol, d.work_h = p(self.work_rvol), p(self.work_h)
         d.work_vpack = p(self.work_vpack)
@@ -410,6 +450,12 @@ class DeviceMesh:
         bcp = np.asarray(bc_params, dtype=np.float64)
         if bcp.shape != (len(parts), 8, 5):
             raise ValueError("bc_params must be (n_cases, 8, 5)")
+        offs = [q.state_ref for q in parts]
+        if any(o is None for o in offs) and not all(o is None for o in offs):
+            raise ValueError("parts must all store offsets (state_ref) or all store the state itself")
+        for q in parts:
+            if q.state_ref is not None and q.n_cases != 1:
+                raise ValueError("a part must be a single case")
         self = cls.__new__(cls)
         dv = a.device
         self.lib, self.turbulence, self.mesh, self.slots = a.lib, a.turbulence, None, None
@@ -472,6 +518,7 @@ class DeviceMesh:
         self.cell_case = torch.as_tensor(cc).to(dv)
         self.bc_params = torch.as_tensor(bcp).to(dv)
         self.ref_state = ref
+        self._set_state_ref(None if offs[0] is None else np.concatenate(offs, axis=0))
         dt = self.dtype
         ns = self.ns
         self.work_q = torch.empty((C, ns, 5), dtype=dt, device=dv)
@@ -802,6 +849,8 @@ class DeviceMesh:
     def primitives(self, U, dQ=None):
         """dQ = primitive(U) - ref_state into dQ (default: the mesh work buffer)."""
         self._check_state(U)
+        if self.state_ref is not None:
+            return self.prepare_state(U, dQ)[0]
         dQ = self.work_q if dQ is None else dQ
         ref = (ctypes.c_double * 5)(*[float(x) for x in self.ref_state])
         _lib.check(self.lib.afr_primitives(ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(dQ.data_ptr()),

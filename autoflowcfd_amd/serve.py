@@ -91,7 +91,7 @@ def build_solver(req, dtype=torch.float32, turb_model="NONE", mu=1.8e-5, ref_sta
                     turb_model=turb_model, dtype=dtype, k_inf=c.k_inf, omega_inf=c.omega_inf, mu_molecular=mu,
                     bc_overrides={BC_FARFIELD: Q, BC_INLET: Q}, ref_state=ref_state, slots=slots)
     U0 = np.broadcast_to(cons(Q), (s.dm.n_cells, s.dm.ns, 5))
-    s.U.copy_(torch.as_tensor(np.ascontiguousarray(U0), dtype=dtype))
+    s.U.copy_(s.dm.to_storage(torch.as_tensor(np.ascontiguousarray(U0))))
     return s
 
 

@@ -49,7 +49,7 @@ class GPUFRSolver:
                  p_inf: float = 101325.0, mu_molecular: float = 1.8e-5, boundary_ghost_provider=None,
                  turb_model: str = "NONE", dtype=torch.float32, k_inf: float = 1e-6, omega_inf: float = 1.0,
                  bc_overrides=None, Pr: float = 0.72, Pr_t: float = 0.9, ref_state=None, slots=None,
-                 global_time_step: bool = False, storage: str = "locality"):
+                 global_time_step: bool = False, storage: str = "locality", offset_storage=None):
         """mesh: autoflowcfd_amd.mesh.Mesh with face records; ops is accepted for signature
         compatibility (the tables are generated for ``order``).  boundary_ghost_provider /
         bc_overrides: tabulated reference states {BC code: (5,) primitive state} (FARFIELD, INLET)
@@ -66,6 +66,12 @@ class GPUFRSolver:
         ref_state: constant primitive state of the perturbation form (default: the FARFIELD state);
         a case stepped alone with the ref_state of a batch reproduces its batched trajectory bit
         for bit (batch.CaseBatch).
+        offset_storage: store the state on the device as its offset from a conserved reference
+        (DeviceMesh state_ref): True for the FARFIELD state (bc_overrides may set it, e.g. a yawed
+        free stream; else rho_inf, vel_inf along +x, p_inf), a (5,) conserved state for that
+        state, None for True with FP32 storage (it removes the free-stream rounding floor) and
+        False with FP64.  ``U`` then holds the offset; get_state_cpu, set_state_from_cpu,
+        checkpoints and the residual entry points take and return the conserved state.
         """
         if time_scheme.lower().replace("-", "_") != "ssp_rk3":
             raise NotImplementedError(f"time_scheme {time_scheme!r}: this backend implements ssp_rk3")
@@ -106,11 +112,20 @@ class GPUFRSolver:
             raise ValueError(f"storage {storage!r}: 'locality' or 'input'")
         self._new_of_old = torch.as_tensor(new_of_old, dtype=torch.int64, device=self.device)
         self._old_of_new = torch.as_tensor(np.argsort(new_of_old), dtype=torch.int64, device=self.device)
+        U_inf = _cons(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]))
+        if offset_storage is None:
+            offset_storage = dtype == torch.float32
+        if isinstance(offset_storage, (bool, np.bool_)):
+            Q_far = np.asarray(overrides.get(BC_FARFIELD, Q_free), dtype=np.float64).reshape(5)
+            state_ref = _cons(Q_far) if offset_storage else None
+        else:
+            state_ref = np.asarray(offset_storage, dtype=np.float64).reshape(5)
         self.dm = DeviceMesh(dev_mesh, order, dtype=dtype, device=self.device, bc_params=bcp, ref_state=ref_state,
-                             slots=slots, turbulence="SST" if self.turb_model_name == "SST" else None)
+                             slots=slots, turbulence="SST" if self.turb_model_name == "SST" else None,
+                             state_ref=state_ref)
         C, ns = self.dm.n_cells, self.dm.ns
-        U0 = _cons(np.broadcast_to(np.array([rho_inf, vel_inf, 0.0, 0.0, p_inf]), (C, ns, 5)))
-        self.U = torch.as_tensor(U0, dtype=dtype).to(self.device).contiguous()
+        U0 = torch.as_tensor(np.broadcast_to(U_inf, (C, ns, 5)).copy(), device=self.device)
+        self.U = self.dm.to_storage(U0)
         if self.turb_model_name == "SST":
             self.k, self.omega, self.nu_t, self.mu_t = self.dm.sst_fields(k_inf, omega_inf)
         else:
@@ -204,14 +219,19 @@ class GPUFRSolver:
         """Device cell order -> mesh cell order (rows)."""
         return x.index_select(0, self._new_of_old).contiguous()
 
+    def _trial(self, U_trial):
+        """The stored state, or the stored form of a conserved trial state given in mesh order."""
+        if U_trial is None:
+            return self.U
+        return self.dm.to_storage(self._to_dev(torch.as_tensor(U_trial, device=self.device)))
+
     def compute_inviscid_residual_gpu(self, U_trial=None):
         """Inviscid dU/dt (device tensor, mesh cell order) of the state or of U_trial (mesh order)."""
-        U = self.U if U_trial is None else self._to_dev(U_trial)
-        return self._to_user(self.dm.inviscid_residual(U))
+        return self._to_user(self.dm.inviscid_residual(self._trial(U_trial)))
 
     def compute_viscous_residual_gpu(self, U_trial=None, mu_t_field=None):
         """Viscous dU/dt (device tensor, mesh cell order); U_trial and mu_t_field in mesh order."""
-        U = self.U if U_trial is None else self._to_dev(U_trial)
+        U = self._trial(U_trial)
         mut = None if mu_t_field is None else self._to_dev(mu_t_field)
         return self._to_user(self.dm.viscous_residual(U, mut, mu=self.mu_molecular, Pr=self.Pr, Pr_t=self.Pr_t))
 
@@ -237,7 +257,7 @@ class GPUFRSolver:
     def get_state_cpu(self):
         """{'U', 'Q'} as FP64 NumPy arrays (n_cells, n_sps, n_vars); with SST the last two
         components are rho k, rho omega (U) and k, omega (Q)."""
-        U = self._to_user(self.U).double().cpu().numpy()
+        U = self._to_user(self.dm.to_absolute(self.U)).cpu().numpy()
         Q = self._primitives(U)
         if self.n_vars > 5:
             k = self._to_user(self.k).double().cpu().numpy() if self.k is not None else np.zeros(U.shape[:2])
@@ -251,7 +271,7 @@ class GPUFRSolver:
         if U_np.shape[:2] != tuple(self.U.shape[:2]):
             raise ValueError(f"state shape {U_np.shape} does not match ({self.U.shape[0]}, {self.U.shape[1]}, ...)")
         dev = lambda a: self._to_dev(torch.as_tensor(np.ascontiguousarray(a), device=self.device))  # noqa: E731
-        self.U.copy_(dev(U_np[..., :5]).to(self.U.dtype))
+        self.U.copy_(self.dm.to_storage(dev(U_np[..., :5])))
         if U_np.shape[-1] >= 7 and self.k is not None:
             rho = np.maximum(U_np[..., 0], 1e-10)
             self.k.copy_(dev(U_np[..., 5] / rho).to(self.k.dtype))
@@ -277,7 +297,7 @@ class GPUFRSolver:
             if int(f["n_cells"]) != self.dm.n_cells or int(f["n_sps"]) != self.dm.ns:
                 raise ValueError("checkpoint does not match this mesh / order")
             dev = lambda a: self._to_dev(torch.as_tensor(np.ascontiguousarray(a), device=self.device))  # noqa: E731
-            self.U.copy_(dev(f["U"][..., :5]).to(self.U.dtype))
+            self.U.copy_(self.dm.to_storage(dev(np.asarray(f["U"][..., :5], dtype=np.float64))))
             self.iteration = int(f["iteration"])
             self.residual_history = f["residual_history"].tolist()
             if self.k is not None and "k" in f:

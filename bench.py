@@ -467,11 +467,12 @@ def main():
         mesh, _ = renumber_cells(mesh, morton_order(mesh.nodes[mesh.prism_conn].mean(axis=1)))
     Q0 = freestream()
     t0 = time.time()
+    # FP32 storage as the solver keeps it: the offset from the free stream
     dm = DeviceMesh(mesh, args.order, dtype=torch.float32, bc_params=bc_params_table(Q0), metrics="gpu",
-                    turbulence="SST")
+                    turbulence="SST", state_ref=cons(Q0))
     torch.cuda.synchronize()
     log(f"[bench] device set-up {time.time() - t0:.1f}s, {dm.slots.n_blocks} row blocks")
-    U = device_state(mesh, dm.ns, Q0, torch.float32)
+    U = dm.to_storage(device_state(mesh, dm.ns, Q0, torch.float64))
     k, w, nut, mut = dm.sst_fields(K_INF, OMEGA_INF)
     R = torch.empty_like(U)
     rates = torch.empty((mesh.n_cells, dm.ns, 2), dtype=torch.float32, device="cuda")

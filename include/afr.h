@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define AFR_ABI_VERSION 5
+#define AFR_ABI_VERSION 6
 
 enum { AFR_F32 = 0, AFR_F64 = 1 };
 
@@ -110,6 +110,15 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *   wrec_geom  (W, 4) FP64             record unit normal, record area
  *   w1d        (order+1) FP64          Gauss-Legendre weights
  *   work_fpart (W, 6) FP64             per-record force partial sums
+ * State storage:
+ *   state_ref  (n_cases, 5) FP64 or NULL  conserved offset of each case: every state array of
+ *                                      this mesh (U, U0, the stage states) then holds
+ *                                      U - state_ref[case of the cell], and every kernel adds the
+ *                                      offset back in FP64 when it loads a point.  NULL: the
+ *                                      arrays hold U itself.  With the case's free stream as the
+ *                                      offset, FP32 storage resolves the flow's variation about
+ *                                      the free stream instead of rounding the dimensional state
+ *                                      (rho E ~ 2.5e5 rounds to 0.016).
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
@@ -166,6 +175,7 @@ typedef struct AfrMeshDesc {
     /* kernel variants (tile shapes) per kernel, indexed by AfrKernel; 0 = the built-in default.
      * Set by afr_mesh_set_variant (the autotuner), not read from the caller's descriptor. */
     int32_t variant[8];
+    const double *state_ref;
 } AfrMeshDesc;
 
 /* Kernels whose persistent grid size is tunable (afr_mesh_set_grid). */
@@ -415,6 +425,9 @@ typedef struct AfrP0Desc {
     const double *rec_w;
     const void *d_wall;
     double *work_sst;
+    /* (5) FP64 or NULL: the conserved offset of the stored state (as AfrMeshDesc.state_ref, one
+     * case) */
+    const double *state_ref;
 } AfrP0Desc;
 
 /* P0 residual of the conserved state U (C, 1, 5) into R: the finite-volume inviscid residual of

@@ -92,7 +92,8 @@ def test_p1_euler_pulse_full_configuration_matches_oracle_fixture():
     assert np.allclose((U ** 2).sum(axis=(0, 1)), ref["checksum2"], rtol=1e-12, atol=0.0)
 
 
-def test_c2_workload_residual_at_12_cubed_matches_oracle():
+@pytest.mark.parametrize("storage", ["offset", "absolute"])
+def test_c2_workload_residual_at_12_cubed_matches_oracle(storage):
     """The headline workload built by bench.py's own definitions (graded wall box of y-extruded
     prisms, no-slip y = 0, inlet / outlet / symmetry, P2 SST RANS in FP32, Mach 0.15, Re 1e6 per
     unit length, k and omega from 1 % intensity and viscosity ratio 10, the seeded perturbed
@@ -100,7 +101,9 @@ def test_c2_workload_residual_at_12_cubed_matches_oracle():
     evaluation as the bench times it (local time step, SST sources + transport, mean-flow residual
     with the new mu_t) on the GPU vs the FP64 oracle on the same FP32-rounded state.  Mean-flow
     residual within 1e-5 relative L2 (the stated FP32 tolerance); nu_t and the updated k, omega
-    within 1e-5 of their largest value."""
+    within 1e-5 of their largest value.  storage "offset": the state stored as its offset from the
+    free stream, as the bench and the FP32 solvers keep it (the oracle then sees the conserved
+    state that offset + FP32 offset represents); "absolute": the conserved state itself."""
     import torch
     import bench
     from autoflowcfd_amd.device import DeviceMesh, bc_params_table
@@ -110,12 +113,16 @@ def test_c2_workload_residual_at_12_cubed_matches_oracle():
     from oracle.geometry import Geometry
     from oracle.residual import inviscid_residual, viscous_residual
     from oracle.time_step import local_time_step, update_primitives
-    print("[gpu] C2 workload at 12^3: P2 SST FP32 residual vs oracle", flush=True)
+    print(f"[gpu] C2 workload at 12^3: P2 SST FP32 residual vs oracle ({storage} storage)", flush=True)
     mesh = bench.wall_box_mesh(12)
     Q0 = bench.freestream()
     bcp = bc_params_table(Q0)
-    dm = DeviceMesh(mesh, 2, dtype=torch.float32, bc_params=bcp, turbulence="SST")
-    U = bench.device_state(mesh, dm.ns, Q0, torch.float32)
+    if storage == "offset":
+        dm = DeviceMesh(mesh, 2, dtype=torch.float32, bc_params=bcp, turbulence="SST", state_ref=bench.cons(Q0))
+        U = dm.to_storage(bench.device_state(mesh, dm.ns, Q0, torch.float64))
+    else:
+        dm = DeviceMesh(mesh, 2, dtype=torch.float32, bc_params=bcp, turbulence="SST")
+        U = bench.device_state(mesh, dm.ns, Q0, torch.float32)
     k, w, nut, mut = dm.sst_fields(bench.K_INF, bench.OMEGA_INF)
     k0 = k.double().cpu().numpy()
     w0 = w.double().cpu().numpy()
@@ -123,7 +130,7 @@ def test_c2_workload_residual_at_12_cubed_matches_oracle():
     dm.local_time_step_nut(U, nut, dt, mu=bench.MU)
     dm.sst_step(U, k, w, dt, nut, mut, mu=bench.MU)
     R = dm.residual(U, mut, torch.empty_like(U), mu=bench.MU).double().cpu().numpy()
-    Uh = U.double().cpu().numpy()
+    Uh = dm.to_absolute(U).cpu().numpy()
     ops = fr_ops.generate_operators(2)
     g = Geometry(mesh, ops)
     fl = build_flat_faces(mesh, ops)
