@@ -596,7 +596,8 @@ def main():
         "config": {"workload": f"P{args.order} SST RANS residual (AUSM+up inviscid + BR1 viscous + SST sources and "
                                f"transport), {args.n}^3-hex graded wall box split into {mesh.n_cells} prisms, "
                                f"{n_sp} solution points x 7 variables",
-                   "cells": mesh.n_cells, "order": args.order, "n_vars": 7},
+                   "cells": mesh.n_cells, "order": args.order, "n_vars": 7,
+                   "state_storage": "FP32 offsets from the free stream (conserved, FP64 reference)"},
         # headline: one full residual evaluation (all seven kernels) against the HBM roofline
         "roofline": {"bound": "hbm", "achieved": b_res / t_res / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": b_res / t_res / HBM_PEAK,
