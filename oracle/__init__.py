@@ -22,6 +22,10 @@ trilinear field exactly (inside and outside the old points) and its rows sum to 
 finite-volume residual preserves the free stream and is discretely conservative against the
 boundary flux with the per-cell ghost quirk; the P0 viscous residual equals the closed-form wall
 penalty g'(+-1/2)(-eta |a| side) 2u / det J and vanishes without walls.
+``topology`` (tests/test_oracle_topology.py): the face records close every cell (outward area
+vectors sum to zero: a missing, duplicated or wrongly split triangle or a wrong orientation
+fails), with 8 records per prism and 4 per tet; planted mistakes (the other quad diagonal, owner
+and neighbour swapped, first-group-wins tagging) are caught against the product.
 
 Parity unpinned (no reference fixture and no closed form pins them; each says so in its
 docstring): the pointwise distribution of the tet interface corrections (the collapsed tet scheme
