@@ -307,6 +307,23 @@ def _resolve_codes(mesh: Mesh, cells: np.ndarray, tri: np.ndarray) -> np.ndarray
     return codes
 
 
+def check_cells(mesh: Mesh) -> None:
+    """Refuse cells whose vertices repeat.  The reference's BL extrusion can collapse one side edge
+    of a prism into a point (a wedge with v_i == w_i), and its face extraction then drops the two
+    zero-area triangles (face_extraction_kernels.py:310-318); the slot tables and flux-point
+    location of this package assume five real faces per prism, so such a mesh is refused here
+    rather than given zero-area records.  Tets with a repeated vertex have no volume."""
+    for name, conn in (("prism", mesh.prism_conn), ("tet", mesh.tet_conn)):
+        c = np.asarray(conn)
+        if not c.size:
+            continue
+        srt = np.sort(c.reshape(len(c), -1), axis=1)
+        bad = np.flatnonzero(np.any(srt[:, 1:] == srt[:, :-1], axis=1))
+        if len(bad):
+            raise ValueError(f"{len(bad)} {name} cell(s) with a repeated vertex (first: cell {int(bad[0])}, "
+                             f"nodes {c[bad[0]].tolist()}): collapsed cells are not supported")
+
+
 def topology_of(mesh: Mesh, topology="host") -> FaceRecords:
     """The face records of ``mesh``: "host" builds them with NumPy (build_topology), "gpu" on the
     device (topology.build_topology_gpu, the same records bit for bit)."""
@@ -335,6 +352,7 @@ def mesh_from_cells(nodes, prism_conn=None, tet_conn=None, default_bc=BC_EXTRAP,
 
 def build_topology(mesh: Mesh) -> FaceRecords:
     """Deduplicated triangle records with owner/neighbour, cube-face codes and normals."""
+    check_cells(mesh)
     tris = []
     owners = []
     if mesh.n_prism:

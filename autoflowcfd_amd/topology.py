@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .mesh import BC_INTERIOR, FaceRecords
+from .mesh import BC_INTERIOR, FaceRecords, check_cells
 
 
 def build_topology_gpu(mesh, device=None) -> FaceRecords:
@@ -32,6 +32,7 @@ def build_topology_gpu(mesh, device=None) -> FaceRecords:
     for conn in (mesh.prism_conn, mesh.tet_conn):
         if conn.size and (conn.min() < 0 or conn.max() >= n_nodes):
             raise ValueError("cell connectivity names a node outside the node array")
+    check_cells(mesh)
     nodes = torch.as_tensor(np.ascontiguousarray(mesh.nodes, dtype=np.float64)).to(dv)
     pc = torch.as_tensor(np.ascontiguousarray(mesh.prism_conn.reshape(-1, 6), dtype=np.int32)).to(dv)
     tc = torch.as_tensor(np.ascontiguousarray(mesh.tet_conn.reshape(-1, 4), dtype=np.int32)).to(dv)

@@ -145,3 +145,17 @@ def test_oracle_records_close_every_cell(kind):
     bad["normal"] = rec["normal"].copy()
     bad["normal"][0] *= -1.0
     assert np.abs(_closure(bad, mesh.n_cells)).max() > 1e-3 * scale
+
+
+def test_collapsed_wedge_is_refused_by_the_product_and_dropped_by_the_oracle():
+    """A prism whose side edge v2-w2 collapses to a point: the reference drops its two zero-area
+    triangles (6 records), the oracle does the same, the product refuses the mesh."""
+    from autoflowcfd_amd.mesh import Mesh, build_topology
+    from oracle.topology import face_records
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1]], dtype=float)
+    conn = np.array([[0, 1, 2, 3, 4, 2]])
+    rec = face_records(nodes, conn, np.zeros((0, 4), np.int64))
+    assert len(rec["owner"]) == 6 and np.all(rec["area"] > 0)
+    assert np.abs(_closure(rec, 1)).max() <= 1e-15
+    with pytest.raises(ValueError, match="repeated vertex"):
+        build_topology(Mesh(nodes=nodes, prism_conn=conn, tet_conn=np.zeros((0, 4), np.int64)))
