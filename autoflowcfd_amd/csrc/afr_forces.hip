@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_wall_forces(ForceArgs<real> a) 
         if (t < NS) {
             const int64_t i = c * NS + t;
             double u5[5];
-            afr::load_state(a.U, i, a.so.of(c), u5);
+            afr::load_state(a.U, i, a.so, c, u5);
             const double rho = fmax(u5[0], 1e-10);
             const double u = u5[1] / rho, v = u5[2] / rho, w = u5[3] / rho;
             const double ke = 0.5 * (u * u + v * v + w * w);

@@ -47,7 +47,8 @@ constexpr double kCip = 4.0;   // interior-penalty coefficient, viscous_flux.py:
 template <typename real>
 __device__ __forceinline__ void prim_at(const real* U, int64_t c, const double* off, double q[5]) {
     double u[5];
-    load_state(U, c, off, u);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = off ? (double)U[c * 5 + v] + off[v] : (double)U[c * 5 + v];
     cons2prim(u, q);
 }
 
@@ -252,7 +253,8 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_transport(AfrP0Desc d, cons
             const double g = (code & 1) ? 0.5 : -0.5;   // g'(0) of the side's Radau correction
             const int64_t o = d.rec_owner[r];
             double uo[5];
-            load_state(U, o, d.state_ref, uo);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) uo[v] = d.state_ref ? (double)U[o * 5 + v] + d.state_ref[v] : (double)U[o * 5 + v];
             const double ro = dmax(uo[0], 1e-10);
             const double* n = d.rec_geom + (int64_t)r * 4;
             const double vo[3] = {uo[1] / ro, uo[2] / ro, uo[3] / ro};
@@ -318,6 +320,14 @@ __global__ void __launch_bounds__(kThreads) k_p0_sst_update(AfrP0Desc d, const r
     }
 }
 
+// the P0 state's storage for the shared local-time-step kernel: the offset row on the device, or none
+StateOff p0_state_off(const AfrP0Desc& d) {
+    StateOff so{};
+    so.table = d.state_ref;
+    so.mode = d.state_ref ? 2 : 0;
+    return so;
+}
+
 int grid_of(int64_t n) {
     int64_t b = (n + kThreads - 1) / kThreads;
     return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -367,7 +377,7 @@ int stages_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_var
 
 template <typename real>
 int step_t(const AfrP0Desc& d, void* U, const void* mu_t, double mu, int n_vars_total, double* norm, hipStream_t st) {
-    if (afr_step::launch_dt_local<real>((const real*)U, StateOff{d.state_ref, nullptr}, (const real*)mu_t,
+    if (afr_step::launch_dt_local<real>((const real*)U, p0_state_off(d), (const real*)mu_t,
                                         (const real*)d.det, (const real*)d.mfs,
                                         (const real*)d.vol1, (real*)d.work_dt, d.n_cells, 1, mu, 0, 0, st))
         return -1;
@@ -380,7 +390,7 @@ int sst_step_t(const AfrP0Desc& d, void* U, void* k, void* w, void* nu_t, void* 
     const int g = grid_of(d.n_cells);
     real* dt = (real*)d.work_dt;
     // local time step with the previous step's eddy viscosity (nu_t held, mu_eff = mu + rho nu_t)
-    if (afr_step::launch_dt_local<real>((const real*)U, StateOff{d.state_ref, nullptr}, (const real*)nu_t,
+    if (afr_step::launch_dt_local<real>((const real*)U, p0_state_off(d), (const real*)nu_t,
                                         (const real*)d.det, (const real*)d.mfs,
                                         (const real*)d.vol1, dt, d.n_cells, 1, mu, 0, 1, st))
         return -1;

@@ -81,24 +81,30 @@ template <typename real>
 __host__ __device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : b; }
 
 // Stored state -> conserved state.  A mesh may store every state as the offset from a per-case
-// conserved reference (sref, n_cases x 5, FP64; the case's free stream): the kernels add the offset
-// back in FP64 when they load a point.  Without a table the storage holds the state itself and the
-// load is a plain widening (bit for bit the same values as before the offset existed).
+// conserved reference (the case's free stream): the kernels add the offset back in FP64 when they
+// load a point.  A single-case mesh carries its offset by value (a kernel argument, held in scalar
+// registers); a batched mesh reads the row of the cell's case from a device table.  Without an
+// offset the storage holds the state itself and the load is a plain widening (bit for bit the same
+// values as before the offset existed).
 struct StateOff {
-    const double* sref;    // null: absolute storage
-    const int* cell_case;  // null: one case
-    __device__ __forceinline__ const double* of(int64_t cell) const {
-        return sref ? sref + 5 * (int64_t)(cell_case ? cell_case[cell] : 0) : nullptr;
+    const double* table;   // mode 2: (n_cases, 5) FP64 device table
+    const int* cell_case;  // mode 2: the case of every cell (null: row 0)
+    double one[5];         // mode 1: the offset
+    int mode;              // 0 absolute storage, 1 one offset by value, 2 per-case table
+    __device__ __forceinline__ double get(int64_t cell, int v) const {
+        if (mode == 1) return one[v];
+        if (mode == 2) return table[5 * (int64_t)(cell_case ? cell_case[cell] : 0) + v];
+        return 0.0;
     }
 };
 
 template <typename real>
-__device__ __forceinline__ void load_state(const real* U, int64_t i, const double* off, double u[5]) {
+__device__ __forceinline__ void load_state(const real* U, int64_t i, const StateOff& so, int64_t cell, double u[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) u[v] = (double)U[i * 5 + v];
-    if (off) {
+    if (so.mode) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] += off[v];
+        for (int v = 0; v < 5; ++v) u[v] += so.get(cell, v);
     }
 }
 

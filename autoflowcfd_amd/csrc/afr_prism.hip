@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(256) k_prim_umean(const real* __restrict__ U, 
         if (ok) {
             const int64_t i = (c0 + c) * NS + s;
             double u[5], q[5];
-            load_state(U, i, so.of(c0 + c), u);
+            load_state(U, i, so, c0 + c, u);
             cons2prim(u, q);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {

@@ -58,15 +58,15 @@ struct AfrMesh {
     AfrMeshDesc d;
     // host copies of the case offsets of a batched mesh (cell_case set)
     std::vector<int64_t> case_prism_off, case_tet_off, case_wall_off;
+    // the state storage the kernels see: a single-case offset copied here at creation (passed by
+    // value), a batched mesh's device table, or none
+    afr::StateOff so{};
 };
 
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kSlots = 5;
-
-// the mesh's state storage: offsets from its per-case reference table, or the state itself
-StateOff state_off(const AfrMeshDesc& d) { return StateOff{d.state_ref, d.cell_case}; }
 
 template <int P> struct Ord {
     static constexpr int N1 = P + 1;
@@ -510,8 +510,21 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
             if (po[i + 1] < po[i] || to[i + 1] < to[i] || (!wo.empty() && wo[i + 1] < wo[i]))
                 return fail("afr_mesh_create: case offsets not ascending");
     }
+    afr::StateOff so{};
+    if (desc->state_ref) {
+        if (desc->cell_case) {
+            so.table = desc->state_ref;
+            so.cell_case = desc->cell_case;
+            so.mode = 2;
+        } else {
+            if (hipMemcpy(so.one, desc->state_ref, sizeof(so.one), hipMemcpyDeviceToHost) != hipSuccess)
+                return fail("afr_mesh_create: state_ref is not a readable device array of 5 doubles");
+            so.mode = 1;
+        }
+    }
     AfrMesh* m = new AfrMesh;
     m->d = *desc;
+    m->so = so;
     m->case_prism_off = std::move(po);
     m->case_tet_off = std::move(to);
     m->case_wall_off = std::move(wo);
@@ -613,16 +626,16 @@ int afr_prepare_state(const AfrMesh* mesh, const void* U, void* dQ, void* umean,
         auto* u = (const float*)U;
         auto* q = (float*)dQ;
         auto* m = (float*)umean;
-        rc = ns == 8    ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
-             : ns == 27 ? afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
-                        : afr_prism::launch_prim_umean<64, float>(u, q, m, d.n_cells, d.ref_state, state_off(d), st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, float>(u, q, m, d.n_cells, d.ref_state, mesh->so, st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, float>(u, q, m, d.n_cells, d.ref_state, mesh->so, st)
+                        : afr_prism::launch_prim_umean<64, float>(u, q, m, d.n_cells, d.ref_state, mesh->so, st);
     } else {
         auto* u = (const double*)U;
         auto* q = (double*)dQ;
         auto* m = (double*)umean;
-        rc = ns == 8    ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
-             : ns == 27 ? afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st)
-                        : afr_prism::launch_prim_umean<64, double>(u, q, m, d.n_cells, d.ref_state, state_off(d), st);
+        rc = ns == 8    ? afr_prism::launch_prim_umean<8, double>(u, q, m, d.n_cells, d.ref_state, mesh->so, st)
+             : ns == 27 ? afr_prism::launch_prim_umean<27, double>(u, q, m, d.n_cells, d.ref_state, mesh->so, st)
+                        : afr_prism::launch_prim_umean<64, double>(u, q, m, d.n_cells, d.ref_state, mesh->so, st);
     }
     return rc ? fail("k_prim_umean launch failed") : 0;
 }
@@ -658,8 +671,8 @@ int afr_inviscid_residual(const AfrMesh* mesh, const void* U, void* R, int32_t s
 namespace {
 
 template <typename real>
-int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, double mu, double Pr, double Prt,
-                int suppress, int accumulate, int phase, hipStream_t st) {
+int run_viscous(const AfrMeshDesc& d, const StateOff& so, const void* U, const void* mu_t, void* R, double mu,
+                double Pr, double Prt, int suppress, int accumulate, int phase, hipStream_t st) {
     // phase 0: volume kernel only; 1: interface kernel only
     for (int ct = 0; ct < 2; ++ct) {
         afr_visc::ViscArgs<real> a;
@@ -667,7 +680,7 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
         a.n_cells = ct == 0 ? d.n_prism : d.n_cells - d.n_prism;
         if (a.n_cells <= 0) continue;
         a.U = (const real*)U;
-        a.so = state_off(d);
+        a.so = so;
         a.mu_t = (const real*)mu_t;
         a.dQ = (real*)d.work_q;
         a.umean = (real*)d.work_umean;
@@ -722,10 +735,12 @@ int run_viscous(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, 
     return 0;
 }
 
-int viscous_phase(const AfrMeshDesc& d, const void* U, const void* mu_t, void* R, double mu, double Pr, double Prt,
+int viscous_phase(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr, double Prt,
                   int suppress, int accumulate, int phase, hipStream_t st) {
-    return d.dtype == AFR_F32 ? run_viscous<float>(d, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st)
-                              : run_viscous<double>(d, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st);
+    const AfrMeshDesc& d = mesh->d;
+    return d.dtype == AFR_F32
+               ? run_viscous<float>(d, mesh->so, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st)
+               : run_viscous<double>(d, mesh->so, U, mu_t, R, mu, Pr, Prt, suppress, accumulate, phase, st);
 }
 
 bool viscous_ready(const AfrMeshDesc& d) {
@@ -743,8 +758,8 @@ int afr_viscous_residual(const AfrMesh* mesh, const void* U, const void* mu_t, v
     const AfrMeshDesc& d = mesh->d;
     if (!viscous_ready(d)) return fail("afr_viscous_residual: mesh has no viscous work buffers");
     hipStream_t st = (hipStream_t)stream;
-    if (viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
-    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 1, st);
+    if (viscous_phase(mesh, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
+    return viscous_phase(mesh, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 1, st);
 }
 
 int afr_viscous_volume(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, double Pr, double Pr_t,
@@ -752,7 +767,7 @@ int afr_viscous_volume(const AfrMesh* mesh, const void* U, const void* mu_t, dou
     if (!mesh || !U) return fail("afr_viscous_volume: null argument");
     const AfrMeshDesc& d = mesh->d;
     if (!viscous_ready(d)) return fail("afr_viscous_volume: mesh has no viscous work buffers");
-    return viscous_phase(d, U, mu_t, nullptr, mu, Pr, Pr_t, 0, 0, 0, (hipStream_t)stream);
+    return viscous_phase(mesh, U, mu_t, nullptr, mu, Pr, Pr_t, 0, 0, 0, (hipStream_t)stream);
 }
 
 int afr_viscous_interface(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr,
@@ -760,7 +775,7 @@ int afr_viscous_interface(const AfrMesh* mesh, const void* U, const void* mu_t, 
     if (!mesh || !U || !R) return fail("afr_viscous_interface: null argument");
     const AfrMeshDesc& d = mesh->d;
     if (!viscous_ready(d)) return fail("afr_viscous_interface: mesh has no viscous work buffers");
-    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, accumulate, 1, (hipStream_t)stream);
+    return viscous_phase(mesh, U, mu_t, R, mu, Pr, Pr_t, suppress, accumulate, 1, (hipStream_t)stream);
 }
 
 int afr_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, double mu, double Pr, double Pr_t,
@@ -770,9 +785,9 @@ int afr_residual(const AfrMesh* mesh, const void* U, const void* mu_t, void* R, 
     if (!viscous_ready(d)) return fail("afr_residual: mesh has no viscous work buffers");
     hipStream_t st = (hipStream_t)stream;
     // volume kernel: prepared state (dQ, umean), gradients, viscous volume term
-    if (viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
+    if (viscous_phase(mesh, U, mu_t, R, mu, Pr, Pr_t, suppress, 0, 0, st)) return -1;
     if (afr_inviscid_from_primitives(mesh, d.work_q, d.work_umean, R, suppress, stream)) return -1;
-    return viscous_phase(d, U, mu_t, R, mu, Pr, Pr_t, suppress, 1, 1, st);
+    return viscous_phase(mesh, U, mu_t, R, mu, Pr, Pr_t, suppress, 1, 1, st);
 }
 
 namespace {
@@ -793,10 +808,10 @@ static int local_time_step(const AfrMesh* mesh, const void* U, const void* mu_t,
     if (!d.metric_flux_scale || !d.volume) return fail("afr_local_time_step: mesh has no metric_flux_scale/volume");
     hipStream_t st = (hipStream_t)stream;
     const int rc = d.dtype == AFR_F32
-                       ? afr_step::launch_dt_local<float>((const float*)U, state_off(d), (const float*)mu_t, (const float*)d.det,
+                       ? afr_step::launch_dt_local<float>((const float*)U, mesh->so, (const float*)mu_t, (const float*)d.det,
                                                           (const float*)d.metric_flux_scale, (const float*)d.volume,
                                                           (float*)dt, d.n_cells, ns_of(d), mu, d.order, nut_mode, st)
-                       : afr_step::launch_dt_local<double>((const double*)U, state_off(d), (const double*)mu_t, (const double*)d.det,
+                       : afr_step::launch_dt_local<double>((const double*)U, mesh->so, (const double*)mu_t, (const double*)d.det,
                                                            (const double*)d.metric_flux_scale,
                                                            (const double*)d.volume, (double*)dt, d.n_cells, ns_of(d),
                                                            mu, d.order, nut_mode, st);
@@ -820,12 +835,12 @@ int afr_rk_stage(const AfrMesh* mesh, const void* U0, const void* Ua, const void
     const void* ua = Ua ? Ua : U0;
     const int rc = d.dtype == AFR_F32
                        ? afr_step::launch_rk_stage<float>(ns_of(d), (const float*)U0, (const float*)ua, (const float*)R,
-                                                          (const float*)dt, (float*)U_out, state_off(d), (const float*)d.filt[0],
+                                                          (const float*)dt, (float*)U_out, mesh->so, (const float*)d.filt[0],
                                                           (const float*)d.filt[1], d.n_cells, d.n_prism, c0, c1, c2,
                                                           1.0, st)
                        : afr_step::launch_rk_stage<double>(ns_of(d), (const double*)U0, (const double*)ua,
                                                            (const double*)R, (const double*)dt, (double*)U_out,
-                                                           state_off(d), (const double*)d.filt[0], (const double*)d.filt[1],
+                                                           mesh->so, (const double*)d.filt[0], (const double*)d.filt[1],
                                                            d.n_cells, d.n_prism, c0, c1, c2, 1.0, st);
     return rc ? fail("k_rk_stage launch failed") : 0;
 }
@@ -989,13 +1004,13 @@ int afr_wall_distance(const double* sp_xyz, int64_t n_pts, const double* wall_xy
 namespace {
 
 template <typename real>
-int run_sst(const AfrMeshDesc& d, const void* U, void* k, void* w, const void* dw, const void* dt, void* nut,
-            void* mut, void* rates, double mu, hipStream_t st) {
+int run_sst(const AfrMeshDesc& d, const StateOff& so, const void* U, void* k, void* w, const void* dw,
+            const void* dt, void* nut, void* mut, void* rates, double mu, hipStream_t st) {
     if (d.order > 2) return fail("SST is supported at orders 1 and 2");
     afr_sst::SstArgs<real> a;
     a.n_records = d.n_records;
     a.U = (const real*)U;
-    a.so = state_off(d);
+    a.so = so;
     a.k = (real*)k;
     a.w = (real*)w;
     a.d_wall = (const real*)dw;
@@ -1078,8 +1093,9 @@ int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const
         !d.work_pack || !d.work_nbp || !d.work_tvol || !d.work_tsrc || !d.work_crec || !d.adj)
         return fail("afr_sst_step: mesh has no SST record tables / work buffers");
     hipStream_t st = (hipStream_t)stream;
-    return d.dtype == AFR_F32 ? run_sst<float>(d, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st)
-                              : run_sst<double>(d, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st);
+    return d.dtype == AFR_F32
+               ? run_sst<float>(d, mesh->so, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st)
+               : run_sst<double>(d, mesh->so, U, k, omega, d_wall, dt, nu_t, mu_t, rates, mu_molecular, st);
 }
 
 // wall_off == nullptr: one sum over every record into out[6]; else per case i the records
@@ -1102,7 +1118,7 @@ static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, dou
         a.n_wall = d.n_wall_rec;
         a.n_prism = d.n_prism;
         a.U = (const real*)U;
-        a.so = state_off(d);
+        a.so = mesh->so;
         a.mu_t = (const real*)mu_t;
         a.det = (const real*)d.det;
         a.adj = (const real*)d.adj;

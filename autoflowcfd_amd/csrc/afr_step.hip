@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = i / ns;
         double u5[5];
-        load_state(U, i, so.of(c), u5);
+        load_state(U, i, so, c, u5);
         const double rho = u5[0] > 1e-10 ? u5[0] : 1e-10;
         const double ir = 1.0 / rho;
         const double u = u5[1] * ir, v = u5[2] * ir, w = u5[3] * ir;
@@ -85,14 +85,14 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
         if (t < NS) {
             const int64_t i = cell * NS + t;
             const double d = (double)dt[i];
-            double q[5], qa[5];
-            const double* off = so.of(cell);
+            double q[5], qa[5], off[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
                 if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
                 q[v] = (double)(real)val;  // stage value in the storage precision
-                qa[v] = off ? q[v] + off[v] : q[v];
+                off[v] = so.get(cell, v);
+                qa[v] = so.mode ? q[v] + off[v] : q[v];
             }
             const bool low_rho = !(qa[0] > 1e-6);
             const double rho = low_rho ? 1e-6 : qa[0];
@@ -111,13 +111,11 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
             if (low_p) E = p_floor / 0.4 + ke;
             // a component the projection left alone is stored as it was (exact); a projected one
             // as its (absolute) value less the offset
-            const double o0 = off ? off[0] : 0.0, o1 = off ? off[1] : 0.0, o2 = off ? off[2] : 0.0;
-            const double o3 = off ? off[3] : 0.0, o4 = off ? off[4] : 0.0;
-            x[t * 5 + 0] = low_rho ? (real)(rho - o0) : (real)q[0];
-            x[t * 5 + 1] = fast ? (real)(m1 - o1) : (real)q[1];
-            x[t * 5 + 2] = fast ? (real)(m2 - o2) : (real)q[2];
-            x[t * 5 + 3] = fast ? (real)(m3 - o3) : (real)q[3];
-            x[t * 5 + 4] = low_p ? (real)(E - o4) : (real)q[4];
+            x[t * 5 + 0] = low_rho ? (real)(rho - off[0]) : (real)q[0];
+            x[t * 5 + 1] = fast ? (real)(m1 - off[1]) : (real)q[1];
+            x[t * 5 + 2] = fast ? (real)(m2 - off[2]) : (real)q[2];
+            x[t * 5 + 3] = fast ? (real)(m3 - off[3]) : (real)q[3];
+            x[t * 5 + 4] = low_p ? (real)(E - off[4]) : (real)q[4];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_VOLUME) 
 #pragma unroll
             for (int q = 0; q < 9; ++q) adj_s[q] = a.adj[(cell * NS + s) * 9 + q];
             double u[5], q[5];
-            load_state(a.U, cell * NS + s, a.so.of(cell), u);
+            load_state(a.U, cell * NS + s, a.so, cell, u);
             cons2prim(u, q);
             real* vp = sPk + s * kVP;
 #pragma unroll

@@ -118,7 +118,10 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *                                      arrays hold U itself.  With the case's free stream as the
  *                                      offset, FP32 storage resolves the flow's variation about
  *                                      the free stream instead of rounding the dimensional state
- *                                      (rho E ~ 2.5e5 rounds to 0.016).
+ *                                      (rho E ~ 2.5e5 rounds to 0.016).  Without cell_case the
+ *                                      one row is copied at afr_mesh_create (the kernels take it
+ *                                      by value); with cell_case the table is read by the kernels
+ *                                      and must stay valid and unchanged while the mesh exists.
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
