@@ -81,7 +81,9 @@ def p0_record_sources(mesh):
     interior record takes its neighbour; the primary record of a prism quad group takes the
     group's first neighbour with weight 1 when its triangle is the quad's lower half (the centre
     point lies on the diagonal, which belongs to the lower half) and the second record's neighbour
-    with the complementary weight; boundary and non-primary records have none (-1)."""
+    with the complementary weight; a quad held by one record (a collapsed-edge wedge's) takes its
+    neighbour with weight 1, as any single-source face; boundary and non-primary records have none
+    (-1)."""
     fr = mesh.faces
     F, P = fr.n_faces, mesh.n_prism
     src = np.full((F, 2), -1, dtype=np.int64)
@@ -103,7 +105,7 @@ def p0_record_sources(mesh):
         g0 = qs[start]
         low = _quad_lower(mesh, fr.owner[g0], fr.owner_code[g0], fr.node_ids[g0]).astype(np.float64)
         src[g0, 0] = fr.neighbor[g0]
-        wgt[g0, 0] = low
+        wgt[g0, 0] = np.where(size == 1, 1.0, low)
         two = size == 2
         g1 = qs[start[two] + 1]
         src[g0[two], 1] = fr.neighbor[g1]

@@ -98,6 +98,12 @@ __device__ __forceinline__ void sort3(int32_t v[3]) {
     if (v[1] < v[0]) { int32_t x = v[0]; v[0] = v[1]; v[1] = x; }
 }
 
+// a zero-area triangle of a collapsed-edge wedge (two equal node ids) is stored as this triple: it
+// sorts after every real triangle in both sort forms and is never a record
+constexpr int32_t kNoTri = 0x7fffffff;
+
+__device__ __forceinline__ bool no_tri(const int32_t* tri3, int32_t t) { return tri3[(int64_t)t * 3] == kNoTri; }
+
 // pass 0: sorted triples, keys (one-pass packed key, or the low pair of a two-pass sort)
 __global__ void __launch_bounds__(kThreads) k_tri_keys(TopoArgs a, int32_t* __restrict__ tri3,
                                                        uint64_t* __restrict__ key, int32_t* __restrict__ val,
@@ -106,6 +112,13 @@ __global__ void __launch_bounds__(kThreads) k_tri_keys(TopoArgs a, int32_t* __re
         int32_t v[3];
         tri_nodes(a, t, v);
         sort3(v);
+        if (v[0] == v[1] || v[1] == v[2]) {
+            // face_extraction_kernels.py:310-318: dropped
+            tri3[t * 3 + 0] = tri3[t * 3 + 1] = tri3[t * 3 + 2] = kNoTri;
+            key[t] = ~(uint64_t)0;
+            val[t] = (int32_t)t;
+            continue;
+        }
         tri3[t * 3 + 0] = v[0];
         tri3[t * 3 + 1] = v[1];
         tri3[t * 3 + 2] = v[2];
@@ -132,7 +145,7 @@ __global__ void __launch_bounds__(kThreads) k_run_flags(int64_t n, const int32_t
                                                         const int32_t* __restrict__ sorted_t,
                                                         int32_t* __restrict__ flag) {
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-        flag[i] = (i == 0 || !same3(tri3, sorted_t[i], sorted_t[i - 1])) ? 1 : 0;
+        flag[i] = (!no_tri(tri3, sorted_t[i]) && (i == 0 || !same3(tri3, sorted_t[i], sorted_t[i - 1]))) ? 1 : 0;
 }
 
 // cube-face code of the triangle t's node set on cell c; -1 unless exactly one face matches
@@ -174,10 +187,10 @@ __global__ void __launch_bounds__(kThreads) k_records(TopoArgs a, const double* 
         const int64_t owner = cell_of(a, t0, &j);
         int64_t neighbor = -1;
         int32_t t1 = -1;
-        if (i + 1 < n && !flag[i + 1]) {
+        if (i + 1 < n && !flag[i + 1] && !no_tri(tri3, sorted_t[i + 1])) {
             t1 = sorted_t[i + 1];
             neighbor = cell_of(a, t1, &j);
-            if (i + 2 < n && !flag[i + 2]) atomicOr(d.status, 1);
+            if (i + 2 < n && !flag[i + 2] && !no_tri(tri3, sorted_t[i + 2])) atomicOr(d.status, 1);
         }
         int32_t v[3];
         tri_nodes(a, t0, v);

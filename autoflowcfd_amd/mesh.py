@@ -307,21 +307,34 @@ def _resolve_codes(mesh: Mesh, cells: np.ndarray, tri: np.ndarray) -> np.ndarray
     return codes
 
 
+def collapsed_prisms(prism_conn: np.ndarray) -> np.ndarray:
+    """(n_prism,) bool: prisms with one side edge collapsed to a point (v_i == w_i for exactly one
+    i, every other vertex distinct): the wedges the reference's boundary-layer extrusion leaves
+    where one bottom vertex stops growing (face_extraction_kernels.py:310-318)."""
+    c = np.asarray(prism_conn).reshape(-1, 6)
+    return np.sum(c[:, :3] == c[:, 3:], axis=1) == 1
+
+
 def check_cells(mesh: Mesh) -> None:
-    """Refuse cells whose vertices repeat.  The reference's BL extrusion can collapse one side edge
-    of a prism into a point (a wedge with v_i == w_i), and its face extraction then drops the two
-    zero-area triangles (face_extraction_kernels.py:310-318); the slot tables and flux-point
-    location of this package assume five real faces per prism, so such a mesh is refused here
-    rather than given zero-area records.  Tets with a repeated vertex have no volume."""
+    """Refuse cells without volume: a tet with a repeated vertex, a prism with a repeated vertex
+    other than one collapsed side edge.  Collapsed-edge wedges are accepted: their two zero-area
+    triangles are dropped from the face records as the reference drops them, so the collapsed quad
+    keeps one triangle record and its slot one source."""
     for name, conn in (("prism", mesh.prism_conn), ("tet", mesh.tet_conn)):
         c = np.asarray(conn)
         if not c.size:
             continue
-        srt = np.sort(c.reshape(len(c), -1), axis=1)
-        bad = np.flatnonzero(np.any(srt[:, 1:] == srt[:, :-1], axis=1))
+        c = c.reshape(len(c), -1)
+        srt = np.sort(c, axis=1)
+        n_rep = np.sum(srt[:, 1:] == srt[:, :-1], axis=1)
+        ok = n_rep == 0
+        if name == "prism":
+            ok |= (n_rep == 1) & collapsed_prisms(c)
+        bad = np.flatnonzero(~ok)
         if len(bad):
-            raise ValueError(f"{len(bad)} {name} cell(s) with a repeated vertex (first: cell {int(bad[0])}, "
-                             f"nodes {c[bad[0]].tolist()}): collapsed cells are not supported")
+            raise ValueError(f"{len(bad)} {name} cell(s) with repeated vertices (first: cell {int(bad[0])}, "
+                             f"nodes {c[bad[0]].tolist()}): only a prism with one collapsed side edge "
+                             "(v_i == w_i) is accepted")
 
 
 def topology_of(mesh: Mesh, topology="host") -> FaceRecords:
@@ -356,8 +369,12 @@ def build_topology(mesh: Mesh) -> FaceRecords:
     tris = []
     owners = []
     if mesh.n_prism:
-        tris.append(_prism_triangles(mesh.prism_conn))
-        owners.append(np.repeat(np.arange(mesh.n_prism), 8))
+        pt = _prism_triangles(mesh.prism_conn)
+        po = np.repeat(np.arange(mesh.n_prism), 8)
+        # the zero-area triangles of collapsed-edge wedges are dropped (face_extraction_kernels.py:310-318)
+        keep = (pt[:, 0] != pt[:, 1]) & (pt[:, 0] != pt[:, 2]) & (pt[:, 1] != pt[:, 2])
+        tris.append(pt[keep])
+        owners.append(po[keep])
     if mesh.n_tet:
         tris.append(_tet_triangles(mesh.tet_conn))
         owners.append(np.repeat(np.arange(mesh.n_tet) + mesh.n_prism, 4))

@@ -226,7 +226,11 @@ class FlatFaces:
 def _side_sources(mesh, ops, fr, self_cells, self_codes, faces_idx, role, groups):
     """Sources for the native FP grid of ``self`` sides of records ``faces_idx``.
 
-    role 'owner': self = owner, other = neighbour (nb_src); 'neighbor' the reverse.
+    role 'owner': self = owner, other = neighbour (nb_src); 'neighbor' the reverse.  A prism quad
+    held by two records takes each record's diagonal half from that record's other cell
+    (face_flux_points_merge.py:251-276); a quad held by one record (a collapsed-edge wedge's, whose
+    other triangle has no area) takes every point from it, as any single-source face
+    (face_flux_points_merge.py:99-110, 168-200).
     Returns src0_cell, src0_mat, list of (face, cell, mat) for source 1.
     """
     n1d, nfp, nsp = ops.n1d, ops.n_fp, ops.n_sps
@@ -247,14 +251,13 @@ def _side_sources(mesh, ops, fr, self_cells, self_codes, faces_idx, role, groups
     tasks = []  # (face, fp_mask, target_cell, target_code)
     for f in faces_idx:
         sc, code = int(self_cells[f]), int(self_codes[f])
-        if sc < mesh.n_prism and code in PRISM_QUAD_CODES:
+        if sc < mesh.n_prism and code in PRISM_QUAD_CODES and len(groups[(sc, code)]) > 1:
             grp = sorted(groups[(sc, code)])
             half, std = _quad_half(mesh, sc, code, fr.node_ids[grp[0]])
             low = lower_std if std else lower_flip
             m0 = low if half == "lower" else ~low
             tasks.append((f, m0, int(other_cell[grp[0]]), int(other_code[grp[0]]), 0))
-            if len(grp) > 1:
-                tasks.append((f, ~m0, int(other_cell[grp[1]]), int(other_code[grp[1]]), 1))
+            tasks.append((f, ~m0, int(other_cell[grp[1]]), int(other_code[grp[1]]), 1))
         else:
             tasks.append((f, np.ones(nfp, dtype=bool), int(other_cell[f]), int(other_code[f]), 0))
 

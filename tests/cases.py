@@ -84,3 +84,25 @@ def smooth_state(xyz, Q0, amp=1e-2, seed=0):
         ph = rng.uniform(0, 2 * np.pi)
         Q[..., v] += amp * scale[v] * np.sin(2 * np.pi * (x @ k) + ph)
     return prim_to_cons(Q)
+
+
+def wedge_box(n=(4, 3, 3), seed=1):
+    """A graded wall box whose first prism layer has one side edge collapsed to a point: the top
+    node above an interior wall vertex is merged into it, so the prisms around that edge become
+    wedges with v_i == w_i (the cells the reference's boundary-layer extrusion leaves where a layer
+    stops growing) and the prisms above them are stretched down to the wall.  Records on y = 0 are
+    WALL, every other boundary record FARFIELD."""
+    from autoflowcfd_amd.mesh import BC_FARFIELD, mesh_from_cells
+    src = wall_box(n=n, seed=seed)
+    pc = src.prism_conn.copy()
+    y = src.nodes[:, 1]
+    lo, hi = src.nodes.min(axis=0), src.nodes.max(axis=0)
+    edges = sorted({(int(v), int(w)) for c in pc for v, w in zip(c[:3], c[3:]) if abs(y[v]) < 1e-12 and y[w] > 0
+                    and lo[0] < src.nodes[v, 0] < hi[0] and lo[2] < src.nodes[v, 2] < hi[2]})
+    v, w = edges[len(edges) // 2]
+    m = mesh_from_cells(src.nodes, np.where(pc == w, v, pc), default_bc=BC_FARFIELD)
+    fr = m.faces
+    on_wall = fr.is_boundary & (np.abs(fr.center[:, 1]) < 1e-12)
+    fr.bc[on_wall] = BC_WALL
+    m.wall_nodes = np.unique(fr.node_ids[on_wall])
+    return m

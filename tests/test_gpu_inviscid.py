@@ -8,7 +8,7 @@ the oracle evaluated on the same FP32 state.
 import numpy as np
 import pytest
 
-from cases import freestream, mixed_mesh, rotated_prism_box, smooth_state, tet_box, wall_box
+from cases import freestream, mixed_mesh, rotated_prism_box, smooth_state, tet_box, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +50,7 @@ CASES = [
     ("wallbox", lambda: wall_box(seed=2), 1), ("wallbox", lambda: wall_box(seed=2), 2),
     ("periodic", lambda: tet_box(3, seed=3, displacement=0.0, periodic=True), 2),
     ("rotprisms", rotated_prism_box, 2),
+    ("wedges", wedge_box, 1), ("wedges", wedge_box, 2),
 ]
 
 
@@ -62,7 +63,7 @@ def test_inviscid_fp64_matches_oracle(name, make, order):
     assert err < tol, f"{name} P{order}: rel L2 {err:.3e}"
 
 
-@pytest.mark.parametrize("name,make,order", [c for c in CASES if c[0] in ("wallbox", "mixed", "rotprisms") and not (c[0] == "mixed" and c[2] == 2)])
+@pytest.mark.parametrize("name,make,order", [c for c in CASES if c[0] in ("wallbox", "mixed", "rotprisms", "wedges") and not (c[0] == "mixed" and c[2] == 2)])
 def test_inviscid_fp32_matches_oracle(name, make, order):
     import torch
     print(f"[gpu] inviscid fp32 {name} P{order}", flush=True)

@@ -6,12 +6,12 @@ interpolation rows themselves, at P1 and P2.
 Meshes: graded prism wall boxes (permuted), displaced tet boxes, periodic tet and x-extruded prism
 boxes (corner correspondence by position), the mixed prism / tet mesh, rotated prisms, the odd
 23-prism mesh, a prism quad split between two tets (the split-slot path), a mesh with boundary-group
-rows, and a 30^3-hex wall box."""
+rows, collapsed-edge wedges (a quad face held by one triangle record), and a 30^3-hex wall box."""
 
 import numpy as np
 import pytest
 
-from cases import mixed_mesh, odd_prism_mesh, rotated_prism_box, tet_box, wall_box
+from cases import mixed_mesh, odd_prism_mesh, rotated_prism_box, tet_box, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +41,8 @@ def _mesh(kind):
         return odd_prism_mesh()
     if kind == "split_quad":
         return _split_quad_mesh()
+    if kind == "wedges":
+        return wedge_box()
     if kind == "group_rows":
         m = wall_box(n=(3, 3, 2), seed=5)
         m.faces.bc_row = np.where(m.faces.neighbor < 0, np.arange(m.faces.n_faces) % 8, -1)
@@ -51,7 +53,7 @@ def _mesh(kind):
 
 @pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("kind", ["wallbox", "tets", "tets_periodic", "prisms_periodic_x", "mixed", "rotated",
-                                  "odd", "split_quad", "group_rows", "wallbox30"])
+                                  "odd", "split_quad", "group_rows", "wallbox30", "wedges"])
 def test_device_slots_equal_host(kind, order):
     from autoflowcfd_amd.face_slots import build_slots
     from autoflowcfd_amd.fr_tables import tables

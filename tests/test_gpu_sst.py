@@ -11,7 +11,7 @@ norm 1e-5.
 import numpy as np
 import pytest
 
-from cases import WALL_BOX_BC, freestream, odd_prism_mesh, smooth_state, wall_box
+from cases import WALL_BOX_BC, freestream, odd_prism_mesh, smooth_state, wall_box, wedge_box
 from autoflowcfd_amd.mesh import BC_WALL, hex_box
 
 pytestmark = pytest.mark.gpu
@@ -23,7 +23,7 @@ def _tet_wall_box():
 
 
 CASES = [("wallbox", lambda: wall_box(seed=2), 1), ("wallbox", lambda: wall_box(seed=2), 2),
-         ("tetwall", _tet_wall_box, 1), ("oddprisms", odd_prism_mesh, 2)]
+         ("tetwall", _tet_wall_box, 1), ("oddprisms", odd_prism_mesh, 2), ("wedges", wedge_box, 2)]
 
 
 def _fields(g, seed):

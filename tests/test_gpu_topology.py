@@ -4,13 +4,14 @@ node triples exactly, the FP64 normals, areas and centres as the same correctly 
 
 Meshes: permuted graded prism boxes, displaced tet boxes, the mixed prism/tet mesh of the
 reference's residual tests, a periodic box through hex_box (boundary classification and pairing
-on top of either topology), node ids above 2^21 (the two-pass sort), and a face shared by three
-cells (refused)."""
+on top of either topology), node ids above 2^21 (the two-pass sort), collapsed-edge wedges (their
+zero-area triangles dropped, also with ids above 2^21), and a face shared by three cells
+(refused)."""
 
 import numpy as np
 import pytest
 
-from cases import mixed_mesh, tet_box, wall_box
+from cases import mixed_mesh, tet_box, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +35,7 @@ def _shifted(mesh, shift):
     return Mesh(nodes=nodes, prism_conn=mesh.prism_conn + shift, tet_conn=mesh.tet_conn + shift)
 
 
-@pytest.mark.parametrize("kind", ["wallbox", "tets", "mixed", "big_ids", "wallbox30"])
+@pytest.mark.parametrize("kind", ["wallbox", "tets", "mixed", "big_ids", "wallbox30", "wedges", "wedges_big_ids"])
 def test_device_topology_equals_host(kind):
     from autoflowcfd_amd.mesh import build_topology, hex_box
     from autoflowcfd_amd.topology import build_topology_gpu
@@ -47,6 +48,10 @@ def test_device_topology_equals_host(kind):
         mesh = mixed_mesh()
     elif kind == "big_ids":
         mesh = _shifted(wall_box(n=(3, 3, 2), seed=1), 2 ** 21 + 5)
+    elif kind == "wedges":
+        mesh = wedge_box()
+    elif kind == "wedges_big_ids":
+        mesh = _shifted(wedge_box(n=(3, 3, 2)), 2 ** 21 + 5)
     else:
         mesh = hex_box(30, 30, 30, 4.0, 1.0, 1.0, split="prism", grading_y=1.1, seed=7)
     _same(build_topology_gpu(mesh), build_topology(mesh))

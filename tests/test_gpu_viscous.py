@@ -8,7 +8,7 @@ rel. L2 <= 1e-5 against the oracle evaluated on the same FP32-rounded state.
 import numpy as np
 import pytest
 
-from cases import freestream, mixed_mesh, rotated_prism_box, smooth_state, tet_box, wall_box
+from cases import freestream, mixed_mesh, rotated_prism_box, smooth_state, tet_box, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -60,6 +60,7 @@ CASES = [
     ("mixed", mixed_mesh, 1), ("mixed", mixed_mesh, 2),
     ("tetbox", lambda: tet_box(3, seed=1), 1), ("tetbox", lambda: tet_box(3, seed=1), 2),
     ("rotprisms", rotated_prism_box, 1), ("rotprisms", rotated_prism_box, 2),
+    ("wedges", wedge_box, 1), ("wedges", wedge_box, 2),
 ]
 
 
@@ -73,7 +74,7 @@ def test_viscous_fp64_matches_oracle(name, make, order):
     assert ef < tol, f"{name} P{order}: full residual rel L2 {ef:.3e}"
 
 
-@pytest.mark.parametrize("name,make,order", [c for c in CASES if c[0] in ("wallbox", "rotprisms")] + [CASES[2]])
+@pytest.mark.parametrize("name,make,order", [c for c in CASES if c[0] in ("wallbox", "rotprisms", "wedges")] + [CASES[2]])
 def test_viscous_fp32_matches_oracle(name, make, order):
     import torch
     print(f"[gpu] viscous fp32 {name} P{order}", flush=True)

@@ -143,15 +143,15 @@ def test_p0_transient_amplifies_round_off():
     assert rel[-1] > 1e-8
 
 
-@pytest.mark.parametrize("kind", ["wallbox", "mixed", "rotated", "tets"])
+@pytest.mark.parametrize("kind", ["wallbox", "mixed", "rotated", "tets", "wedges"])
 def test_p0_record_sources_equal_the_oracle_flat_faces(kind):
     """The product's P0 neighbour sources (continuation.p0_record_sources: one flux point per record,
     a prism quad's centre on its diagonal) equal the oracle's flat faces built with the one-point
     operators: source cells, and weights equal to the oracle's 1 x 1 interpolation rows."""
     from autoflowcfd_amd.continuation import p0_record_sources
-    from cases import mixed_mesh, rotated_prism_box, tet_box
+    from cases import mixed_mesh, rotated_prism_box, tet_box, wedge_box
     mesh = {"wallbox": lambda: wall_box(seed=3), "mixed": mixed_mesh, "rotated": lambda: rotated_prism_box(seed=1),
-            "tets": lambda: tet_box(3, seed=2)}[kind]()
+            "tets": lambda: tet_box(3, seed=2), "wedges": wedge_box}[kind]()
     fl = build_flat_faces(mesh, fr_ops.generate_operators(0))
     src, wgt = p0_record_sources(mesh)
     assert np.array_equal(src[:, 0], fl.nb_src0_cell)

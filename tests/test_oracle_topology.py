@@ -11,7 +11,7 @@ of the prism quads, owner and neighbour swapped, the last group not winning) is 
 import numpy as np
 import pytest
 
-from cases import mixed_mesh, odd_prism_mesh, rotated_prism_box, tet_box, wall_box
+from cases import mixed_mesh, odd_prism_mesh, rotated_prism_box, tet_box, wall_box, wedge_box
 
 
 def _same(prod, ref):
@@ -27,6 +27,7 @@ MESHES = {
     "mixed": mixed_mesh,
     "rotated": lambda: rotated_prism_box(seed=4),
     "odd": odd_prism_mesh,
+    "wedges": wedge_box,
 }
 
 
@@ -136,10 +137,11 @@ def test_oracle_records_close_every_cell(kind):
     rec = face_records(mesh.nodes, mesh.prism_conn, mesh.tet_conn)
     scale = float(rec["area"].max())
     assert np.abs(_closure(rec, mesh.n_cells)).max() <= 1e-13 * scale
-    # a prism has 8 triangle records (2 per quad), a tet 4; each cube face code of a tet once
+    # a prism has 8 triangle records (2 per quad; 6 for a collapsed-edge wedge), a tet 4
     count = np.bincount(rec["owner"], minlength=mesh.n_cells) + np.bincount(
         rec["neighbor"][rec["neighbor"] >= 0], minlength=mesh.n_cells)
-    assert np.all(count[:mesh.n_prism] == 8) and np.all(count[mesh.n_prism:] == 4)
+    wedge = np.sum(mesh.prism_conn[:, :3] == mesh.prism_conn[:, 3:], axis=1) == 1
+    assert np.all(count[:mesh.n_prism] == np.where(wedge, 6, 8)) and np.all(count[mesh.n_prism:] == 4)
     # planted: one record's normal turned round breaks the closure of its cells
     bad = dict(rec)
     bad["normal"] = rec["normal"].copy()
@@ -147,15 +149,21 @@ def test_oracle_records_close_every_cell(kind):
     assert np.abs(_closure(bad, mesh.n_cells)).max() > 1e-3 * scale
 
 
-def test_collapsed_wedge_is_refused_by_the_product_and_dropped_by_the_oracle():
+def test_collapsed_wedge_records():
     """A prism whose side edge v2-w2 collapses to a point: the reference drops its two zero-area
-    triangles (6 records), the oracle does the same, the product refuses the mesh."""
+    triangles (6 records), the oracle and the product do the same; other repeated vertices (a
+    degenerate cap, two collapsed edges, a tet with a repeated vertex) are refused."""
     from autoflowcfd_amd.mesh import Mesh, build_topology
     from oracle.topology import face_records
-    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1]], dtype=float)
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0.3, 0.3, 1.0]], dtype=float)
+    none = np.zeros((0, 4), np.int64)
     conn = np.array([[0, 1, 2, 3, 4, 2]])
-    rec = face_records(nodes, conn, np.zeros((0, 4), np.int64))
+    rec = face_records(nodes, conn, none)
     assert len(rec["owner"]) == 6 and np.all(rec["area"] > 0)
     assert np.abs(_closure(rec, 1)).max() <= 1e-15
-    with pytest.raises(ValueError, match="repeated vertex"):
-        build_topology(Mesh(nodes=nodes, prism_conn=conn, tet_conn=np.zeros((0, 4), np.int64)))
+    _same(build_topology(Mesh(nodes=nodes, prism_conn=conn, tet_conn=none)), rec)
+    for bad_p, bad_t in (([[0, 1, 1, 3, 4, 5]], None), ([[0, 1, 2, 0, 4, 2]], None), (None, [[0, 1, 2, 2]])):
+        m = Mesh(nodes=nodes, prism_conn=np.array(bad_p if bad_p else np.zeros((0, 6), np.int64)),
+                 tet_conn=np.array(bad_t if bad_t else none))
+        with pytest.raises(ValueError, match="repeated vertices"):
+            build_topology(m)
