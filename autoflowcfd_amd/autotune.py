@@ -22,7 +22,19 @@ from . import _lib
 
 KERNELS = {"prism_inviscid": 0, "tet_inviscid": 1, "visc_volume": 2, "visc_iface": 3, "sst_cell": 4,
            "sst_face": 5, "sst_update": 6}
-CANDIDATES = (0, 256, 512, 1024, 2048, 4096, 8192, 16384)   # workgroups; 0 = built-in default
+# workgroups per launch, as multiples of the compute-unit count: a kernel whose grid equals the
+# workgroups resident at once (CUs x resident workgroups per CU, e.g. 6 x 256 = 1536 on a 256-CU
+# device at 6 waves per SIMD) walks each XCD's cell range in one contiguous sweep, so neighbour
+# data is reused from L2 as the sweep passes; larger grids hide latency better.  0 = built-in default.
+CU_MULTIPLES = (1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32, 64, 96)
+
+
+def candidates_for(device=None):
+    """Grid candidates of this device: 0 and CU_MULTIPLES x its compute-unit count."""
+    n_cu = torch.cuda.get_device_properties(device if device is not None else torch.cuda.current_device()) \
+        .multi_processor_count
+    return (0,) + tuple(int(n_cu) * k for k in CU_MULTIPLES)
+
 VARIANTS = {"prism_inviscid": (0, 1), "sst_cell": (0, 1)}   # tile shapes (afr_mesh_set_variant)
 
 _memo = {}
@@ -71,7 +83,7 @@ def _time(fn, reps):
     return ev[0].elapsed_time(ev[1]) / reps
 
 
-def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CANDIDATES, cache_path=None,
+def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=None, cache_path=None,
              reuse=True):
     """Pick the grid size of every residual kernel present on this mesh.
 
@@ -81,6 +93,8 @@ def autotune(dm, U, mu_t=None, mu=1.8e-5, sst_fields=None, reps=3, candidates=CA
     (False times every candidate again and replaces the stored choice).  Returns
     {kernel: {"grid": chosen, "variant": v, "ms": {candidate: time} or None when reused}}; the
     choice stays set on the mesh."""
+    if candidates is None:
+        candidates = candidates_for(dm.device)
     cache = {}
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:

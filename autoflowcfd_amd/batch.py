@@ -171,13 +171,13 @@ class CaseBatch:
         precision and size) on copies of the state: the batch's own fields are not touched, and
         results do not depend on the choice.  Call before capture_graph (a graph keeps the launch
         shapes it was captured with)."""
-        from .autotune import CANDIDATES, autotune
+        from .autotune import autotune
         U = self.U.clone()
         sst = None
         if self.turb == "SST":
             sst = tuple(x.clone() for x in (self.k, self.omega, self.nu_t, self.mu_t))
         return autotune(self.dm, U, None if sst is None else sst[3], mu=self.mu, sst_fields=sst, reps=reps,
-                        candidates=candidates or CANDIDATES)
+                        candidates=candidates)
 
     def capture_graph(self):
         """Record one batched pseudo-step into a HIP graph (after one eager warm-up step, counted)."""
