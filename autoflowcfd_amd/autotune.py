@@ -26,7 +26,7 @@ KERNELS = {"prism_inviscid": 0, "tet_inviscid": 1, "visc_volume": 2, "visc_iface
 # workgroups resident at once (CUs x resident workgroups per CU, e.g. 6 x 256 = 1536 on a 256-CU
 # device at 6 waves per SIMD) walks each XCD's cell range in one contiguous sweep, so neighbour
 # data is reused from L2 as the sweep passes; larger grids hide latency better.  0 = built-in default.
-CU_MULTIPLES = (1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32, 64, 96)
+CU_MULTIPLES = (1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32, 64, 96)
 
 
 def candidates_for(device=None):
