@@ -58,32 +58,36 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
 }
 
 // U_out = c0 U0 + c1 Ua + c2 dt R, then positivity (rho >= 1e-6, |u| <= 1e4,
-// p >= p_floor) per solution point and the modal filter per cell.  One
-// wavefront per cell; Ua may alias U_out, U0 may alias U_out (each cell is
-// read completely before it is written).  With a state offset the three states
-// are offsets from the cell's reference: the stage combination is the same (c0 +
-// c1 = 1), the positivity test sees the absolute state, a projected component
-// is stored back as its offset, and the filter (which keeps constants) acts on
-// the offset.
+// p >= p_floor) per solution point and the modal filter per cell.  A workgroup
+// takes CPB = 256 / NS consecutive cells at a time: one thread per solution point
+// forms the stage value and its projection into LDS, then one thread per output
+// value applies the cell's filter row and writes it (consecutive threads,
+// consecutive addresses).  Ua may alias U_out, U0 may alias U_out (the cells of a
+// pass are read completely before any of them is written).  With a state offset
+// the three states are offsets from the cell's reference: the stage combination
+// is the same (c0 + c1 = 1), the positivity test sees the absolute state, a
+// projected component is stored back as its offset, and the filter (which keeps
+// constants) acts on the offset.
 template <int NS, typename real>
 __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const real* Ua, const real* __restrict__ R,
                                                        const real* __restrict__ dt, real* U_out, StateOff so,
                                                        const real* __restrict__ filt0, const real* __restrict__ filt1,
                                                        int64_t n_cells, int64_t n_prism, double c0, double c1,
                                                        double c2, double p_floor) {
-    constexpr int W = kThreads / 64, NSV = NS * 5;
+    constexpr int NSV = NS * 5, CPB = kThreads / NS;
     __shared__ real s_f[2][NS * NS];
-    __shared__ real s_x[W][NSV];
+    __shared__ real s_x[CPB * NSV];
     for (int i = threadIdx.x; i < NS * NS; i += kThreads) {
         s_f[0][i] = filt0[i];
         s_f[1][i] = filt1[i];
     }
     __syncthreads();
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
-    real* x = s_x[wid];
-    for (int64_t cell = (int64_t)blockIdx.x * W + wid; cell < n_cells; cell += (int64_t)gridDim.x * W) {
-        if (t < NS) {
-            const int64_t i = cell * NS + t;
+    const int t = threadIdx.x;
+    for (int64_t cb = (int64_t)blockIdx.x * CPB; cb < n_cells; cb += (int64_t)gridDim.x * CPB) {
+        const int nc = (int)(n_cells - cb < CPB ? n_cells - cb : CPB);
+        if (t < nc * NS) {
+            const int64_t cell = cb + t / NS;
+            const int64_t i = cb * NS + t;
             const double d = (double)dt[i];
             double q[5], qa[5], off[5];
 #pragma unroll
@@ -111,25 +115,23 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
             if (low_p) E = p_floor / 0.4 + ke;
             // a component the projection left alone is stored as it was (exact); a projected one
             // as its (absolute) value less the offset
-            x[t * 5 + 0] = low_rho ? (real)(rho - off[0]) : (real)q[0];
-            x[t * 5 + 1] = fast ? (real)(m1 - off[1]) : (real)q[1];
-            x[t * 5 + 2] = fast ? (real)(m2 - off[2]) : (real)q[2];
-            x[t * 5 + 3] = fast ? (real)(m3 - off[3]) : (real)q[3];
-            x[t * 5 + 4] = low_p ? (real)(E - off[4]) : (real)q[4];
+            real* x = s_x + t * 5;
+            x[0] = low_rho ? (real)(rho - off[0]) : (real)q[0];
+            x[1] = fast ? (real)(m1 - off[1]) : (real)q[1];
+            x[2] = fast ? (real)(m2 - off[2]) : (real)q[2];
+            x[3] = fast ? (real)(m3 - off[3]) : (real)q[3];
+            x[4] = low_p ? (real)(E - off[4]) : (real)q[4];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const real* F = s_f[cell < n_prism ? 0 : 1];
-        for (int o = t; o < NSV; o += 64) {
-            const int s = o / 5, v = o % 5;
+        __syncthreads();
+        for (int o = t; o < nc * NSV; o += kThreads) {
+            const int cc = o / NSV, r = o % NSV, s = r / 5, v = r % 5;
+            const real* F = s_f[cb + cc < n_prism ? 0 : 1];
+            const real* x = s_x + cc * NSV;
             real acc = 0;
             for (int j = 0; j < NS; ++j) acc += F[s * NS + j] * x[j * 5 + v];
-            U_out[cell * NSV + o] = acc;
+            U_out[cb * NSV + o] = acc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __syncthreads();
     }
 }
 
@@ -186,7 +188,7 @@ template <typename real>
 int launch_rk_stage(int ns, const real* U0, const real* Ua, const real* R, const real* dt, real* U_out,
                     StateOff so, const real* filt0, const real* filt1, int64_t n_cells, int64_t n_prism,
                     double c0, double c1, double c2, double p_floor, hipStream_t st) {
-    const int grid = grid_for(n_cells, kThreads / 64);
+    const int grid = grid_for(n_cells, ns > 0 ? kThreads / ns : 1);
     if (ns == 8)
         hipLaunchKernelGGL((k_rk_stage<8, real>), dim3(grid), dim3(kThreads), 0, st, U0, Ua, R, dt, U_out, so,
                            filt0, filt1, n_cells, n_prism, c0, c1, c2, p_floor);
