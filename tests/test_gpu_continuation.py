@@ -14,7 +14,7 @@ oracle (oracle/continuation.py).
 import numpy as np
 import pytest
 
-from cases import freestream, mixed_mesh, prim_to_cons, wall_box
+from cases import freestream, mixed_mesh, prim_to_cons, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -42,14 +42,14 @@ def _state(mesh, Q0, seed=1):
     return prim_to_cons(Q)[:, None, :]
 
 
-@pytest.mark.parametrize("kind", ["wallbox", "mixed"])
+@pytest.mark.parametrize("kind", ["wallbox", "mixed", "wedges"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
 def test_p0_residual_matches_oracle(prec, tol, kind):
     import torch
     from oracle import continuation as OC
     dtype = torch.float64 if prec == "f64" else torch.float32
     print(f"[gpu] P0 residual {prec} {kind} vs oracle", flush=True)
-    mesh = wall_box(seed=5) if kind == "wallbox" else mixed_mesh()
+    mesh = {"wallbox": lambda: wall_box(seed=5), "mixed": mixed_mesh, "wedges": wedge_box}[kind]()
     Q0 = freestream(0.2, 5.0)
     dev, g0, fl, bcd = _p0_setup(mesh, Q0, dtype)
     U = _state(mesh, Q0)
@@ -171,8 +171,9 @@ def test_run_steady_runs_the_continuation_for_laminar_p2():
     assert res_sst.final_residual == out_sst["final_residual"]
 
 
+@pytest.mark.parametrize("kind", ["wallbox", "wedges"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
-def test_p0_sst_step_matches_oracle(prec, tol):
+def test_p0_sst_step_matches_oracle(prec, tol, kind):
     """One P0 pseudo-step with SST (afr_p0_sst_step) against the oracle's general turbulence step
     with the one-point operators and the P0 stages: k, omega, nu_t, the state and the norm, on a
     perturbed wall-box state with non-uniform k, omega (so that convection, diffusion, the WALL
@@ -184,9 +185,9 @@ def test_p0_sst_step_matches_oracle(prec, tol):
     from oracle import fr_ops, sst
     from oracle.faces import build_flat_faces
     from oracle.geometry import Geometry
-    print(f"[gpu] P0 SST step {prec} vs oracle", flush=True)
+    print(f"[gpu] P0 SST step {prec} {kind} vs oracle", flush=True)
     dtype = torch.float64 if prec == "f64" else torch.float32
-    mesh = wall_box(seed=7)
+    mesh = wall_box(seed=7) if kind == "wallbox" else wedge_box()
     Q0 = freestream(0.2, 6.0)
     bcp = bc_params_table(Q0)
     bcd = {c: bcp[c] for c in range(8)}

@@ -19,7 +19,7 @@ AfrMeshDesc.state_ref / AfrP0Desc.state_ref), the storage of every FP32 solver.
 import numpy as np
 import pytest
 
-from cases import freestream, prim_to_cons, smooth_state, wall_box
+from cases import freestream, mixed_mesh, prim_to_cons, smooth_state, tet_box, wall_box, wedge_box
 
 pytestmark = pytest.mark.gpu
 
@@ -133,16 +133,17 @@ def test_p0_phase_equal_on_offset_storage():
         assert _rel(x.cpu(), y.cpu()) < 1e-9
 
 
-@pytest.mark.parametrize("order", [1, 2])
-def test_fp32_offset_step_matches_oracle(order):
+@pytest.mark.parametrize("kind,order", [("wallbox", 1), ("wallbox", 2), ("tets", 1), ("mixed", 2), ("wedges", 2)])
+def test_fp32_offset_step_matches_oracle(kind, order):
     import torch
     from autoflowcfd_amd.device import DeviceMesh, bc_params_table
     from oracle import fr_ops
     from oracle.faces import build_flat_faces
     from oracle.geometry import Geometry
     from oracle.time_step import ns_step
-    print(f"[gpu] offset storage FP32 pseudo-step vs oracle P{order}", flush=True)
-    mesh = wall_box(seed=2)
+    print(f"[gpu] offset storage FP32 pseudo-step vs oracle {kind} P{order}", flush=True)
+    mesh = {"wallbox": lambda: wall_box(seed=2), "tets": lambda: tet_box(3, seed=1), "mixed": mixed_mesh,
+            "wedges": wedge_box}[kind]()
     Q0 = freestream(0.15)
     bcp = bc_params_table(Q0)
     ops = fr_ops.generate_operators(order)
@@ -156,7 +157,8 @@ def test_fp32_offset_step_matches_oracle(order):
     inc = dm.to_absolute(U).cpu().numpy() - Urep
     print(f"[gpu] P{order} norm rel {abs(norm - norm_ref) / norm_ref:.3e}, increment rel "
           f"{_rel(inc, U_ref - Urep):.3e}", flush=True)
-    assert abs(norm - norm_ref) <= 1e-5 * norm_ref
+    tol = 1e-4 if kind == "mixed" else 1e-5     # P2 tets: their operators amplify FP32 rounding
+    assert abs(norm - norm_ref) <= tol * norm_ref
     assert _rel(inc, U_ref - Urep) < 1e-3
 
 
