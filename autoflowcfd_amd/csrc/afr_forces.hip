@@ -157,5 +157,8 @@ template int launch_wall_forces<1, float>(const ForceArgs<float>&, hipStream_t);
 template int launch_wall_forces<2, float>(const ForceArgs<float>&, hipStream_t);
 template int launch_wall_forces<1, double>(const ForceArgs<double>&, hipStream_t);
 template int launch_wall_forces<2, double>(const ForceArgs<double>&, hipStream_t);
+// P3 (tet meshes, FP64 storage)
+template int launch_wall_forces_records<3, double>(const ForceArgs<double>&, hipStream_t);
+template int launch_wall_forces<3, double>(const ForceArgs<double>&, hipStream_t);
 
 }  // namespace afr_forces

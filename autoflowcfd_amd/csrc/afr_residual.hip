@@ -1102,7 +1102,8 @@ int afr_sst_step(const AfrMesh* mesh, const void* U, void* k, void* omega, const
 // [wall_off[i], wall_off[i+1]) into out[6 i .. 6 i + 5]
 static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, double mu, bool per_case, double* out,
                        void* stream) {
-    if (mesh->d.order > 2) return fail("afr_forces_f64: wall forces are supported at orders 1 and 2");
+    if (mesh->d.order > 3 || (mesh->d.order == 3 && mesh->d.dtype != AFR_F64))
+        return fail("afr_forces_f64: wall forces are supported at orders 1 and 2, and at 3 in FP64");
     const int64_t* wall_off = per_case ? mesh->case_wall_off.data() : nullptr;
     const int n_cases = mesh->d.n_cases;
     if (per_case && mesh->case_wall_off.empty())
@@ -1133,12 +1134,18 @@ static int wall_forces(const AfrMesh* mesh, const void* U, const void* mu_t, dou
         a.mu = mu;
         a.part = d.work_fpart;
         a.out = out;
-        if (!wall_off)
-            return d.order == 1 ? afr_forces::launch_wall_forces<1, real>(a, st)
-                                : afr_forces::launch_wall_forces<2, real>(a, st);
-        if (d.order == 1 ? afr_forces::launch_wall_forces_records<1, real>(a, st)
-                         : afr_forces::launch_wall_forces_records<2, real>(a, st))
-            return -1;
+        auto records = [&]() {
+            if constexpr (std::is_same<real, double>::value) {
+                if (d.order == 3) return afr_forces::launch_wall_forces_records<3, double>(a, st);
+            }
+            return d.order == 1 ? afr_forces::launch_wall_forces_records<1, real>(a, st)
+                                : afr_forces::launch_wall_forces_records<2, real>(a, st);
+        };
+        if (!wall_off) {
+            if (records()) return -1;
+            return afr_forces::launch_sum_forces(a.part, a.n_wall, a.out, st);
+        }
+        if (records()) return -1;
         for (int i = 0; i < n_cases; ++i)
             if (afr_forces::launch_sum_forces(d.work_fpart + wall_off[i] * 6, wall_off[i + 1] - wall_off[i],
                                               out + 6 * i, st))

@@ -154,3 +154,29 @@ def test_c_abi_graph_replays_equal_eager_steps():
         assert torch.equal(UA, UB), r
         assert torch.equal(dm.norm_dev, nB), r
     graph.close()
+
+
+def test_p3_wall_forces_match_oracle():
+    """P3 (FP64, tets): the wall-force integral of a graded tet box with a no-slip wall at y = 0
+    against the oracle's: pressure to 1e-9 of its largest component (measured 9e-11), viscous to
+    1e-3 (measured 2.6e-4).  The shipped P3 tables and the oracle's are separate FP64 LU solves of
+    the same ill-conditioned collapsed basis that agree to 1e-6 of their largest entry
+    (tests/test_oracle_operators.py), and the tet derivative rows, which reach 6e7, carry that
+    into the wall velocity gradients (the P3 viscous residual's tolerance is 2e-4 for the same
+    reason)."""
+    import torch
+    from autoflowcfd_amd.mesh import hex_box
+    from cases import WALL_BOX_BC
+    from oracle.forces import wall_forces
+    print("[gpu] wall forces f64 P3 tets", flush=True)
+    mesh = hex_box(3, 2, 2, 2.0, 1.0, 1.0, split="tet", grading_y=1.2, seed=4, displacement=0.05,
+                   bc_by_plane=WALL_BOX_BC)
+    dm, ops, g, fl, bcd, U = _setup(mesh, 3, torch.float64)
+    mut = 1.8e-5 * np.random.default_rng(9).uniform(0, 20, size=U.shape[:2])
+    fp, fv = wall_forces(U, g, ops, fl, 1.8e-5, mu_t=mut)
+    out = dm.wall_forces(torch.as_tensor(U, device="cuda").contiguous(),
+                         torch.as_tensor(mut, device="cuda").contiguous()).cpu().numpy()
+    print(f"[gpu] P3 forces: pressure rel {np.abs(out[:3] - fp).max() / np.abs(fp).max():.3e}, viscous rel "
+          f"{np.abs(out[3:] - fv).max() / np.abs(fv).max():.3e}", flush=True)
+    assert np.abs(out[:3] - fp).max() <= 1e-9 * np.abs(fp).max()
+    assert np.abs(out[3:] - fv).max() <= 1e-3 * np.abs(fv).max()
