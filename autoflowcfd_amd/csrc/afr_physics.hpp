@@ -86,15 +86,65 @@ __host__ __device__ __forceinline__ real rmin(real a, real b) { return a < b ? a
 // registers); a batched mesh reads the row of the cell's case from a device table.  Without an
 // offset the storage holds the state itself and the load is a plain widening (bit for bit the same
 // values as before the offset existed).
+typedef const __attribute__((address_space(4))) int* ConstIntPtr;       // read-only for the kernel:
+typedef const __attribute__((address_space(4))) double* ConstDblPtr;    // uniform loads allowed
+
+__device__ __forceinline__ int64_t lane_value(int64_t x, int lane) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 struct StateOff {
     const double* table;   // mode 2: (n_cases, 5) FP64 device table
     const int* cell_case;  // mode 2: the case of every cell (null: row 0)
     double one[5];         // mode 1: the offset
     int mode;              // 0 absolute storage, 1 one offset by value, 2 per-case table
-    __device__ __forceinline__ double get(int64_t cell, int v) const {
-        if (mode == 1) return one[v];
-        if (mode == 2) return table[5 * (int64_t)(cell_case ? cell_case[cell] : 0) + v];
-        return 0.0;
+    int sorted;            // mode 2: cell_case is non-decreasing on [0, split) and on [split, n_cells)
+    int64_t split;         // first cell of the second ascending range (the first tet of a batch)
+
+    // The offset of the case of `cell` (lanes may hold different cells; call it where every lane
+    // of the wave that needs a row is active).  With sorted cell_case, a wave whose active lanes
+    // all hold cells between those of its first and last active lanes, both on one side of
+    // `split` and of one case, takes that case's row with wave-uniform loads; otherwise every
+    // lane looks up its own case and row.  Both give the same values.
+    __device__ __forceinline__ void row(int64_t cell, double off[5]) const {
+        if (mode == 1) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) off[v] = one[v];
+            return;
+        }
+        if (mode != 2) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) off[v] = 0.0;
+            return;
+        }
+        if (!cell_case) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) off[v] = ((ConstDblPtr)table)[v];
+            return;
+        }
+        bool wave_row = false;
+        if (sorted) {
+            const uint64_t act = __builtin_amdgcn_read_exec();
+            const int64_t a = lane_value(cell, __builtin_ctzll(act));
+            const int64_t b = lane_value(cell, 63 - __builtin_clzll(act));
+            const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+            if ((lo < split) == (hi < split) && __all(lo <= cell && cell <= hi)) {
+                const int ka = ((ConstIntPtr)cell_case)[lo], kb = ((ConstIntPtr)cell_case)[hi];
+                if (ka == kb) {
+                    const ConstDblPtr r = (ConstDblPtr)table + 5 * (int64_t)ka;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) off[v] = r[v];
+                    wave_row = true;
+                }
+            }
+        }
+        if (!wave_row) {
+            const double* r = table + 5 * (int64_t)cell_case[cell];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) off[v] = r[v];
+        }
     }
 };
 
@@ -103,8 +153,10 @@ __device__ __forceinline__ void load_state(const real* U, int64_t i, const State
 #pragma unroll
     for (int v = 0; v < 5; ++v) u[v] = (double)U[i * 5 + v];
     if (so.mode) {
+        double off[5];
+        so.row(cell, off);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) u[v] += so.get(cell, v);
+        for (int v = 0; v < 5; ++v) u[v] += off[v];
     }
 }
 

@@ -516,6 +516,17 @@ int afr_mesh_create(const AfrMeshDesc* desc, AfrMesh** out) {
             so.table = desc->state_ref;
             so.cell_case = desc->cell_case;
             so.mode = 2;
+            // a wave may share one row when cell_case ascends over the prisms and over the tets
+            // (the concatenated layout: every case's prisms, then every case's tets)
+            std::vector<int> cc((size_t)desc->n_cells);
+            if (!cc.empty() && hipMemcpy(cc.data(), desc->cell_case, cc.size() * sizeof(int),
+                                         hipMemcpyDeviceToHost) != hipSuccess)
+                return fail("afr_mesh_create: cell_case is not a readable device array of n_cells ints");
+            bool asc = true;
+            for (int64_t c = 1; c < desc->n_cells && asc; ++c)
+                if (c != desc->n_prism && cc[c] < cc[c - 1]) asc = false;
+            so.sorted = asc ? 1 : 0;
+            so.split = desc->n_prism;
         } else {
             if (hipMemcpy(so.one, desc->state_ref, sizeof(so.one), hipMemcpyDeviceToHost) != hipSuccess)
                 return fail("afr_mesh_create: state_ref is not a readable device array of 5 doubles");

@@ -121,9 +121,11 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_CELL) k_s
 #pragma unroll
             for (int q = 0; q < 9; ++q) adj_s[q] = a.adj[i * 9 + q];
             real u5[5];
+            double off[5];
+            a.so.row(cell, off);
 #pragma unroll
             for (int v = 0; v < 5; ++v)
-                u5[v] = a.so.mode ? (real)((double)a.U[i * 5 + v] + a.so.get(cell, v)) : a.U[i * 5 + v];
+                u5[v] = a.so.mode ? (real)((double)a.U[i * 5 + v] + off[v]) : a.U[i * 5 + v];
             const real rho = u5[0] > real(1e-10) ? u5[0] : real(1e-10);
             real* r = sv + tl * NF;
             r[0] = rho;
@@ -427,7 +429,9 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
                                             : (axis == 0 ? gl[0] : (axis == 1 ? gl[1] : gl[2]));
                 acc += (role == 0 ? -1.0 : 1.0) * g * cv[q];
             }
-            const double rho = dmax((double)a.U[i * 5] + a.so.get(cell, 0), 1e-10);
+            double off[5];
+            a.so.row(cell, off);
+            const double rho = dmax((double)a.U[i * 5] + off[0], 1e-10);
             st[t] = ((double)a.vol[i * 2 + f] + acc / det) / rho;
             sf[t] = fabs((double)(f == 0 ? a.k[i] : a.w[i]));
         }

@@ -90,12 +90,12 @@ __global__ void __launch_bounds__(kThreads) k_rk_stage(const real* U0, const rea
             const int64_t i = cb * NS + t;
             const double d = (double)dt[i];
             double q[5], qa[5], off[5];
+            so.row(cell, off);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double val = c0 * (double)U0[i * 5 + v] + c2 * d * (double)R[i * 5 + v];
                 if (c1 != 0.0) val += c1 * (double)Ua[i * 5 + v];
                 q[v] = (double)(real)val;  // stage value in the storage precision
-                off[v] = so.get(cell, v);
                 qa[v] = so.mode ? q[v] + off[v] : q[v];
             }
             const bool low_rho = !(qa[0] > 1e-6);
