@@ -20,6 +20,38 @@ namespace afr_step {
 
 constexpr int kThreads = 256;
 
+// local time step of one solution point i of cell c (h = cbrt |vol[c]|)
+template <typename real>
+__device__ __forceinline__ real dt_point(const real* __restrict__ U, const StateOff& so, const real* __restrict__ mu_t,
+                                         const real* __restrict__ det, const real* __restrict__ mfs, int64_t i,
+                                         int64_t c, double h, double mu, int order, int nut_mode) {
+    const double cfl = 0.1, fa = 1.0 / (2 * order + 1), fv = fa * fa;
+    double u5[5];
+    load_state(U, i, so, c, u5);
+    const double rho = u5[0] > 1e-10 ? u5[0] : 1e-10;
+    const double ir = 1.0 / rho;
+    const double u = u5[1] * ir, v = u5[2] * ir, w = u5[3] * ir;
+    const double ke = 0.5 * (u * u + v * v + w * w);
+    double p = (u5[4] - rho * ke) * 0.4;
+    p = p > 1.0 ? p : 1.0;
+    double a2 = 1.4 * p / rho;
+    a2 = a2 > 1e-10 ? a2 : 1e-10;
+    double ws = sqrt(u * u + v * v + w * w) + sqrt(a2);
+    ws = ws > 1e-10 ? ws : 1e-10;
+    const double dta = cfl * fa * h / ws;
+    // nut_mode: the eddy-viscosity array holds nu_t and mu_t = rho nu_t
+    double me = mu + (mu_t ? (nut_mode ? rho * (double)mu_t[i] : (double)mu_t[i]) : 0.0);
+    me = me > 1e-30 ? me : 1e-30;
+    const double dtv = 0.25 * cfl * fv * rho * h * h / me;
+    double den = (double)mfs[i] * ws;
+    den = den > 1e-300 ? den : 1e-300;
+    const double dtg = cfl * fabs((double)det[i]) / den;
+    double d = dta < dtv ? dta : dtv;
+    d = d < dtg ? d : dtg;
+    return (real)d;
+}
+
+// any points per cell: one thread per point, grid-stride
 template <typename real>
 __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ U, StateOff so,
                                                        const real* __restrict__ mu_t,
@@ -27,33 +59,34 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
                                                        const real* __restrict__ vol, real* __restrict__ dt,
                                                        int64_t n_cells, int ns, double mu, int order, int nut_mode) {
     const int64_t n = n_cells * ns;
-    const double cfl = 0.1, fa = 1.0 / (2 * order + 1), fv = fa * fa;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = i / ns;
-        double u5[5];
-        load_state(U, i, so, c, u5);
-        const double rho = u5[0] > 1e-10 ? u5[0] : 1e-10;
-        const double ir = 1.0 / rho;
-        const double u = u5[1] * ir, v = u5[2] * ir, w = u5[3] * ir;
-        const double ke = 0.5 * (u * u + v * v + w * w);
-        double p = (u5[4] - rho * ke) * 0.4;
-        p = p > 1.0 ? p : 1.0;
-        double a2 = 1.4 * p / rho;
-        a2 = a2 > 1e-10 ? a2 : 1e-10;
-        double ws = sqrt(u * u + v * v + w * w) + sqrt(a2);
-        ws = ws > 1e-10 ? ws : 1e-10;
-        const double h = cbrt(fabs((double)vol[c]));
-        const double dta = cfl * fa * h / ws;
-        // nut_mode: the eddy-viscosity array holds nu_t and mu_t = rho nu_t
-        double me = mu + (mu_t ? (nut_mode ? rho * (double)mu_t[i] : (double)mu_t[i]) : 0.0);
-        me = me > 1e-30 ? me : 1e-30;
-        const double dtv = 0.25 * cfl * fv * rho * h * h / me;
-        double den = (double)mfs[i] * ws;
-        den = den > 1e-300 ? den : 1e-300;
-        const double dtg = cfl * fabs((double)det[i]) / den;
-        double d = dta < dtv ? dta : dtv;
-        d = d < dtg ? d : dtg;
-        dt[i] = (real)d;
+        dt[i] = dt_point(U, so, mu_t, det, mfs, i, c, cbrt(fabs((double)vol[c])), mu, order, nut_mode);
+    }
+}
+
+// NS points per cell: a workgroup takes CPB = 256 / NS consecutive cells at a time, forms each
+// cell's length scale once (one thread per cell, into LDS), then one thread per point (no 64-bit
+// division, no per-point cube root; the same values as k_dt_local)
+template <int NS, typename real>
+__global__ void __launch_bounds__(kThreads) k_dt_cells(const real* __restrict__ U, StateOff so,
+                                                       const real* __restrict__ mu_t,
+                                                       const real* __restrict__ det, const real* __restrict__ mfs,
+                                                       const real* __restrict__ vol, real* __restrict__ dt,
+                                                       int64_t n_cells, double mu, int order, int nut_mode) {
+    constexpr int CPB = kThreads / NS;
+    __shared__ double s_h[CPB];
+    const int t = threadIdx.x;
+    for (int64_t cb = (int64_t)blockIdx.x * CPB; cb < n_cells; cb += (int64_t)gridDim.x * CPB) {
+        const int nc = (int)(n_cells - cb < CPB ? n_cells - cb : CPB);
+        if (t < nc) s_h[t] = cbrt(fabs((double)vol[cb + t]));
+        __syncthreads();
+        if (t < nc * NS) {
+            const int lc = t / NS;
+            const int64_t i = cb * NS + t;
+            dt[i] = dt_point(U, so, mu_t, det, mfs, i, cb + lc, s_h[lc], mu, order, nut_mode);
+        }
+        __syncthreads();
     }
 }
 
@@ -179,8 +212,21 @@ template <typename real>
 int launch_dt_local(const real* U, StateOff so, const real* mu_t, const real* det, const real* mfs,
                     const real* vol, real* dt, int64_t n_cells, int ns, double mu, int order, int nut_mode,
                     hipStream_t st) {
-    hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U, so,
-                       mu_t, det, mfs, vol, dt, n_cells, ns, mu, order, nut_mode);
+#define AFR_DT_CELLS(NSV)                                                                                    \
+    hipLaunchKernelGGL((k_dt_cells<NSV, real>), dim3(grid_for(n_cells, kThreads / NSV)), dim3(kThreads), 0, st, U, \
+                       so, mu_t, det, mfs, vol, dt, n_cells, mu, order, nut_mode)
+    if (ns == 1)
+        AFR_DT_CELLS(1);
+    else if (ns == 8)
+        AFR_DT_CELLS(8);
+    else if (ns == 27)
+        AFR_DT_CELLS(27);
+    else if (ns == 64)
+        AFR_DT_CELLS(64);
+    else
+        hipLaunchKernelGGL((k_dt_local<real>), dim3(grid_for(n_cells * ns, kThreads)), dim3(kThreads), 0, st, U,
+                           so, mu_t, det, mfs, vol, dt, n_cells, ns, mu, order, nut_mode);
+#undef AFR_DT_CELLS
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
