@@ -65,9 +65,9 @@ __global__ void __launch_bounds__(kThreads) k_dt_local(const real* __restrict__ 
     }
 }
 
-// NS points per cell: a workgroup takes CPB = 256 / NS consecutive cells at a time, forms each
-// cell's length scale once (one thread per cell, into LDS), then one thread per point (no 64-bit
-// division, no per-point cube root; the same values as k_dt_local)
+// NS points per cell: a workgroup takes CPB = 256 / NS consecutive cells at a time, one thread
+// per point (a 32-bit division by the constant NS instead of a 64-bit one; no barrier, so the
+// loads of successive passes overlap; the same values as k_dt_local)
 template <int NS, typename real>
 __global__ void __launch_bounds__(kThreads) k_dt_cells(const real* __restrict__ U, StateOff so,
                                                        const real* __restrict__ mu_t,
@@ -75,18 +75,15 @@ __global__ void __launch_bounds__(kThreads) k_dt_cells(const real* __restrict__ 
                                                        const real* __restrict__ vol, real* __restrict__ dt,
                                                        int64_t n_cells, double mu, int order, int nut_mode) {
     constexpr int CPB = kThreads / NS;
-    __shared__ double s_h[CPB];
     const int t = threadIdx.x;
+    if (t >= CPB * NS) return;
+    const int lc = t / NS;
     for (int64_t cb = (int64_t)blockIdx.x * CPB; cb < n_cells; cb += (int64_t)gridDim.x * CPB) {
-        const int nc = (int)(n_cells - cb < CPB ? n_cells - cb : CPB);
-        if (t < nc) s_h[t] = cbrt(fabs((double)vol[cb + t]));
-        __syncthreads();
-        if (t < nc * NS) {
-            const int lc = t / NS;
+        const int64_t c = cb + lc;
+        if (c < n_cells) {
             const int64_t i = cb * NS + t;
-            dt[i] = dt_point(U, so, mu_t, det, mfs, i, cb + lc, s_h[lc], mu, order, nut_mode);
+            dt[i] = dt_point(U, so, mu_t, det, mfs, i, c, cbrt(fabs((double)vol[c])), mu, order, nut_mode);
         }
-        __syncthreads();
     }
 }
 

@@ -562,7 +562,8 @@ def main():
                    f"afr_sst::k_sst_update<{P}, 0, float>"]
     kr = {}
     for kname, phase, b, parts in (
-            ("afr_step::k_dt_local<float>", "dt_local", algorithmic_bytes_dt_local(n_sp, mesh.n_cells), None),
+            (f"afr_step::k_dt_cells<{dm.ns}, float>" if dm.ns in (1, 8, 27, 64) else "afr_step::k_dt_local<float>",
+             "dt_local", algorithmic_bytes_dt_local(n_sp, mesh.n_cells), None),
             (" + ".join(sst_kernels), "sst", algorithmic_bytes_sst(n_sp, mesh.n_cells, fr.n_faces), sst_kernels),
             (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
              algorithmic_bytes_visc_volume(n_sp, mesh.n_cells), None),
