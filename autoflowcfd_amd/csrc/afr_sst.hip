@@ -388,6 +388,17 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
     __shared__ double s_t[kWaves][2 * NS];
     __shared__ double s_fld[kWaves][2 * NS];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), t = threadIdx.x & 63;
+    // per-point tables of the three cube axes (flux point of the point on each axis and the
+    // correction-function values at its coordinate), the same for every cell: staged once
+    __shared__ int s_fps[3 * NS];
+    __shared__ real s_gl[3 * NS], s_gr[3 * NS];
+    for (int q = threadIdx.x; q < 3 * NS; q += 64 * kWaves) {
+        s_fps[q] = a.fp_of_sp[q];
+        const int cs = a.coord_of_sp[q];
+        s_gl[q] = a.gL[cs];
+        s_gr[q] = a.gR[cs];
+    }
+    __syncthreads();
     double* st = s_t[wid];
     double* sf = s_fld[wid];
     for (CellWalk cw(a.n_cells, kWaves, wid); cw.it < cw.hi; cw.it += cw.step) {
@@ -396,15 +407,13 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_SST_UPDATE) k
             const int s = t >> 1, f = t & 1;
             const int64_t i = cell * NS + s;
             const double det = (double)a.det[i];
-            // per-lane lookups of the three cube axes, hoisted out of the incidence loop
             int fps[3];
             double gl[3], gr[3];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                fps[ax] = a.fp_of_sp[ax * NS + s];
-                const int cs = a.coord_of_sp[ax * NS + s];
-                gl[ax] = (double)a.gL[cs];
-                gr[ax] = (double)a.gR[cs];
+                fps[ax] = s_fps[ax * NS + s];
+                gl[ax] = (double)s_gl[ax * NS + s];
+                gr[ax] = (double)s_gr[ax * NS + s];
             }
             int ev[kInc];
 #pragma unroll
