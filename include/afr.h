@@ -122,6 +122,11 @@ enum { AFR_F32 = 0, AFR_F64 = 1 };
  *                                      one row is copied at afr_mesh_create (the kernels take it
  *                                      by value); with cell_case the table is read by the kernels
  *                                      and must stay valid and unchanged while the mesh exists.
+ *                                      afr_mesh_create also reads cell_case once: when it ascends
+ *                                      over the prisms and over the tets (the concatenated
+ *                                      layout), a wavefront whose cells share one case reads that
+ *                                      case's row once; any other cell_case gives the same
+ *                                      results through per-cell lookups.
  */
 typedef struct AfrMeshDesc {
     int32_t abi_version;
