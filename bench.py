@@ -176,6 +176,21 @@ def cpu_baseline(order, n_sample=6, budget_s=20.0, workers=None):
                       f"({res[0][1]:.3f} s each on one core)"}
 
 
+def residual_kernel_names(P, knz, ns, v_inv=0, v_sst=0):
+    """{phase: [kernel names as the profiler prints them]} of one prism-mesh residual evaluation at
+    order P with knz-wide interpolation rows, ns stored points per cell and the tile variants in
+    force (afr_mesh_set_variant: prism variant v stages the slot sources when v = 1; SST cell
+    variant v pairs prisms when v = 0)."""
+    return {
+        "dt_local": [f"afr_step::k_dt_cells<{ns}, float>" if ns in (1, 8, 27, 64) else "afr_step::k_dt_local<float>"],
+        "sst": [f"afr_sst::k_sst_cell<{P}, 0, float, {1 - v_sst}>", f"afr_sst::k_sst_face<{P}, {knz}, float>",
+                f"afr_sst::k_sst_update<{P}, 0, float>"],
+        "visc_volume": [f"afr_visc::k_visc_volume<{P}, 0, float>"],
+        "inviscid": [f"afr_prism::k_prism_inviscid<{P}, {knz}, float, {v_inv}>"],
+        "visc_interface": [f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>"],
+    }
+
+
 def traffic_from_profiles(kernel, workload="wallbox82_p2_sst_fp32"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
     WRITE_SIZE scaled by a plain copy's), or None."""
@@ -554,22 +569,17 @@ def main():
     # per-phase rooflines (prism-only mesh: one launch per phase, three for SST), timed with HIP
     # events on the launch stream; PMC traffic from the committed rocprofv3 summary
     P, knz = args.order, dm.rows_knz
-    # kernel names as the profiler prints them, with the tile variant in force (afr_mesh_set_variant:
-    # prism variant v stages the slot sources when v = 1; SST cell variant v pairs prisms when v = 0)
     from autoflowcfd_amd.autotune import get_variant
     v_inv, v_sst = get_variant(dm, "prism_inviscid"), get_variant(dm, "sst_cell")
-    sst_kernels = [f"afr_sst::k_sst_cell<{P}, 0, float, {1 - v_sst}>", f"afr_sst::k_sst_face<{P}, {knz}, float>",
-                   f"afr_sst::k_sst_update<{P}, 0, float>"]
+    kn = residual_kernel_names(P, knz, dm.ns, v_inv, v_sst)
+    phase_bytes = {"dt_local": algorithmic_bytes_dt_local(n_sp, mesh.n_cells),
+                   "sst": algorithmic_bytes_sst(n_sp, mesh.n_cells, fr.n_faces),
+                   "visc_volume": algorithmic_bytes_visc_volume(n_sp, mesh.n_cells), "inviscid": b_inv,
+                   "visc_interface": algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp)}
     kr = {}
-    for kname, phase, b, parts in (
-            (f"afr_step::k_dt_cells<{dm.ns}, float>" if dm.ns in (1, 8, 27, 64) else "afr_step::k_dt_local<float>",
-             "dt_local", algorithmic_bytes_dt_local(n_sp, mesh.n_cells), None),
-            (" + ".join(sst_kernels), "sst", algorithmic_bytes_sst(n_sp, mesh.n_cells, fr.n_faces), sst_kernels),
-            (f"afr_visc::k_visc_volume<{P}, 0, float>", "visc_volume",
-             algorithmic_bytes_visc_volume(n_sp, mesh.n_cells), None),
-            (f"afr_prism::k_prism_inviscid<{P}, {knz}, float, {v_inv}>", "inviscid", b_inv, None),
-            (f"afr_visc::k_visc_iface<{P}, 0, {knz}, float>", "visc_interface",
-             algorithmic_bytes_visc_iface(n_sp, mesh.n_cells, dm.nfp), None)):
+    for phase, parts in kn.items():
+        kname, b = " + ".join(parts), phase_bytes[phase]
+        parts = parts if len(parts) > 1 else None
         t = float(ph[names.index(phase)]) * 1e-3
         tr = [traffic_from_profiles(k) for k in (parts or [kname])]
         kr[kname] = {"phase": phase, "bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK / 1e9,
