@@ -397,7 +397,14 @@ __global__ void __launch_bounds__(64 * kWaves) AFR_WPE_SEL(AFR_WPE_VISC_IFACE) k
                 m_nb1 = a.slot_nb[sidx * 2 + 1];
             }
         }
-        for (int o = t; o < NS * kVP; o += 64) sP[o] = a.vpack[cell * NS * kVP + o];
+        if constexpr (sizeof(real) == 4 && (NS * kVP) % 4 == 0 &&
+                      (NS * kVP + 5 * NFP * 5 + NSV) % 4 == 0) {   // 16-byte aligned rows and waves
+            const float4* src = reinterpret_cast<const float4*>(a.vpack + cell * NS * kVP);
+            float4* dst = reinterpret_cast<float4*>(sP);
+            for (int o = t; o < NS * kVP / 4; o += 64) dst[o] = src[o];
+        } else {
+            for (int o = t; o < NS * kVP; o += 64) sP[o] = a.vpack[cell * NS * kVP + o];
+        }
         wave_sync();
         t = lane_fresh(t);
         // 1. BR1 jumps: lane = (slot, flux point)
