@@ -163,6 +163,15 @@ def test_p3_tet_residual_and_step_match_oracle(prec):
     tol = 2e-4
     assert e(ri, Ri) < tol, f"inviscid {e(ri, Ri):.3e}"
     assert e(rv, Rv) < tol, f"viscous {e(rv, Rv):.3e}"
+    # local time step at 64 stored points per cell (cfl.py:12-147 via the oracle): on straight-sided
+    # tets the metric terms are constant per cell and do not pass through the P3 tables'
+    # conditioning, so the P1 / P2 bound holds (measured 3.7e-16)
+    from oracle.time_step import local_time_step, update_primitives
+    dt_ref = local_time_step(update_primitives(U), g, 3, MU)
+    dt_dev = s._to_user(s.dm.local_time_step(s.U, None, mu=MU)).double().cpu().numpy()
+    e_dt = np.abs(dt_dev - dt_ref).max() / np.abs(dt_ref).max()
+    print(f"[gpu] P3 local dt rel {e_dt:.3e}", flush=True)
+    assert e_dt < 1e-12, f"local dt {e_dt:.3e}"
     dt = 1e-5
     U_ref = ssp_rk3(U, lambda X: inviscid_residual(X, g, ops, fl, bcd) + viscous_residual(X, g, ops, fl, bcd, MU),
                     np.full(U.shape[:2], dt), ops, 0)
